@@ -1,0 +1,330 @@
+"""MI355X (gfx950) H-matrix BEM hot path: Python binding of the C ABI in
+include/bem_hip.h (lib3dbem_hip.so next to this file).
+
+There is no CPU fallback: every call goes to the HIP library, and importing
+the binding raises if the library has not been built (``make -C 3d_bem_amd``
+or ``__graft_entry__.build()``).  The package directory name starts with a
+digit, so import it with ``importlib.import_module("3d_bem_amd")``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import meshes  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib3dbem_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "bem_hip.h")
+
+BEM_OK, BEM_ERR_ARG, BEM_ERR_HIP, BEM_ERR_STATE, BEM_ERR_NAN, BEM_ERR_NOCONV = 0, -1, -2, -3, -4, -5
+PHI, DPHI = 0, 1
+FP64, FP32 = 0, 1
+
+_i64, _dbl, _p, _int, _u64 = ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+_SIGS = {
+    "bem_create": (_int, [ctypes.POINTER(_p), _int]),
+    "bem_destroy": (_int, [_p]),
+    "bem_last_error": (ctypes.c_char_p, [_p]),
+    "bem_upload_mesh": (_int, [_p, _i64, _p, _p, _p, _p, _dbl]),
+    "bem_set_bc": (_int, [_p, _p, _p, _p]),
+    "bem_upload_partition": (_int, [_p, _i64, _p, _p, _p, _p, _p]),
+    "bem_set_frequency": (_int, [_p, _dbl, _dbl, _dbl]),
+    "bem_assemble": (_int, [_p, _int, _dbl, _i64, _int, _u64, ctypes.POINTER(_p)]),
+    "bem_hmat_destroy": (_int, [_p]),
+    "bem_hmvm": (_int, [_p, _p, _p, _p, _int]),
+    "bem_block_info": (_int, [_p, _i64, ctypes.POINTER(_i64)]),
+    "bem_block_download": (_int, [_p, _i64, _p, _p, _p]),
+    "bem_hmat_stats": (_int, [_p, _p]),
+    "bem_gmres": (_int, [_p, _p, _p, _p, _dbl, _i64, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)]),
+    "bem_reconstruct": (_int, [_p, _p, _dbl, _dbl, _p, _p, _p]),
+    "bem_entries4": (_int, [_p, _i64, _p, _p]),
+    "bem_entries": (_int, [_p, _int, _i64, _p, _p]),
+    "bem_malloc": (_int, [_p, ctypes.c_size_t, ctypes.POINTER(_p)]),
+    "bem_free": (_int, [_p, _p]),
+    "bem_memcpy_htod": (_int, [_p, _p, _p, ctypes.c_size_t]),
+    "bem_memcpy_dtoh": (_int, [_p, _p, _p, ctypes.c_size_t]),
+    "bem_synchronize": (_int, [_p]),
+    "bem_profile": (_int, [_p, _int, _p]),
+    "bem_solver_create": (_int, [ctypes.POINTER(_p), _int, _i64, _p, _p, _i64, _dbl]),
+    "bem_solver_destroy": (_int, [_p]),
+    "bem_solver_last_error": (ctypes.c_char_p, [_p]),
+    "bem_solver_context": (_p, [_p]),
+    "bem_solver_layout": (_int, [_p, _p, ctypes.POINTER(_i64), _p, _i64]),
+    "bem_solver_set_sources": (_int, [_p, _i64, _p, _int, _p]),
+    "bem_solver_solve": (_int, [_p, _dbl, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _p, _p, _p, _p]),
+}
+EXPORTS = tuple(_SIGS)
+
+_LIB = None
+
+
+class BemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library; raises if it is missing (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C 3d_bem_amd` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class DeviceArray:
+    """Complex128 vector in device memory owned by a Context."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        p = _p()
+        ctx._check(lib().bem_malloc(ctx.h, 16 * self.n, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, x):
+        x = _c(x, np.complex128)
+        assert x.size == self.n
+        self.ctx._check(lib().bem_memcpy_htod(self.ctx.h, self.ptr, _ptr(x), 16 * self.n))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, np.complex128)
+        self.ctx._check(lib().bem_memcpy_dtoh(self.ctx.h, _ptr(out), self.ptr, 16 * self.n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().bem_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device + stream (bem_ctx_t)."""
+
+    def __init__(self, device=0):
+        h = _p()
+        rc = lib().bem_create(ctypes.byref(h), int(device))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_create failed (no HIP device?)")
+        self.h = h
+        self.n = 0
+        self.blocks = None
+
+    def _check(self, rc):
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_mesh(self, nodes9, normal3, area, colloc3, hbar):
+        nodes9, normal3 = _c(nodes9, np.float64), _c(normal3, np.float64)
+        area, colloc3 = _c(area, np.float64), _c(colloc3, np.float64)
+        self.n = len(area)
+        self._check(lib().bem_upload_mesh(self.h, self.n, _ptr(nodes9), _ptr(normal3), _ptr(area), _ptr(colloc3),
+                                          float(hbar)))
+
+    def set_bc(self, alpha, beta, f):
+        a, b, g = (_c(v, np.complex128) for v in (alpha, beta, f))
+        self._check(lib().bem_set_bc(self.h, _ptr(a), _ptr(b), _ptr(g)))
+
+    def upload_partition(self, blocks5):
+        B = _c(blocks5, np.int64)
+        cols = [np.ascontiguousarray(B[:, q]) for q in range(4)]
+        adm = np.ascontiguousarray(B[:, 4].astype(np.uint8))
+        self._check(lib().bem_upload_partition(self.h, len(B), *[_ptr(c) for c in cols], _ptr(adm)))
+        self.blocks = B.copy()
+
+    def set_frequency(self, k, cc):
+        self._check(lib().bem_set_frequency(self.h, float(k), float(np.real(cc)), float(np.imag(cc))))
+
+    def assemble(self, kind, tol, maxrank=0, precision=FP64, seed=1):
+        h = _p()
+        self._check(lib().bem_assemble(self.h, int(kind), float(tol), int(maxrank), int(precision), int(seed),
+                                       ctypes.byref(h)))
+        return HMatrix(self, h)
+
+    def entries4(self, ij):
+        ij = _c(ij, np.int64).reshape(-1, 2)
+        out = np.empty((len(ij), 4), np.complex128)
+        self._check(lib().bem_entries4(self.h, len(ij), _ptr(ij), _ptr(out)))
+        return out
+
+    def entries(self, kind, ij):
+        ij = _c(ij, np.int64).reshape(-1, 2)
+        out = np.empty(len(ij), np.complex128)
+        self._check(lib().bem_entries(self.h, int(kind), len(ij), _ptr(ij), _ptr(out)))
+        return out
+
+    def array(self, x=None, n=None):
+        d = DeviceArray(self, self.n if n is None else n)
+        if x is not None:
+            d.upload(x)
+        return d
+
+    def reconstruct(self, x_dev, rho, freq):
+        phi, dphi, p = self.array(), self.array(), self.array()
+        self._check(lib().bem_reconstruct(self.h, x_dev.ptr, float(rho), float(freq), phi.ptr, dphi.ptr, p.ptr))
+        return phi.download(), dphi.download(), p.download()
+
+    def synchronize(self):
+        self._check(lib().bem_synchronize(self.h))
+
+    def profile(self, enable=-1):
+        out = np.zeros(12)
+        self._check(lib().bem_profile(self.h, int(enable), _ptr(out)))
+        return out
+
+
+class HMatrix:
+    """Assembled H-matrix on the device (bem_hmat_t)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bem_hmat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = np.zeros(8)
+        lib().bem_hmat_stats(self.h, _ptr(s))
+        keys = ["evaluated", "max_rank", "mean_rank", "bytes", "assembly_ms", "overflow_blocks", "dense_ms", "aca_ms"]
+        return dict(zip(keys, s.tolist()))
+
+    def rank(self, b):
+        r = _i64()
+        self.ctx._check(lib().bem_block_info(self.h, int(b), ctypes.byref(r)))
+        return r.value
+
+    def block(self, b):
+        """Dense block (m x n) or (U, alpha, V) in the reference layout."""
+        r0, m, c0, n, adm = self.ctx.blocks[b]
+        k = self.rank(b)
+        if k < 0:
+            F = np.empty((m, n), np.complex128, order="F")
+            self.ctx._check(lib().bem_block_download(self.h, int(b), _ptr(F), None, None))
+            return F
+        U = np.empty((m, k), np.complex128, order="F")
+        al = np.empty(k, np.complex128)
+        V = np.empty((k, n), np.complex128, order="F")
+        self.ctx._check(lib().bem_block_download(self.h, int(b), _ptr(U), _ptr(al), _ptr(V)))
+        return U, al, V
+
+    def mvm(self, x, y=None, accumulate=False):
+        """y (+)= H x with host arrays (copies through device buffers)."""
+        xd = self.ctx.array(x)
+        yd = self.ctx.array(np.zeros(self.ctx.n, np.complex128) if y is None else y)
+        self.mvm_dev(xd, yd, accumulate)
+        return yd.download()
+
+    def mvm_dev(self, xd, yd, accumulate=False):
+        self.ctx._check(lib().bem_hmvm(self.ctx.h, self.h, xd.ptr, yd.ptr, 1 if accumulate else 0))
+
+    def gmres(self, b, tol, restart=300, maxit=100000, x0=None):
+        bd = self.ctx.array(b)
+        xd = self.ctx.array(np.zeros(self.ctx.n, np.complex128) if x0 is None else x0)
+        its, rel = _i64(), _dbl()
+        rc = lib().bem_gmres(self.ctx.h, self.h, bd.ptr, xd.ptr, float(tol), int(restart), int(maxit),
+                             ctypes.byref(its), ctypes.byref(rel))
+        if rc not in (BEM_OK, BEM_ERR_NOCONV):
+            self.ctx._check(rc)
+        return xd.download(), its.value, rel.value, rc
+
+
+STAT_KEYS = ["t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres", "t_total", "gmres_iters", "residual",
+             "max_rank_phi", "max_rank_dphi", "evaluated_phi", "evaluated_dphi", "near_blocks", "far_blocks",
+             "refine_steps", "fp32_gmres_iters", "reserved"]
+
+
+class Solver:
+    """BoundaryElementSolver::hMatrixSolve on the GPU (bem_solver_t).
+
+    tri: (n, 9) nodes in input order; bc: (n, 6) Robin a, b, g (re/im)."""
+
+    def __init__(self, tri, bc, leaf=50, eta=3.0, device=0):
+        tri, bc = _c(tri, np.float64), _c(bc, np.float64)
+        h = _p()
+        rc = lib().bem_solver_create(ctypes.byref(h), int(device), len(tri), _ptr(tri), _ptr(bc), int(leaf),
+                                     float(eta))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_create failed")
+        self.h = h
+        self.n = len(tri)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bem_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self):
+        perm = np.empty(self.n, np.int64)
+        nb = _i64()
+        lib().bem_solver_layout(self.h, _ptr(perm), ctypes.byref(nb), None, 0)
+        blocks = np.empty((nb.value, 5), np.int64)
+        lib().bem_solver_layout(self.h, None, ctypes.byref(nb), _ptr(blocks), nb.value)
+        return perm, blocks
+
+    def set_sources(self, points=None, plane=None):
+        pts = _c(points if points is not None else np.zeros((0, 6)), np.float64)
+        pl = _c(plane if plane is not None else np.zeros(5), np.float64)
+        rc = lib().bem_solver_set_sources(self.h, len(pts), _ptr(pts), 1 if plane is not None else 0, _ptr(pl))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_sources")
+
+    def solve(self, freq, c=343.0, rho=1.21, coupling=1, aca_tol=1e-6, maxrank=0, gmres_tol=None, restart=300,
+              seed=1, precision=FP64):
+        n = self.n
+        phi, dphi, pr = (np.empty(n, np.complex128) for _ in range(3))
+        st = np.zeros(16)
+        gt = -1.0 if gmres_tol is None else float(gmres_tol)
+        rc = lib().bem_solver_solve(self.h, float(freq), float(c), float(rho), int(coupling), float(aca_tol),
+                                    int(maxrank), gt, int(restart), int(seed), int(precision), _ptr(phi),
+                                    _ptr(dphi), _ptr(pr), _ptr(st))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
+        return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))

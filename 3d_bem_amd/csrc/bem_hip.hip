@@ -1,0 +1,1558 @@
+// gfx950 kernels and the C ABI (include/bem_hip.h) of the H-matrix BEM path.
+//
+// Layout in HBM (all cluster order):
+//   mesh      SoA doubles (collocation point, normal, area, 16 quadrature
+//             points per element as q * n + j), near-singular CSR with its
+//             polar sub-triangle geometry, self-integral radii
+//   dense     every non-admissible block column-major, packed back to back,
+//             complex FP64; processed as 64 x 64 tiles
+//   low rank  U (m x k col-major), alpha (k), V (k x n row-major) per block,
+//             packed back to back after compaction to the final ranks
+//   MVM       per tile / per low-rank block partial vectors, summed per row
+//             in a fixed order (no atomics: deterministic results)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bem_hip.h"
+#include "bem_kernels.hpp"
+#include "bem_setup.hpp"
+
+using namespace bemdev;
+
+namespace {
+
+constexpr int kWG = 256;        // workgroup size of the block kernels (4 wavefronts)
+constexpr int kTile = 64;       // dense tile edge
+constexpr int kCapFirst = 48;   // first-pass ACA rank capacity per block
+constexpr int kCapMax = 1024;   // second-pass capacity limit (LDS: (3 cap + 2) x 16 B <= 48 KiB)
+constexpr double kTiny = 10.0 * 2.220446049250313e-16;  // global.h:53
+
+struct Tile {
+    int64_t dest;      // element offset of the tile origin in dense storage
+    int64_t part;      // offset of this tile's partial vector (MVM)
+    int32_t ld;        // rows of the owning block
+    int32_t row0, col0, rows, cols;   // absolute first row/col, extent
+    int32_t pad;
+};
+
+struct Fix {
+    int64_t dest;      // element offset in dense storage
+    int32_t i, j;
+    int64_t t;         // near-singular pair index, -1 for the self entry
+};
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_quadpoints(int64_t n, const double* __restrict__ nodes, double* qx, double* qy, double* qz)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double* v = nodes + 9 * j;
+    for (int q = 0; q < 16; q++) {
+        double x1 = kTriX[q][0], x2 = kTriX[q][1];
+        double x3 = (1.0 - x2) - x1;
+        qx[q * n + j] = (x1 * v[0] + x2 * v[3]) + x3 * v[6];
+        qy[q * n + j] = (x1 * v[1] + x2 * v[4]) + x3 * v[7];
+        qz[q * n + j] = (x1 * v[2] + x2 * v[5]) + x3 * v[8];
+    }
+}
+
+// Dense near-field tiles (K1): regular quadrature of every (row, col) pair of
+// a 64 x 64 tile with the column data staged in LDS.  Self and
+// near-singular pairs are skipped here and written by k_fixup.
+__global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile* __restrict__ tiles,
+                                               bem_c64* outPhi, bem_c64* outDPhi)
+{
+    __shared__ double sq[3][16][kTile];
+    __shared__ double sn[3][kTile], sp[3][kTile], sa[kTile];
+    const Tile T = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int64_t n = M.n;
+    for (int idx = tid; idx < 16 * kTile; idx += kWG) {
+        int q = idx / kTile, c = idx % kTile;
+        if (c < T.cols) {
+            int64_t j = T.col0 + c;
+            sq[0][q][c] = M.qx[q * n + j];
+            sq[1][q][c] = M.qy[q * n + j];
+            sq[2][q][c] = M.qz[q * n + j];
+        }
+    }
+    for (int c = tid; c < T.cols; c += kWG) {
+        int64_t j = T.col0 + c;
+        sn[0][c] = M.nx[j]; sn[1][c] = M.ny[j]; sn[2][c] = M.nz[j];
+        sp[0][c] = M.px[j]; sp[1][c] = M.py[j]; sp[2][c] = M.pz[j];
+        sa[c] = M.area[j];
+    }
+    __syncthreads();
+    const int r = tid & (kTile - 1), cg = tid >> 6;
+    if (r >= T.rows) return;
+    const int64_t i = T.row0 + r;
+    const double px = M.px[i], py = M.py[i], pz = M.pz[i];
+    const double npx = M.nx[i], npy = M.ny[i], npz = M.nz[i];
+    for (int c = cg; c < T.cols; c += 4) {
+        const int64_t j = T.col0 + c;
+        if (i == j) continue;
+        double dx = px - sp[0][c], dy = py - sp[1][c], dz = pz - sp[2][c];
+        if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
+        bem_ops4 o = regular_ops(F.k, px, py, pz, npx, npy, npz, sn[0][c], sn[1][c], sn[2][c], sa[c], &sq[0][0][c],
+                                 &sq[1][0][c], &sq[2][0][c], kTile);
+        bem_c64 X, Y;
+        bem_xy(&o, F.cc, 0, &X, &Y);
+        int64_t d = T.dest + r + (int64_t)c * T.ld;
+        if (outPhi) outPhi[d] = bem_phi_entry(X, Y, F.subst[j], F.cphi[j]);
+        if (outDPhi) outDPhi[d] = bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
+    }
+}
+
+// Self and near-singular entries of the dense blocks (K2), one per thread,
+// over a list sorted by cost.
+__global__ __launch_bounds__(kWG) void k_fixup(DevMesh M, DevFreq F, const Fix* __restrict__ fx, int64_t nfix,
+                                               bem_c64* outPhi, bem_c64* outDPhi)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfix) return;
+    Fix f = fx[t];
+    int64_t i = f.i, j = f.j;
+    bem_ops4 o;
+    if (f.t < 0) o = self_ops(F.k, M.self_geo + kSelf * i);
+    else
+        o = near_ops(F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
+                     M.nr_geo + M.nr_off[f.t], M.nr_nsub[f.t]);
+    if (outPhi) outPhi[f.dest] = bc_entry(F, 0, o, i, j);
+    if (outDPhi) outDPhi[f.dest] = bc_entry(F, 1, o, i, j);
+}
+
+__global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int64_t* ij, bem_c64* out)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    int64_t i = ij[2 * t], j = ij[2 * t + 1];
+    bem_ops4 o = ops_general(M, F.k, i, j);
+    if (mode == 2) {
+        out[4 * t] = o.L; out[4 * t + 1] = o.M; out[4 * t + 2] = o.Mt; out[4 * t + 3] = o.N;
+    } else {
+        out[t] = bc_entry(F, mode, o, i, j);
+    }
+}
+
+// -------------------------------------------------- workgroup reductions
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int s = 32; s > 0; s >>= 1) x = x + __shfl_down(x, s, 64);
+    return x;
+}
+__device__ __forceinline__ bem_c64 wave_sum(bem_c64 x)
+{
+    for (int s = 32; s > 0; s >>= 1) {
+        x.re = x.re + __shfl_down(x.re, s, 64);
+        x.im = x.im + __shfl_down(x.im, s, 64);
+    }
+    return x;
+}
+
+struct ArgMax { double v; int64_t i; };
+
+__device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b)
+{
+    if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+    return a;
+}
+
+// first index of the largest |v_i|^2 over [0, len); index `excl` counts as 0
+__device__ ArgMax block_argmax(const bem_c64* v, int64_t len, int64_t excl, ArgMax* sred)
+{
+    const int tid = threadIdx.x;
+    ArgMax a{-1.0, (int64_t)1 << 62};
+    for (int64_t t = tid; t < len; t += kWG) {
+        double x = t == excl ? 0.0 : c64_abs2(v[t]);
+        if (x > a.v) { a.v = x; a.i = t; }
+    }
+    for (int s = 32; s > 0; s >>= 1) {
+        ArgMax o;
+        o.v = __shfl_xor(a.v, s, 64);
+        o.i = __shfl_xor(a.i, s, 64);
+        a = better(a, o);
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) sred[tid >> 6] = a;
+    __syncthreads();
+    ArgMax r = sred[0];
+    for (int w = 1; w < kWG / 64; w++) r = better(r, sred[w]);
+    return r;
+}
+
+struct AcaArgs {
+    DevMesh M;
+    DevFreq F;
+    int kind;
+    const int32_t* blist;
+    const int64_t *row0, *rows, *col0, *cols;
+    const int64_t *uoff, *voff, *aoff;
+    const int32_t* cap;
+    bem_c64 *U, *V, *A;
+    int32_t* rank_out;
+    int32_t* flag_out;    // 1: rank capacity reached, 2: non-finite
+    int64_t* evals_out;
+    double tol;
+    int64_t maxrank;
+    uint64_t seed;
+    const int64_t* piv_off;
+    const int32_t* piv;
+};
+
+// one 64-lane canonical reduction per task, tasks spread over the waves:
+// task 0 |row|^2, 1 |col|^2, 2 + 2l col^H U_l, 3 + 2l row^H V_l  (l < r)
+__device__ void aca_dots(const bem_c64* col, const bem_c64* row, const bem_c64* U, const bem_c64* V, int64_t m,
+                         int64_t n, int r, bem_c64* sdots)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntask = 2 + 2 * r;
+    for (int task = w; task < ntask; task += kWG / 64) {
+        if (task < 2) {
+            const bem_c64* v = task == 0 ? row : col;
+            int64_t len = task == 0 ? n : m;
+            double p = 0.0;
+            for (int64_t t = lane; t < len; t += 64) p = p + c64_abs2(v[t]);
+            p = wave_sum(p);
+            if (lane == 0) sdots[task] = c64(p, 0.0);
+        } else {
+            int l = (task - 2) >> 1;
+            bool isc = ((task - 2) & 1) == 0;
+            const bem_c64* a = isc ? col : row;
+            const bem_c64* b = isc ? U + (int64_t)l * m : V + (int64_t)l * n;
+            int64_t len = isc ? m : n;
+            bem_c64 p = c64(0, 0);
+            for (int64_t t = lane; t < len; t += 64) p = c64_fma(c64_conj(a[t]), b[t], p);
+            p = wave_sum(p);
+            if (lane == 0) sdots[task] = p;
+        }
+    }
+}
+
+// Frobenius-norm estimate update of partialPivotACA (hmatrix.cpp:568-572);
+// every thread computes it identically from the LDS dot products
+__device__ __forceinline__ double aca_update_norm(bem_c64 alpha, const bem_c64* A, int r, const bem_c64* sdots,
+                                                  bem_c64& S)
+{
+    double gain = (c64_abs2(alpha) * sdots[0].re) * sdots[1].re;
+    bem_c64 s = S;
+    s.re = s.re + gain;
+    bem_c64 ca2 = c64_scale(c64_conj(alpha), 2.0);
+    for (int l = 0; l < r; l++) {
+        bem_c64 t = c64_mul(c64_mul(c64_mul(ca2, A[l]), sdots[2 + 2 * l]), sdots[3 + 2 * l]);
+        s = c64_add(s, t);
+    }
+    S = s;
+    return gain;
+}
+
+// row i of the block minus the current cross approximation, into dst
+__device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
+                        const bem_c64* U, const bem_c64* V, const bem_c64* A, bem_c64* dst, bem_c64* sw)
+{
+    for (int l = threadIdx.x; l < r; l += kWG) sw[l] = c64_mul(U[(int64_t)l * m + i], A[l]);
+    __syncthreads();
+    for (int64_t j = threadIdx.x; j < n; j += kWG) {
+        bem_c64 e = entry_general(a.M, a.F, a.kind, r0 + i, c0 + j);
+        bem_c64 s = c64(0, 0);
+        for (int l = 0; l < r; l++) s = c64_fma(sw[l], V[(int64_t)l * n + j], s);
+        dst[j] = c64_sub(e, s);
+    }
+    __syncthreads();
+}
+
+// column j of the block minus the current cross approximation, into dst
+__device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
+                        const bem_c64* U, const bem_c64* V, const bem_c64* A, bem_c64* dst, bem_c64* sw)
+{
+    for (int l = threadIdx.x; l < r; l += kWG) sw[l] = c64_mul(A[l], V[(int64_t)l * n + j]);
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < m; i += kWG) {
+        bem_c64 e = entry_general(a.M, a.F, a.kind, r0 + i, c0 + j);
+        bem_c64 s = c64(0, 0);
+        for (int l = 0; l < r; l++) s = c64_fma(U[(int64_t)l * m + i], sw[l], s);
+        dst[i] = c64_sub(e, s);
+    }
+    __syncthreads();
+}
+
+// ACA (K3): one workgroup per admissible block.  Plain variant =
+// BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra variant =
+// BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).  Dynamic LDS:
+// sw[cap] then sdots[2 cap + 2].
+template <bool EXTRA>
+__global__ __launch_bounds__(kWG) void k_aca(AcaArgs a)
+{
+    extern __shared__ bem_c64 dyn[];
+    __shared__ ArgMax sred[kWG / 64];
+    const int b = a.blist[blockIdx.x];
+    const int64_t m = a.rows[b], n = a.cols[b], r0 = a.row0[b], c0 = a.col0[b];
+    const int cap = a.cap[b];
+    int64_t maxRank = m < n ? m : n;
+    if (a.maxrank > 0 && a.maxrank < maxRank) maxRank = a.maxrank;
+    bem_c64* U = a.U + a.uoff[b];
+    bem_c64* V = a.V + a.voff[b];
+    bem_c64* A = a.A + a.aoff[b];
+    bem_c64* sw = dyn;
+    bem_c64* sdots = dyn + cap;
+    const double tiny2 = kTiny * kTiny, tol2 = a.tol * a.tol;
+    int r = 0, flag = 0;
+    int64_t ev = 0;
+    bem_c64 S = c64(0, 0);
+    if (!EXTRA) {
+        int64_t j = bem_rand_bounded(a.seed, 0, (uint64_t)b, 0, n);
+        aca_col(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
+        ev += m;
+        int64_t i = -1;
+        while (r < maxRank) {
+            if (r >= cap) { flag = 1; break; }
+            const bem_c64* wcol = r >= 1 ? U + (int64_t)(r - 1) * m : U;
+            ArgMax am = block_argmax(wcol, m, r >= 1 ? i : -1, sred);
+            i = am.i;
+            if (am.v <= tiny2 && r >= 1) break;
+            bem_c64* row = V + (int64_t)r * n;
+            aca_row(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+            ev += n;
+            ArgMax ar = block_argmax(row, n, -1, sred);
+            if (ar.v <= tiny2 && r >= 1) break;
+            bem_c64* col = U + (int64_t)r * m;
+            aca_col(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+            ev += m;
+            bem_c64 alpha = c64_div(c64(1.0, 0.0), row[ar.i]);
+            if (threadIdx.x == 0) A[r] = alpha;
+            aca_dots(col, row, U, V, m, n, r, sdots);
+            __syncthreads();
+            double gain = aca_update_norm(alpha, A, r, sdots, S);
+            r++;
+            __syncthreads();
+            if (!(isfinite(gain))) { flag = 2; break; }
+            if (a.tol > 0 && gain < tol2 * S.re) break;
+        }
+    } else {
+        const int64_t p0 = a.piv_off[b], p1 = a.piv_off[b + 1];
+        bool done = false;
+        for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
+            int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
+            bem_c64 pe = entry_general(a.M, a.F, a.kind, r0 + i, c0 + j);
+            ev += 1;
+            if (c64_abs2(pe) < tiny2) continue;
+            bem_c64 approx = c64(0, 0);
+            for (int l = 0; l < r; l++)
+                approx = c64_add(approx, c64_mul(c64_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
+            if (c64_abs2(c64_sub(pe, approx)) < tol2 * c64_abs2(pe)) continue;
+            if (r >= cap) { flag = 1; break; }
+            aca_col(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
+            ev += m;
+            bool first = true;
+            while (r < maxRank) {
+                if (r >= cap) { flag = 1; done = true; break; }
+                const bem_c64* wcol = first ? U + (int64_t)r * m : U + (int64_t)(r - 1) * m;
+                ArgMax am = block_argmax(wcol, m, first ? -1 : i, sred);
+                i = am.i;
+                first = false;
+                if (c64_abs2(wcol[i]) <= tiny2 && r >= 1) break;
+                bem_c64* row = V + (int64_t)r * n;
+                aca_row(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+                ev += n;
+                ArgMax ar = block_argmax(row, n, -1, sred);
+                if (ar.v <= tiny2 && r >= 1) break;
+                bem_c64* col = U + (int64_t)r * m;
+                aca_col(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+                ev += m;
+                bem_c64 alpha = c64_div(c64(1.0, 0.0), row[ar.i]);
+                if (threadIdx.x == 0) A[r] = alpha;
+                aca_dots(col, row, U, V, m, n, r, sdots);
+                __syncthreads();
+                bem_c64 oldS = S;
+                double gain = aca_update_norm(alpha, A, r, sdots, S);
+                r++;
+                __syncthreads();
+                if (!(isfinite(gain))) { flag = 2; done = true; break; }
+                double target = tol2 * S.re;
+                if (a.tol > 0 && gain < target) {
+                    if (gain < 0.1 * target) { r--; S = oldS; }
+                    break;
+                }
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        a.rank_out[b] = r;
+        a.flag_out[b] = flag;
+        a.evals_out[b] = ev;
+    }
+}
+
+// copy the first rank columns/rows of the ACA scratch into the packed
+// storage (rank 0 becomes a zero rank-1 block, hmatrix.cpp:587-592)
+__global__ __launch_bounds__(kWG) void k_compact(const int32_t* blist, const int64_t* rows, const int64_t* cols,
+                                                 const int64_t* suoff, const int64_t* svoff, const int64_t* saoff,
+                                                 const bem_c64* sU, const bem_c64* sV, const bem_c64* sA,
+                                                 const int32_t* srank, const int64_t* duoff, const int64_t* dvoff,
+                                                 const int64_t* daoff, bem_c64* dU, bem_c64* dV, bem_c64* dA)
+{
+    const int b = blist[blockIdx.x];
+    const int64_t m = rows[b], n = cols[b];
+    const int r = srank[b];
+    const int64_t nu = (int64_t)(r > 0 ? r : 1) * m, nv = (int64_t)(r > 0 ? r : 1) * n;
+    for (int64_t t = threadIdx.x; t < nu; t += kWG) dU[duoff[b] + t] = r > 0 ? sU[suoff[b] + t] : c64(0, 0);
+    for (int64_t t = threadIdx.x; t < nv; t += kWG) dV[dvoff[b] + t] = r > 0 ? sV[svoff[b] + t] : c64(0, 0);
+    for (int t = threadIdx.x; t < (r > 0 ? r : 1); t += kWG) dA[daoff[b] + t] = r > 0 ? sA[saoff[b] + t] : c64(0, 0);
+}
+
+// ------------------------------------------------------------- H-MVM
+
+// near-field tiles (K4): partial[tile][row] = sum_c F[row, c] x[c]
+__global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tiles, const bem_c64* __restrict__ F,
+                                                   const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+{
+    __shared__ bem_c64 red[4][kTile];
+    const Tile T = tiles[blockIdx.x];
+    const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
+    bem_c64 acc = c64(0, 0);
+    if (r < T.rows) {
+        const bem_c64* f = F + T.dest + r;
+        for (int c = cg; c < T.cols; c += 4) acc = c64_fma(f[(int64_t)c * T.ld], x[T.col0 + c], acc);
+    }
+    red[cg][r] = acc;
+    __syncthreads();
+    if (cg == 0 && r < T.rows)
+        part[T.part + r] = c64_add(c64_add(c64_add(red[0][r], red[1][r]), red[2][r]), red[3][r]);
+}
+
+// low-rank blocks (K5): t = alpha o (V x_J) in LDS, partial = U t
+__global__ __launch_bounds__(kWG) void k_mvm_lowrank(const int32_t* blist, const int64_t* row0, const int64_t* rows,
+                                                     const int64_t* col0, const int64_t* cols, const int32_t* rank,
+                                                     const int64_t* uoff, const int64_t* voff, const int64_t* aoff,
+                                                     const bem_c64* __restrict__ U, const bem_c64* __restrict__ V,
+                                                     const bem_c64* __restrict__ A, const bem_c64* __restrict__ x,
+                                                     const int64_t* poff, bem_c64* __restrict__ part)
+{
+    extern __shared__ bem_c64 st[];
+    const int b = blist[blockIdx.x];
+    const int64_t m = rows[b], n = cols[b];
+    const int k = rank[b];
+    const bem_c64* Ub = U + uoff[b];
+    const bem_c64* Vb = V + voff[b];
+    const bem_c64* xs = x + col0[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int l = w; l < k; l += kWG / 64) {
+        const bem_c64* v = Vb + (int64_t)l * n;
+        bem_c64 p = c64(0, 0);
+        for (int64_t j = lane; j < n; j += 64) p = c64_fma(v[j], xs[j], p);
+        p = wave_sum(p);
+        if (lane == 0) st[l] = c64_mul(A[aoff[b] + l], p);
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < m; i += kWG) {
+        bem_c64 acc = c64(0, 0);
+        for (int l = 0; l < k; l++) acc = c64_fma(Ub[(int64_t)l * m + i], st[l], acc);
+        part[poff[b] + i] = acc;
+    }
+}
+
+// y_i (+)= sum of the partial vectors covering row i, in partition order
+__global__ void k_mvm_gather(int64_t n, const int64_t* __restrict__ gptr, const int64_t* __restrict__ gidx,
+                             const bem_c64* __restrict__ part, bem_c64* y, int accumulate)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bem_c64 acc = accumulate ? y[i] : c64(0, 0);
+    for (int64_t p = gptr[i]; p < gptr[i + 1]; p++) acc = c64_add(acc, part[gidx[p]]);
+    y[i] = acc;
+}
+
+// ------------------------------------------------------------- GMRES
+
+// out[l] = V_l^H w for l < cnt, one workgroup per l
+__global__ __launch_bounds__(kWG) void k_dots(int64_t n, const bem_c64* V, const bem_c64* w, bem_c64* out)
+{
+    __shared__ bem_c64 red[kWG / 64];
+    const bem_c64* v = V + (int64_t)blockIdx.x * n;
+    bem_c64 p = c64(0, 0);
+    for (int64_t i = threadIdx.x; i < n; i += kWG) p = c64_fma(c64_conj(v[i]), w[i], p);
+    p = wave_sum(p);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bem_c64 s = red[0];
+        for (int q = 1; q < kWG / 64; q++) s = c64_add(s, red[q]);
+        out[blockIdx.x] = s;
+    }
+}
+
+// w -= sum_l V_l h_l ; hacc[l] += h[l]
+__global__ void k_orth_update(int64_t n, int cnt, const bem_c64* V, const bem_c64* h, bem_c64* w)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bem_c64 acc = w[i];
+    for (int l = 0; l < cnt; l++) acc = c64_fms(h[l], V[(int64_t)l * n + i], acc);
+    w[i] = acc;
+}
+
+// partial sums of |w|^2 per workgroup
+__global__ __launch_bounds__(kWG) void k_norm2_part(int64_t n, const bem_c64* w, double* part)
+{
+    __shared__ double red[kWG / 64];
+    double p = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG)
+        p = p + c64_abs2(w[i]);
+    p = wave_sum(p);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void k_norm2_final(int nparts, const double* part, double* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int q = 0; q < nparts; q++) s += part[q];
+        *out = sqrt(s);
+    }
+}
+
+// dst = src / *nrm (0 if the norm is 0)
+__global__ void k_scale_by_norm(int64_t n, const bem_c64* src, const double* nrm, bem_c64* dst)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = *nrm;
+    dst[i] = d > 0 ? c64_scale(src[i], 1.0 / d) : c64(0, 0);
+}
+
+// x += sum_l V_l y_l
+__global__ void k_combine(int64_t n, int cnt, const bem_c64* V, const bem_c64* y, bem_c64* x)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bem_c64 acc = x[i];
+    for (int l = 0; l < cnt; l++) acc = c64_fma(V[(int64_t)l * n + i], y[l], acc);
+    x[i] = acc;
+}
+
+// r = b - r
+__global__ void k_residual(int64_t n, const bem_c64* b, bem_c64* r)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = c64_sub(b[i], r[i]);
+}
+
+__global__ void k_reconstruct(int64_t n, const bem_c64* x, const int32_t* subst, const bem_c64* al,
+                              const bem_c64* be, const bem_c64* f, double s, bem_c64* phi, bem_c64* dphi,
+                              bem_c64* pr)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bem_c64 ph = x[i], dp = x[i];
+    if (subst[i]) dp = c64_div(c64_sub(f[i], c64_mul(al[i], ph)), be[i]);
+    else ph = c64_div(c64_sub(f[i], c64_mul(be[i], dp)), al[i]);
+    if (phi) phi[i] = ph;
+    if (dphi) dphi[i] = dp;
+    if (pr) pr[i] = c64_mul(c64(0.0, s), ph);
+}
+
+__global__ void k_has_nonfinite(int64_t len, const bem_c64* v, int32_t* flag)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len && !(isfinite(v[i].re) && isfinite(v[i].im))) *flag = 1;
+}
+
+// ------------------------------------------------------------ host side
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct bem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // mesh
+    int64_t n = 0;
+    double hbar = 0;
+    std::vector<double> h_nodes, h_normal, h_area, h_colloc;
+    std::vector<void*> allocs_mesh;
+    DevMesh mesh{};
+    bemsetup::NearList near;
+    // bc
+    std::vector<bem_c64> h_alpha, h_beta, h_f;
+    bem_c64 *d_alpha = nullptr, *d_beta = nullptr, *d_f = nullptr;
+    bool have_bc = false;
+    // frequency
+    double k = 0;
+    bem_c64 cc{0, 0};
+    int32_t* d_subst = nullptr;
+    bem_c64 *d_cphi = nullptr, *d_cd = nullptr;
+    std::vector<int32_t> h_subst;
+    bool have_freq = false;
+    // partition
+    std::vector<bemsetup::Block> blocks;
+    int64_t *d_row0 = nullptr, *d_rows = nullptr, *d_col0 = nullptr, *d_cols = nullptr;
+    std::vector<int64_t> dense_off;          // per block (-1 if admissible)
+    int64_t dense_total = 0;
+    std::vector<Tile> tiles;
+    Tile* d_tiles = nullptr;
+    std::vector<Fix> fixes;
+    Fix* d_fixes = nullptr;
+    std::vector<int32_t> adm_list;           // admissible block ids, large first
+    int32_t* d_adm_list = nullptr;
+    std::vector<int64_t> lr_part;            // per block partial offset (low rank)
+    int64_t* d_lr_part = nullptr;
+    int64_t part_total = 0;
+    bem_c64* d_part = nullptr;
+    int64_t *d_gptr = nullptr, *d_gidx = nullptr;
+    bool have_part = false;
+    // GMRES workspace
+    bem_c64* d_gm = nullptr;
+    int64_t gm_cap = 0;
+    // profiling
+    bool prof = false;
+    double prof_ms[6] = {0};
+    int64_t prof_cnt[6] = {0};
+    std::vector<void*> allocs_part;
+};
+
+struct bem_hmat {
+    bem_ctx* ctx = nullptr;
+    int kind = 0, prec = 0;
+    bem_c64* dense = nullptr;
+    bem_c64 *U = nullptr, *V = nullptr, *A = nullptr;
+    int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
+    int32_t* d_rank = nullptr;
+    std::vector<int64_t> uoff, voff, aoff;
+    std::vector<int32_t> rank;
+    double stats[8] = {0};
+};
+
+namespace {
+
+int set_err(bem_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPC(ctx, call)                                                                                     \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return set_err(ctx, BEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+template <class T>
+int dalloc(bem_ctx* c, T** p, size_t count, std::vector<void*>* owner)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (owner) owner->push_back(*p);
+    return BEM_OK;
+}
+
+template <class T>
+int dupload(bem_ctx* c, T** p, const T* src, size_t count, std::vector<void*>* owner)
+{
+    int rc = dalloc(c, p, count, owner);
+    if (rc) return rc;
+    if (count) HIPC(c, hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return BEM_OK;
+}
+
+void free_list(std::vector<void*>& v)
+{
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+}
+
+DevFreq dev_freq(const bem_ctx* c)
+{
+    DevFreq F;
+    F.k = c->k;
+    F.cc = c->cc;
+    F.subst = c->d_subst;
+    F.cphi = c->d_cphi;
+    F.cd = c->d_cd;
+    return F;
+}
+
+struct Timer {
+    bem_ctx* c;
+    int slot;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Timer(bem_ctx* c_, int s) : c(c_), slot(s)
+    {
+        if (c->prof) {
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            (void)hipEventRecord(e0, c->stream);
+        }
+    }
+    ~Timer()
+    {
+        if (c->prof) {
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            c->prof_ms[slot] += ms;
+            c->prof_cnt[slot]++;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+};
+
+int launch_check(bem_ctx* c, const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return BEM_OK;
+}
+
+unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int bem_create(bem_ctx_t* out, int device)
+{
+    if (!out) return BEM_ERR_ARG;
+    *out = nullptr;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return BEM_ERR_HIP;
+    if (device < 0 || device >= nd) return BEM_ERR_ARG;
+    auto c = new bem_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BEM_ERR_HIP;
+    }
+    *out = c;
+    return BEM_OK;
+}
+
+int bem_destroy(bem_ctx_t c)
+{
+    if (!c) return BEM_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_list(c->allocs_mesh);
+    free_list(c->allocs_part);
+    if (c->d_alpha) (void)hipFree(c->d_alpha);
+    if (c->d_beta) (void)hipFree(c->d_beta);
+    if (c->d_f) (void)hipFree(c->d_f);
+    if (c->d_subst) (void)hipFree(c->d_subst);
+    if (c->d_cphi) (void)hipFree(c->d_cphi);
+    if (c->d_cd) (void)hipFree(c->d_cd);
+    if (c->d_gm) (void)hipFree(c->d_gm);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BEM_OK;
+}
+
+const char* bem_last_error(bem_ctx_t c) { return c ? c->err.c_str() : "null context"; }
+
+int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* normal3, const double* area,
+                    const double* colloc3, double hbar)
+{
+    if (!c || n <= 0 || !nodes9 || !normal3 || !area || !colloc3 || n > (int64_t)INT32_MAX)
+        return set_err(c, BEM_ERR_ARG, "bem_upload_mesh: bad arguments");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    free_list(c->allocs_mesh);
+    c->have_part = false;
+    c->n = n;
+    c->hbar = hbar;
+    c->h_nodes.assign(nodes9, nodes9 + 9 * n);
+    c->h_normal.assign(normal3, normal3 + 3 * n);
+    c->h_area.assign(area, area + n);
+    c->h_colloc.assign(colloc3, colloc3 + 3 * n);
+    // frequency-independent geometry of the singular and near-singular rules
+    bemsetup::Elements E;
+    E.n = n;
+    E.nodes = c->h_nodes;
+    E.normal.resize(n);
+    E.colloc.resize(n);
+    E.area = c->h_area;
+    E.hbar = hbar;
+    for (int64_t e = 0; e < n; e++) {
+        E.normal[e] = bemsetup::Vec3{normal3[3 * e], normal3[3 * e + 1], normal3[3 * e + 2]};
+        E.colloc[e] = bemsetup::Vec3{colloc3[3 * e], colloc3[3 * e + 1], colloc3[3 * e + 2]};
+    }
+    c->near = bemsetup::build_near(E);
+    std::vector<double> self = bemsetup::build_self(E);
+    std::vector<double> soa(7 * n);
+    for (int64_t e = 0; e < n; e++) {
+        soa[e] = colloc3[3 * e]; soa[n + e] = colloc3[3 * e + 1]; soa[2 * n + e] = colloc3[3 * e + 2];
+        soa[3 * n + e] = normal3[3 * e]; soa[4 * n + e] = normal3[3 * e + 1]; soa[5 * n + e] = normal3[3 * e + 2];
+        soa[6 * n + e] = area[e];
+    }
+    double *d_soa, *d_nodes, *d_q;
+    int rc;
+    if ((rc = dupload(c, &d_soa, soa.data(), soa.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_nodes, c->h_nodes.data(), c->h_nodes.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dalloc(c, &d_q, (size_t)48 * n, &c->allocs_mesh))) return rc;
+    DevMesh& M = c->mesh;
+    M.n = n;
+    M.px = d_soa; M.py = d_soa + n; M.pz = d_soa + 2 * n;
+    M.nx = d_soa + 3 * n; M.ny = d_soa + 4 * n; M.nz = d_soa + 5 * n;
+    M.area = d_soa + 6 * n;
+    M.qx = d_q; M.qy = d_q + 16 * n; M.qz = d_q + 32 * n;
+    hipLaunchKernelGGL(k_quadpoints, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, d_nodes, d_q, d_q + 16 * n,
+                       d_q + 32 * n);
+    if ((rc = launch_check(c, "k_quadpoints"))) return rc;
+    int64_t *d_ptr, *d_off;
+    int32_t *d_col, *d_nsub;
+    double *d_geo, *d_self;
+    if ((rc = dupload(c, &d_ptr, c->near.rowptr.data(), c->near.rowptr.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_col, c->near.col.data(), c->near.col.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_nsub, c->near.nsub.data(), c->near.nsub.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_off, c->near.geo_off.data(), c->near.geo_off.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_geo, c->near.geo.data(), c->near.geo.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_self, self.data(), self.size(), &c->allocs_mesh))) return rc;
+    M.nr_ptr = d_ptr; M.nr_col = d_col; M.nr_nsub = d_nsub; M.nr_off = d_off; M.nr_geo = d_geo;
+    M.self_geo = d_self;
+    M.near_lim = 4.0 * hbar;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+int bem_set_bc(bem_ctx_t c, const double* a, const double* b, const double* f)
+{
+    if (!c || c->n <= 0 || !a || !b || !f) return set_err(c, BEM_ERR_ARG, "bem_set_bc: bad arguments / no mesh");
+    HIPC(c, hipSetDevice(c->device));
+    int64_t n = c->n;
+    c->h_alpha.assign((const bem_c64*)a, (const bem_c64*)a + n);
+    c->h_beta.assign((const bem_c64*)b, (const bem_c64*)b + n);
+    c->h_f.assign((const bem_c64*)f, (const bem_c64*)f + n);
+    for (bem_c64** p : {&c->d_alpha, &c->d_beta, &c->d_f})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    int rc;
+    if ((rc = dupload(c, &c->d_alpha, c->h_alpha.data(), n, nullptr))) return rc;
+    if ((rc = dupload(c, &c->d_beta, c->h_beta.data(), n, nullptr))) return rc;
+    if ((rc = dupload(c, &c->d_f, c->h_f.data(), n, nullptr))) return rc;
+    c->have_bc = true;
+    c->have_freq = false;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+int bem_set_frequency(bem_ctx_t c, double k, double cc_re, double cc_im)
+{
+    if (!c || !c->have_bc) return set_err(c, BEM_ERR_STATE, "bem_set_frequency: mesh/boundary conditions missing");
+    if (!(k > 0) || !std::isfinite(k)) return set_err(c, BEM_ERR_ARG, "bem_set_frequency: k must be > 0");
+    HIPC(c, hipSetDevice(c->device));
+    int64_t n = c->n;
+    c->k = k;
+    c->cc = c64(cc_re, cc_im);
+    // substitution rule and column coefficients (bem_detmath.h bem_xy notes)
+    std::vector<int32_t> subst(n);
+    std::vector<bem_c64> cphi(n), cd(n);
+    for (int64_t j = 0; j < n; j++) {
+        bem_c64 al = c->h_alpha[j], be = c->h_beta[j];
+        subst[j] = std::hypot(al.re, al.im) < std::hypot(be.re, be.im);
+        if (subst[j]) {
+            cphi[j] = c64_div(c64(-1.0, 0.0), be);
+            cd[j] = c64_div(c64_neg(al), be);
+        } else {
+            cphi[j] = c64_div(c64(1.0, 0.0), al);
+            cd[j] = c64_div(be, al);
+        }
+    }
+    c->h_subst = subst;
+    if (!c->d_subst) {
+        int rc;
+        if ((rc = dalloc(c, &c->d_subst, n, nullptr))) return rc;
+        if ((rc = dalloc(c, &c->d_cphi, n, nullptr))) return rc;
+        if ((rc = dalloc(c, &c->d_cd, n, nullptr))) return rc;
+    }
+    HIPC(c, hipMemcpyAsync(c->d_subst, subst.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->d_cphi, cphi.data(), n * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->d_cd, cd.data(), n * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->have_freq = true;
+    return BEM_OK;
+}
+
+int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int64_t* rows, const int64_t* col0,
+                         const int64_t* cols, const uint8_t* adm)
+{
+    if (!c || c->n <= 0 || nb <= 0 || !row0 || !rows || !col0 || !cols || !adm)
+        return set_err(c, BEM_ERR_ARG, "bem_upload_partition: bad arguments / no mesh");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    int64_t n = c->n;
+    // every entry covered exactly once and every block inside the matrix
+    std::vector<int64_t> cover(n, 0);
+    for (int64_t b = 0; b < nb; b++) {
+        if (rows[b] <= 0 || cols[b] <= 0 || row0[b] < 0 || col0[b] < 0 || row0[b] + rows[b] > n ||
+            col0[b] + cols[b] > n || rows[b] > INT32_MAX || cols[b] > INT32_MAX)
+            return set_err(c, BEM_ERR_ARG, "bem_upload_partition: block outside the matrix");
+        for (int64_t i = row0[b]; i < row0[b] + rows[b]; i++) cover[i] += cols[b];
+    }
+    for (int64_t i = 0; i < n; i++)
+        if (cover[i] != n) return set_err(c, BEM_ERR_ARG, "bem_upload_partition: blocks do not tile the matrix");
+    free_list(c->allocs_part);
+    c->blocks.resize(nb);
+    for (int64_t b = 0; b < nb; b++) c->blocks[b] = bemsetup::Block{row0[b], rows[b], col0[b], cols[b], adm[b] != 0};
+    // dense storage and tiles
+    c->dense_off.assign(nb, -1);
+    c->tiles.clear();
+    c->adm_list.clear();
+    c->lr_part.assign(nb, -1);
+    int64_t doff = 0, poff = 0;
+    std::vector<std::vector<int64_t>> contrib(n);  // partial offsets per row, partition order
+    for (int64_t b = 0; b < nb; b++) {
+        const auto& B = c->blocks[b];
+        if (!B.adm) {
+            c->dense_off[b] = doff;
+            for (int64_t tc = 0; tc < B.cols; tc += kTile)
+                for (int64_t tr = 0; tr < B.rows; tr += kTile) {
+                    Tile T;
+                    T.dest = doff + tr + tc * B.rows;
+                    T.ld = (int32_t)B.rows;
+                    T.row0 = (int32_t)(B.row0 + tr);
+                    T.col0 = (int32_t)(B.col0 + tc);
+                    T.rows = (int32_t)std::min<int64_t>(kTile, B.rows - tr);
+                    T.cols = (int32_t)std::min<int64_t>(kTile, B.cols - tc);
+                    T.part = poff;
+                    T.pad = 0;
+                    for (int r = 0; r < T.rows; r++) contrib[T.row0 + r].push_back(poff + r);
+                    poff += T.rows;
+                    c->tiles.push_back(T);
+                }
+            doff += B.rows * B.cols;
+        } else {
+            c->adm_list.push_back((int32_t)b);
+            c->lr_part[b] = poff;
+            for (int64_t r = 0; r < B.rows; r++) contrib[B.row0 + r].push_back(poff + r);
+            poff += B.rows;
+        }
+    }
+    c->dense_total = doff;
+    c->part_total = poff;
+    // largest admissible blocks first (load balance of the one-block-per-
+    // workgroup kernels)
+    std::stable_sort(c->adm_list.begin(), c->adm_list.end(), [&](int32_t a, int32_t b) {
+        return c->blocks[a].rows + c->blocks[a].cols > c->blocks[b].rows + c->blocks[b].cols;
+    });
+    // self / near-singular entries inside dense blocks
+    c->fixes.clear();
+    const auto& NL = c->near;
+    for (int64_t b = 0; b < nb; b++) {
+        const auto& B = c->blocks[b];
+        if (B.adm) continue;
+        for (int64_t i = B.row0; i < B.row0 + B.rows; i++) {
+            if (i >= B.col0 && i < B.col0 + B.cols)
+                c->fixes.push_back(Fix{c->dense_off[b] + (i - B.row0) + (i - B.col0) * B.rows, (int32_t)i, (int32_t)i, -1});
+            for (int64_t t = NL.rowptr[i]; t < NL.rowptr[i + 1]; t++) {
+                int64_t j = NL.col[t];
+                if (j >= B.col0 && j < B.col0 + B.cols)
+                    c->fixes.push_back(Fix{c->dense_off[b] + (i - B.row0) + (j - B.col0) * B.rows, (int32_t)i, (int32_t)j, t});
+            }
+        }
+    }
+    std::stable_sort(c->fixes.begin(), c->fixes.end(), [&](const Fix& a, const Fix& b) {
+        int ca = a.t < 0 ? 4 : NL.nsub[a.t], cb = b.t < 0 ? 4 : NL.nsub[b.t];
+        return ca > cb;
+    });
+    std::vector<int64_t> gptr(n + 1, 0), gidx;
+    for (int64_t i = 0; i < n; i++) gptr[i + 1] = gptr[i] + (int64_t)contrib[i].size();
+    gidx.reserve(gptr[n]);
+    for (int64_t i = 0; i < n; i++) gidx.insert(gidx.end(), contrib[i].begin(), contrib[i].end());
+    std::vector<int64_t> r0(nb), rs(nb), c0(nb), cs(nb);
+    for (int64_t b = 0; b < nb; b++) { r0[b] = row0[b]; rs[b] = rows[b]; c0[b] = col0[b]; cs[b] = cols[b]; }
+    int rc;
+    auto* ap = &c->allocs_part;
+    if ((rc = dupload(c, &c->d_row0, r0.data(), nb, ap))) return rc;
+    if ((rc = dupload(c, &c->d_rows, rs.data(), nb, ap))) return rc;
+    if ((rc = dupload(c, &c->d_col0, c0.data(), nb, ap))) return rc;
+    if ((rc = dupload(c, &c->d_cols, cs.data(), nb, ap))) return rc;
+    if ((rc = dupload(c, &c->d_tiles, c->tiles.data(), c->tiles.size(), ap))) return rc;
+    if ((rc = dupload(c, &c->d_fixes, c->fixes.data(), c->fixes.size(), ap))) return rc;
+    if ((rc = dupload(c, &c->d_adm_list, c->adm_list.data(), c->adm_list.size(), ap))) return rc;
+    if ((rc = dupload(c, &c->d_lr_part, c->lr_part.data(), nb, ap))) return rc;
+    if ((rc = dalloc(c, &c->d_part, c->part_total, ap))) return rc;
+    if ((rc = dupload(c, &c->d_gptr, gptr.data(), gptr.size(), ap))) return rc;
+    if ((rc = dupload(c, &c->d_gidx, gidx.data(), gidx.size(), ap))) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->have_part = true;
+    return BEM_OK;
+}
+
+int bem_hmat_destroy(bem_hmat_t h)
+{
+    if (!h) return BEM_ERR_ARG;
+    (void)hipSetDevice(h->ctx->device);
+    (void)hipStreamSynchronize(h->ctx->stream);
+    for (void* p : {(void*)h->dense, (void*)h->U, (void*)h->V, (void*)h->A, (void*)h->d_uoff, (void*)h->d_voff,
+                    (void*)h->d_aoff, (void*)h->d_rank})
+        if (p) (void)hipFree(p);
+    delete h;
+    return BEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ACA over the blocks in `list` with per-block capacity `capf(b)` into freshly
+// allocated scratch; returns ranks/flags/evals per partition block
+struct AcaScratch {
+    bem_c64 *U = nullptr, *V = nullptr, *A = nullptr;
+    int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
+    int32_t *d_cap = nullptr, *d_rank = nullptr, *d_flag = nullptr, *d_list = nullptr;
+    int64_t* d_ev = nullptr;
+    std::vector<int64_t> uoff, voff, aoff;
+    std::vector<void*> owned;
+    ~AcaScratch() { free_list(owned); }
+};
+
+int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
+            const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv, AcaScratch& S,
+            std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev)
+{
+    int64_t nb = (int64_t)c->blocks.size();
+    S.uoff.assign(nb, 0); S.voff.assign(nb, 0); S.aoff.assign(nb, 0);
+    int64_t tu = 0, tv = 0, ta = 0;
+    int maxcap = 1;
+    for (int32_t b : list) {
+        const auto& B = c->blocks[b];
+        S.uoff[b] = tu; S.voff[b] = tv; S.aoff[b] = ta;
+        tu += B.rows * cap[b]; tv += B.cols * cap[b]; ta += cap[b];
+        maxcap = std::max(maxcap, cap[b]);
+    }
+    int rc;
+    if ((rc = dalloc(c, &S.U, tu, &S.owned))) return rc;
+    if ((rc = dalloc(c, &S.V, tv, &S.owned))) return rc;
+    if ((rc = dalloc(c, &S.A, ta, &S.owned))) return rc;
+    if ((rc = dupload(c, &S.d_uoff, S.uoff.data(), nb, &S.owned))) return rc;
+    if ((rc = dupload(c, &S.d_voff, S.voff.data(), nb, &S.owned))) return rc;
+    if ((rc = dupload(c, &S.d_aoff, S.aoff.data(), nb, &S.owned))) return rc;
+    if ((rc = dupload(c, &S.d_cap, cap.data(), nb, &S.owned))) return rc;
+    if ((rc = dupload(c, &S.d_list, list.data(), list.size(), &S.owned))) return rc;
+    if ((rc = dalloc(c, &S.d_rank, nb, &S.owned))) return rc;
+    if ((rc = dalloc(c, &S.d_flag, nb, &S.owned))) return rc;
+    if ((rc = dalloc(c, &S.d_ev, nb, &S.owned))) return rc;
+    HIPC(c, hipMemsetAsync(S.d_rank, 0, nb * sizeof(int32_t), c->stream));
+    HIPC(c, hipMemsetAsync(S.d_flag, 0, nb * sizeof(int32_t), c->stream));
+    HIPC(c, hipMemsetAsync(S.d_ev, 0, nb * sizeof(int64_t), c->stream));
+    AcaArgs a;
+    a.M = c->mesh;
+    a.F = dev_freq(c);
+    a.kind = kind;
+    a.blist = S.d_list;
+    a.row0 = c->d_row0; a.rows = c->d_rows; a.col0 = c->d_col0; a.cols = c->d_cols;
+    a.uoff = S.d_uoff; a.voff = S.d_voff; a.aoff = S.d_aoff; a.cap = S.d_cap;
+    a.U = S.U; a.V = S.V; a.A = S.A;
+    a.rank_out = S.d_rank; a.flag_out = S.d_flag; a.evals_out = S.d_ev;
+    a.tol = tol; a.maxrank = maxrank; a.seed = seed;
+    a.piv_off = d_pivoff; a.piv = d_piv;
+    size_t shm = (size_t)(3 * maxcap + 2) * sizeof(bem_c64);
+    if (!list.empty()) {
+        Timer t(c, 2);
+        if (kind == BEM_KIND_DPHI)
+            hipLaunchKernelGGL(k_aca<true>, dim3((unsigned)list.size()), dim3(kWG), shm, c->stream, a);
+        else
+            hipLaunchKernelGGL(k_aca<false>, dim3((unsigned)list.size()), dim3(kWG), shm, c->stream, a);
+        if ((rc = launch_check(c, "k_aca"))) return rc;
+    }
+    rank.assign(nb, 0); flag.assign(nb, 0); ev.assign(nb, 0);
+    HIPC(c, hipMemcpyAsync(rank.data(), S.d_rank, nb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(flag.data(), S.d_flag, nb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(ev.data(), S.d_ev, nb * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precision, uint64_t seed, bem_hmat_t* out)
+{
+    if (!c || !out) return set_err(c, BEM_ERR_ARG, "bem_assemble: null argument");
+    *out = nullptr;
+    if (!c->have_part || !c->have_freq) return set_err(c, BEM_ERR_STATE, "bem_assemble: partition or frequency missing");
+    if (kind != BEM_KIND_PHI && kind != BEM_KIND_DPHI) return set_err(c, BEM_ERR_ARG, "bem_assemble: bad kind");
+    if (precision != BEM_PREC_FP64) return set_err(c, BEM_ERR_ARG, "bem_assemble: precision not available in this entry point");
+    if (!(tol >= 0)) return set_err(c, BEM_ERR_ARG, "bem_assemble: negative tolerance");
+    HIPC(c, hipSetDevice(c->device));
+    double t0 = now_ms();
+    auto h = new bem_hmat();
+    h->ctx = c;
+    h->kind = kind;
+    h->prec = precision;
+    int64_t nb = (int64_t)c->blocks.size();
+    int rc;
+    auto fail = [&](int code) { bem_hmat_destroy(h); return code; };
+    // dense near field
+    if ((rc = dalloc(c, &h->dense, c->dense_total, nullptr))) return fail(rc);
+    bem_c64* outPhi = kind == BEM_KIND_PHI ? h->dense : nullptr;
+    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? h->dense : nullptr;
+    DevFreq F = dev_freq(c);
+    hipEvent_t ev0, ev1, ev2;
+    HIPC(c, hipEventCreate(&ev0)); HIPC(c, hipEventCreate(&ev1)); HIPC(c, hipEventCreate(&ev2));
+    HIPC(c, hipEventRecord(ev0, c->stream));
+    if (!c->tiles.empty()) {
+        Timer t(c, 0);
+        hipLaunchKernelGGL(k_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F, c->d_tiles,
+                           outPhi, outDPhi);
+        if ((rc = launch_check(c, "k_dense"))) return fail(rc);
+    }
+    if (!c->fixes.empty()) {
+        Timer t(c, 1);
+        hipLaunchKernelGGL(k_fixup, dim3(nblk((int64_t)c->fixes.size(), kWG)), dim3(kWG), 0, c->stream, c->mesh, F,
+                           c->d_fixes, (int64_t)c->fixes.size(), outPhi, outDPhi);
+        if ((rc = launch_check(c, "k_fixup"))) return fail(rc);
+    }
+    HIPC(c, hipEventRecord(ev1, c->stream));
+    // pivot lists of the DPhi variant
+    std::vector<int64_t> pivoff;
+    std::vector<int32_t> piv;
+    std::vector<void*> tmp_owned;
+    int64_t* d_pivoff = nullptr;
+    int32_t* d_piv = nullptr;
+    if (kind == BEM_KIND_DPHI) {
+        bemsetup::Elements E;
+        E.n = c->n;
+        E.normal.resize(c->n);
+        for (int64_t e = 0; e < c->n; e++)
+            E.normal[e] = bemsetup::Vec3{c->h_normal[3 * e], c->h_normal[3 * e + 1], c->h_normal[3 * e + 2]};
+        bemsetup::build_pivot_lists(E, c->blocks, seed, pivoff, piv);
+        if ((rc = dupload(c, &d_pivoff, pivoff.data(), pivoff.size(), &tmp_owned))) return fail(rc);
+        if ((rc = dupload(c, &d_piv, piv.data(), piv.size(), &tmp_owned))) return fail(rc);
+    }
+    // ACA, first pass with a modest rank capacity, then the overflowing
+    // blocks again with their full capacity
+    std::vector<int32_t> cap(nb, 0);
+    auto full_cap = [&](int32_t b) {
+        int64_t m = std::min(c->blocks[b].rows, c->blocks[b].cols);
+        if (maxrank > 0) m = std::min(m, maxrank);
+        return (int32_t)m;
+    };
+    for (int32_t b : c->adm_list) cap[b] = std::min<int32_t>(full_cap(b), kCapFirst);
+    AcaScratch S1, S2;
+    std::vector<int32_t> rank, flag, rank2, flag2;
+    std::vector<int64_t> evs, evs2;
+    if ((rc = run_aca(c, kind, tol, maxrank, seed, c->adm_list, cap, d_pivoff, d_piv, S1, rank, flag, evs)))
+        return fail(rc);
+    std::vector<int32_t> over, keep;
+    for (int32_t b : c->adm_list) {
+        if (flag[b] == 2) {
+            free_list(tmp_owned);
+            bem_hmat_destroy(h);
+            return set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite value in ACA block " + std::to_string(b));
+        }
+        (flag[b] == 1 ? over : keep).push_back(b);
+    }
+    if (!over.empty()) {
+        std::vector<int32_t> cap2(nb, 0);
+        for (int32_t b : over) cap2[b] = std::min<int32_t>(full_cap(b), kCapMax);
+        if ((rc = run_aca(c, kind, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, S2, rank2, flag2, evs2)))
+            return fail(rc);
+        for (int32_t b : over) {
+            rank[b] = rank2[b];
+            evs[b] = evs2[b];
+            if (flag2[b]) {
+                free_list(tmp_owned);
+                bem_hmat_destroy(h);
+                return set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
+                               "bem_assemble: ACA block " + std::to_string(b) +
+                                   (flag2[b] == 2 ? " non-finite" : " needs rank > 1024; lower the tolerance or set max_rank"));
+            }
+        }
+    }
+    free_list(tmp_owned);
+    // packed storage at the final ranks
+    h->uoff.assign(nb, 0); h->voff.assign(nb, 0); h->aoff.assign(nb, 0); h->rank.assign(nb, -1);
+    int64_t tu = 0, tv = 0, ta = 0;
+    double maxr = 0, sumr = 0;
+    int64_t evaluated = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        const auto& B = c->blocks[b];
+        if (!B.adm) { evaluated += B.rows * B.cols; continue; }
+        int32_t r = rank[b] > 0 ? rank[b] : 1;
+        h->rank[b] = r;
+        h->uoff[b] = tu; h->voff[b] = tv; h->aoff[b] = ta;
+        tu += B.rows * r; tv += B.cols * r; ta += r;
+        maxr = std::max(maxr, (double)r);
+        sumr += r;
+        evaluated += evs[b];
+    }
+    if ((rc = dalloc(c, &h->U, tu, nullptr))) return fail(rc);
+    if ((rc = dalloc(c, &h->V, tv, nullptr))) return fail(rc);
+    if ((rc = dalloc(c, &h->A, ta, nullptr))) return fail(rc);
+    if ((rc = dupload(c, &h->d_uoff, h->uoff.data(), nb, nullptr))) return fail(rc);
+    if ((rc = dupload(c, &h->d_voff, h->voff.data(), nb, nullptr))) return fail(rc);
+    if ((rc = dupload(c, &h->d_aoff, h->aoff.data(), nb, nullptr))) return fail(rc);
+    std::vector<int32_t> rk(nb);
+    for (int64_t b = 0; b < nb; b++) rk[b] = rank[b];
+    int32_t* d_srank;
+    std::vector<void*> own2;
+    if ((rc = dupload(c, &d_srank, rk.data(), nb, &own2))) return fail(rc);
+    if ((rc = dupload(c, &h->d_rank, h->rank.data(), nb, nullptr))) return fail(rc);
+    int32_t *d_keep = nullptr, *d_over = nullptr;
+    if (!keep.empty()) {
+        if ((rc = dupload(c, &d_keep, keep.data(), keep.size(), &own2))) return fail(rc);
+        hipLaunchKernelGGL(k_compact, dim3((unsigned)keep.size()), dim3(kWG), 0, c->stream, d_keep, c->d_rows,
+                           c->d_cols, S1.d_uoff, S1.d_voff, S1.d_aoff, S1.U, S1.V, S1.A, d_srank, h->d_uoff,
+                           h->d_voff, h->d_aoff, h->U, h->V, h->A);
+        if ((rc = launch_check(c, "k_compact"))) return fail(rc);
+    }
+    if (!over.empty()) {
+        if ((rc = dupload(c, &d_over, over.data(), over.size(), &own2))) return fail(rc);
+        hipLaunchKernelGGL(k_compact, dim3((unsigned)over.size()), dim3(kWG), 0, c->stream, d_over, c->d_rows,
+                           c->d_cols, S2.d_uoff, S2.d_voff, S2.d_aoff, S2.U, S2.V, S2.A, d_srank, h->d_uoff,
+                           h->d_voff, h->d_aoff, h->U, h->V, h->A);
+        if ((rc = launch_check(c, "k_compact"))) return fail(rc);
+    }
+    HIPC(c, hipEventRecord(ev2, c->stream));
+    // NaN check of the dense part (the reference reports NaN blocks,
+    // hmatrix.cpp:812-820)
+    int32_t* d_flag;
+    if ((rc = dalloc(c, &d_flag, 1, &own2))) return fail(rc);
+    HIPC(c, hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
+    if (c->dense_total)
+        hipLaunchKernelGGL(k_has_nonfinite, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream, c->dense_total,
+                           h->dense, d_flag);
+    int32_t hflag = 0;
+    HIPC(c, hipMemcpyAsync(&hflag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    free_list(own2);
+    float ms_dense = 0, ms_aca = 0;
+    (void)hipEventElapsedTime(&ms_dense, ev0, ev1);
+    (void)hipEventElapsedTime(&ms_aca, ev1, ev2);
+    (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1); (void)hipEventDestroy(ev2);
+    if (hflag) {
+        bem_hmat_destroy(h);
+        return set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite entry in the near field");
+    }
+    size_t nadm = c->adm_list.size();
+    h->stats[0] = (double)evaluated;
+    h->stats[1] = maxr;
+    h->stats[2] = nadm ? sumr / (double)nadm : 0;
+    h->stats[3] = (double)((c->dense_total + tu + tv + ta) * sizeof(bem_c64));
+    h->stats[4] = now_ms() - t0;
+    h->stats[5] = (double)over.size();
+    h->stats[6] = ms_dense;
+    h->stats[7] = ms_aca;
+    *out = h;
+    return BEM_OK;
+}
+
+int bem_hmvm(bem_ctx_t c, bem_hmat_t h, const void* x, void* y, int accumulate)
+{
+    if (!c || !h || !x || !y || h->ctx != c) return set_err(c, BEM_ERR_ARG, "bem_hmvm: bad arguments");
+    HIPC(c, hipSetDevice(c->device));
+    const bem_c64* xd = (const bem_c64*)x;
+    bem_c64* yd = (bem_c64*)y;
+    int rc;
+    if (!c->tiles.empty()) {
+        Timer t(c, 3);
+        hipLaunchKernelGGL(k_mvm_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->d_tiles, h->dense,
+                           xd, c->d_part);
+        if ((rc = launch_check(c, "k_mvm_dense"))) return rc;
+    }
+    if (!c->adm_list.empty()) {
+        int maxr = 1;
+        for (int32_t b : c->adm_list) maxr = std::max(maxr, h->rank[b]);
+        Timer t(c, 4);
+        hipLaunchKernelGGL(k_mvm_lowrank, dim3((unsigned)c->adm_list.size()), dim3(kWG), maxr * sizeof(bem_c64),
+                           c->stream, c->d_adm_list, c->d_row0, c->d_rows, c->d_col0, c->d_cols, h->d_rank,
+                           h->d_uoff, h->d_voff, h->d_aoff, h->U, h->V, h->A, xd, c->d_lr_part, c->d_part);
+        if ((rc = launch_check(c, "k_mvm_lowrank"))) return rc;
+    }
+    {
+        Timer t(c, 5);
+        hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr, c->d_gidx,
+                           c->d_part, yd, accumulate);
+        if ((rc = launch_check(c, "k_mvm_gather"))) return rc;
+    }
+    return BEM_OK;
+}
+
+int bem_block_info(bem_hmat_t h, int64_t b, int64_t* rank)
+{
+    if (!h || !rank || b < 0 || b >= (int64_t)h->ctx->blocks.size()) return BEM_ERR_ARG;
+    *rank = h->ctx->blocks[b].adm ? h->rank[b] : -1;
+    return BEM_OK;
+}
+
+int bem_block_download(bem_hmat_t h, int64_t b, double* full_or_U, double* alpha, double* V)
+{
+    if (!h || b < 0 || b >= (int64_t)h->ctx->blocks.size() || !full_or_U) return BEM_ERR_ARG;
+    bem_ctx* c = h->ctx;
+    HIPC(c, hipSetDevice(c->device));
+    const auto& B = c->blocks[b];
+    if (!B.adm) {
+        HIPC(c, hipMemcpyAsync(full_or_U, h->dense + c->dense_off[b], B.rows * B.cols * sizeof(bem_c64),
+                               hipMemcpyDeviceToHost, c->stream));
+    } else {
+        int64_t r = h->rank[b];
+        HIPC(c, hipMemcpyAsync(full_or_U, h->U + h->uoff[b], B.rows * r * sizeof(bem_c64), hipMemcpyDeviceToHost,
+                               c->stream));
+        if (alpha)
+            HIPC(c, hipMemcpyAsync(alpha, h->A + h->aoff[b], r * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+        if (V) {
+            std::vector<bem_c64> rowmajor(r * B.cols);
+            HIPC(c, hipMemcpyAsync(rowmajor.data(), h->V + h->voff[b], r * B.cols * sizeof(bem_c64),
+                                   hipMemcpyDeviceToHost, c->stream));
+            HIPC(c, hipStreamSynchronize(c->stream));
+            bem_c64* Vo = (bem_c64*)V;  // reference layout: k x n column-major
+            for (int64_t l = 0; l < r; l++)
+                for (int64_t j = 0; j < B.cols; j++) Vo[j * r + l] = rowmajor[l * B.cols + j];
+        }
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+int bem_hmat_stats(bem_hmat_t h, double* s)
+{
+    if (!h || !s) return BEM_ERR_ARG;
+    std::memcpy(s, h->stats, sizeof h->stats);
+    return BEM_OK;
+}
+
+int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int64_t restart, int64_t maxit,
+              int64_t* iters, double* rel_resid)
+{
+    if (!c || !h || !b || !x || restart < 1 || maxit < 0) return set_err(c, BEM_ERR_ARG, "bem_gmres: bad arguments");
+    HIPC(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    const int64_t m = restart;
+    // workspace: V (m+1) x n, w, r, small vectors
+    int64_t need = (m + 1) * n + 2 * n + 4 * (m + 2) + 1024;
+    if (c->gm_cap < need) {
+        if (c->d_gm) (void)hipFree(c->d_gm);
+        c->d_gm = nullptr;
+        c->gm_cap = 0;
+        int rc;
+        if ((rc = dalloc(c, &c->d_gm, need, nullptr))) return rc;
+        c->gm_cap = need;
+    }
+    bem_c64* V = c->d_gm;
+    bem_c64* w = V + (m + 1) * n;
+    bem_c64* r = w + n;
+    bem_c64* dh = r + n;               // m + 2
+    bem_c64* dy = dh + (m + 2);        // m + 2
+    double* dn = (double*)(dy + (m + 2));     // norm parts (512) + result
+    double* dnorm = dn + 512;
+    const bem_c64* bd = (const bem_c64*)b;
+    bem_c64* xd = (bem_c64*)x;
+    const int npart = (int)std::min<int64_t>(256, (n + kWG - 1) / kWG);
+    auto norm_of = [&](const bem_c64* v, double* result) -> int {
+        hipLaunchKernelGGL(k_norm2_part, dim3(npart), dim3(kWG), 0, c->stream, n, v, dn);
+        hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, c->stream, npart, dn, dnorm);
+        int rc = launch_check(c, "norm");
+        if (rc) return rc;
+        HIPC(c, hipMemcpyAsync(result, dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return BEM_OK;
+    };
+    int rc;
+    double bn = 0;
+    if ((rc = norm_of(bd, &bn))) return rc;
+    if (bn == 0) {
+        HIPC(c, hipMemsetAsync(xd, 0, n * sizeof(bem_c64), c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return BEM_OK;
+    }
+    std::vector<bem_c64> H((m + 1) * m), g(m + 1), cs_s(m), hcol(m + 2), hh(m + 2);
+    std::vector<double> cs_c(m);
+    int64_t it = 0;
+    double rel = 1;
+    while (true) {
+        if ((rc = bem_hmvm(c, h, xd, r, 0))) return rc;
+        hipLaunchKernelGGL(k_residual, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, bd, r);
+        double beta;
+        if ((rc = norm_of(r, &beta))) return rc;
+        rel = beta / bn;
+        if (rel < tol || it >= maxit) break;
+        hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, r, dnorm, V);
+        std::fill(g.begin(), g.end(), c64(0, 0));
+        g[0] = c64(beta, 0);
+        int64_t j = 0;
+        bool conv = false;
+        for (; j < m && it < maxit; j++, it++) {
+            if ((rc = bem_hmvm(c, h, V + j * n, w, 0))) return rc;
+            std::fill(hcol.begin(), hcol.end(), c64(0, 0));
+            for (int pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL(k_dots, dim3((unsigned)(j + 1)), dim3(kWG), 0, c->stream, n, V, w, dh);
+                hipLaunchKernelGGL(k_orth_update, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)(j + 1), V, dh,
+                                   w);
+                if ((rc = launch_check(c, "cgs"))) return rc;
+                HIPC(c, hipMemcpyAsync(hh.data(), dh, (j + 1) * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+                HIPC(c, hipStreamSynchronize(c->stream));
+                for (int64_t l = 0; l <= j; l++) hcol[l] = c64_add(hcol[l], hh[l]);
+            }
+            double hn;
+            if ((rc = norm_of(w, &hn))) return rc;
+            hcol[j + 1] = c64(hn, 0);
+            hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, w, dnorm,
+                               V + (j + 1) * n);
+            for (int64_t l = 0; l < j; l++) {
+                bem_c64 a0 = hcol[l], b0 = hcol[l + 1];
+                hcol[l] = c64_add(c64_scale(a0, cs_c[l]), c64_mul(cs_s[l], b0));
+                hcol[l + 1] = c64_sub(c64_scale(b0, cs_c[l]), c64_mul(c64_conj(cs_s[l]), a0));
+            }
+            double na = std::sqrt(c64_abs2(hcol[j])), nb2 = std::sqrt(c64_abs2(hcol[j + 1]));
+            double t = std::sqrt(na * na + nb2 * nb2);
+            double cc;
+            bem_c64 ss;
+            if (na == 0) { cc = 0; ss = c64(1, 0); }
+            else {
+                cc = na / t;
+                ss = c64_scale(c64_mul(c64_scale(hcol[j], 1.0 / na), c64_conj(hcol[j + 1])), 1.0 / t);
+            }
+            cs_c[j] = cc; cs_s[j] = ss;
+            hcol[j] = c64_add(c64_scale(hcol[j], cc), c64_mul(ss, hcol[j + 1]));
+            bem_c64 gj = g[j];
+            g[j] = c64_scale(gj, cc);
+            g[j + 1] = c64_neg(c64_mul(c64_conj(ss), gj));
+            for (int64_t l = 0; l <= j; l++) H[j * (m + 1) + l] = hcol[l];
+            rel = std::sqrt(c64_abs2(g[j + 1])) / bn;
+            if (rel < tol) { conv = true; j++; it++; break; }
+        }
+        (void)conv;
+        std::vector<bem_c64> y(j);
+        for (int64_t l = j - 1; l >= 0; l--) {
+            bem_c64 s = g[l];
+            for (int64_t q = l + 1; q < j; q++) s = c64_sub(s, c64_mul(H[q * (m + 1) + l], y[q]));
+            y[l] = c64_div(s, H[l * (m + 1) + l]);
+        }
+        if (j > 0) {
+            HIPC(c, hipMemcpyAsync(dy, y.data(), j * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)j, V, dy, xd);
+            if ((rc = launch_check(c, "k_combine"))) return rc;
+        }
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (iters) *iters = it;
+    if (rel_resid) *rel_resid = rel;
+    if (rel >= tol) return set_err(c, BEM_ERR_NOCONV, "bem_gmres: maxit reached above tolerance");
+    return BEM_OK;
+}
+
+int bem_reconstruct(bem_ctx_t c, const void* x, double rho, double freq, void* phi, void* dphi, void* pr)
+{
+    if (!c || !x || !c->have_freq) return set_err(c, BEM_ERR_ARG, "bem_reconstruct: bad arguments");
+    HIPC(c, hipSetDevice(c->device));
+    double s = freq != 0 ? (rho * (2.0 * BEM_PI)) * freq : rho * (2.0 * BEM_PI);
+    hipLaunchKernelGGL(k_reconstruct, dim3(nblk(c->n, 256)), dim3(256), 0, c->stream, c->n, (const bem_c64*)x,
+                       c->d_subst, c->d_alpha, c->d_beta, c->d_f, s, (bem_c64*)phi, (bem_c64*)dphi, (bem_c64*)pr);
+    int rc = launch_check(c, "k_reconstruct");
+    if (rc) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+static int entries_impl(bem_ctx_t c, int mode, int64_t np, const int64_t* ij, double* out)
+{
+    if (!c || np < 0 || (np && (!ij || !out))) return set_err(c, BEM_ERR_ARG, "bem_entries: bad arguments");
+    if (!c->have_freq) return set_err(c, BEM_ERR_STATE, "bem_entries: frequency not set");
+    for (int64_t t = 0; t < 2 * np; t++)
+        if (ij[t] < 0 || ij[t] >= c->n) return set_err(c, BEM_ERR_ARG, "bem_entries: index out of range");
+    if (np == 0) return BEM_OK;
+    HIPC(c, hipSetDevice(c->device));
+    int64_t* d_ij;
+    bem_c64* d_out;
+    std::vector<void*> own;
+    int rc;
+    size_t per = mode == 2 ? 4 : 1;
+    if ((rc = dupload(c, &d_ij, ij, 2 * np, &own))) return rc;
+    if ((rc = dalloc(c, &d_out, per * np, &own))) { free_list(own); return rc; }
+    hipLaunchKernelGGL(k_entries, dim3(nblk(np, 128)), dim3(128), 0, c->stream, c->mesh, dev_freq(c), mode, np, d_ij,
+                       d_out);
+    if ((rc = launch_check(c, "k_entries"))) { free_list(own); return rc; }
+    hipError_t e = hipMemcpyAsync(out, d_out, per * np * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    free_list(own);
+    if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, hipGetErrorString(e));
+    return BEM_OK;
+}
+
+int bem_entries4(bem_ctx_t c, int64_t np, const int64_t* ij, double* out8) { return entries_impl(c, 2, np, ij, out8); }
+int bem_entries(bem_ctx_t c, int kind, int64_t np, const int64_t* ij, double* out2)
+{
+    if (kind != 0 && kind != 1) return set_err(c, BEM_ERR_ARG, "bem_entries: bad kind");
+    return entries_impl(c, kind, np, ij, out2);
+}
+
+int bem_malloc(bem_ctx_t c, size_t bytes, void** p)
+{
+    if (!c || !p) return BEM_ERR_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMalloc(p, bytes ? bytes : 1));
+    return BEM_OK;
+}
+int bem_free(bem_ctx_t c, void* p)
+{
+    if (!c) return BEM_ERR_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (p) HIPC(c, hipFree(p));
+    return BEM_OK;
+}
+int bem_memcpy_htod(bem_ctx_t c, void* dst, const void* src, size_t bytes)
+{
+    if (!c) return BEM_ERR_ARG;
+    HIPC(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+int bem_memcpy_dtoh(bem_ctx_t c, void* dst, const void* src, size_t bytes)
+{
+    if (!c) return BEM_ERR_ARG;
+    HIPC(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+int bem_synchronize(bem_ctx_t c)
+{
+    if (!c) return BEM_ERR_ARG;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+int bem_profile(bem_ctx_t c, int enable, double* out)
+{
+    if (!c) return BEM_ERR_ARG;
+    if (out)
+        for (int q = 0; q < 6; q++) {
+            out[q] = c->prof_cnt[q] ? c->prof_ms[q] / (double)c->prof_cnt[q] : 0.0;
+            out[6 + q] = (double)c->prof_cnt[q];
+        }
+    if (enable >= 0) {
+        c->prof = enable != 0;
+        for (int q = 0; q < 6; q++) { c->prof_ms[q] = 0; c->prof_cnt[q] = 0; }
+    }
+    return BEM_OK;
+}
+
+}  // extern "C"

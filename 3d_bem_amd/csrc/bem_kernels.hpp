@@ -1,0 +1,188 @@
+// Device-side entry generator of the collocation BEM operators (gfx950).
+// Every function here follows the operation order of the host restatement
+// so that FP64 entries are bit-identical to it (see include/bem_detmath.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/bem_detmath.h"
+
+namespace bemdev {
+
+// order-8 16-point triangle rule weights (trianglequadraturerules.cpp:91-108)
+__constant__ static const double kTriW[16] = {
+    0.14431560767779, 0.09509163426728, 0.09509163426728, 0.09509163426728, 0.10321737053472, 0.10321737053472,
+    0.10321737053472, 0.03245849762320, 0.03245849762320, 0.03245849762320, 0.02723031417443, 0.02723031417443,
+    0.02723031417443, 0.02723031417443, 0.02723031417443, 0.02723031417443};
+// barycentric (xi1, xi2) of the same rule
+__constant__ static const double kTriX[16][2] = {
+    {0.33333333333333, 0.33333333333333}, {0.45929258829272, 0.45929258829272}, {0.45929258829272, 0.08141482341455},
+    {0.08141482341455, 0.45929258829272}, {0.17056930775176, 0.17056930775176}, {0.17056930775176, 0.65886138449648},
+    {0.65886138449648, 0.17056930775176}, {0.05054722831703, 0.05054722831703}, {0.05054722831703, 0.89890554336594},
+    {0.89890554336594, 0.05054722831703}, {0.26311282963464, 0.72849239295540}, {0.72849239295540, 0.00839477740996},
+    {0.00839477740996, 0.26311282963464}, {0.72849239295540, 0.26311282963464}, {0.26311282963464, 0.00839477740996},
+    {0.00839477740996, 0.72849239295540}};
+// 5-point Gauss-Legendre (linequadraturerules.cpp:63-86)
+__constant__ static const double kGL5X[5] = {-0.906179845938664, -0.5384693101056831, 0.0, 0.5384693101056831,
+                                             0.906179845938664};
+__constant__ static const double kGL5W[5] = {0.23692688505618908, 0.47862867049936647, 0.5688888888888889,
+                                             0.47862867049936647, 0.23692688505618908};
+// positive half of 32-point Gauss-Legendre weights (boundaryelementsolver.cpp:906)
+__constant__ static const double kHGW[16] = {
+    0.0965400885147278,   0.09563872007927486, 0.09384439908080457, 0.09117387869576389, 0.08765209300440381,
+    0.08331192422694675,  0.07819389578707031, 0.0723457941088485,  0.06582222277636185, 0.058684093478535544,
+    0.050998059262376175, 0.04283589802222668, 0.03427386291302143, 0.02539206530926206, 0.01627439473090567,
+    0.007018610009470096};
+
+constexpr int kNearSub = 25;   // doubles per near-singular sub-triangle
+constexpr int kSelf = 99;      // doubles per element of the self integral
+
+// Mesh in HBM, structure of arrays, cluster order.  q* hold the 16
+// quadrature points of every element, index q * n + j (coalesced over j).
+struct DevMesh {
+    int64_t n;
+    const double *px, *py, *pz;   // collocation points
+    const double *nx, *ny, *nz;   // unit normals
+    const double* area;
+    const double *qx, *qy, *qz;
+    const int64_t* nr_ptr;        // near-singular CSR (n + 1)
+    const int32_t* nr_col;
+    const int32_t* nr_nsub;
+    const int64_t* nr_off;
+    const double* nr_geo;
+    const double* self_geo;       // kSelf per element
+    double near_lim;              // 4 * hbar
+};
+
+// frequency-dependent boundary-condition coefficients (bem_detmath.h)
+struct DevFreq {
+    double k;
+    bem_c64 cc;
+    const int32_t* subst;
+    const bem_c64* cphi;
+    const bem_c64* cd;
+};
+
+// regular branch of BemOperatorsConst (boundaryelementsolver.cpp:856-893);
+// quadrature points read with stride qs from qx/qy/qz
+__device__ __forceinline__ bem_ops4 regular_ops(double k, double px, double py, double pz, double npx, double npy,
+                                                double npz, double nqx, double nqy, double nqz, double area,
+                                                const double* qx, const double* qy, const double* qz, int64_t qs)
+{
+    double upnq = fma(npz, nqz, fma(npy, nqy, npx * nqx));
+    bem_ops4 o = bem_ops4_zero();
+#pragma unroll 2
+    for (int q = 0; q < 16; q++)
+        bem_point_accumulate(&o, k, px, py, pz, npx, npy, npz, nqx, nqy, nqz, upnq, qx[q * qs], qy[q * qs],
+                             qz[q * qs], kTriW[q]);
+    o.L = c64_scale(o.L, area);
+    o.M = c64_scale(o.M, area);
+    o.Mt = c64_scale(o.Mt, area);
+    o.N = c64_scale(o.N, area);
+    return o;
+}
+
+// BemOperatorsNearSing (boundaryelementsolver.cpp:1387-1514) on precomputed
+// sub-triangle geometry
+__device__ __noinline__ bem_ops4 near_ops(double k, double px, double py, double pz, double npx, double npy,
+                                          double npz, double nqx, double nqy, double nqz, const double* geo,
+                                          int nsub)
+{
+    double upnq = fma(npz, nqz, fma(npy, nqy, npx * nqx));
+    bem_ops4 o = bem_ops4_zero();
+    for (int s = 0; s < nsub; s++) {
+        const double* g = geo + s * kNearSub;
+        bem_ops4 t = bem_ops4_zero();
+        for (int ia = 0; ia < 5; ia++) {
+            double maxR = g[10 + 3 * ia], ca = g[11 + 3 * ia], sa = g[12 + 3 * ia];
+            for (int ib = 0; ib < 5; ib++) {
+                double rs = (maxR + kGL5X[ib] * maxR) / 2.0;
+                double x = rs * ca, y = rs * sa;
+                double qx = (g[0] + x * g[3]) + y * g[6];
+                double qy = (g[1] + x * g[4]) + y * g[7];
+                double qz = (g[2] + x * g[5]) + y * g[8];
+                double fw = ((maxR / 2.0 * rs) * kGL5W[ia]) * kGL5W[ib];
+                bem_point_accumulate(&t, k, px, py, pz, npx, npy, npz, nqx, nqy, nqz, upnq, qx, qy, qz, fw);
+            }
+        }
+        double h = g[9];
+        o.L = c64_axpy(h, t.L, o.L);
+        o.M = c64_axpy(h, t.M, o.M);
+        o.Mt = c64_axpy(h, t.Mt, o.Mt);
+        o.N = c64_axpy(h, t.N, o.N);
+    }
+    return o;
+}
+
+// BemOperatorsSingularPolarInt (boundaryelementsolver.cpp:896-1029) on
+// precomputed polar radii
+__device__ __noinline__ bem_ops4 self_ops(double k, const double* g)
+{
+    bem_c64 Ls[3], Ns[3];
+    for (int s = 0; s < 3; s++) {
+        const double* gs = g + 33 * s;
+        bem_c64 L = c64(0, 0), N = c64(0, 0);
+        for (int q = 0; q < 16; q++) {
+            double r1 = gs[1 + q], r2 = gs[17 + q];
+            bem_c64 e1 = det_expi(k * r1), e2 = det_expi(k * r2);
+            L = c64_axpy(kHGW[q], c64_add(e1, e2), L);
+            N = c64_axpy(kHGW[q], c64_add(c64(e1.re / r1, e1.im / r1), c64(e2.re / r2, e2.im / r2)), N);
+        }
+        Ls[s] = c64_scale(L, gs[0]);
+        Ns[s] = c64_scale(N, gs[0]);
+    }
+    bem_c64 LT = c64_add(c64_add(Ls[0], Ls[1]), Ls[2]);
+    LT = c64(LT.re / (2.0 * BEM_PI), LT.im / (2.0 * BEM_PI));
+    bem_c64 NT = c64_add(c64_add(Ns[0], Ns[1]), Ns[2]);
+    NT = c64(NT.re / (4.0 * BEM_PI), NT.im / (4.0 * BEM_PI));
+    bem_ops4 o = bem_ops4_zero();
+    o.L = c64_mul(c64(0.0, 1.0 / (2.0 * k)), c64(1.0 - LT.re, -LT.im));
+    o.N = c64(0.0 - NT.re, k / 2.0 - NT.im);
+    return o;
+}
+
+__device__ __forceinline__ int64_t near_find(const DevMesh& M, int64_t i, int32_t j)
+{
+    int64_t lo = M.nr_ptr[i], hi = M.nr_ptr[i + 1];
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        int32_t c = M.nr_col[mid];
+        if (c == j) return mid;
+        if (c < j) lo = mid + 1; else hi = mid;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ bool is_near(const DevMesh& M, int64_t i, int64_t j)
+{
+    double dx = M.px[i] - M.px[j], dy = M.py[i] - M.py[j], dz = M.pz[i] - M.pz[j];
+    return sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim;
+}
+
+// BemOperatorsConst (boundaryelementsolver.cpp:844-894), any pair
+__device__ bem_ops4 ops_general(const DevMesh& M, double k, int64_t i, int64_t j)
+{
+    if (i == j) return self_ops(k, M.self_geo + kSelf * i);
+    if (is_near(M, i, j)) {
+        int64_t t = near_find(M, i, (int32_t)j);
+        if (t >= 0)
+            return near_ops(k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
+                            M.nr_geo + M.nr_off[t], M.nr_nsub[t]);
+    }
+    return regular_ops(k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
+                       M.area[j], M.qx + j, M.qy + j, M.qz + j, M.n);
+}
+
+// implicitPhiMatrix2 / implicitDPhiMatrix2 (boundaryelementsolver.cpp:2004-2040)
+__device__ __forceinline__ bem_c64 bc_entry(const DevFreq& F, int kind, const bem_ops4& o, int64_t i, int64_t j)
+{
+    bem_c64 X, Y;
+    bem_xy(&o, F.cc, i == j, &X, &Y);
+    return kind == 0 ? bem_phi_entry(X, Y, F.subst[j], F.cphi[j]) : bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
+}
+
+__device__ __forceinline__ bem_c64 entry_general(const DevMesh& M, const DevFreq& F, int kind, int64_t i, int64_t j)
+{
+    bem_ops4 o = ops_general(M, F.k, i, j);
+    return bc_entry(F, kind, o, i, j);
+}
+
+}  // namespace bemdev

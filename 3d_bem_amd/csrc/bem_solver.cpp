@@ -1,0 +1,407 @@
+// Host mirror of the reference solver (see bem_solver.hpp) and the
+// solver-level C ABI (bem_solver_*).  Compiled with -ffp-contract=off.
+#include "bem_solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/bem_detmath.h"
+
+namespace bem {
+namespace {
+
+void check(bem_ctx_t ctx, int rc, const char* what)
+{
+    if (rc != BEM_OK) throw Error(rc, std::string(what) + ": " + bem_last_error(ctx));
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+void BoundaryElements::prepare()
+{
+    bemsetup::Elements E;
+    E.n = n;
+    E.nodes = nodes;
+    bemsetup::prepare_elements(E);
+    normal.resize(3 * n);
+    colloc.resize(3 * n);
+    area = E.area;
+    for (int64_t e = 0; e < n; e++) {
+        normal[3 * e] = E.normal[e].x; normal[3 * e + 1] = E.normal[e].y; normal[3 * e + 2] = E.normal[e].z;
+        colloc[3 * e] = E.colloc[e].x; colloc[3 * e + 1] = E.colloc[e].y; colloc[3 * e + 2] = E.colloc[e].z;
+    }
+    averageTriangleDim = E.hbar;
+}
+
+ClusterTree::ClusterTree(BoundaryElements* be, int64_t leaf)
+{
+    tree_ = bemsetup::build_cluster_tree(be->n, be->nodes.data(), leaf);
+    // ClusterTree::recursiveReorder (clustertree.cpp:265-283)
+    std::vector<double> nodes(be->nodes.size());
+    std::vector<RobinBoundaryCondition> bc(be->n);
+    std::vector<int64_t> idx(be->n);
+    for (int64_t e = 0; e < be->n; e++) {
+        int64_t o = tree_.perm[e];
+        std::memcpy(&nodes[9 * e], &be->nodes[9 * o], 9 * sizeof(double));
+        bc[e] = be->bc[o];
+        idx[e] = be->elementIndex.empty() ? o : be->elementIndex[o];
+    }
+    be->nodes.swap(nodes);
+    be->bc.swap(bc);
+    be->elementIndex.swap(idx);
+    be->prepare();
+}
+
+DeviceVector::DeviceVector(bem_ctx_t ctx, int64_t n) : ctx_(ctx), n_(n)
+{
+    check(ctx_, bem_malloc(ctx_, (size_t)n * sizeof(cplx), &p_), "DeviceVector");
+}
+DeviceVector::~DeviceVector()
+{
+    if (p_) bem_free(ctx_, p_);
+}
+void DeviceVector::upload(const std::vector<cplx>& h)
+{
+    if ((int64_t)h.size() != n_) throw Error(BEM_ERR_ARG, "DeviceVector::upload: size");
+    check(ctx_, bem_memcpy_htod(ctx_, p_, h.data(), n_ * sizeof(cplx)), "DeviceVector::upload");
+}
+std::vector<cplx> DeviceVector::download() const
+{
+    std::vector<cplx> h(n_);
+    check(ctx_, bem_memcpy_dtoh(ctx_, h.data(), p_, n_ * sizeof(cplx)), "DeviceVector::download");
+    return h;
+}
+
+void HMatrix::setClusterTrees(std::shared_ptr<ClusterTree> rows, std::shared_ptr<ClusterTree> cols)
+{
+    rows_ = std::move(rows);
+    cols_ = std::move(cols);
+}
+
+void HMatrix::populateBlockClusterTree(double eta)
+{
+    if (!rows_ || rows_ != cols_) throw Error(BEM_ERR_STATE, "populateBlockClusterTree: one cluster tree expected");
+    partition_ = bemsetup::build_partition(rows_->tree_, eta);
+    std::vector<int64_t> r0, rs, c0, cs;
+    std::vector<uint8_t> adm;
+    for (auto& B : partition_) {
+        r0.push_back(B.row0); rs.push_back(B.rows); c0.push_back(B.col0); cs.push_back(B.cols);
+        adm.push_back(B.adm ? 1 : 0);
+    }
+    check(ctx_, bem_upload_partition(ctx_, (int64_t)partition_.size(), r0.data(), rs.data(), c0.data(), cs.data(),
+                                     adm.data()),
+          "populateBlockClusterTree");
+}
+
+void HMatrix::assemble(int kind, long rank, double relErr, uint64_t seed, int precision)
+{
+    clear();
+    check(ctx_, bem_assemble(ctx_, kind, relErr, rank, precision, seed, &h_), "assembleBlocks");
+}
+void HMatrix::assembleBlocks(long rank, double relErr, uint64_t seed, int precision)
+{
+    assemble(BEM_KIND_PHI, rank, relErr, seed, precision);
+}
+void HMatrix::assembleBlocksExtra(long rank, double relErr, uint64_t seed, int precision)
+{
+    assemble(BEM_KIND_DPHI, rank, relErr, seed, precision);
+}
+void HMatrix::clear()
+{
+    if (h_) bem_hmat_destroy(h_);
+    h_ = nullptr;
+}
+double HMatrix::stat(int q) const
+{
+    double s[8] = {0};
+    if (h_) bem_hmat_stats(h_, s);
+    return s[q];
+}
+long HMatrix::calculateMaxBlockRank() const { return (long)stat(1); }
+int64_t HMatrix::evaluatedEntries() const { return (int64_t)stat(0); }
+
+void HArithm::MVM(DeviceVector& y, HMatrix& h, const DeviceVector& x)
+{
+    check(h.context(), bem_hmvm(h.context(), h.handle(), x.data(), y.data(), 1), "HArithm::MVM");
+    check(h.context(), bem_synchronize(h.context()), "HArithm::MVM");
+}
+
+int64_t GMRES::gmresSolve(HMatrix& A, DeviceVector& x, const DeviceVector& b, double tol, long restart, double* rel)
+{
+    int64_t its = 0;
+    double r = 0;
+    int rc = bem_gmres(A.context(), A.handle(), b.data(), x.data(), tol, restart, 100000, &its, &r);
+    if (rc != BEM_OK && rc != BEM_ERR_NOCONV) check(A.context(), rc, "GMRES::gmresSolve");
+    if (rel) *rel = r;
+    return its;
+}
+
+BoundaryElementSolver::BoundaryElementSolver(int device, BoundaryElements elements, int64_t leaf, double eta)
+    : boundaryElements(std::move(elements)), eta_(eta)
+{
+    int rc = bem_create(&ctx_, device);
+    if (rc != BEM_OK) throw Error(rc, "bem_create failed (no HIP device?)");
+    try {
+        clusterTree = std::make_shared<ClusterTree>(&boundaryElements, leaf);
+        auto& E = boundaryElements;
+        check(ctx_, bem_upload_mesh(ctx_, E.n, E.nodes.data(), E.normal.data(), E.area.data(), E.colloc.data(),
+                                    E.averageTriangleDim),
+              "bem_upload_mesh");
+        std::vector<cplx> a(E.n), b(E.n), g(E.n);
+        for (int64_t e = 0; e < E.n; e++) { a[e] = E.bc[e].a; b[e] = E.bc[e].b; g[e] = E.bc[e].g; }
+        check(ctx_, bem_set_bc(ctx_, (const double*)a.data(), (const double*)b.data(), (const double*)g.data()),
+              "bem_set_bc");
+        phiHMat = std::make_unique<HMatrix>(ctx_);
+        dPhiHMat = std::make_unique<HMatrix>(ctx_);
+        phiHMat->setClusterTrees(clusterTree, clusterTree);
+        dPhiHMat->setClusterTrees(clusterTree, clusterTree);
+        phiHMat->populateBlockClusterTree(eta_);
+    } catch (...) {
+        phiHMat.reset();
+        dPhiHMat.reset();
+        bem_destroy(ctx_);
+        ctx_ = nullptr;
+        throw;
+    }
+}
+
+BoundaryElementSolver::~BoundaryElementSolver()
+{
+    phiHMat.reset();
+    dPhiHMat.reset();
+    if (ctx_) bem_destroy(ctx_);
+}
+
+// calculateBoundaryConditionAndSourceTermVector + sourceTerm
+// (boundaryelementsolver.cpp:3386-3441; the point-source term keeps the
+// reference's division by (ikr - 1)); a plane wave A e^{ik d.x} contributes
+// phi_inc + cc ik (d.n) phi_inc
+void BoundaryElementSolver::calculateBoundaryConditionAndSourceTermVector()
+{
+    auto& E = boundaryElements;
+    sourceTermVector.assign(E.n, cplx(0, 0));
+    const bem_c64 cc = c64(couplingParameter.real(), couplingParameter.imag());
+    for (int64_t i = 0; i < E.n; i++) {
+        double x = E.colloc[3 * i], y = E.colloc[3 * i + 1], z = E.colloc[3 * i + 2];
+        double nx = E.normal[3 * i], ny = E.normal[3 * i + 1], nz = E.normal[3 * i + 2];
+        bem_c64 acc = c64(0, 0);
+        for (auto& s : pointSources) {
+            double Rx = x - s.x, Ry = y - s.y, Rz = z - s.z;
+            double r = std::sqrt(fma(Rz, Rz, fma(Ry, Ry, Rx * Rx)));
+            double rup = fma(Rz, nz, fma(Ry, ny, Rx * nx)) / r;
+            bem_c64 G = c64_mul(c64(s.weight.real(), s.weight.imag()),
+                                c64_scale(det_expi(wavenumber * r), 1.0 / (4.0 * BEM_PI * r)));
+            bem_c64 dG = c64_div(c64_scale(G, 1.0 / r), c64(-1.0, wavenumber * r));
+            acc = c64_add(acc, c64_add(G, c64_mul(cc, c64_scale(dG, rup))));
+        }
+        for (auto& w : planeWaves) {
+            double dx = fma(w.dz, z, fma(w.dy, y, w.dx * x));
+            double dn = fma(w.dz, nz, fma(w.dy, ny, w.dx * nx));
+            bem_c64 phi = c64_mul(c64(w.amplitude.real(), w.amplitude.imag()), det_expi(wavenumber * dx));
+            bem_c64 dphi = c64_mul(c64(0.0, wavenumber * dn), phi);
+            acc = c64_add(acc, c64_add(phi, c64_mul(cc, dphi)));
+        }
+        sourceTermVector[i] = cplx(acc.re, acc.im);
+    }
+}
+
+void BoundaryElementSolver::calculateBoundarySolution()
+{
+    if (!(frequency > 0)) throw Error(BEM_ERR_ARG, "frequency must be > 0 (the f = 0 Laplace branch is not on this path)");
+    wavenumber = (2.0 * frequency * BEM_PI) / waveSpeed;
+    switch (bemSolverParameters.coupling) {
+    case NoCoupling: couplingParameter = cplx(0, 0); break;
+    case BurtonMillerCoupling: couplingParameter = cplx(0.0, 1.0 / wavenumber); break;
+    case KirkupCoupling: {
+        bem_c64 q = c64_div(c64(0, 1), c64(wavenumber + 1.0, 0));
+        couplingParameter = cplx(q.re, q.im);
+        break;
+    }
+    }
+    check(ctx_, bem_set_frequency(ctx_, wavenumber, couplingParameter.real(), couplingParameter.imag()),
+          "bem_set_frequency");
+    hMatrixSolve();
+}
+
+// BoundaryElementSolver::hMatrixSolve (boundaryelementsolver.cpp:526-744)
+void BoundaryElementSolver::hMatrixSolve()
+{
+    auto& E = boundaryElements;
+    const auto& P = bemSolverParameters;
+    std::memset(stats, 0, sizeof stats);
+    double t0 = now_ms();
+    calculateBoundaryConditionAndSourceTermVector();
+    phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    double t1 = now_ms();
+    stats[7] = phiHMat->stat(1);
+    stats[9] = phiHMat->stat(0);
+    // b = s + Phi f (:578-579)
+    DeviceVector b(ctx_, E.n), f(ctx_, E.n), x(ctx_, E.n);
+    std::vector<cplx> fv(E.n);
+    for (int64_t e = 0; e < E.n; e++) fv[e] = E.bc[e].g;
+    f.upload(fv);
+    b.upload(sourceTermVector);
+    HArithm::MVM(b, *phiHMat, f);
+    phiHMat->clear();
+    double t2 = now_ms();
+    dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    double t3 = now_ms();
+    stats[8] = dPhiHMat->stat(1);
+    stats[10] = dPhiHMat->stat(0);
+    // GMRES with tolerance 1e-3 acaRelativeError by default (:670); start
+    // vector zero
+    x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
+    double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
+    double est = 0;
+    int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est);
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    double t4 = now_ms();
+    // true residual (:682-684)
+    DeviceVector r(ctx_, E.n);
+    check(ctx_, bem_hmvm(ctx_, dPhiHMat->handle(), x.data(), r.data(), 0), "residual");
+    std::vector<cplx> rv = r.download(), bv = b.download();
+    double num = 0, den = 0;
+    for (int64_t e = 0; e < E.n; e++) { num += std::norm(rv[e] - bv[e]); den += std::norm(bv[e]); }
+    // reconstruction (:729-740) and pressure (:2251-2261)
+    DeviceVector phi(ctx_, E.n), dphi(ctx_, E.n), p(ctx_, E.n);
+    check(ctx_, bem_reconstruct(ctx_, x.data(), airDensity, frequency, phi.data(), dphi.data(), p.data()),
+          "bem_reconstruct");
+    phiSolution = phi.download();
+    dPhiSolution = dphi.download();
+    soundPressure = p.download();
+    dPhiHMat->clear();
+    double t5 = now_ms();
+    int64_t nnear = 0, nfar = 0;
+    for (auto& B : phiHMat->getMinPartition()) (B.adm ? nfar : nnear)++;
+    stats[0] = t1 - t0; stats[1] = t2 - t1; stats[2] = t3 - t2; stats[3] = t4 - t3; stats[4] = t5 - t0;
+    stats[5] = (double)its;
+    stats[6] = den > 0 ? std::sqrt(num / den) : 0;
+    stats[11] = (double)nnear;
+    stats[12] = (double)nfar;
+}
+
+}  // namespace bem
+
+// ========================================================== solver C ABI
+
+struct bem_solver {
+    std::unique_ptr<bem::BoundaryElementSolver> s;
+    std::string err;
+};
+
+extern "C" {
+
+int bem_solver_create(bem_solver_t* out, int device, int64_t n, const double* nodes9, const double* bc6, int64_t leaf,
+                      double eta)
+{
+    if (!out) return BEM_ERR_ARG;
+    *out = nullptr;
+    if (n <= 0 || !nodes9 || !bc6 || leaf < 1 || !(eta > 0)) return BEM_ERR_ARG;
+    auto S = new bem_solver();
+    try {
+        bem::BoundaryElements E;
+        E.n = n;
+        E.nodes.assign(nodes9, nodes9 + 9 * n);
+        E.bc.resize(n);
+        for (int64_t e = 0; e < n; e++) {
+            const double* q = bc6 + 6 * e;
+            E.bc[e].a = bem::cplx(q[0], q[1]);
+            E.bc[e].b = bem::cplx(q[2], q[3]);
+            E.bc[e].g = bem::cplx(q[4], q[5]);
+        }
+        S->s = std::make_unique<bem::BoundaryElementSolver>(device, std::move(E), leaf, eta);
+    } catch (const bem::Error& e) {
+        delete S;
+        return e.code;
+    } catch (...) {
+        delete S;
+        return BEM_ERR_HIP;
+    }
+    *out = S;
+    return BEM_OK;
+}
+
+int bem_solver_destroy(bem_solver_t s)
+{
+    if (!s) return BEM_ERR_ARG;
+    delete s;
+    return BEM_OK;
+}
+
+const char* bem_solver_last_error(bem_solver_t s) { return s ? s->err.c_str() : "null solver"; }
+
+bem_ctx_t bem_solver_context(bem_solver_t s) { return s && s->s ? s->s->context() : nullptr; }
+
+int bem_solver_layout(bem_solver_t s, int64_t* perm, int64_t* nblocks, int64_t* blocks5, int64_t cap)
+{
+    if (!s || !s->s) return BEM_ERR_ARG;
+    auto& E = s->s->boundaryElements;
+    if (perm) std::memcpy(perm, E.elementIndex.data(), E.n * sizeof(int64_t));
+    const auto& P = s->s->partition();
+    if (nblocks) *nblocks = (int64_t)P.size();
+    if (blocks5) {
+        if (cap < (int64_t)P.size()) return BEM_ERR_ARG;
+        for (size_t b = 0; b < P.size(); b++) {
+            blocks5[5 * b] = P[b].row0; blocks5[5 * b + 1] = P[b].rows;
+            blocks5[5 * b + 2] = P[b].col0; blocks5[5 * b + 3] = P[b].cols;
+            blocks5[5 * b + 4] = P[b].adm ? 1 : 0;
+        }
+    }
+    return BEM_OK;
+}
+
+int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* pts, int has_plane, const double* pl)
+{
+    if (!s || !s->s || npoint < 0 || (npoint && !pts) || (has_plane && !pl)) return BEM_ERR_ARG;
+    s->s->pointSources.clear();
+    s->s->planeWaves.clear();
+    for (int64_t q = 0; q < npoint; q++) {
+        const double* v = pts + 6 * q;
+        s->s->pointSources.push_back(bem::PointSource{v[0], v[1], v[2], bem::cplx(v[3], v[4])});
+    }
+    if (has_plane) s->s->planeWaves.push_back(bem::PlaneWave{pl[0], pl[1], pl[2], bem::cplx(pl[3], pl[4])});
+    return BEM_OK;
+}
+
+int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
+                     int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision, double* phi,
+                     double* dphi, double* pressure, double* stats16)
+{
+    if (!s || !s->s || coupling < 0 || coupling > 2 || max_rank < 0 || restart < 1) return BEM_ERR_ARG;
+    try {
+        auto& S = *s->s;
+        S.setFrequency(freq);
+        S.setWaveSpeed(c);
+        S.setAirDensity(rho);
+        S.bemSolverParameters.acaRelativeError = aca_tol;
+        S.bemSolverParameters.acaMaxRank = (unsigned long)max_rank;
+        S.bemSolverParameters.coupling = (bem::BemCoupling)coupling;
+        S.bemSolverParameters.gmresTolerance = gmres_tol;
+        S.bemSolverParameters.gmresRestart = (long)restart;
+        S.bemSolverParameters.pivotSeed = seed;
+        S.bemSolverParameters.precision = precision;
+        S.calculateBoundarySolution();
+        int64_t n = S.boundaryElements.n;
+        if (phi) std::memcpy(phi, S.phiSolution.data(), n * sizeof(bem::cplx));
+        if (dphi) std::memcpy(dphi, S.dPhiSolution.data(), n * sizeof(bem::cplx));
+        if (pressure) std::memcpy(pressure, S.soundPressure.data(), n * sizeof(bem::cplx));
+        if (stats16) std::memcpy(stats16, S.stats, sizeof S.stats);
+    } catch (const bem::Error& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return BEM_ERR_HIP;
+    }
+    return BEM_OK;
+}
+
+}  // extern "C"

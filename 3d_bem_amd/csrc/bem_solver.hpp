@@ -1,0 +1,155 @@
+// C++ mirror of the reference's solver-side API for the H-matrix BEM path.
+// Class and method names follow the reference so that its callers read the
+// same (boundaryelementsolver.h, HMatrix/clustertree.h, HMatrix/hmatrix.h,
+// HMatrix/harithm.h, HMatrix/gmres.h); underneath, every operator entry,
+// block, product and Krylov step runs on the GPU through include/bem_hip.h.
+#pragma once
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bem_hip.h"
+#include "bem_setup.hpp"
+
+namespace bem {
+
+using cplx = std::complex<double>;
+
+// robinboundarycondition.h: a phi + b dphi/dn = g
+struct RobinBoundaryCondition {
+    cplx a{0.0, 0.0}, b{1.0, 0.0}, g{0.0, 0.0};
+};
+
+// boundaryelements.h:35-170 (triangles + derived geometry), SoA here
+struct BoundaryElements {
+    int64_t n = 0;
+    std::vector<double> nodes;                      // n x 9
+    std::vector<RobinBoundaryCondition> bc;          // n
+    std::vector<double> normal, colloc, area;        // n x 3, n x 3, n
+    double averageTriangleDim = 0;
+    std::vector<int64_t> elementIndex;               // input index of each element
+    void prepare();                                  // prepareBoundaryElements (:3463-3468)
+};
+
+// HMatrix/clustertree.h:77-86: builds the tree and reorders the elements
+class ClusterTree {
+public:
+    explicit ClusterTree(BoundaryElements* elements, int64_t triangleThreshold = 50);
+    const std::vector<bemsetup::Cluster>& clusters() const { return tree_.clusters; }
+    const std::vector<int64_t>& permutation() const { return tree_.perm; }
+    bemsetup::ClusterTree tree_;
+};
+
+// A device-resident complex vector (Eigen::VectorXcd's role in the reference)
+class DeviceVector {
+public:
+    DeviceVector(bem_ctx_t ctx, int64_t n);
+    ~DeviceVector();
+    DeviceVector(const DeviceVector&) = delete;
+    DeviceVector& operator=(const DeviceVector&) = delete;
+    void upload(const std::vector<cplx>& h);
+    std::vector<cplx> download() const;
+    void* data() const { return p_; }
+    int64_t size() const { return n_; }
+private:
+    bem_ctx_t ctx_;
+    int64_t n_;
+    void* p_ = nullptr;
+};
+
+// HMatrix/hmatrix.h:157-260 -- blocks live on the GPU
+class HMatrix {
+public:
+    HMatrix(bem_ctx_t ctx) : ctx_(ctx) {}
+    ~HMatrix() { clear(); }
+    void setClusterTrees(std::shared_ptr<ClusterTree> rows, std::shared_ptr<ClusterTree> cols);
+    // HMatrix::populateBlockClusterTree (hmatrix.cpp:922-1007); admissibility
+    // constant eta (reference constexpr 3, hmatrix.h:240)
+    void populateBlockClusterTree(double eta = 3.0);
+    // assembleBlocks with the BEM Phi operator (implicitPhiMatrix2)
+    void assembleBlocks(long rank, double relativeError, uint64_t seed, int precision = BEM_PREC_FP64);
+    // assembleBlocksExtra with the BEM DPhi operator (implicitDPhiMatrix2) and
+    // getNormalFilteredPivotIndices
+    void assembleBlocksExtra(long rank, double relativeError, uint64_t seed, int precision = BEM_PREC_FP64);
+    void clear();
+    long calculateMaxBlockRank() const;
+    int64_t evaluatedEntries() const;
+    const std::vector<bemsetup::Block>& getMinPartition() const { return partition_; }
+    bem_hmat_t handle() const { return h_; }
+    bem_ctx_t context() const { return ctx_; }
+    double stat(int q) const;
+private:
+    void assemble(int kind, long rank, double relErr, uint64_t seed, int precision);
+    bem_ctx_t ctx_;
+    bem_hmat_t h_ = nullptr;
+    std::shared_ptr<ClusterTree> rows_, cols_;
+    std::vector<bemsetup::Block> partition_;
+};
+
+// HMatrix/harithm.h:32: y += H x
+struct HArithm {
+    static void MVM(DeviceVector& y, HMatrix& h, const DeviceVector& x);
+};
+
+// HMatrix/gmres.h:170-180
+struct GMRES {
+    static int64_t gmresSolve(HMatrix& A, DeviceVector& x, const DeviceVector& b, double tolerance = 0.01,
+                              long iterations = 300, double* relResidual = nullptr);
+};
+
+enum BemCoupling : int { NoCoupling = 0, BurtonMillerCoupling = 1, KirkupCoupling = 2 };
+
+// bemparameters.h:21-34 (the fields this path uses)
+struct BemSolverParameters {
+    double acaRelativeError = 0.01;
+    unsigned long acaMaxRank = 0;
+    BemCoupling coupling = KirkupCoupling;
+    double gmresTolerance = -1;   // < 0: 1e-3 * acaRelativeError (boundaryelementsolver.cpp:670)
+    long gmresRestart = 300;
+    uint64_t pivotSeed = 1;
+    int precision = BEM_PREC_FP64;
+};
+
+struct PointSource { double x, y, z; cplx weight; };
+struct PlaneWave { double dx, dy, dz; cplx amplitude; };
+
+// boundaryelementsolver.h: the H-matrix solve of one frequency point
+class BoundaryElementSolver {
+public:
+    BoundaryElementSolver(int device, BoundaryElements elements, int64_t leaf, double eta);
+    ~BoundaryElementSolver();
+    void setFrequency(double f) { frequency = f; }
+    void setWaveSpeed(double c) { waveSpeed = c; }
+    void setAirDensity(double rho) { airDensity = rho; }
+    void calculateBoundarySolution();      // boundaryelementsolver.cpp:12-138 (H-matrix branch)
+    BoundaryElements boundaryElements;     // cluster order after construction
+    BemSolverParameters bemSolverParameters;
+    std::vector<PointSource> pointSources;
+    std::vector<PlaneWave> planeWaves;
+    std::vector<cplx> phiSolution, dPhiSolution, soundPressure;
+    double frequency = 0, waveSpeed = 343.0, airDensity = 1.2041;
+    double stats[16] = {0};
+    bem_ctx_t context() const { return ctx_; }
+    const std::vector<bemsetup::Block>& partition() const { return phiHMat->getMinPartition(); }
+    std::shared_ptr<ClusterTree> clusterTree;
+private:
+    void hMatrixSolve();
+    void calculateBoundaryConditionAndSourceTermVector();
+    bem_ctx_t ctx_ = nullptr;
+    double wavenumber = 0;
+    cplx couplingParameter{0, 0};
+    std::vector<cplx> sourceTermVector;
+    std::unique_ptr<HMatrix> phiHMat, dPhiHMat;
+    double eta_;
+};
+
+struct Error : std::exception {
+    int code;
+    std::string msg;
+    Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+}  // namespace bem

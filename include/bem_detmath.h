@@ -38,8 +38,8 @@
 #include <math.h>
 #include <stdint.h>
 
-#if defined(__HIPCC__)
-#define BEM_HD __host__ __device__ __forceinline__
+#if defined(__HIP__)
+#define BEM_HD __host__ __device__ inline __attribute__((always_inline))
 #else
 #define BEM_HD static inline
 #endif

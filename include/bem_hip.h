@@ -1,0 +1,149 @@
+/*
+ * bem_hip.h -- C ABI of the MI355X (gfx950) H-matrix BEM path.
+ *
+ * The reference runs this path behind callback-parameterised C++ methods:
+ *   HMatrix::assembleBlocks(rank, relErr, std::function<complex(long,long)>)
+ *       HMatrix/hmatrix.h:200, HMatrix/hmatrix.cpp:793-823
+ *   HMatrix::assembleBlocksExtra(rank, relErr, getPivotIndices, implicitMatrix)
+ *       HMatrix/hmatrix.h:201, HMatrix/hmatrix.cpp:825-846
+ *   HArithm::MVM(VectorXcd& y, HMatrix&, const VectorXcd& x)   (y += H x)
+ *       HMatrix/harithm.h:32, HMatrix/harithm.cpp:3-28
+ *   GMRES::gmresSolve(HMatrixWrapper, x0, b, tol, restart)
+ *       HMatrix/gmres.h:170-180
+ * A per-entry host callback cannot run on the GPU, so this ABI takes the
+ * description of the entry generator instead (geometry, boundary
+ * conditions, wavenumber, coupling, operator kind) and owns all device
+ * memory.  Handles are opaque; every call returns 0 on success or a negative
+ * BEM_ERR_* code and records a message for bem_last_error().  No call throws.
+ * One context = one HIP stream; a context is not re-entrant.
+ *
+ * Numbers: complex values are interleaved (re, im) doubles (Eigen's
+ * std::complex<double> layout).  Block storage follows the reference's
+ * BlockCluster fields (hmatrix.h:130-133): dense blocks column-major;
+ * low-rank blocks A ~ U diag(alpha) V with U m x k column-major, alpha the
+ * ACA pivots' reciprocals (the reference's "singularValues") and V k x n
+ * whose rows are the ACA residual rows (the reference's "VAdjMat").
+ */
+#ifndef BEM_HIP_H
+#define BEM_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEM_OK 0
+#define BEM_ERR_ARG -1      /* bad argument or size */
+#define BEM_ERR_HIP -2      /* HIP runtime error (message has the HIP string) */
+#define BEM_ERR_STATE -3    /* call order (e.g. assemble before upload) */
+#define BEM_ERR_NAN -4      /* NaN detected in an assembled operator */
+#define BEM_ERR_NOCONV -5   /* GMRES stopped at maxit above tolerance */
+
+#define BEM_KIND_PHI 0      /* implicitPhiMatrix2,  boundaryelementsolver.cpp:2004-2020 */
+#define BEM_KIND_DPHI 1     /* implicitDPhiMatrix2, boundaryelementsolver.cpp:2022-2040 */
+#define BEM_PREC_FP64 0
+#define BEM_PREC_FP32 1     /* FP32 regular quadrature and block storage; FP64 singular/near-singular */
+
+typedef struct bem_ctx* bem_ctx_t;
+typedef struct bem_hmat* bem_hmat_t;
+typedef struct bem_solver* bem_solver_t;
+
+/* context on HIP device `device` (BoundaryElementSolver owns one per solve) */
+int bem_create(bem_ctx_t* out, int device);
+int bem_destroy(bem_ctx_t ctx);
+const char* bem_last_error(bem_ctx_t ctx);
+
+/* Mesh in cluster order (after ClusterTree's reordering,
+   HMatrix/clustertree.cpp:265-283): nodes n x 9 (a, b, c), normal n x 3,
+   area n, colloc n x 3 (centroids), hbar = averageTriangleDim
+   (boundaryelements.cpp:35-46).  Replaces the geometry read by
+   BemOperatorsConst (boundaryelementsolver.cpp:844-894). */
+int bem_upload_mesh(bem_ctx_t ctx, int64_t n, const double* nodes9, const double* normal3, const double* area,
+                    const double* colloc3, double hbar);
+/* Robin coefficients a phi + b dphi = g per element (complex, n each):
+   the alpha/beta/f vectors of calculateBoundaryConditionAndSourceTermVector
+   (boundaryelementsolver.cpp:3386-3429). */
+int bem_set_bc(bem_ctx_t ctx, const double* alpha_ri, const double* beta_ri, const double* f_ri);
+/* Block partition in minPartition order (HMatrix::populateBlockClusterTree,
+   hmatrix.cpp:922-1007). */
+int bem_upload_partition(bem_ctx_t ctx, int64_t nblocks, const int64_t* row0, const int64_t* rows,
+                         const int64_t* col0, const int64_t* cols, const uint8_t* admissible);
+/* wavenumber k = 2 pi f / c and coupling parameter (boundaryelementsolver.cpp:30-59) */
+int bem_set_frequency(bem_ctx_t ctx, double k, double cc_re, double cc_im);
+
+/* assembleBlocks (kind PHI: dense + partialPivotACA) or assembleBlocksExtra
+   (kind DPHI: dense + partialPivotACAextra with normal-filtered pivots).
+   max_rank 0 = unlimited; seed keys the deterministic pivot generator. */
+int bem_assemble(bem_ctx_t ctx, int kind, double aca_tol, int64_t max_rank, int precision, uint64_t seed,
+                 bem_hmat_t* out);
+int bem_hmat_destroy(bem_hmat_t h);
+/* HArithm::MVM: y (+)= H x on device pointers (complex double, cluster
+   order); accumulate = 1 adds to y as the reference does. */
+int bem_hmvm(bem_ctx_t ctx, bem_hmat_t h, const void* x_dev, void* y_dev, int accumulate);
+/* rank of block b (-1 dense), and its factors in the reference layout (host
+   buffers: dense m x n col-major into full_or_U; else U m x k, alpha k, V k x n
+   col-major), for the host H-LU (HMatrix(const HMatrix&, rank, err),
+   hmatrix.cpp:770-776) and for inspection. */
+int bem_block_info(bem_hmat_t h, int64_t b, int64_t* rank);
+int bem_block_download(bem_hmat_t h, int64_t b, double* full_or_U, double* alpha, double* V);
+/* stats[0] evaluated entries, [1] max rank, [2] mean rank, [3] bytes of block
+   storage, [4] assembly ms (device), [5] blocks re-run after rank-capacity
+   overflow, [6] dense ms, [7] ACA ms */
+int bem_hmat_stats(bem_hmat_t h, double* stats8);
+
+/* GMRES::gmresSolve replacement: restarted GMRES(restart) with CGS2 Arnoldi
+   on the device; x_dev holds the start vector on entry and the solution on
+   return.  Returns BEM_ERR_NOCONV when maxit is reached above tol. */
+int bem_gmres(bem_ctx_t ctx, bem_hmat_t h, const void* b_dev, void* x_dev, double tol, int64_t restart,
+              int64_t maxit, int64_t* iters, double* rel_resid);
+/* boundary-condition reconstruction (boundaryelementsolver.cpp:729-740) and
+   p = i rho 2 pi f phi (phiSolutionToSoundPressure, :2251-2261) */
+int bem_reconstruct(bem_ctx_t ctx, const void* x_dev, double rho, double freq, void* phi_dev, void* dphi_dev,
+                    void* pressure_dev);
+
+/* L, M, Mt, N (out npairs x 8) or Phi/DPhi entries (out npairs x 2) of
+   (row, col) pairs given in host memory, evaluated on the device
+   (BemOperatorsConst / implicitPhiMatrix2 / implicitDPhiMatrix2). */
+int bem_entries4(bem_ctx_t ctx, int64_t npairs, const int64_t* ij, double* out8);
+int bem_entries(bem_ctx_t ctx, int kind, int64_t npairs, const int64_t* ij, double* out2);
+
+/* device memory helpers for callers without a HIP runtime binding */
+int bem_malloc(bem_ctx_t ctx, size_t bytes, void** dev_ptr);
+int bem_free(bem_ctx_t ctx, void* dev_ptr);
+int bem_memcpy_htod(bem_ctx_t ctx, void* dst_dev, const void* src_host, size_t bytes);
+int bem_memcpy_dtoh(bem_ctx_t ctx, void* dst_host, const void* src_dev, size_t bytes);
+int bem_synchronize(bem_ctx_t ctx);
+/* average device time (ms) of the last `bem_profile` window per kernel
+   family: [0] dense assembly [1] singular/near-singular fix-up [2] ACA
+   [3] MVM dense [4] MVM low-rank [5] MVM gather; counts in [6..11] */
+int bem_profile(bem_ctx_t ctx, int enable, double* out12);
+
+/* BoundaryElementSolver::hMatrixSolve (boundaryelementsolver.cpp:526-744) as
+   one call: cluster tree + partition on the host, everything else on the
+   device.  Mesh in input order: nodes n x 9, bc n x 6 (a, b, g complex). */
+int bem_solver_create(bem_solver_t* out, int device, int64_t n, const double* nodes9, const double* bc6,
+                      int64_t leaf, double eta);
+int bem_solver_destroy(bem_solver_t s);
+const char* bem_solver_last_error(bem_solver_t s);
+bem_ctx_t bem_solver_context(bem_solver_t s);
+/* perm[new] = old; blocks nb x 5 (row0, rows, col0, cols, admissible) */
+int bem_solver_layout(bem_solver_t s, int64_t* perm, int64_t* nblocks, int64_t* blocks5, int64_t cap);
+/* point sources (x, y, z, weight re, im) and an optional plane wave
+   (dir x, y, z, amplitude re, im) for the source-term vector */
+int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* points6, int has_plane,
+                           const double* plane5);
+/* one frequency point.  coupling: 0 none, 1 Burton-Miller, 2 Kirkup.
+   Outputs (cluster order, complex n each): phi, dphi, pressure.
+   stats[0..15]: t_phi_asm, t_rhs, t_dphi_asm, t_gmres, t_total (ms),
+   gmres iterations, true relative residual, max rank phi, max rank dphi,
+   evaluated phi, evaluated dphi, near blocks, far blocks, refinement steps,
+   fp32 gmres iterations, reserved. */
+int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
+                     int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
+                     double* phi, double* dphi, double* pressure, double* stats16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

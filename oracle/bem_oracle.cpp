@@ -476,8 +476,10 @@ void make_blocks(orc_problem& P, int rc, int cc, double eta)
 
 /* ------------------------------------------------ deterministic reductions */
 
-const int NLANE = 256;
-/* lane-strided partial sums + pairwise tree (the device reduction order) */
+const int NLANE = 64;
+/* lane-strided partial sums over one 64-wide wavefront + pairwise tree (the
+   device reduction order: lane l takes elements l, l+64, ..., then
+   part[l] += part[l+s] for s = 32, 16, ..., 1) */
 double det_sum_abs2(const cplx* v, int64_t n)
 {
     double part[NLANE];

@@ -19,7 +19,7 @@
  *      det_sincos (<= 1.5 ulp) instead of libm's complex exp;
  *   D5 ACA argmax compares |z|^2 (reference |z|), ties keep the first index
  *      as Eigen's maxCoeff does; reductions over a block row/column use a
- *      fixed 256-lane strided partial sum followed by a pairwise tree.
+ *      fixed 64-lane (one wavefront) strided partial sum followed by a pairwise tree.
  */
 #ifndef BEM_ORACLE_H
 #define BEM_ORACLE_H

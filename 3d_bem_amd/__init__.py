@@ -272,7 +272,7 @@ class HMatrix:
 
 STAT_KEYS = ["t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres", "t_total", "gmres_iters", "residual",
              "max_rank_phi", "max_rank_dphi", "evaluated_phi", "evaluated_dphi", "near_blocks", "far_blocks",
-             "refine_steps", "fp32_gmres_iters", "reserved"]
+             "refine_steps", "fp32_gmres_iters", "dphi_bytes"]
 
 
 class Solver:

@@ -623,6 +623,9 @@ struct bem_ctx {
     bool prof = false;
     double prof_ms[6] = {0};
     int64_t prof_cnt[6] = {0};
+    struct Pending { int slot; hipEvent_t e0, e1; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
     std::vector<void*> allocs_part;
 };
 
@@ -639,6 +642,63 @@ struct bem_hmat {
 };
 
 namespace {
+
+// Device time per kernel family: a pair of events is recorded around each
+// launch on the context stream and resolved only when bem_profile() reads
+// the totals, so profiling adds no synchronisation to the timed work.
+struct Timer {
+    bem_ctx* c;
+    int slot;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Timer(bem_ctx* c_, int s);
+    ~Timer();
+};
+
+hipEvent_t pool_get(bem_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void resolve_pending(bem_ctx* c)
+{
+    if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
+            c->prof_ms[p.slot] += ms;
+            c->prof_cnt[p.slot]++;
+        }
+        c->event_pool.push_back(p.e0);
+        c->event_pool.push_back(p.e1);
+    }
+    c->pending.clear();
+}
+
+Timer::Timer(bem_ctx* c_, int s) : c(c_), slot(s)
+{
+    if (c->prof) {
+        e0 = pool_get(c);
+        e1 = pool_get(c);
+        (void)hipEventRecord(e0, c->stream);
+    }
+}
+
+Timer::~Timer()
+{
+    if (c->prof) {
+        (void)hipEventRecord(e1, c->stream);
+        c->pending.push_back(bem_ctx::Pending{slot, e0, e1});
+        if (c->pending.size() > 4096) resolve_pending(c);
+    }
+}
 
 int set_err(bem_ctx* c, int code, const std::string& msg)
 {
@@ -690,33 +750,6 @@ DevFreq dev_freq(const bem_ctx* c)
     return F;
 }
 
-struct Timer {
-    bem_ctx* c;
-    int slot;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    Timer(bem_ctx* c_, int s) : c(c_), slot(s)
-    {
-        if (c->prof) {
-            (void)hipEventCreate(&e0);
-            (void)hipEventCreate(&e1);
-            (void)hipEventRecord(e0, c->stream);
-        }
-    }
-    ~Timer()
-    {
-        if (c->prof) {
-            (void)hipEventRecord(e1, c->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            c->prof_ms[slot] += ms;
-            c->prof_cnt[slot]++;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        }
-    }
-};
-
 int launch_check(bem_ctx* c, const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -763,6 +796,8 @@ int bem_destroy(bem_ctx_t c)
     if (c->d_cphi) (void)hipFree(c->d_cphi);
     if (c->d_cd) (void)hipFree(c->d_cd);
     if (c->d_gm) (void)hipFree(c->d_gm);
+    resolve_pending(c);
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return BEM_OK;
@@ -1543,6 +1578,7 @@ int bem_synchronize(bem_ctx_t c)
 int bem_profile(bem_ctx_t c, int enable, double* out)
 {
     if (!c) return BEM_ERR_ARG;
+    resolve_pending(c);
     if (out)
         for (int q = 0; q < 6; q++) {
             out[q] = c->prof_cnt[q] ? c->prof_ms[q] / (double)c->prof_cnt[q] : 0.0;

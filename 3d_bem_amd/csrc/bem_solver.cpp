@@ -256,6 +256,7 @@ void BoundaryElementSolver::hMatrixSolve()
     double t3 = now_ms();
     stats[8] = dPhiHMat->stat(1);
     stats[10] = dPhiHMat->stat(0);
+    stats[15] = dPhiHMat->stat(3);
     // GMRES with tolerance 1e-3 acaRelativeError by default (:670); start
     // vector zero
     x.upload(std::vector<cplx>(E.n, cplx(0, 0)));

@@ -138,7 +138,7 @@ int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* points6
    stats[0..15]: t_phi_asm, t_rhs, t_dphi_asm, t_gmres, t_total (ms),
    gmres iterations, true relative residual, max rank phi, max rank dphi,
    evaluated phi, evaluated dphi, near blocks, far blocks, refinement steps,
-   fp32 gmres iterations, reserved. */
+   fp32 gmres iterations, bytes of the DPhi block storage. */
 int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);

@@ -247,8 +247,11 @@ bem_ops4 ops_regular(const orc_problem& P, int64_t i, int64_t j)
 
 /* BemOperatorsNearSing (boundaryelementsolver.cpp:1387-1514): split the
    source triangle at the projection of the collocation point, 5x5 polar
-   Gauss-Legendre on each sub-triangle. */
-bem_ops4 ops_near(const orc_problem& P, int64_t i, int64_t j)
+   Gauss-Legendre on each sub-triangle (nq x nq with line rule x/w when given:
+   the test suite's convergence check of the same polar decomposition). */
+bem_ops4 ops_near_rule(const orc_problem& P, int64_t i, int64_t j, int nrule, const double* LX, const double* LW);
+bem_ops4 ops_near(const orc_problem& P, int64_t i, int64_t j) { return ops_near_rule(P, i, j, 5, GL5X, GL5W); }
+bem_ops4 ops_near_rule(const orc_problem& P, int64_t i, int64_t j, int nrule, const double* LX, const double* LW)
 {
     bem_ops4 o = bem_ops4_zero();
     V3 p = P.colloc[i], np = P.normal[i], nq = P.normal[j];
@@ -277,17 +280,17 @@ bem_ops4 ops_near(const orc_problem& P, int64_t i, int64_t j)
         V3 c0 = normalized(e12);
         V3 c1 = normalized(sub(e13, scl(dot(c0, e13), c0)));
         bem_ops4 t = bem_ops4_zero();
-        for (int ia = 0; ia < 5; ia++) {
-            double angle1 = (angP + GL5X[ia] * angP) / 2.0;
+        for (int ia = 0; ia < nrule; ia++) {
+            double angle1 = (angP + LX[ia] * angP) / 2.0;
             double maxR = s12 * std::sin(ang2) / (std::sin(angle1 + ang2));
             double ca = std::cos(angle1), sa = std::sin(angle1);
-            for (int ib = 0; ib < 5; ib++) {
-                double rs = (maxR + GL5X[ib] * maxR) / 2.0;
+            for (int ib = 0; ib < nrule; ib++) {
+                double rs = (maxR + LX[ib] * maxR) / 2.0;
                 double x = rs * ca, y = rs * sa;
                 double qx = (c.a.x + x * c0.x) + y * c1.x;
                 double qy = (c.a.y + x * c0.y) + y * c1.y;
                 double qz = (c.a.z + x * c0.z) + y * c1.z;
-                double fw = ((maxR / 2.0 * rs) * GL5W[ia]) * GL5W[ib];
+                double fw = ((maxR / 2.0 * rs) * LW[ia]) * LW[ib];
                 bem_point_accumulate(&t, P.k, p.x, p.y, p.z, np.x, np.y, np.z, nq.x, nq.y, nq.z, upnq, qx, qy, qz, fw);
             }
         }
@@ -1007,6 +1010,16 @@ void orc_entries4(const orc_problem* p, int64_t np, const int64_t* ij, double* o
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t t = 0; t < np; t++) {
         bem_ops4 o = ops4(*p, ij[2 * t], ij[2 * t + 1]);
+        double v[8] = {o.L.re, o.L.im, o.M.re, o.M.im, o.Mt.re, o.Mt.im, o.N.re, o.N.im};
+        std::memcpy(out + 8 * t, v, sizeof v);
+    }
+}
+void orc_entries4_polar(const orc_problem* p, int64_t np, const int64_t* ij, int nq, const double* x,
+                        const double* w, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t t = 0; t < np; t++) {
+        bem_ops4 o = ops_near_rule(*p, ij[2 * t], ij[2 * t + 1], nq, x, w);
         double v[8] = {o.L.re, o.L.im, o.M.re, o.M.im, o.Mt.re, o.Mt.im, o.N.re, o.N.im};
         std::memcpy(out + 8 * t, v, sizeof v);
     }

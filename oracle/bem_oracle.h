@@ -59,6 +59,10 @@ void orc_get_params(const orc_problem* p, double* out4); /* k, cc.re, cc.im, hba
 
 /* the four operators L, M, Mt, N of pairs (row, col): out npairs x 8 */
 void orc_entries4(const orc_problem* p, int64_t npairs, const int64_t* ij, double* out8);
+/* the near-singular polar decomposition of pairs (row, col) forced for every
+   pair, with an nq-point line rule (x, w on [-1, 1]) in angle and radius */
+void orc_entries4_polar(const orc_problem* p, int64_t npairs, const int64_t* ij, int nq, const double* x,
+                        const double* w, double* out8);
 /* kind 0 Phi, 1 DPhi: out npairs x 2 */
 void orc_entries(const orc_problem* p, int kind, int64_t npairs, const int64_t* ij, double* out2);
 /* dense n x n matrix (column-major complex) */

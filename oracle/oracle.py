@@ -42,6 +42,7 @@ def lib():
             "orc_get_params": (None, [_p, _p]),
             "orc_entries4": (None, [_p, _i64, _p, _p]),
             "orc_entries": (None, [_p, ctypes.c_int, _i64, _p, _p]),
+            "orc_entries4_polar": (None, [_p, _i64, _p, ctypes.c_int, _p, _p, _p]),
             "orc_dense_matrix": (None, [_p, ctypes.c_int, _p]),
             "orc_assemble": (_p, [_p, ctypes.c_int, _dbl, _i64, ctypes.c_uint64, ctypes.c_int]),
             "orc_hmat_free": (None, [_p]),
@@ -141,6 +142,13 @@ class Problem:
         ij = _c(ij, np.int64).reshape(-1, 2)
         out = np.empty((len(ij), 4), np.complex128)
         lib().orc_entries4(self.h, len(ij), _ptr(ij), _ptr(out))
+        return out
+
+    def entries4_polar(self, ij, x, w):
+        ij = _c(ij, np.int64).reshape(-1, 2)
+        x, w = _c(x, np.float64), _c(w, np.float64)
+        out = np.empty((len(ij), 4), np.complex128)
+        lib().orc_entries4_polar(self.h, len(ij), _ptr(ij), len(x), _ptr(x), _ptr(w), _ptr(out))
         return out
 
     def entries(self, kind, ij):

@@ -111,3 +111,14 @@ def test_small_meshes(bem, O, meshes, level, leaf, eta):
     P.set_frequency(200.0, 343.0, 1)
     _, _, pro, _ = P.solve(1e-8, gmres_tol=1e-12, rho=1.21, freq=200.0)
     assert np.linalg.norm(pr - pro) / np.linalg.norm(pro) < 1e-10
+
+
+def test_golden_entries_on_device(bem, O, meshes):
+    """The committed fixture's entries reproduced bit for bit by the device."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz"),
+                allow_pickle=False)
+    P = O.Problem(g["tri"], meshes.robin_bc(len(g["tri"]), 0, 1, 1), leaf=32, eta=2.0)
+    P.set_frequency(500.0, 343.0, 1)
+    ctx = upload_problem(bem, P)
+    assert same_bits(ctx.entries4(g["ij"]), g["entries4"])

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,8 @@ constexpr int kWG = 256;        // workgroup size of the block kernels (4 wavefr
 constexpr int kTile = 64;       // dense tile edge
 constexpr int kCapFirst = 48;   // first-pass ACA rank capacity per block
 constexpr int kCapMax = 1024;   // second-pass capacity limit (LDS: (3 cap + 2) x 16 B <= 48 KiB)
+constexpr int64_t kSmallBlock = 384;   // m + n up to which an ACA block gets one wavefront
+constexpr int64_t kHugeBlock = 1024;   // m + n above which an ACA block gets eight wavefronts
 constexpr double kTiny = 10.0 * 2.220446049250313e-16;  // global.h:53
 
 struct Tile {
@@ -111,22 +114,39 @@ __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile*
     }
 }
 
-// Self and near-singular entries of the dense blocks (K2), one per thread,
-// over a list sorted by cost.
-__global__ __launch_bounds__(kWG) void k_fixup(DevMesh M, DevFreq F, const Fix* __restrict__ fx, int64_t nfix,
-                                               bem_c64* outPhi, bem_c64* outDPhi)
+// Self and near-singular entries (K2): every near-singular pair of the CSR
+// (items 0..nnz-1, in the cost order `order`) and every self entry (items
+// nnz..nnz+n-1), one per thread, into the per-pair table `sing` of the
+// operator `kind`.  Dense blocks scatter from the table (k_scatter), the ACA
+// reads it for the rare singular entries of admissible blocks.
+__global__ __launch_bounds__(kWG) void k_singular(DevMesh M, DevFreq F, int kind, const int32_t* __restrict__ order,
+                                                  int64_t nitems, int64_t nnz, bem_c64* sing)
+{
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nitems) return;
+    int64_t t = order[u];
+    bem_ops4 o;
+    int64_t i, j;
+    if (t >= nnz) {
+        i = j = t - nnz;
+        o = self_ops(F.k, M.self_geo + kSelf * i);
+    } else {
+        i = M.nr_row[t];
+        j = M.nr_col[t];
+        o = near_ops(F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
+                     M.nr_geo + M.nr_off[t], M.nr_nsub[t]);
+    }
+    sing[t] = bc_entry(F, kind, o, i, j);
+}
+
+// singular entries of the dense blocks from the table
+__global__ void k_scatter(const Fix* __restrict__ fx, int64_t nfix, int64_t nnz, const bem_c64* __restrict__ sing,
+                          bem_c64* out)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nfix) return;
     Fix f = fx[t];
-    int64_t i = f.i, j = f.j;
-    bem_ops4 o;
-    if (f.t < 0) o = self_ops(F.k, M.self_geo + kSelf * i);
-    else
-        o = near_ops(F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
-                     M.nr_geo + M.nr_off[f.t], M.nr_nsub[f.t]);
-    if (outPhi) outPhi[f.dest] = bc_entry(F, 0, o, i, j);
-    if (outDPhi) outDPhi[f.dest] = bc_entry(F, 1, o, i, j);
+    out[f.dest] = sing[f.t >= 0 ? f.t : nnz + f.i];
 }
 
 __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int64_t* ij, bem_c64* out)
@@ -166,12 +186,14 @@ __device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b)
     return a;
 }
 
-// first index of the largest |v_i|^2 over [0, len); index `excl` counts as 0
+// first index of the largest |v_i|^2 over [0, len); index `excl` counts as 0.
+// NT threads (NT / 64 wavefronts); every thread gets the result.
+template <int NT>
 __device__ ArgMax block_argmax(const bem_c64* v, int64_t len, int64_t excl, ArgMax* sred)
 {
     const int tid = threadIdx.x;
     ArgMax a{-1.0, (int64_t)1 << 62};
-    for (int64_t t = tid; t < len; t += kWG) {
+    for (int64_t t = tid; t < len; t += NT) {
         double x = t == excl ? 0.0 : c64_abs2(v[t]);
         if (x > a.v) { a.v = x; a.i = t; }
     }
@@ -181,11 +203,12 @@ __device__ ArgMax block_argmax(const bem_c64* v, int64_t len, int64_t excl, ArgM
         o.i = __shfl_xor(a.i, s, 64);
         a = better(a, o);
     }
+    if (NT == 64) return a;
     __syncthreads();
     if ((tid & 63) == 0) sred[tid >> 6] = a;
     __syncthreads();
     ArgMax r = sred[0];
-    for (int w = 1; w < kWG / 64; w++) r = better(r, sred[w]);
+    for (int w = 1; w < NT / 64; w++) r = better(r, sred[w]);
     return r;
 }
 
@@ -206,35 +229,61 @@ struct AcaArgs {
     uint64_t seed;
     const int64_t* piv_off;
     const int32_t* piv;
+    const bem_c64* sing;   // self / near-singular entries (k_singular)
+    int64_t nnz;
 };
 
-// one 64-lane canonical reduction per task, tasks spread over the waves:
-// task 0 |row|^2, 1 |col|^2, 2 + 2l col^H U_l, 3 + 2l row^H V_l  (l < r)
+// The dot products are processed kDotGroup at a time per wavefront: the
+// new vector is loaded once per group and the shuffle trees of the group are
+// interleaved (independent latency chains); each task keeps the canonical
+// lane-strided / pairwise order.
+constexpr int kDotGroup = 8;
+
+// tasks over vector v (length len) against basis W (stride len): k = 0 is
+// |v|^2 into out[base], k >= 1 is v^H W_{k-1} into out[base + 2k]
+template <int NT>
+__device__ __forceinline__ void dot_group_pass(const bem_c64* v, const bem_c64* W, int64_t len, int ntask,
+                                               bem_c64* out, int base)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int kb = w * kDotGroup; kb < ntask; kb += (NT / 64) * kDotGroup) {
+        bem_c64 acc[kDotGroup];
+#pragma unroll
+        for (int g = 0; g < kDotGroup; g++) acc[g] = c64(0, 0);
+        for (int64_t t = lane; t < len; t += 64) {
+            bem_c64 cv = v[t];
+#pragma unroll
+            for (int g = 0; g < kDotGroup; g++) {
+                int k = kb + g;
+                if (k == 0) acc[g].re = acc[g].re + c64_abs2(cv);
+                else if (k < ntask) acc[g] = c64_fma(c64_conj(cv), W[(int64_t)(k - 1) * len + t], acc[g]);
+            }
+        }
+        for (int s = 32; s > 0; s >>= 1) {
+#pragma unroll
+            for (int g = 0; g < kDotGroup; g++) {
+                acc[g].re = acc[g].re + __shfl_down(acc[g].re, s, 64);
+                acc[g].im = acc[g].im + __shfl_down(acc[g].im, s, 64);
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int g = 0; g < kDotGroup; g++) {
+                int k = kb + g;
+                if (k == 0) out[base] = c64(acc[g].re, 0.0);
+                else if (k < ntask) out[base + 2 * k] = acc[g];
+            }
+        }
+    }
+}
+
+// sdots[0] |row|^2, [1] |col|^2, [2 + 2l] row^H V_l, [3 + 2l] col^H U_l (l < r)
+template <int NT>
 __device__ void aca_dots(const bem_c64* col, const bem_c64* row, const bem_c64* U, const bem_c64* V, int64_t m,
                          int64_t n, int r, bem_c64* sdots)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ntask = 2 + 2 * r;
-    for (int task = w; task < ntask; task += kWG / 64) {
-        if (task < 2) {
-            const bem_c64* v = task == 0 ? row : col;
-            int64_t len = task == 0 ? n : m;
-            double p = 0.0;
-            for (int64_t t = lane; t < len; t += 64) p = p + c64_abs2(v[t]);
-            p = wave_sum(p);
-            if (lane == 0) sdots[task] = c64(p, 0.0);
-        } else {
-            int l = (task - 2) >> 1;
-            bool isc = ((task - 2) & 1) == 0;
-            const bem_c64* a = isc ? col : row;
-            const bem_c64* b = isc ? U + (int64_t)l * m : V + (int64_t)l * n;
-            int64_t len = isc ? m : n;
-            bem_c64 p = c64(0, 0);
-            for (int64_t t = lane; t < len; t += 64) p = c64_fma(c64_conj(a[t]), b[t], p);
-            p = wave_sum(p);
-            if (lane == 0) sdots[task] = p;
-        }
-    }
+    dot_group_pass<NT>(col, U, m, r + 1, sdots, 1);
+    dot_group_pass<NT>(row, V, n, r + 1, sdots, 0);
 }
 
 // Frobenius-norm estimate update of partialPivotACA (hmatrix.cpp:568-572);
@@ -247,21 +296,66 @@ __device__ __forceinline__ double aca_update_norm(bem_c64 alpha, const bem_c64* 
     s.re = s.re + gain;
     bem_c64 ca2 = c64_scale(c64_conj(alpha), 2.0);
     for (int l = 0; l < r; l++) {
-        bem_c64 t = c64_mul(c64_mul(c64_mul(ca2, A[l]), sdots[2 + 2 * l]), sdots[3 + 2 * l]);
+        bem_c64 t = c64_mul(c64_mul(c64_mul(ca2, A[l]), sdots[3 + 2 * l]), sdots[2 + 2 * l]);
         s = c64_add(s, t);
     }
     S = s;
     return gain;
 }
 
-// row i of the block minus the current cross approximation, into dst
+// regular entry of (i, j), or false for a self / near-singular pair
+__device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c64* e)
+{
+    const DevMesh& M = a.M;
+    if (i == j) return false;
+    const double px = M.px[i], py = M.py[i], pz = M.pz[i];
+    double dx = px - M.px[j], dy = py - M.py[j], dz = pz - M.pz[j];
+    if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) return false;
+    bem_ops4 o = regular_ops<1>(a.F.k, px, py, pz, M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j], M.area[j],
+                                M.qx + j, M.qy + j, M.qz + j, M.n);
+    *e = bc_entry(a.F, a.kind, o, i, j);
+    return true;
+}
+
+// self / near-singular entry of (i, j) from the table; t = pair index or
+// -1 for the self entry
+__device__ __forceinline__ bem_c64 singular_entry(const AcaArgs& a, int64_t i, int64_t t)
+{
+    return a.sing[t >= 0 ? t : a.nnz + i];
+}
+
+// any entry (the pivot probe of the extra variant)
+__device__ __forceinline__ bem_c64 entry_any(const AcaArgs& a, int64_t i, int64_t j)
+{
+    bem_c64 e;
+    if (regular_entry(a, i, j, &e)) return e;
+    return singular_entry(a, i, i == j ? -1 : near_find(a.M, i, (int32_t)j));
+}
+
+// Row i of the block minus the current cross approximation, into dst.
+// Regular entries first; the few self / near-singular entries of the row
+// (CSR row r0 + i) in a second pass, so that the main loop has no divergent
+// branch.
+template <int NT>
 __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
                         const bem_c64* U, const bem_c64* V, const bem_c64* A, bem_c64* dst, bem_c64* sw)
 {
-    for (int l = threadIdx.x; l < r; l += kWG) sw[l] = c64_mul(U[(int64_t)l * m + i], A[l]);
+    for (int l = threadIdx.x; l < r; l += NT) sw[l] = c64_mul(U[(int64_t)l * m + i], A[l]);
     __syncthreads();
-    for (int64_t j = threadIdx.x; j < n; j += kWG) {
-        bem_c64 e = entry_general(a.M, a.F, a.kind, r0 + i, c0 + j);
+    const int64_t gi = r0 + i;
+    for (int64_t j = threadIdx.x; j < n; j += NT) {
+        bem_c64 e;
+        if (!regular_entry(a, gi, c0 + j, &e)) continue;
+        bem_c64 s = c64(0, 0);
+        for (int l = 0; l < r; l++) s = c64_fma(sw[l], V[(int64_t)l * n + j], s);
+        dst[j] = c64_sub(e, s);
+    }
+    const int64_t p0 = a.M.nr_ptr[gi], p1 = a.M.nr_ptr[gi + 1];
+    for (int64_t t = p0 + threadIdx.x; t <= p1; t += NT) {
+        int64_t gj = t < p1 ? a.M.nr_col[t] : gi;
+        if (gj < c0 || gj >= c0 + n) continue;
+        int64_t j = gj - c0;
+        bem_c64 e = singular_entry(a, gi, t < p1 ? t : -1);
         bem_c64 s = c64(0, 0);
         for (int l = 0; l < r; l++) s = c64_fma(sw[l], V[(int64_t)l * n + j], s);
         dst[j] = c64_sub(e, s);
@@ -269,14 +363,30 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     __syncthreads();
 }
 
-// column j of the block minus the current cross approximation, into dst
+// column j of the block minus the current cross approximation, into dst;
+// the near-singular rows of column c0 + j are the near list of row c0 + j
+// (the distance criterion is symmetric)
+template <int NT>
 __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
                         const bem_c64* U, const bem_c64* V, const bem_c64* A, bem_c64* dst, bem_c64* sw)
 {
-    for (int l = threadIdx.x; l < r; l += kWG) sw[l] = c64_mul(A[l], V[(int64_t)l * n + j]);
+    for (int l = threadIdx.x; l < r; l += NT) sw[l] = c64_mul(A[l], V[(int64_t)l * n + j]);
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < m; i += kWG) {
-        bem_c64 e = entry_general(a.M, a.F, a.kind, r0 + i, c0 + j);
+    const int64_t gj = c0 + j;
+    for (int64_t i = threadIdx.x; i < m; i += NT) {
+        bem_c64 e;
+        if (!regular_entry(a, r0 + i, gj, &e)) continue;
+        bem_c64 s = c64(0, 0);
+        for (int l = 0; l < r; l++) s = c64_fma(U[(int64_t)l * m + i], sw[l], s);
+        dst[i] = c64_sub(e, s);
+    }
+    const int64_t p0 = a.M.nr_ptr[gj], p1 = a.M.nr_ptr[gj + 1];
+    for (int64_t t = p0 + threadIdx.x; t <= p1; t += NT) {
+        int64_t gi = t < p1 ? a.M.nr_col[t] : gj;
+        if (gi < r0 || gi >= r0 + m) continue;
+        int64_t i = gi - r0;
+        int64_t pair = t < p1 ? near_find(a.M, gi, (int32_t)gj) : -1;
+        bem_c64 e = singular_entry(a, gi, pair);
         bem_c64 s = c64(0, 0);
         for (int l = 0; l < r; l++) s = c64_fma(U[(int64_t)l * m + i], sw[l], s);
         dst[i] = c64_sub(e, s);
@@ -284,16 +394,13 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     __syncthreads();
 }
 
-// ACA (K3): one workgroup per admissible block.  Plain variant =
-// BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra variant =
-// BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).  Dynamic LDS:
-// sw[cap] then sdots[2 cap + 2].
-template <bool EXTRA>
-__global__ __launch_bounds__(kWG) void k_aca(AcaArgs a)
+// ACA of one admissible block b by a workgroup of NT threads.  Plain
+// variant = BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra
+// variant = BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).
+// Dynamic LDS: sw[cap] then sdots[2 cap + 2].
+template <bool EXTRA, int NT>
+__device__ void aca_block(const AcaArgs& a, const int b, bem_c64* dyn, ArgMax* sred)
 {
-    extern __shared__ bem_c64 dyn[];
-    __shared__ ArgMax sred[kWG / 64];
-    const int b = a.blist[blockIdx.x];
     const int64_t m = a.rows[b], n = a.cols[b], r0 = a.row0[b], c0 = a.col0[b];
     const int cap = a.cap[b];
     int64_t maxRank = m < n ? m : n;
@@ -309,26 +416,26 @@ __global__ __launch_bounds__(kWG) void k_aca(AcaArgs a)
     bem_c64 S = c64(0, 0);
     if (!EXTRA) {
         int64_t j = bem_rand_bounded(a.seed, 0, (uint64_t)b, 0, n);
-        aca_col(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
+        aca_col<NT>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
         ev += m;
         int64_t i = -1;
         while (r < maxRank) {
             if (r >= cap) { flag = 1; break; }
             const bem_c64* wcol = r >= 1 ? U + (int64_t)(r - 1) * m : U;
-            ArgMax am = block_argmax(wcol, m, r >= 1 ? i : -1, sred);
+            ArgMax am = block_argmax<NT>(wcol, m, r >= 1 ? i : -1, sred);
             i = am.i;
             if (am.v <= tiny2 && r >= 1) break;
             bem_c64* row = V + (int64_t)r * n;
-            aca_row(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+            aca_row<NT>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
             ev += n;
-            ArgMax ar = block_argmax(row, n, -1, sred);
+            ArgMax ar = block_argmax<NT>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
             bem_c64* col = U + (int64_t)r * m;
-            aca_col(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+            aca_col<NT>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
             ev += m;
             bem_c64 alpha = c64_div(c64(1.0, 0.0), row[ar.i]);
             if (threadIdx.x == 0) A[r] = alpha;
-            aca_dots(col, row, U, V, m, n, r, sdots);
+            aca_dots<NT>(col, row, U, V, m, n, r, sdots);
             __syncthreads();
             double gain = aca_update_norm(alpha, A, r, sdots, S);
             r++;
@@ -341,7 +448,7 @@ __global__ __launch_bounds__(kWG) void k_aca(AcaArgs a)
         bool done = false;
         for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
             int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
-            bem_c64 pe = entry_general(a.M, a.F, a.kind, r0 + i, c0 + j);
+            bem_c64 pe = entry_any(a, r0 + i, c0 + j);
             ev += 1;
             if (c64_abs2(pe) < tiny2) continue;
             bem_c64 approx = c64(0, 0);
@@ -349,27 +456,27 @@ __global__ __launch_bounds__(kWG) void k_aca(AcaArgs a)
                 approx = c64_add(approx, c64_mul(c64_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
             if (c64_abs2(c64_sub(pe, approx)) < tol2 * c64_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
-            aca_col(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
+            aca_col<NT>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
             ev += m;
             bool first = true;
             while (r < maxRank) {
                 if (r >= cap) { flag = 1; done = true; break; }
                 const bem_c64* wcol = first ? U + (int64_t)r * m : U + (int64_t)(r - 1) * m;
-                ArgMax am = block_argmax(wcol, m, first ? -1 : i, sred);
+                ArgMax am = block_argmax<NT>(wcol, m, first ? -1 : i, sred);
                 i = am.i;
                 first = false;
                 if (c64_abs2(wcol[i]) <= tiny2 && r >= 1) break;
                 bem_c64* row = V + (int64_t)r * n;
-                aca_row(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+                aca_row<NT>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
                 ev += n;
-                ArgMax ar = block_argmax(row, n, -1, sred);
+                ArgMax ar = block_argmax<NT>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
                 bem_c64* col = U + (int64_t)r * m;
-                aca_col(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+                aca_col<NT>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
                 ev += m;
                 bem_c64 alpha = c64_div(c64(1.0, 0.0), row[ar.i]);
                 if (threadIdx.x == 0) A[r] = alpha;
-                aca_dots(col, row, U, V, m, n, r, sdots);
+                aca_dots<NT>(col, row, U, V, m, n, r, sdots);
                 __syncthreads();
                 bem_c64 oldS = S;
                 double gain = aca_update_norm(alpha, A, r, sdots, S);
@@ -389,6 +496,15 @@ __global__ __launch_bounds__(kWG) void k_aca(AcaArgs a)
         a.flag_out[b] = flag;
         a.evals_out[b] = ev;
     }
+}
+
+// ACA (K3): one workgroup of NT threads per listed admissible block
+template <bool EXTRA, int NT>
+__global__ __launch_bounds__(NT) void k_aca(AcaArgs a)
+{
+    extern __shared__ bem_c64 dyn[];
+    __shared__ ArgMax sred[NT / 64];
+    aca_block<EXTRA, NT>(a, a.blist[blockIdx.x], dyn, sred);
 }
 
 // copy the first rank columns/rows of the ACA scratch into the packed
@@ -580,6 +696,8 @@ struct DBuf {
 struct bem_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};   // concurrent ACA launches (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     std::string err;
     // mesh
     int64_t n = 0;
@@ -608,6 +726,14 @@ struct bem_ctx {
     Tile* d_tiles = nullptr;
     std::vector<Fix> fixes;
     Fix* d_fixes = nullptr;
+    bool piv_ok = false;                 // cached DPhi pivot lists
+    uint64_t piv_seed = 0;
+    int64_t* d_pivoff = nullptr;
+    int32_t* d_piv = nullptr;
+    std::vector<void*> allocs_piv;
+    int32_t* d_sing_order = nullptr;     // evaluation order of the singular items
+    bem_c64* d_sing = nullptr;           // per near pair, then per element (self)
+    int64_t nnz = 0;
     std::vector<int32_t> adm_list;           // admissible block ids, large first
     int32_t* d_adm_list = nullptr;
     std::vector<int64_t> lr_part;            // per block partial offset (low rank)
@@ -649,8 +775,9 @@ namespace {
 struct Timer {
     bem_ctx* c;
     int slot;
+    hipStream_t st;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    Timer(bem_ctx* c_, int s);
+    Timer(bem_ctx* c_, int s, hipStream_t stream = nullptr);
     ~Timer();
 };
 
@@ -670,6 +797,7 @@ void resolve_pending(bem_ctx* c)
 {
     if (c->pending.empty()) return;
     (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t t : c->side) (void)hipStreamSynchronize(t);
     for (auto& p : c->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
@@ -682,19 +810,19 @@ void resolve_pending(bem_ctx* c)
     c->pending.clear();
 }
 
-Timer::Timer(bem_ctx* c_, int s) : c(c_), slot(s)
+Timer::Timer(bem_ctx* c_, int s, hipStream_t stream) : c(c_), slot(s), st(stream ? stream : c_->stream)
 {
     if (c->prof) {
         e0 = pool_get(c);
         e1 = pool_get(c);
-        (void)hipEventRecord(e0, c->stream);
+        (void)hipEventRecord(e0, st);
     }
 }
 
 Timer::~Timer()
 {
     if (c->prof) {
-        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventRecord(e1, st);
         c->pending.push_back(bem_ctx::Pending{slot, e0, e1});
         if (c->pending.size() > 4096) resolve_pending(c);
     }
@@ -774,7 +902,12 @@ int bem_create(bem_ctx_t* out, int device)
     if (device < 0 || device >= nd) return BEM_ERR_ARG;
     auto c = new bem_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess) {
         delete c;
         return BEM_ERR_HIP;
     }
@@ -789,6 +922,7 @@ int bem_destroy(bem_ctx_t c)
     (void)hipStreamSynchronize(c->stream);
     free_list(c->allocs_mesh);
     free_list(c->allocs_part);
+    free_list(c->allocs_piv);
     if (c->d_alpha) (void)hipFree(c->d_alpha);
     if (c->d_beta) (void)hipFree(c->d_beta);
     if (c->d_f) (void)hipFree(c->d_f);
@@ -798,6 +932,9 @@ int bem_destroy(bem_ctx_t c)
     if (c->d_gm) (void)hipFree(c->d_gm);
     resolve_pending(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    for (hipStream_t t : c->side) { (void)hipStreamSynchronize(t); (void)hipStreamDestroy(t); }
+    (void)hipEventDestroy(c->ev_fork);
+    for (hipEvent_t e : c->ev_join) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return BEM_OK;
@@ -863,7 +1000,22 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     if ((rc = dupload(c, &d_off, c->near.geo_off.data(), c->near.geo_off.size(), &c->allocs_mesh))) return rc;
     if ((rc = dupload(c, &d_geo, c->near.geo.data(), c->near.geo.size(), &c->allocs_mesh))) return rc;
     if ((rc = dupload(c, &d_self, self.data(), self.size(), &c->allocs_mesh))) return rc;
-    M.nr_ptr = d_ptr; M.nr_col = d_col; M.nr_nsub = d_nsub; M.nr_off = d_off; M.nr_geo = d_geo;
+    // row of every pair, and the evaluation order of the singular items
+    // (self entries first, then near pairs by decreasing sub-triangle count)
+    const int64_t nnz = c->near.rowptr[n];
+    std::vector<int32_t> prow(nnz), order(nnz + n);
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t t = c->near.rowptr[i]; t < c->near.rowptr[i + 1]; t++) prow[t] = (int32_t)i;
+    for (int64_t i = 0; i < n; i++) order[i] = (int32_t)(nnz + i);
+    for (int64_t t = 0; t < nnz; t++) order[n + t] = (int32_t)t;
+    std::stable_sort(order.begin() + n, order.end(),
+                     [&](int32_t a, int32_t b) { return c->near.nsub[a] > c->near.nsub[b]; });
+    int32_t* d_prow;
+    if ((rc = dupload(c, &d_prow, prow.data(), prow.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &c->d_sing_order, order.data(), order.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dalloc(c, &c->d_sing, (size_t)(nnz + n), &c->allocs_mesh))) return rc;
+    c->nnz = nnz;
+    M.nr_ptr = d_ptr; M.nr_row = d_prow; M.nr_col = d_col; M.nr_nsub = d_nsub; M.nr_off = d_off; M.nr_geo = d_geo;
     M.self_geo = d_self;
     M.near_lim = 4.0 * hbar;
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -946,6 +1098,8 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
     for (int64_t i = 0; i < n; i++)
         if (cover[i] != n) return set_err(c, BEM_ERR_ARG, "bem_upload_partition: blocks do not tile the matrix");
     free_list(c->allocs_part);
+    free_list(c->allocs_piv);
+    c->piv_ok = false;
     c->blocks.resize(nb);
     for (int64_t b = 0; b < nb; b++) c->blocks[b] = bemsetup::Block{row0[b], rows[b], col0[b], cols[b], adm[b] != 0};
     // dense storage and tiles
@@ -1005,10 +1159,6 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
             }
         }
     }
-    std::stable_sort(c->fixes.begin(), c->fixes.end(), [&](const Fix& a, const Fix& b) {
-        int ca = a.t < 0 ? 4 : NL.nsub[a.t], cb = b.t < 0 ? 4 : NL.nsub[b.t];
-        return ca > cb;
-    });
     std::vector<int64_t> gptr(n + 1, 0), gidx;
     for (int64_t i = 0; i < n; i++) gptr[i + 1] = gptr[i] + (int64_t)contrib[i].size();
     gidx.reserve(gptr[n]);
@@ -1062,7 +1212,8 @@ struct AcaScratch {
 };
 
 int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
-            const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv, AcaScratch& S,
+            const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv,
+            const std::function<int()>& main_work, AcaScratch& S,
             std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev)
 {
     int64_t nb = (int64_t)c->blocks.size();
@@ -1101,14 +1252,49 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
     a.rank_out = S.d_rank; a.flag_out = S.d_flag; a.evals_out = S.d_ev;
     a.tol = tol; a.maxrank = maxrank; a.seed = seed;
     a.piv_off = d_pivoff; a.piv = d_piv;
+    a.sing = c->d_sing; a.nnz = c->nnz;
     size_t shm = (size_t)(3 * maxcap + 2) * sizeof(bem_c64);
-    if (!list.empty()) {
-        Timer t(c, 2);
-        if (kind == BEM_KIND_DPHI)
-            hipLaunchKernelGGL(k_aca<true>, dim3((unsigned)list.size()), dim3(kWG), shm, c->stream, a);
-        else
-            hipLaunchKernelGGL(k_aca<false>, dim3((unsigned)list.size()), dim3(kWG), shm, c->stream, a);
+    // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock four,
+    // larger eight.  The huge and large classes run on their own streams so
+    // that their long tails overlap the small blocks (and the caller's dense
+    // tiles); within each class the largest blocks start first.
+    std::vector<int32_t> cls[3];
+    for (int32_t b : list) {
+        int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
+        cls[s2 > kHugeBlock ? 0 : (s2 > kSmallBlock ? 1 : 2)].push_back(b);
+    }
+    std::vector<int32_t> ordered;
+    for (auto& v : cls) ordered.insert(ordered.end(), v.begin(), v.end());
+    HIPC(c, hipMemcpyAsync(S.d_list, ordered.data(), ordered.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
+    HIPC(c, hipEventRecord(c->ev_fork, c->stream));
+    size_t off = 0;
+    for (int q = 0; q < 3; q++) {
+        AcaArgs aq = a;
+        aq.blist = S.d_list + off;
+        off += cls[q].size();
+        if (q == 2 && (rc = main_work())) return rc;   // e.g. the dense tiles, overlapping the large classes
+        if (cls[q].empty()) continue;
+        hipStream_t st = q < 2 ? c->side[q] : c->stream;
+        if (q < 2) HIPC(c, hipStreamWaitEvent(st, c->ev_fork, 0));
+        Timer t(c, 2, st);
+        dim3 g((unsigned)cls[q].size());
+        bool x = kind == BEM_KIND_DPHI;
+        if (q == 0) {
+            if (x) hipLaunchKernelGGL((k_aca<true, 512>), g, dim3(512), shm, st, aq);
+            else hipLaunchKernelGGL((k_aca<false, 512>), g, dim3(512), shm, st, aq);
+        } else if (q == 1) {
+            if (x) hipLaunchKernelGGL((k_aca<true, 256>), g, dim3(256), shm, st, aq);
+            else hipLaunchKernelGGL((k_aca<false, 256>), g, dim3(256), shm, st, aq);
+        } else {
+            if (x) hipLaunchKernelGGL((k_aca<true, 64>), g, dim3(64), shm, st, aq);
+            else hipLaunchKernelGGL((k_aca<false, 64>), g, dim3(64), shm, st, aq);
+        }
         if ((rc = launch_check(c, "k_aca"))) return rc;
+    }
+    for (int q = 0; q < 2; q++) {
+        HIPC(c, hipEventRecord(c->ev_join[q], c->side[q]));
+        HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join[q], 0));
     }
     rank.assign(nb, 0); flag.assign(nb, 0); ev.assign(nb, 0);
     HIPC(c, hipMemcpyAsync(rank.data(), S.d_rank, nb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
@@ -1152,34 +1338,53 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     hipEvent_t ev0, ev1, ev2;
     HIPC(c, hipEventCreate(&ev0)); HIPC(c, hipEventCreate(&ev1)); HIPC(c, hipEventCreate(&ev2));
     HIPC(c, hipEventRecord(ev0, c->stream));
-    if (!c->tiles.empty()) {
-        Timer t(c, 0);
-        hipLaunchKernelGGL(k_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F, c->d_tiles,
-                           outPhi, outDPhi);
-        if ((rc = launch_check(c, "k_dense"))) return fail(rc);
-    }
-    if (!c->fixes.empty()) {
+    {
         Timer t(c, 1);
-        hipLaunchKernelGGL(k_fixup, dim3(nblk((int64_t)c->fixes.size(), kWG)), dim3(kWG), 0, c->stream, c->mesh, F,
-                           c->d_fixes, (int64_t)c->fixes.size(), outPhi, outDPhi);
-        if ((rc = launch_check(c, "k_fixup"))) return fail(rc);
+        int64_t nitems = c->nnz + c->n;
+        hipLaunchKernelGGL(k_singular, dim3(nblk(nitems, kWG)), dim3(kWG), 0, c->stream, c->mesh, F, kind,
+                           c->d_sing_order, nitems, c->nnz, c->d_sing);
+        if ((rc = launch_check(c, "k_singular"))) return fail(rc);
     }
-    HIPC(c, hipEventRecord(ev1, c->stream));
-    // pivot lists of the DPhi variant
-    std::vector<int64_t> pivoff;
-    std::vector<int32_t> piv;
+    // dense near-field tiles + scatter of their singular entries: enqueued on
+    // the main stream while the large ACA classes run on the side streams
+    auto dense_work = [&]() -> int {
+        int rc2;
+        if (!c->tiles.empty()) {
+            Timer t(c, 0);
+            hipLaunchKernelGGL(k_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F,
+                               c->d_tiles, outPhi, outDPhi);
+            if ((rc2 = launch_check(c, "k_dense"))) return rc2;
+        }
+        if (!c->fixes.empty()) {
+            hipLaunchKernelGGL(k_scatter, dim3(nblk((int64_t)c->fixes.size(), kWG)), dim3(kWG), 0, c->stream,
+                               c->d_fixes, (int64_t)c->fixes.size(), c->nnz, c->d_sing, h->dense);
+            if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
+        }
+        return BEM_OK;
+    };
+    // pivot lists of the DPhi variant (per mesh, partition and seed; cached)
     std::vector<void*> tmp_owned;
     int64_t* d_pivoff = nullptr;
     int32_t* d_piv = nullptr;
     if (kind == BEM_KIND_DPHI) {
-        bemsetup::Elements E;
-        E.n = c->n;
-        E.normal.resize(c->n);
-        for (int64_t e = 0; e < c->n; e++)
-            E.normal[e] = bemsetup::Vec3{c->h_normal[3 * e], c->h_normal[3 * e + 1], c->h_normal[3 * e + 2]};
-        bemsetup::build_pivot_lists(E, c->blocks, seed, pivoff, piv);
-        if ((rc = dupload(c, &d_pivoff, pivoff.data(), pivoff.size(), &tmp_owned))) return fail(rc);
-        if ((rc = dupload(c, &d_piv, piv.data(), piv.size(), &tmp_owned))) return fail(rc);
+        if (!c->piv_ok || c->piv_seed != seed) {
+            free_list(c->allocs_piv);
+            c->piv_ok = false;
+            bemsetup::Elements E;
+            E.n = c->n;
+            E.normal.resize(c->n);
+            for (int64_t e = 0; e < c->n; e++)
+                E.normal[e] = bemsetup::Vec3{c->h_normal[3 * e], c->h_normal[3 * e + 1], c->h_normal[3 * e + 2]};
+            std::vector<int64_t> pivoff;
+            std::vector<int32_t> piv;
+            bemsetup::build_pivot_lists(E, c->blocks, seed, pivoff, piv);
+            if ((rc = dupload(c, &c->d_pivoff, pivoff.data(), pivoff.size(), &c->allocs_piv))) return fail(rc);
+            if ((rc = dupload(c, &c->d_piv, piv.data(), piv.size(), &c->allocs_piv))) return fail(rc);
+            c->piv_ok = true;
+            c->piv_seed = seed;
+        }
+        d_pivoff = c->d_pivoff;
+        d_piv = c->d_piv;
     }
     // ACA, first pass with a modest rank capacity, then the overflowing
     // blocks again with their full capacity
@@ -1193,8 +1398,10 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     AcaScratch S1, S2;
     std::vector<int32_t> rank, flag, rank2, flag2;
     std::vector<int64_t> evs, evs2;
-    if ((rc = run_aca(c, kind, tol, maxrank, seed, c->adm_list, cap, d_pivoff, d_piv, S1, rank, flag, evs)))
+    if ((rc = run_aca(c, kind, tol, maxrank, seed, c->adm_list, cap, d_pivoff, d_piv, dense_work, S1, rank, flag,
+                      evs)))
         return fail(rc);
+    HIPC(c, hipEventRecord(ev1, c->stream));
     std::vector<int32_t> over, keep;
     for (int32_t b : c->adm_list) {
         if (flag[b] == 2) {
@@ -1207,7 +1414,8 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     if (!over.empty()) {
         std::vector<int32_t> cap2(nb, 0);
         for (int32_t b : over) cap2[b] = std::min<int32_t>(full_cap(b), kCapMax);
-        if ((rc = run_aca(c, kind, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, S2, rank2, flag2, evs2)))
+        if ((rc = run_aca(c, kind, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, [] { return BEM_OK; }, S2, rank2,
+                          flag2, evs2)))
             return fail(rc);
         for (int32_t b : over) {
             rank[b] = rank2[b];

@@ -44,6 +44,7 @@ struct DevMesh {
     const double* area;
     const double *qx, *qy, *qz;
     const int64_t* nr_ptr;        // near-singular CSR (n + 1)
+    const int32_t* nr_row;        // row of each pair
     const int32_t* nr_col;
     const int32_t* nr_nsub;
     const int64_t* nr_off;
@@ -63,13 +64,14 @@ struct DevFreq {
 
 // regular branch of BemOperatorsConst (boundaryelementsolver.cpp:856-893);
 // quadrature points read with stride qs from qx/qy/qz
+template <int UNROLL = 2>
 __device__ __forceinline__ bem_ops4 regular_ops(double k, double px, double py, double pz, double npx, double npy,
                                                 double npz, double nqx, double nqy, double nqz, double area,
                                                 const double* qx, const double* qy, const double* qz, int64_t qs)
 {
     double upnq = fma(npz, nqz, fma(npy, nqy, npx * nqx));
     bem_ops4 o = bem_ops4_zero();
-#pragma unroll 2
+#pragma unroll UNROLL
     for (int q = 0; q < 16; q++)
         bem_point_accumulate(&o, k, px, py, pz, npx, npy, npz, nqx, nqy, nqz, upnq, qx[q * qs], qy[q * qs],
                              qz[q * qs], kTriW[q]);

@@ -88,8 +88,9 @@ int bem_hmvm(bem_ctx_t ctx, bem_hmat_t h, const void* x_dev, void* y_dev, int ac
 int bem_block_info(bem_hmat_t h, int64_t b, int64_t* rank);
 int bem_block_download(bem_hmat_t h, int64_t b, double* full_or_U, double* alpha, double* V);
 /* stats[0] evaluated entries, [1] max rank, [2] mean rank, [3] bytes of block
-   storage, [4] assembly ms (device), [5] blocks re-run after rank-capacity
-   overflow, [6] dense ms, [7] ACA ms */
+   storage, [4] assembly ms (host wall clock), [5] blocks re-run after
+   rank-capacity overflow, [6] device ms of the singular, dense and first ACA
+   pass, [7] device ms of the re-run and the packing */
 int bem_hmat_stats(bem_hmat_t h, double* stats8);
 
 /* GMRES::gmresSolve replacement: restarted GMRES(restart) with CGS2 Arnoldi

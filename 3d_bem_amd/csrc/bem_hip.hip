@@ -18,7 +18,9 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bem_hip.h"
@@ -525,9 +527,19 @@ __global__ __launch_bounds__(kWG) void k_compact(const int32_t* blist, const int
 }
 
 // ------------------------------------------------------------- H-MVM
+// Block storage is complex FP64 (bem_c64) or FP32 (bem_c32); products
+// accumulate in FP64 and vectors are FP64 in both cases.
+
+__device__ __forceinline__ bem_c64 ld64(const bem_c64* p) { return *p; }
+__device__ __forceinline__ bem_c64 ld64(const bem_c32* p)
+{
+    bem_c32 v = *p;
+    return c64((double)v.re, (double)v.im);
+}
 
 // near-field tiles (K4): partial[tile][row] = sum_c F[row, c] x[c]
-__global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tiles, const bem_c64* __restrict__ F,
+template <class S>
+__global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tiles, const S* __restrict__ F,
                                                    const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
 {
     __shared__ bem_c64 red[4][kTile];
@@ -535,8 +547,8 @@ __global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tile
     const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
     bem_c64 acc = c64(0, 0);
     if (r < T.rows) {
-        const bem_c64* f = F + T.dest + r;
-        for (int c = cg; c < T.cols; c += 4) acc = c64_fma(f[(int64_t)c * T.ld], x[T.col0 + c], acc);
+        const S* f = F + T.dest + r;
+        for (int c = cg; c < T.cols; c += 4) acc = c64_fma(ld64(f + (int64_t)c * T.ld), x[T.col0 + c], acc);
     }
     red[cg][r] = acc;
     __syncthreads();
@@ -544,68 +556,137 @@ __global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tile
         part[T.part + r] = c64_add(c64_add(c64_add(red[0][r], red[1][r]), red[2][r]), red[3][r]);
 }
 
-// low-rank blocks (K5): t = alpha o (V x_J) in LDS, partial = U t
-__global__ __launch_bounds__(kWG) void k_mvm_lowrank(const int32_t* blist, const int64_t* row0, const int64_t* rows,
-                                                     const int64_t* col0, const int64_t* cols, const int32_t* rank,
-                                                     const int64_t* uoff, const int64_t* voff, const int64_t* aoff,
-                                                     const bem_c64* __restrict__ U, const bem_c64* __restrict__ V,
-                                                     const bem_c64* __restrict__ A, const bem_c64* __restrict__ x,
-                                                     const int64_t* poff, bem_c64* __restrict__ part)
+// low-rank blocks, phase 1 (K5a): t_b[l] = alpha_l (V_b[l, :] . x_J), one
+// wavefront per (block, l) task
+template <class S>
+__global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks, int64_t ntask,
+                                              const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
+                                              const int64_t* __restrict__ voff, const int64_t* __restrict__ aoff,
+                                              const int64_t* __restrict__ toff, const S* __restrict__ V,
+                                              const S* __restrict__ A, const bem_c64* __restrict__ x,
+                                              bem_c64* __restrict__ t)
 {
-    extern __shared__ bem_c64 st[];
-    const int b = blist[blockIdx.x];
-    const int64_t m = rows[b], n = cols[b];
-    const int k = rank[b];
-    const bem_c64* Ub = U + uoff[b];
-    const bem_c64* Vb = V + voff[b];
+    const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+    if (task >= ntask) return;
+    const int lane = threadIdx.x & 63;
+    const int b = tasks[2 * task], l = tasks[2 * task + 1];
+    const int64_t n = cols[b];
+    const S* v = V + voff[b] + (int64_t)l * n;
     const bem_c64* xs = x + col0[b];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int l = w; l < k; l += kWG / 64) {
-        const bem_c64* v = Vb + (int64_t)l * n;
-        bem_c64 p = c64(0, 0);
-        for (int64_t j = lane; j < n; j += 64) p = c64_fma(v[j], xs[j], p);
-        p = wave_sum(p);
-        if (lane == 0) st[l] = c64_mul(A[aoff[b] + l], p);
-    }
-    __syncthreads();
-    for (int64_t i = threadIdx.x; i < m; i += kWG) {
-        bem_c64 acc = c64(0, 0);
-        for (int l = 0; l < k; l++) acc = c64_fma(Ub[(int64_t)l * m + i], st[l], acc);
-        part[poff[b] + i] = acc;
-    }
+    bem_c64 p = c64(0, 0);
+    for (int64_t j = lane; j < n; j += 64) p = c64_fma(ld64(v + j), xs[j], p);
+    p = wave_sum(p);
+    if (lane == 0) t[toff[b] + l] = c64_mul(ld64(A + aoff[b] + l), p);
 }
 
-// y_i (+)= sum of the partial vectors covering row i, in partition order
+// low-rank blocks, phase 2 (K5b): partial_b[i] = sum_l U_b[i, l] t_b[l] over a
+// chunk of kWG rows of one block
+template <class S>
+__global__ __launch_bounds__(kWG) void k_lr_u(const int32_t* __restrict__ tasks, const int64_t* __restrict__ rows,
+                                              const int32_t* __restrict__ rank, const int64_t* __restrict__ uoff,
+                                              const int64_t* __restrict__ toff, const int64_t* __restrict__ poff,
+                                              const S* __restrict__ U, const bem_c64* __restrict__ t,
+                                              bem_c64* __restrict__ part)
+{
+    extern __shared__ bem_c64 st[];
+    const int b = tasks[2 * blockIdx.x], chunk = tasks[2 * blockIdx.x + 1];
+    const int64_t m = rows[b];
+    const int k = rank[b];
+    for (int l = threadIdx.x; l < k; l += kWG) st[l] = t[toff[b] + l];
+    __syncthreads();
+    const int64_t i = (int64_t)chunk * kWG + threadIdx.x;
+    if (i >= m) return;
+    const S* u = U + uoff[b] + i;
+    bem_c64 acc = c64(0, 0);
+    for (int l = 0; l < k; l++) acc = c64_fma(ld64(u + (int64_t)l * m), st[l], acc);
+    part[poff[b] + i] = acc;
+}
+
+// y_i (+)= sum of the partial vectors covering row i in partition order:
+// 16 lanes per row take every 16th partial, then a fixed shuffle tree
 __global__ void k_mvm_gather(int64_t n, const int64_t* __restrict__ gptr, const int64_t* __restrict__ gidx,
                              const bem_c64* __restrict__ part, bem_c64* y, int accumulate)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    bem_c64 acc = accumulate ? y[i] : c64(0, 0);
-    for (int64_t p = gptr[i]; p < gptr[i + 1]; p++) acc = c64_add(acc, part[gidx[p]]);
-    y[i] = acc;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = g >> 4;
+    const int q = (int)(g & 15);
+    bem_c64 acc = c64(0, 0);
+    if (i < n)
+        for (int64_t p = gptr[i] + q; p < gptr[i + 1]; p += 16) acc = c64_add(acc, part[gidx[p]]);
+    for (int s = 8; s > 0; s >>= 1) {
+        acc.re = acc.re + __shfl_down(acc.re, s, 16);
+        acc.im = acc.im + __shfl_down(acc.im, s, 16);
+    }
+    if (i < n && q == 0) y[i] = accumulate ? c64_add(y[i], acc) : acc;
 }
 
 // ------------------------------------------------------------- GMRES
 
-// out[l] = V_l^H w for l < cnt, one workgroup per l
-__global__ __launch_bounds__(kWG) void k_dots(int64_t n, const bem_c64* V, const bem_c64* w, bem_c64* out)
+constexpr int64_t kDotChunk = 4096;   // rows per workgroup of the Arnoldi dot products
+
+// part[l * nchunk + chunk] = sum over the chunk's rows of conj(V_l) w; one
+// wavefront per (chunk, l)
+__global__ __launch_bounds__(kWG) void k_cgs_dots(int64_t n, int cnt, const bem_c64* __restrict__ V,
+                                                  const bem_c64* __restrict__ w, bem_c64* __restrict__ part)
 {
-    __shared__ bem_c64 red[kWG / 64];
-    const bem_c64* v = V + (int64_t)blockIdx.x * n;
+    const int l = blockIdx.y * (kWG / 64) + (threadIdx.x >> 6);
+    if (l >= cnt) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t i0 = (int64_t)blockIdx.x * kDotChunk, i1 = i0 + kDotChunk < n ? i0 + kDotChunk : n;
+    const bem_c64* v = V + (int64_t)l * n;
     bem_c64 p = c64(0, 0);
-    for (int64_t i = threadIdx.x; i < n; i += kWG) p = c64_fma(c64_conj(v[i]), w[i], p);
+    for (int64_t i = i0 + lane; i < i1; i += 64) p = c64_fma(c64_conj(v[i]), w[i], p);
     p = wave_sum(p);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = p;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        bem_c64 s = red[0];
-        for (int q = 1; q < kWG / 64; q++) s = c64_add(s, red[q]);
-        out[blockIdx.x] = s;
-    }
+    if (lane == 0) part[(int64_t)l * gridDim.x + blockIdx.x] = p;
 }
 
-// w -= sum_l V_l h_l ; hacc[l] += h[l]
+// h[l] = sum over chunks (fixed order); hacc[l] = h[l] (first pass) or += h[l]
+__global__ void k_cgs_reduce(int cnt, int nchunk, const bem_c64* __restrict__ part, bem_c64* h, bem_c64* hacc,
+                             int first)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= cnt) return;
+    bem_c64 s = part[(int64_t)l * nchunk];
+    for (int q = 1; q < nchunk; q++) s = c64_add(s, part[(int64_t)l * nchunk + q]);
+    h[l] = s;
+    hacc[l] = first ? s : c64_add(hacc[l], s);
+}
+
+// One Arnoldi column on the device: h[0..j] = hacc, h[j+1] = |w|; apply the
+// previous Givens rotations, make the new one, update g and store the
+// column of the triangular factor; resid = |g[j+1]| / |b|.
+__global__ void k_arnoldi(int j, int m, const bem_c64* hacc, const double* hn, bem_c64* H, double* cs_c,
+                          bem_c64* cs_s, bem_c64* g, double bn, double* resid)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    bem_c64* h = H + (int64_t)j * (m + 1);
+    for (int l = 0; l <= j; l++) h[l] = hacc[l];
+    h[j + 1] = c64(*hn, 0.0);
+    for (int l = 0; l < j; l++) {
+        bem_c64 a0 = h[l], b0 = h[l + 1];
+        h[l] = c64_add(c64_scale(a0, cs_c[l]), c64_mul(cs_s[l], b0));
+        h[l + 1] = c64_sub(c64_scale(b0, cs_c[l]), c64_mul(c64_conj(cs_s[l]), a0));
+    }
+    double na = sqrt(c64_abs2(h[j])), nb2 = sqrt(c64_abs2(h[j + 1]));
+    double t = sqrt(na * na + nb2 * nb2);
+    double cc;
+    bem_c64 ss;
+    if (na == 0) { cc = 0; ss = c64(1, 0); }
+    else {
+        cc = na / t;
+        ss = c64_scale(c64_mul(c64_scale(h[j], 1.0 / na), c64_conj(h[j + 1])), 1.0 / t);
+    }
+    cs_c[j] = cc;
+    cs_s[j] = ss;
+    h[j] = c64_add(c64_scale(h[j], cc), c64_mul(ss, h[j + 1]));
+    h[j + 1] = c64(0, 0);
+    bem_c64 gj = g[j];
+    g[j] = c64_scale(gj, cc);
+    g[j + 1] = c64_neg(c64_mul(c64_conj(ss), gj));
+    *resid = sqrt(c64_abs2(g[j + 1])) / bn;
+}
+
+// w -= sum_l V_l h_l
 __global__ void k_orth_update(int64_t n, int cnt, const bem_c64* V, const bem_c64* h, bem_c64* w)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -753,6 +834,10 @@ struct bem_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::vector<void*> allocs_part;
+    // device buffer pool: assemblies reuse their multi-GB scratch instead of
+    // calling hipMalloc/hipFree every time
+    std::multimap<size_t, void*> pool;
+    std::unordered_map<void*, size_t> live;
 };
 
 struct bem_hmat {
@@ -762,6 +847,12 @@ struct bem_hmat {
     bem_c64 *U = nullptr, *V = nullptr, *A = nullptr;
     int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
     int32_t* d_rank = nullptr;
+    // two-phase low-rank MVM: (block, l) and (block, row chunk) work lists
+    int32_t *d_ttask = nullptr, *d_utask = nullptr;
+    int64_t nttask = 0, nutask = 0;
+    int64_t* d_toff = nullptr;
+    bem_c64* d_t = nullptr;
+    int maxrank = 1;
     std::vector<int64_t> uoff, voff, aoff;
     std::vector<int32_t> rank;
     double stats[8] = {0};
@@ -841,14 +932,50 @@ int set_err(bem_ctx* c, int code, const std::string& msg)
             return set_err(ctx, BEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while (0)
 
+void pool_release_all(bem_ctx* c)
+{
+    for (auto& kv : c->pool) (void)hipFree(kv.second);
+    c->pool.clear();
+}
+
+int dalloc_bytes(bem_ctx* c, void** p, size_t bytes)
+{
+    bytes = ((bytes ? bytes : 1) + 255) & ~(size_t)255;
+    auto it = c->pool.lower_bound(bytes);
+    if (it != c->pool.end() && it->first <= 2 * bytes + ((size_t)1 << 20)) {
+        *p = it->second;
+        c->live[*p] = it->first;
+        c->pool.erase(it);
+        return BEM_OK;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {   // give the pooled memory back and try once more
+        (void)hipGetLastError();
+        pool_release_all(c);
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->live[*p] = bytes;
+    return BEM_OK;
+}
+
+void pfree(bem_ctx* c, void* p)
+{
+    if (!p) return;
+    auto it = c->live.find(p);
+    if (it == c->live.end()) { (void)hipFree(p); return; }
+    c->pool.insert({it->second, p});
+    c->live.erase(it);
+}
+
 template <class T>
 int dalloc(bem_ctx* c, T** p, size_t count, std::vector<void*>* owner)
 {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-    if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    if (owner) owner->push_back(*p);
+    void* q = nullptr;
+    int rc = dalloc_bytes(c, &q, count * sizeof(T));
+    *p = (T*)q;
+    if (rc) return rc;
+    if (owner) owner->push_back(q);
     return BEM_OK;
 }
 
@@ -861,9 +988,9 @@ int dupload(bem_ctx* c, T** p, const T* src, size_t count, std::vector<void*>* o
     return BEM_OK;
 }
 
-void free_list(std::vector<void*>& v)
+void free_list(bem_ctx* c, std::vector<void*>& v)
 {
-    for (void* p : v) (void)hipFree(p);
+    for (void* p : v) pfree(c, p);
     v.clear();
 }
 
@@ -920,16 +1047,15 @@ int bem_destroy(bem_ctx_t c)
     if (!c) return BEM_ERR_ARG;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    free_list(c->allocs_mesh);
-    free_list(c->allocs_part);
-    free_list(c->allocs_piv);
-    if (c->d_alpha) (void)hipFree(c->d_alpha);
-    if (c->d_beta) (void)hipFree(c->d_beta);
-    if (c->d_f) (void)hipFree(c->d_f);
-    if (c->d_subst) (void)hipFree(c->d_subst);
-    if (c->d_cphi) (void)hipFree(c->d_cphi);
-    if (c->d_cd) (void)hipFree(c->d_cd);
-    if (c->d_gm) (void)hipFree(c->d_gm);
+    free_list(c, c->allocs_mesh);
+    free_list(c, c->allocs_part);
+    free_list(c, c->allocs_piv);
+    for (void* p : {(void*)c->d_alpha, (void*)c->d_beta, (void*)c->d_f, (void*)c->d_subst, (void*)c->d_cphi,
+                    (void*)c->d_cd, (void*)c->d_gm})
+        pfree(c, p);
+    for (auto& kv : c->live) (void)hipFree(kv.first);
+    c->live.clear();
+    pool_release_all(c);
     resolve_pending(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     for (hipStream_t t : c->side) { (void)hipStreamSynchronize(t); (void)hipStreamDestroy(t); }
@@ -949,7 +1075,7 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
         return set_err(c, BEM_ERR_ARG, "bem_upload_mesh: bad arguments");
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
-    free_list(c->allocs_mesh);
+    free_list(c, c->allocs_mesh);
     c->have_part = false;
     c->n = n;
     c->hbar = hbar;
@@ -1030,8 +1156,10 @@ int bem_set_bc(bem_ctx_t c, const double* a, const double* b, const double* f)
     c->h_alpha.assign((const bem_c64*)a, (const bem_c64*)a + n);
     c->h_beta.assign((const bem_c64*)b, (const bem_c64*)b + n);
     c->h_f.assign((const bem_c64*)f, (const bem_c64*)f + n);
-    for (bem_c64** p : {&c->d_alpha, &c->d_beta, &c->d_f})
-        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (bem_c64** p : {&c->d_alpha, &c->d_beta, &c->d_f}) {
+        pfree(c, *p);
+        *p = nullptr;
+    }
     int rc;
     if ((rc = dupload(c, &c->d_alpha, c->h_alpha.data(), n, nullptr))) return rc;
     if ((rc = dupload(c, &c->d_beta, c->h_beta.data(), n, nullptr))) return rc;
@@ -1097,8 +1225,8 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
     }
     for (int64_t i = 0; i < n; i++)
         if (cover[i] != n) return set_err(c, BEM_ERR_ARG, "bem_upload_partition: blocks do not tile the matrix");
-    free_list(c->allocs_part);
-    free_list(c->allocs_piv);
+    free_list(c, c->allocs_part);
+    free_list(c, c->allocs_piv);
     c->piv_ok = false;
     c->blocks.resize(nb);
     for (int64_t b = 0; b < nb; b++) c->blocks[b] = bemsetup::Block{row0[b], rows[b], col0[b], cols[b], adm[b] != 0};
@@ -1188,9 +1316,11 @@ int bem_hmat_destroy(bem_hmat_t h)
     if (!h) return BEM_ERR_ARG;
     (void)hipSetDevice(h->ctx->device);
     (void)hipStreamSynchronize(h->ctx->stream);
+    for (hipStream_t t : h->ctx->side) (void)hipStreamSynchronize(t);
     for (void* p : {(void*)h->dense, (void*)h->U, (void*)h->V, (void*)h->A, (void*)h->d_uoff, (void*)h->d_voff,
-                    (void*)h->d_aoff, (void*)h->d_rank})
-        if (p) (void)hipFree(p);
+                    (void*)h->d_aoff, (void*)h->d_rank, (void*)h->d_ttask, (void*)h->d_utask, (void*)h->d_toff,
+                    (void*)h->d_t})
+        pfree(h->ctx, p);
     delete h;
     return BEM_OK;
 }
@@ -1208,7 +1338,8 @@ struct AcaScratch {
     int64_t* d_ev = nullptr;
     std::vector<int64_t> uoff, voff, aoff;
     std::vector<void*> owned;
-    ~AcaScratch() { free_list(owned); }
+    bem_ctx* ctx = nullptr;
+    ~AcaScratch() { if (ctx) free_list(ctx, owned); }
 };
 
 int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
@@ -1217,6 +1348,7 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
             std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev)
 {
     int64_t nb = (int64_t)c->blocks.size();
+    S.ctx = c;
     S.uoff.assign(nb, 0); S.voff.assign(nb, 0); S.aoff.assign(nb, 0);
     int64_t tu = 0, tv = 0, ta = 0;
     int maxcap = 1;
@@ -1368,7 +1500,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     int32_t* d_piv = nullptr;
     if (kind == BEM_KIND_DPHI) {
         if (!c->piv_ok || c->piv_seed != seed) {
-            free_list(c->allocs_piv);
+            free_list(c, c->allocs_piv);
             c->piv_ok = false;
             bemsetup::Elements E;
             E.n = c->n;
@@ -1405,7 +1537,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     std::vector<int32_t> over, keep;
     for (int32_t b : c->adm_list) {
         if (flag[b] == 2) {
-            free_list(tmp_owned);
+            free_list(c, tmp_owned);
             bem_hmat_destroy(h);
             return set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite value in ACA block " + std::to_string(b));
         }
@@ -1421,7 +1553,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
             rank[b] = rank2[b];
             evs[b] = evs2[b];
             if (flag2[b]) {
-                free_list(tmp_owned);
+                free_list(c, tmp_owned);
                 bem_hmat_destroy(h);
                 return set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
                                "bem_assemble: ACA block " + std::to_string(b) +
@@ -1429,7 +1561,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
             }
         }
     }
-    free_list(tmp_owned);
+    free_list(c, tmp_owned);
     // packed storage at the final ranks
     h->uoff.assign(nb, 0); h->voff.assign(nb, 0); h->aoff.assign(nb, 0); h->rank.assign(nb, -1);
     int64_t tu = 0, tv = 0, ta = 0;
@@ -1458,6 +1590,25 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     std::vector<void*> own2;
     if ((rc = dupload(c, &d_srank, rk.data(), nb, &own2))) return fail(rc);
     if ((rc = dupload(c, &h->d_rank, h->rank.data(), nb, nullptr))) return fail(rc);
+    {
+        std::vector<int32_t> tt, ut;
+        std::vector<int64_t> toff(nb, 0);
+        int64_t tot = 0;
+        for (int32_t b : c->adm_list) {
+            int r = h->rank[b];
+            toff[b] = tot;
+            tot += r;
+            h->maxrank = std::max(h->maxrank, r);
+            for (int l = 0; l < r; l++) { tt.push_back(b); tt.push_back(l); }
+            for (int64_t q = 0; q * kWG < c->blocks[b].rows; q++) { ut.push_back(b); ut.push_back((int32_t)q); }
+        }
+        h->nttask = (int64_t)tt.size() / 2;
+        h->nutask = (int64_t)ut.size() / 2;
+        if ((rc = dupload(c, &h->d_ttask, tt.data(), tt.size(), nullptr))) return fail(rc);
+        if ((rc = dupload(c, &h->d_utask, ut.data(), ut.size(), nullptr))) return fail(rc);
+        if ((rc = dupload(c, &h->d_toff, toff.data(), nb, nullptr))) return fail(rc);
+        if ((rc = dalloc(c, &h->d_t, tot, nullptr))) return fail(rc);
+    }
     int32_t *d_keep = nullptr, *d_over = nullptr;
     if (!keep.empty()) {
         if ((rc = dupload(c, &d_keep, keep.data(), keep.size(), &own2))) return fail(rc);
@@ -1485,7 +1636,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     int32_t hflag = 0;
     HIPC(c, hipMemcpyAsync(&hflag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    free_list(own2);
+    free_list(c, own2);
     float ms_dense = 0, ms_aca = 0;
     (void)hipEventElapsedTime(&ms_dense, ev0, ev1);
     (void)hipEventElapsedTime(&ms_aca, ev1, ev2);
@@ -1516,23 +1667,23 @@ int bem_hmvm(bem_ctx_t c, bem_hmat_t h, const void* x, void* y, int accumulate)
     int rc;
     if (!c->tiles.empty()) {
         Timer t(c, 3);
-        hipLaunchKernelGGL(k_mvm_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->d_tiles, h->dense,
-                           xd, c->d_part);
+        hipLaunchKernelGGL(k_mvm_dense<bem_c64>, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->d_tiles,
+                           h->dense, xd, c->d_part);
         if ((rc = launch_check(c, "k_mvm_dense"))) return rc;
     }
-    if (!c->adm_list.empty()) {
-        int maxr = 1;
-        for (int32_t b : c->adm_list) maxr = std::max(maxr, h->rank[b]);
+    if (h->nttask > 0) {
         Timer t(c, 4);
-        hipLaunchKernelGGL(k_mvm_lowrank, dim3((unsigned)c->adm_list.size()), dim3(kWG), maxr * sizeof(bem_c64),
-                           c->stream, c->d_adm_list, c->d_row0, c->d_rows, c->d_col0, c->d_cols, h->d_rank,
-                           h->d_uoff, h->d_voff, h->d_aoff, h->U, h->V, h->A, xd, c->d_lr_part, c->d_part);
-        if ((rc = launch_check(c, "k_mvm_lowrank"))) return rc;
+        hipLaunchKernelGGL(k_lr_t<bem_c64>, dim3(nblk(h->nttask, kWG / 64)), dim3(kWG), 0, c->stream, h->d_ttask,
+                           h->nttask, c->d_col0, c->d_cols, h->d_voff, h->d_aoff, h->d_toff, h->V, h->A, xd, h->d_t);
+        hipLaunchKernelGGL(k_lr_u<bem_c64>, dim3((unsigned)h->nutask), dim3(kWG), h->maxrank * sizeof(bem_c64),
+                           c->stream, h->d_utask, c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part, h->U,
+                           h->d_t, c->d_part);
+        if ((rc = launch_check(c, "k_lr"))) return rc;
     }
     {
         Timer t(c, 5);
-        hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr, c->d_gidx,
-                           c->d_part, yd, accumulate);
+        hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(16 * c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr,
+                           c->d_gidx, c->d_part, yd, accumulate);
         if ((rc = launch_check(c, "k_mvm_gather"))) return rc;
     }
     return BEM_OK;
@@ -1584,14 +1735,17 @@ int bem_hmat_stats(bem_hmat_t h, double* s)
 int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int64_t restart, int64_t maxit,
               int64_t* iters, double* rel_resid)
 {
-    if (!c || !h || !b || !x || restart < 1 || maxit < 0) return set_err(c, BEM_ERR_ARG, "bem_gmres: bad arguments");
+    if (!c || !h || !b || !x || restart < 1 || maxit < 0 || h->ctx != c)
+        return set_err(c, BEM_ERR_ARG, "bem_gmres: bad arguments");
     HIPC(c, hipSetDevice(c->device));
     const int64_t n = c->n;
     const int64_t m = restart;
-    // workspace: V (m+1) x n, w, r, small vectors
-    int64_t need = (m + 1) * n + 2 * n + 4 * (m + 2) + 1024;
+    const int nchunk = (int)((n + kDotChunk - 1) / kDotChunk);
+    // workspace (complex slots): V (m+1) n, w n, r n, h, hacc, y, g, cs_s
+    // (m+2 each), H (m+1) m, dot partials (m+1) nchunk, then doubles
+    int64_t need = (m + 1) * n + 2 * n + 5 * (m + 2) + (m + 1) * m + (m + 1) * nchunk + (m + 2) + 640;
     if (c->gm_cap < need) {
-        if (c->d_gm) (void)hipFree(c->d_gm);
+        pfree(c, c->d_gm);
         c->d_gm = nullptr;
         c->gm_cap = 0;
         int rc;
@@ -1601,25 +1755,33 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
     bem_c64* V = c->d_gm;
     bem_c64* w = V + (m + 1) * n;
     bem_c64* r = w + n;
-    bem_c64* dh = r + n;               // m + 2
-    bem_c64* dy = dh + (m + 2);        // m + 2
-    double* dn = (double*)(dy + (m + 2));     // norm parts (512) + result
+    bem_c64* dh = r + n;
+    bem_c64* dhacc = dh + (m + 2);
+    bem_c64* dy = dhacc + (m + 2);
+    bem_c64* dg = dy + (m + 2);
+    bem_c64* dcs_s = dg + (m + 2);
+    bem_c64* dH = dcs_s + (m + 2);
+    bem_c64* dpart = dH + (m + 1) * m;
+    double* dcs_c = (double*)(dpart + (m + 1) * nchunk);
+    double* dn = dcs_c + 2 * (m + 2);      // norm partials (256) + result + residual estimate
     double* dnorm = dn + 512;
+    double* dres = dnorm + 1;
     const bem_c64* bd = (const bem_c64*)b;
     bem_c64* xd = (bem_c64*)x;
     const int npart = (int)std::min<int64_t>(256, (n + kWG - 1) / kWG);
-    auto norm_of = [&](const bem_c64* v, double* result) -> int {
+    auto norm_dev = [&](const bem_c64* v) {
         hipLaunchKernelGGL(k_norm2_part, dim3(npart), dim3(kWG), 0, c->stream, n, v, dn);
         hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, c->stream, npart, dn, dnorm);
-        int rc = launch_check(c, "norm");
-        if (rc) return rc;
-        HIPC(c, hipMemcpyAsync(result, dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    };
+    auto fetch = [&](double* dst, const double* src) -> int {
+        HIPC(c, hipMemcpyAsync(dst, src, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         return BEM_OK;
     };
     int rc;
     double bn = 0;
-    if ((rc = norm_of(bd, &bn))) return rc;
+    norm_dev(bd);
+    if ((rc = launch_check(c, "norm")) || (rc = fetch(&bn, dnorm))) return rc;
     if (bn == 0) {
         HIPC(c, hipMemsetAsync(xd, 0, n * sizeof(bem_c64), c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
@@ -1627,63 +1789,47 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
         if (rel_resid) *rel_resid = 0;
         return BEM_OK;
     }
-    std::vector<bem_c64> H((m + 1) * m), g(m + 1), cs_s(m), hcol(m + 2), hh(m + 2);
-    std::vector<double> cs_c(m);
+    std::vector<bem_c64> H((m + 1) * m), g(m + 2);
     int64_t it = 0;
     double rel = 1;
     while (true) {
+        // r = b - A x, beta = |r|
         if ((rc = bem_hmvm(c, h, xd, r, 0))) return rc;
         hipLaunchKernelGGL(k_residual, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, bd, r);
+        norm_dev(r);
         double beta;
-        if ((rc = norm_of(r, &beta))) return rc;
+        if ((rc = launch_check(c, "residual")) || (rc = fetch(&beta, dnorm))) return rc;
         rel = beta / bn;
         if (rel < tol || it >= maxit) break;
         hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, r, dnorm, V);
         std::fill(g.begin(), g.end(), c64(0, 0));
         g[0] = c64(beta, 0);
+        HIPC(c, hipMemcpyAsync(dg, g.data(), (m + 2) * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
         int64_t j = 0;
-        bool conv = false;
-        for (; j < m && it < maxit; j++, it++) {
+        for (; j < m && it < maxit;) {
             if ((rc = bem_hmvm(c, h, V + j * n, w, 0))) return rc;
-            std::fill(hcol.begin(), hcol.end(), c64(0, 0));
-            for (int pass = 0; pass < 2; pass++) {
-                hipLaunchKernelGGL(k_dots, dim3((unsigned)(j + 1)), dim3(kWG), 0, c->stream, n, V, w, dh);
-                hipLaunchKernelGGL(k_orth_update, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)(j + 1), V, dh,
-                                   w);
-                if ((rc = launch_check(c, "cgs"))) return rc;
-                HIPC(c, hipMemcpyAsync(hh.data(), dh, (j + 1) * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
-                HIPC(c, hipStreamSynchronize(c->stream));
-                for (int64_t l = 0; l <= j; l++) hcol[l] = c64_add(hcol[l], hh[l]);
+            const int cnt = (int)(j + 1);
+            for (int pass = 0; pass < 2; pass++) {   // classical Gram-Schmidt, twice
+                hipLaunchKernelGGL(k_cgs_dots, dim3(nchunk, (cnt + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0,
+                                   c->stream, n, cnt, V, w, dpart);
+                hipLaunchKernelGGL(k_cgs_reduce, dim3(nblk(cnt, 64)), dim3(64), 0, c->stream, cnt, nchunk, dpart, dh,
+                                   dhacc, pass == 0 ? 1 : 0);
+                hipLaunchKernelGGL(k_orth_update, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, cnt, V, dh, w);
             }
-            double hn;
-            if ((rc = norm_of(w, &hn))) return rc;
-            hcol[j + 1] = c64(hn, 0);
+            norm_dev(w);
             hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, w, dnorm,
                                V + (j + 1) * n);
-            for (int64_t l = 0; l < j; l++) {
-                bem_c64 a0 = hcol[l], b0 = hcol[l + 1];
-                hcol[l] = c64_add(c64_scale(a0, cs_c[l]), c64_mul(cs_s[l], b0));
-                hcol[l + 1] = c64_sub(c64_scale(b0, cs_c[l]), c64_mul(c64_conj(cs_s[l]), a0));
-            }
-            double na = std::sqrt(c64_abs2(hcol[j])), nb2 = std::sqrt(c64_abs2(hcol[j + 1]));
-            double t = std::sqrt(na * na + nb2 * nb2);
-            double cc;
-            bem_c64 ss;
-            if (na == 0) { cc = 0; ss = c64(1, 0); }
-            else {
-                cc = na / t;
-                ss = c64_scale(c64_mul(c64_scale(hcol[j], 1.0 / na), c64_conj(hcol[j + 1])), 1.0 / t);
-            }
-            cs_c[j] = cc; cs_s[j] = ss;
-            hcol[j] = c64_add(c64_scale(hcol[j], cc), c64_mul(ss, hcol[j + 1]));
-            bem_c64 gj = g[j];
-            g[j] = c64_scale(gj, cc);
-            g[j + 1] = c64_neg(c64_mul(c64_conj(ss), gj));
-            for (int64_t l = 0; l <= j; l++) H[j * (m + 1) + l] = hcol[l];
-            rel = std::sqrt(c64_abs2(g[j + 1])) / bn;
-            if (rel < tol) { conv = true; j++; it++; break; }
+            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, c->stream, (int)j, (int)m, dhacc, dnorm, dH, dcs_c,
+                               dcs_s, dg, bn, dres);
+            if ((rc = launch_check(c, "arnoldi")) || (rc = fetch(&rel, dres))) return rc;
+            j++;
+            it++;
+            if (rel < tol) break;
         }
-        (void)conv;
+        // y = R^-1 g on the host, x += V y
+        HIPC(c, hipMemcpyAsync(H.data(), dH, (m + 1) * j * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipMemcpyAsync(g.data(), dg, (j + 1) * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
         std::vector<bem_c64> y(j);
         for (int64_t l = j - 1; l >= 0; l--) {
             bem_c64 s = g[l];
@@ -1730,13 +1876,13 @@ static int entries_impl(bem_ctx_t c, int mode, int64_t np, const int64_t* ij, do
     int rc;
     size_t per = mode == 2 ? 4 : 1;
     if ((rc = dupload(c, &d_ij, ij, 2 * np, &own))) return rc;
-    if ((rc = dalloc(c, &d_out, per * np, &own))) { free_list(own); return rc; }
+    if ((rc = dalloc(c, &d_out, per * np, &own))) { free_list(c, own); return rc; }
     hipLaunchKernelGGL(k_entries, dim3(nblk(np, 128)), dim3(128), 0, c->stream, c->mesh, dev_freq(c), mode, np, d_ij,
                        d_out);
-    if ((rc = launch_check(c, "k_entries"))) { free_list(own); return rc; }
+    if ((rc = launch_check(c, "k_entries"))) { free_list(c, own); return rc; }
     hipError_t e = hipMemcpyAsync(out, d_out, per * np * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    free_list(own);
+    free_list(c, own);
     if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, hipGetErrorString(e));
     return BEM_OK;
 }

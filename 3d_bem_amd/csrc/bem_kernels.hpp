@@ -5,6 +5,10 @@
 #include <hip/hip_runtime.h>
 #include "../../include/bem_detmath.h"
 
+// complex FP32 storage of the mixed-precision path (not part of the
+// bit-exact FP64 contract)
+typedef struct { float re, im; } bem_c32;
+
 namespace bemdev {
 
 // order-8 16-point triangle rule weights (trianglequadraturerules.cpp:91-108)

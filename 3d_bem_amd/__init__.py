@@ -249,6 +249,15 @@ class HMatrix:
         self.ctx._check(lib().bem_block_download(self.h, int(b), _ptr(U), _ptr(al), _ptr(V)))
         return U, al, V
 
+    def to_full(self):
+        """Dense n x n matrix of the H-matrix (HMatrix::toFullMat, small n only)."""
+        n = self.ctx.n
+        F = np.zeros((n, n), np.complex128)
+        for b, (r0, m, c0, nn, adm) in enumerate(self.ctx.blocks):
+            blk = self.block(b)
+            F[r0:r0 + m, c0:c0 + nn] = blk if not isinstance(blk, tuple) else (blk[0] * blk[1]) @ blk[2]
+        return F
+
     def mvm(self, x, y=None, accumulate=False):
         """y (+)= H x with host arrays (copies through device buffers)."""
         xd = self.ctx.array(x)

@@ -141,14 +141,62 @@ __global__ __launch_bounds__(kWG) void k_singular(DevMesh M, DevFreq F, int kind
     sing[t] = bc_entry(F, kind, o, i, j);
 }
 
-// singular entries of the dense blocks from the table
+// FP32 dense near-field tiles: as k_dense with the regular quadrature of
+// regular_ops_f32, written as complex FP32
+__global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const int32_t* __restrict__ subst, int kind,
+                                                 const Tile* __restrict__ tiles, bem_c32* out)
+{
+    __shared__ double sq[3][16][kTile];
+    __shared__ double sp[3][kTile];
+    __shared__ float sn[3][kTile], sa[kTile];
+    const Tile T = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int64_t n = M.n;
+    for (int idx = tid; idx < 16 * kTile; idx += kWG) {
+        int q = idx / kTile, c = idx % kTile;
+        if (c < T.cols) {
+            int64_t j = T.col0 + c;
+            sq[0][q][c] = M.qx[q * n + j];
+            sq[1][q][c] = M.qy[q * n + j];
+            sq[2][q][c] = M.qz[q * n + j];
+        }
+    }
+    for (int c = tid; c < T.cols; c += kWG) {
+        int64_t j = T.col0 + c;
+        sn[0][c] = (float)M.nx[j]; sn[1][c] = (float)M.ny[j]; sn[2][c] = (float)M.nz[j];
+        sp[0][c] = M.px[j]; sp[1][c] = M.py[j]; sp[2][c] = M.pz[j];
+        sa[c] = (float)M.area[j];
+    }
+    __syncthreads();
+    const int r = tid & (kTile - 1), cg = tid >> 6;
+    if (r >= T.rows) return;
+    const int64_t i = T.row0 + r;
+    const double px = M.px[i], py = M.py[i], pz = M.pz[i];
+    const float npx = (float)M.nx[i], npy = (float)M.ny[i], npz = (float)M.nz[i];
+    for (int c = cg; c < T.cols; c += 4) {
+        const int64_t j = T.col0 + c;
+        if (i == j) continue;
+        double dx = px - sp[0][c], dy = py - sp[1][c], dz = pz - sp[2][c];
+        if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
+        bem_c32f4 o = regular_ops_f32(F, px, py, pz, npx, npy, npz, sn[0][c], sn[1][c], sn[2][c], sa[c],
+                                      &sq[0][0][c], &sq[1][0][c], &sq[2][0][c], kTile);
+        out[T.dest + r + (int64_t)c * T.ld] = bc_entry_f32(F, subst, kind, o, j);
+    }
+}
+
+// singular entries of the dense blocks from the (FP64) table
+template <class C>
 __global__ void k_scatter(const Fix* __restrict__ fx, int64_t nfix, int64_t nnz, const bem_c64* __restrict__ sing,
-                          bem_c64* out)
+                          C* out)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nfix) return;
     Fix f = fx[t];
-    out[f.dest] = sing[f.t >= 0 ? f.t : nnz + f.i];
+    bem_c64 v = sing[f.t >= 0 ? f.t : nnz + f.i];
+    C o;
+    o.re = v.re;
+    o.im = v.im;
+    out[f.dest] = o;
 }
 
 __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int64_t* ij, bem_c64* out)
@@ -180,6 +228,37 @@ __device__ __forceinline__ bem_c64 wave_sum(bem_c64 x)
     return x;
 }
 
+// ---- complex arithmetic over the ACA storage type: bem_c64 uses the
+// bit-exact FP64 contract of bem_detmath.h, bem_c32 the same formulas in FP32
+__device__ __forceinline__ bem_c64 cx_mul(bem_c64 a, bem_c64 b) { return c64_mul(a, b); }
+__device__ __forceinline__ bem_c32 cx_mul(bem_c32 a, bem_c32 b) { return c32_mul(a, b); }
+__device__ __forceinline__ bem_c64 cx_fma(bem_c64 a, bem_c64 b, bem_c64 acc) { return c64_fma(a, b, acc); }
+__device__ __forceinline__ bem_c32 cx_fma(bem_c32 a, bem_c32 b, bem_c32 acc) { return c32_fma(a, b, acc); }
+__device__ __forceinline__ bem_c64 cx_add(bem_c64 a, bem_c64 b) { return c64_add(a, b); }
+__device__ __forceinline__ bem_c32 cx_add(bem_c32 a, bem_c32 b) { return c32(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ bem_c64 cx_sub(bem_c64 a, bem_c64 b) { return c64_sub(a, b); }
+__device__ __forceinline__ bem_c32 cx_sub(bem_c32 a, bem_c32 b) { return c32(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ double cx_abs2(bem_c64 a) { return c64_abs2(a); }
+__device__ __forceinline__ double cx_abs2(bem_c32 a) { return (double)fmaf(a.re, a.re, a.im * a.im); }
+__device__ __forceinline__ bem_c64 cx_inv(bem_c64 a) { return c64_div(c64(1.0, 0.0), a); }
+__device__ __forceinline__ bem_c32 cx_inv(bem_c32 a)
+{
+    float d = fmaf(a.re, a.re, a.im * a.im);
+    return c32(a.re / d, -a.im / d);
+}
+template <class C> __device__ __forceinline__ C cx_zero();
+template <> __device__ __forceinline__ bem_c64 cx_zero<bem_c64>() { return c64(0, 0); }
+template <> __device__ __forceinline__ bem_c32 cx_zero<bem_c32>() { return c32(0, 0); }
+__device__ __forceinline__ bem_c64 ld64(const bem_c64* p) { return *p; }
+__device__ __forceinline__ bem_c64 ld64(const bem_c32* p)
+{
+    bem_c32 v = *p;
+    return c64((double)v.re, (double)v.im);
+}
+template <class C> __device__ __forceinline__ C from64(bem_c64 z);
+template <> __device__ __forceinline__ bem_c64 from64<bem_c64>(bem_c64 z) { return z; }
+template <> __device__ __forceinline__ bem_c32 from64<bem_c32>(bem_c64 z) { return c32((float)z.re, (float)z.im); }
+
 struct ArgMax { double v; int64_t i; };
 
 __device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b)
@@ -190,13 +269,13 @@ __device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b)
 
 // first index of the largest |v_i|^2 over [0, len); index `excl` counts as 0.
 // NT threads (NT / 64 wavefronts); every thread gets the result.
-template <int NT>
-__device__ ArgMax block_argmax(const bem_c64* v, int64_t len, int64_t excl, ArgMax* sred)
+template <int NT, class C>
+__device__ ArgMax block_argmax(const C* v, int64_t len, int64_t excl, ArgMax* sred)
 {
     const int tid = threadIdx.x;
     ArgMax a{-1.0, (int64_t)1 << 62};
     for (int64_t t = tid; t < len; t += NT) {
-        double x = t == excl ? 0.0 : c64_abs2(v[t]);
+        double x = t == excl ? 0.0 : cx_abs2(v[t]);
         if (x > a.v) { a.v = x; a.i = t; }
     }
     for (int s = 32; s > 0; s >>= 1) {
@@ -217,12 +296,13 @@ __device__ ArgMax block_argmax(const bem_c64* v, int64_t len, int64_t excl, ArgM
 struct AcaArgs {
     DevMesh M;
     DevFreq F;
+    DevFreq32 F32;
     int kind;
     const int32_t* blist;
     const int64_t *row0, *rows, *col0, *cols;
     const int64_t *uoff, *voff, *aoff;
     const int32_t* cap;
-    bem_c64 *U, *V, *A;
+    void *U, *V, *A;      // storage of the ACA type C
     int32_t* rank_out;
     int32_t* flag_out;    // 1: rank capacity reached, 2: non-finite
     int64_t* evals_out;
@@ -231,21 +311,21 @@ struct AcaArgs {
     uint64_t seed;
     const int64_t* piv_off;
     const int32_t* piv;
-    const bem_c64* sing;   // self / near-singular entries (k_singular)
+    const bem_c64* sing;  // self / near-singular entries (k_singular), FP64
     int64_t nnz;
 };
 
 // The dot products are processed kDotGroup at a time per wavefront: the
 // new vector is loaded once per group and the shuffle trees of the group are
 // interleaved (independent latency chains); each task keeps the canonical
-// lane-strided / pairwise order.
-constexpr int kDotGroup = 8;
+// lane-strided / pairwise order.  Accumulation is FP64 for both types.
+constexpr int kDotGroup = 4;
 
 // tasks over vector v (length len) against basis W (stride len): k = 0 is
 // |v|^2 into out[base], k >= 1 is v^H W_{k-1} into out[base + 2k]
-template <int NT>
-__device__ __forceinline__ void dot_group_pass(const bem_c64* v, const bem_c64* W, int64_t len, int ntask,
-                                               bem_c64* out, int base)
+template <int NT, class C>
+__device__ __forceinline__ void dot_group_pass(const C* v, const C* W, int64_t len, int ntask, bem_c64* out,
+                                               int base)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int kb = w * kDotGroup; kb < ntask; kb += (NT / 64) * kDotGroup) {
@@ -253,12 +333,12 @@ __device__ __forceinline__ void dot_group_pass(const bem_c64* v, const bem_c64* 
 #pragma unroll
         for (int g = 0; g < kDotGroup; g++) acc[g] = c64(0, 0);
         for (int64_t t = lane; t < len; t += 64) {
-            bem_c64 cv = v[t];
+            bem_c64 cv = ld64(v + t);
 #pragma unroll
             for (int g = 0; g < kDotGroup; g++) {
                 int k = kb + g;
                 if (k == 0) acc[g].re = acc[g].re + c64_abs2(cv);
-                else if (k < ntask) acc[g] = c64_fma(c64_conj(cv), W[(int64_t)(k - 1) * len + t], acc[g]);
+                else if (k < ntask) acc[g] = c64_fma(c64_conj(cv), ld64(W + (int64_t)(k - 1) * len + t), acc[g]);
             }
         }
         for (int s = 32; s > 0; s >>= 1) {
@@ -280,9 +360,9 @@ __device__ __forceinline__ void dot_group_pass(const bem_c64* v, const bem_c64* 
 }
 
 // sdots[0] |row|^2, [1] |col|^2, [2 + 2l] row^H V_l, [3 + 2l] col^H U_l (l < r)
-template <int NT>
-__device__ void aca_dots(const bem_c64* col, const bem_c64* row, const bem_c64* U, const bem_c64* V, int64_t m,
-                         int64_t n, int r, bem_c64* sdots)
+template <int NT, class C>
+__device__ void aca_dots(const C* col, const C* row, const C* U, const C* V, int64_t m, int64_t n, int r,
+                         bem_c64* sdots)
 {
     dot_group_pass<NT>(col, U, m, r + 1, sdots, 1);
     dot_group_pass<NT>(row, V, n, r + 1, sdots, 0);
@@ -290,77 +370,115 @@ __device__ void aca_dots(const bem_c64* col, const bem_c64* row, const bem_c64* 
 
 // Frobenius-norm estimate update of partialPivotACA (hmatrix.cpp:568-572);
 // every thread computes it identically from the LDS dot products
-__device__ __forceinline__ double aca_update_norm(bem_c64 alpha, const bem_c64* A, int r, const bem_c64* sdots,
-                                                  bem_c64& S)
+template <class C>
+__device__ __forceinline__ double aca_update_norm(C alpha, const C* A, int r, const bem_c64* sdots, bem_c64& S)
 {
-    double gain = (c64_abs2(alpha) * sdots[0].re) * sdots[1].re;
+    double gain = (cx_abs2(alpha) * sdots[0].re) * sdots[1].re;
     bem_c64 s = S;
     s.re = s.re + gain;
-    bem_c64 ca2 = c64_scale(c64_conj(alpha), 2.0);
+    bem_c64 ca2 = c64_scale(c64_conj(ld64(&alpha)), 2.0);
     for (int l = 0; l < r; l++) {
-        bem_c64 t = c64_mul(c64_mul(c64_mul(ca2, A[l]), sdots[3 + 2 * l]), sdots[2 + 2 * l]);
+        bem_c64 t = c64_mul(c64_mul(c64_mul(ca2, ld64(A + l)), sdots[3 + 2 * l]), sdots[2 + 2 * l]);
         s = c64_add(s, t);
     }
     S = s;
     return gain;
 }
 
+// Cross-approximation correction of one residual entry,
+//   s = sum_{l < r} w_l * B[l * ld + idx]   (W_FIRST)   or
+//   s = sum_{l < r} B[l * ld + idx] * w_l   (!W_FIRST),
+// accumulated strictly in the order l = 0, 1, ..., r - 1.  The loads of B are
+// issued kCrossUnroll at a time ahead of their products so that the L2 / HBM
+// latency is paid once per kCrossUnroll terms rather than once per term.
+constexpr int kCrossUnroll = 8;
+template <bool W_FIRST, class C>
+__device__ __forceinline__ C cross_sum(const C* w, const C* B, int64_t ld, int64_t idx, int r)
+{
+    C s = cx_zero<C>();
+    int l = 0;
+    for (; l + kCrossUnroll <= r; l += kCrossUnroll) {
+        C v[kCrossUnroll];
+#pragma unroll
+        for (int u = 0; u < kCrossUnroll; u++) v[u] = B[(int64_t)(l + u) * ld + idx];
+#pragma unroll
+        for (int u = 0; u < kCrossUnroll; u++) s = W_FIRST ? cx_fma(w[l + u], v[u], s) : cx_fma(v[u], w[l + u], s);
+    }
+    for (; l < r; l++) {
+        C v = B[(int64_t)l * ld + idx];
+        s = W_FIRST ? cx_fma(w[l], v, s) : cx_fma(v, w[l], s);
+    }
+    return s;
+}
+
+__device__ __forceinline__ bool near_or_self(const DevMesh& M, int64_t i, int64_t j)
+{
+    if (i == j) return true;
+    double dx = M.px[i] - M.px[j], dy = M.py[i] - M.py[j], dz = M.pz[i] - M.pz[j];
+    return sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim;
+}
+
 // regular entry of (i, j), or false for a self / near-singular pair
 __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c64* e)
 {
     const DevMesh& M = a.M;
-    if (i == j) return false;
-    const double px = M.px[i], py = M.py[i], pz = M.pz[i];
-    double dx = px - M.px[j], dy = py - M.py[j], dz = pz - M.pz[j];
-    if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) return false;
-    bem_ops4 o = regular_ops<1>(a.F.k, px, py, pz, M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j], M.area[j],
-                                M.qx + j, M.qy + j, M.qz + j, M.n);
+    if (near_or_self(M, i, j)) return false;
+    bem_ops4 o = regular_ops<1>(a.F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j],
+                                M.nz[j], M.area[j], M.qx + j, M.qy + j, M.qz + j, M.n);
     *e = bc_entry(a.F, a.kind, o, i, j);
+    return true;
+}
+__device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c32* e)
+{
+    const DevMesh& M = a.M;
+    if (near_or_self(M, i, j)) return false;
+    bem_c32f4 o = regular_ops_f32<1>(a.F32, M.px[i], M.py[i], M.pz[i], (float)M.nx[i], (float)M.ny[i],
+                                     (float)M.nz[i], (float)M.nx[j], (float)M.ny[j], (float)M.nz[j], (float)M.area[j],
+                                     M.qx + j, M.qy + j, M.qz + j, M.n);
+    *e = bc_entry_f32(a.F32, a.F.subst, a.kind, o, j);
     return true;
 }
 
 // self / near-singular entry of (i, j) from the table; t = pair index or
 // -1 for the self entry
-__device__ __forceinline__ bem_c64 singular_entry(const AcaArgs& a, int64_t i, int64_t t)
+template <class C>
+__device__ __forceinline__ C singular_entry(const AcaArgs& a, int64_t i, int64_t t)
 {
-    return a.sing[t >= 0 ? t : a.nnz + i];
+    return from64<C>(a.sing[t >= 0 ? t : a.nnz + i]);
 }
 
 // any entry (the pivot probe of the extra variant)
-__device__ __forceinline__ bem_c64 entry_any(const AcaArgs& a, int64_t i, int64_t j)
+template <class C>
+__device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
 {
-    bem_c64 e;
+    C e;
     if (regular_entry(a, i, j, &e)) return e;
-    return singular_entry(a, i, i == j ? -1 : near_find(a.M, i, (int32_t)j));
+    return singular_entry<C>(a, i, i == j ? -1 : near_find(a.M, i, (int32_t)j));
 }
 
 // Row i of the block minus the current cross approximation, into dst.
 // Regular entries first; the few self / near-singular entries of the row
 // (CSR row r0 + i) in a second pass, so that the main loop has no divergent
 // branch.
-template <int NT>
+template <int NT, class C>
 __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
-                        const bem_c64* U, const bem_c64* V, const bem_c64* A, bem_c64* dst, bem_c64* sw)
+                        const C* U, const C* V, const C* A, C* dst, C* sw)
 {
-    for (int l = threadIdx.x; l < r; l += NT) sw[l] = c64_mul(U[(int64_t)l * m + i], A[l]);
+    for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(U[(int64_t)l * m + i], A[l]);
     __syncthreads();
     const int64_t gi = r0 + i;
     for (int64_t j = threadIdx.x; j < n; j += NT) {
-        bem_c64 e;
+        C e;
         if (!regular_entry(a, gi, c0 + j, &e)) continue;
-        bem_c64 s = c64(0, 0);
-        for (int l = 0; l < r; l++) s = c64_fma(sw[l], V[(int64_t)l * n + j], s);
-        dst[j] = c64_sub(e, s);
+        dst[j] = cx_sub(e, cross_sum<true>(sw, V, n, j, r));
     }
     const int64_t p0 = a.M.nr_ptr[gi], p1 = a.M.nr_ptr[gi + 1];
     for (int64_t t = p0 + threadIdx.x; t <= p1; t += NT) {
         int64_t gj = t < p1 ? a.M.nr_col[t] : gi;
         if (gj < c0 || gj >= c0 + n) continue;
         int64_t j = gj - c0;
-        bem_c64 e = singular_entry(a, gi, t < p1 ? t : -1);
-        bem_c64 s = c64(0, 0);
-        for (int l = 0; l < r; l++) s = c64_fma(sw[l], V[(int64_t)l * n + j], s);
-        dst[j] = c64_sub(e, s);
+        C e = singular_entry<C>(a, gi, t < p1 ? t : -1);
+        dst[j] = cx_sub(e, cross_sum<true>(sw, V, n, j, r));
     }
     __syncthreads();
 }
@@ -368,19 +486,17 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // column j of the block minus the current cross approximation, into dst;
 // the near-singular rows of column c0 + j are the near list of row c0 + j
 // (the distance criterion is symmetric)
-template <int NT>
+template <int NT, class C>
 __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
-                        const bem_c64* U, const bem_c64* V, const bem_c64* A, bem_c64* dst, bem_c64* sw)
+                        const C* U, const C* V, const C* A, C* dst, C* sw)
 {
-    for (int l = threadIdx.x; l < r; l += NT) sw[l] = c64_mul(A[l], V[(int64_t)l * n + j]);
+    for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(A[l], V[(int64_t)l * n + j]);
     __syncthreads();
     const int64_t gj = c0 + j;
     for (int64_t i = threadIdx.x; i < m; i += NT) {
-        bem_c64 e;
+        C e;
         if (!regular_entry(a, r0 + i, gj, &e)) continue;
-        bem_c64 s = c64(0, 0);
-        for (int l = 0; l < r; l++) s = c64_fma(U[(int64_t)l * m + i], sw[l], s);
-        dst[i] = c64_sub(e, s);
+        dst[i] = cx_sub(e, cross_sum<false>(sw, U, m, i, r));
     }
     const int64_t p0 = a.M.nr_ptr[gj], p1 = a.M.nr_ptr[gj + 1];
     for (int64_t t = p0 + threadIdx.x; t <= p1; t += NT) {
@@ -388,10 +504,8 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
         if (gi < r0 || gi >= r0 + m) continue;
         int64_t i = gi - r0;
         int64_t pair = t < p1 ? near_find(a.M, gi, (int32_t)gj) : -1;
-        bem_c64 e = singular_entry(a, gi, pair);
-        bem_c64 s = c64(0, 0);
-        for (int l = 0; l < r; l++) s = c64_fma(U[(int64_t)l * m + i], sw[l], s);
-        dst[i] = c64_sub(e, s);
+        C e = singular_entry<C>(a, gi, pair);
+        dst[i] = cx_sub(e, cross_sum<false>(sw, U, m, i, r));
     }
     __syncthreads();
 }
@@ -399,19 +513,19 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // ACA of one admissible block b by a workgroup of NT threads.  Plain
 // variant = BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra
 // variant = BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).
-// Dynamic LDS: sw[cap] then sdots[2 cap + 2].
-template <bool EXTRA, int NT>
-__device__ void aca_block(const AcaArgs& a, const int b, bem_c64* dyn, ArgMax* sred)
+// Dynamic LDS: sw[cap] (type C) then sdots[2 cap + 2] (FP64).
+template <bool EXTRA, int NT, class C>
+__device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred)
 {
     const int64_t m = a.rows[b], n = a.cols[b], r0 = a.row0[b], c0 = a.col0[b];
     const int cap = a.cap[b];
     int64_t maxRank = m < n ? m : n;
     if (a.maxrank > 0 && a.maxrank < maxRank) maxRank = a.maxrank;
-    bem_c64* U = a.U + a.uoff[b];
-    bem_c64* V = a.V + a.voff[b];
-    bem_c64* A = a.A + a.aoff[b];
-    bem_c64* sw = dyn;
-    bem_c64* sdots = dyn + cap;
+    C* U = (C*)a.U + a.uoff[b];
+    C* V = (C*)a.V + a.voff[b];
+    C* A = (C*)a.A + a.aoff[b];
+    C* sw = (C*)dyn;
+    bem_c64* sdots = (bem_c64*)(dyn + (((size_t)cap * sizeof(C) + 15) & ~(size_t)15));
     const double tiny2 = kTiny * kTiny, tol2 = a.tol * a.tol;
     int r = 0, flag = 0;
     int64_t ev = 0;
@@ -423,19 +537,19 @@ __device__ void aca_block(const AcaArgs& a, const int b, bem_c64* dyn, ArgMax* s
         int64_t i = -1;
         while (r < maxRank) {
             if (r >= cap) { flag = 1; break; }
-            const bem_c64* wcol = r >= 1 ? U + (int64_t)(r - 1) * m : U;
+            const C* wcol = r >= 1 ? U + (int64_t)(r - 1) * m : U;
             ArgMax am = block_argmax<NT>(wcol, m, r >= 1 ? i : -1, sred);
             i = am.i;
             if (am.v <= tiny2 && r >= 1) break;
-            bem_c64* row = V + (int64_t)r * n;
+            C* row = V + (int64_t)r * n;
             aca_row<NT>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
             ev += n;
             ArgMax ar = block_argmax<NT>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
-            bem_c64* col = U + (int64_t)r * m;
+            C* col = U + (int64_t)r * m;
             aca_col<NT>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
             ev += m;
-            bem_c64 alpha = c64_div(c64(1.0, 0.0), row[ar.i]);
+            C alpha = cx_inv(row[ar.i]);
             if (threadIdx.x == 0) A[r] = alpha;
             aca_dots<NT>(col, row, U, V, m, n, r, sdots);
             __syncthreads();
@@ -450,33 +564,33 @@ __device__ void aca_block(const AcaArgs& a, const int b, bem_c64* dyn, ArgMax* s
         bool done = false;
         for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
             int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
-            bem_c64 pe = entry_any(a, r0 + i, c0 + j);
+            C pe = entry_any<C>(a, r0 + i, c0 + j);
             ev += 1;
-            if (c64_abs2(pe) < tiny2) continue;
-            bem_c64 approx = c64(0, 0);
+            if (cx_abs2(pe) < tiny2) continue;
+            C approx = cx_zero<C>();
             for (int l = 0; l < r; l++)
-                approx = c64_add(approx, c64_mul(c64_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
-            if (c64_abs2(c64_sub(pe, approx)) < tol2 * c64_abs2(pe)) continue;
+                approx = cx_add(approx, cx_mul(cx_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
+            if (cx_abs2(cx_sub(pe, approx)) < tol2 * cx_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
             aca_col<NT>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
             ev += m;
             bool first = true;
             while (r < maxRank) {
                 if (r >= cap) { flag = 1; done = true; break; }
-                const bem_c64* wcol = first ? U + (int64_t)r * m : U + (int64_t)(r - 1) * m;
+                const C* wcol = first ? U + (int64_t)r * m : U + (int64_t)(r - 1) * m;
                 ArgMax am = block_argmax<NT>(wcol, m, first ? -1 : i, sred);
                 i = am.i;
                 first = false;
-                if (c64_abs2(wcol[i]) <= tiny2 && r >= 1) break;
-                bem_c64* row = V + (int64_t)r * n;
+                if (cx_abs2(wcol[i]) <= tiny2 && r >= 1) break;
+                C* row = V + (int64_t)r * n;
                 aca_row<NT>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
                 ev += n;
                 ArgMax ar = block_argmax<NT>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
-                bem_c64* col = U + (int64_t)r * m;
+                C* col = U + (int64_t)r * m;
                 aca_col<NT>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
                 ev += m;
-                bem_c64 alpha = c64_div(c64(1.0, 0.0), row[ar.i]);
+                C alpha = cx_inv(row[ar.i]);
                 if (threadIdx.x == 0) A[r] = alpha;
                 aca_dots<NT>(col, row, U, V, m, n, r, sdots);
                 __syncthreads();
@@ -501,41 +615,36 @@ __device__ void aca_block(const AcaArgs& a, const int b, bem_c64* dyn, ArgMax* s
 }
 
 // ACA (K3): one workgroup of NT threads per listed admissible block
-template <bool EXTRA, int NT>
+template <bool EXTRA, int NT, class C>
 __global__ __launch_bounds__(NT) void k_aca(AcaArgs a)
 {
-    extern __shared__ bem_c64 dyn[];
+    extern __shared__ __attribute__((aligned(16))) char dyn[];
     __shared__ ArgMax sred[NT / 64];
-    aca_block<EXTRA, NT>(a, a.blist[blockIdx.x], dyn, sred);
+    aca_block<EXTRA, NT, C>(a, a.blist[blockIdx.x], dyn, sred);
 }
 
 // copy the first rank columns/rows of the ACA scratch into the packed
 // storage (rank 0 becomes a zero rank-1 block, hmatrix.cpp:587-592)
+template <class C>
 __global__ __launch_bounds__(kWG) void k_compact(const int32_t* blist, const int64_t* rows, const int64_t* cols,
                                                  const int64_t* suoff, const int64_t* svoff, const int64_t* saoff,
-                                                 const bem_c64* sU, const bem_c64* sV, const bem_c64* sA,
-                                                 const int32_t* srank, const int64_t* duoff, const int64_t* dvoff,
-                                                 const int64_t* daoff, bem_c64* dU, bem_c64* dV, bem_c64* dA)
+                                                 const C* sU, const C* sV, const C* sA, const int32_t* srank,
+                                                 const int64_t* duoff, const int64_t* dvoff, const int64_t* daoff,
+                                                 C* dU, C* dV, C* dA)
 {
     const int b = blist[blockIdx.x];
     const int64_t m = rows[b], n = cols[b];
     const int r = srank[b];
     const int64_t nu = (int64_t)(r > 0 ? r : 1) * m, nv = (int64_t)(r > 0 ? r : 1) * n;
-    for (int64_t t = threadIdx.x; t < nu; t += kWG) dU[duoff[b] + t] = r > 0 ? sU[suoff[b] + t] : c64(0, 0);
-    for (int64_t t = threadIdx.x; t < nv; t += kWG) dV[dvoff[b] + t] = r > 0 ? sV[svoff[b] + t] : c64(0, 0);
-    for (int t = threadIdx.x; t < (r > 0 ? r : 1); t += kWG) dA[daoff[b] + t] = r > 0 ? sA[saoff[b] + t] : c64(0, 0);
+    const C z = cx_zero<C>();
+    for (int64_t t = threadIdx.x; t < nu; t += kWG) dU[duoff[b] + t] = r > 0 ? sU[suoff[b] + t] : z;
+    for (int64_t t = threadIdx.x; t < nv; t += kWG) dV[dvoff[b] + t] = r > 0 ? sV[svoff[b] + t] : z;
+    for (int t = threadIdx.x; t < (r > 0 ? r : 1); t += kWG) dA[daoff[b] + t] = r > 0 ? sA[saoff[b] + t] : z;
 }
 
 // ------------------------------------------------------------- H-MVM
 // Block storage is complex FP64 (bem_c64) or FP32 (bem_c32); products
 // accumulate in FP64 and vectors are FP64 in both cases.
-
-__device__ __forceinline__ bem_c64 ld64(const bem_c64* p) { return *p; }
-__device__ __forceinline__ bem_c64 ld64(const bem_c32* p)
-{
-    bem_c32 v = *p;
-    return c64((double)v.re, (double)v.im);
-}
 
 // near-field tiles (K4): partial[tile][row] = sum_c F[row, c] x[c]
 template <class S>
@@ -758,7 +867,8 @@ __global__ void k_reconstruct(int64_t n, const bem_c64* x, const int32_t* subst,
     if (pr) pr[i] = c64_mul(c64(0.0, s), ph);
 }
 
-__global__ void k_has_nonfinite(int64_t len, const bem_c64* v, int32_t* flag)
+template <class C>
+__global__ void k_has_nonfinite(int64_t len, const C* v, int32_t* flag)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < len && !(isfinite(v[i].re) && isfinite(v[i].im))) *flag = 1;
@@ -796,6 +906,8 @@ struct bem_ctx {
     bem_c64 cc{0, 0};
     int32_t* d_subst = nullptr;
     bem_c64 *d_cphi = nullptr, *d_cd = nullptr;
+    bem_c32 *d_cphi32 = nullptr, *d_cd32 = nullptr;   // FP32 copies for the FP32 assembly
+    int64_t freq_n = 0;                                // length of the arrays above
     std::vector<int32_t> h_subst;
     bool have_freq = false;
     // partition
@@ -843,8 +955,9 @@ struct bem_ctx {
 struct bem_hmat {
     bem_ctx* ctx = nullptr;
     int kind = 0, prec = 0;
-    bem_c64* dense = nullptr;
-    bem_c64 *U = nullptr, *V = nullptr, *A = nullptr;
+    size_t esz = sizeof(bem_c64);   // bytes per stored entry (FP64 or FP32 complex)
+    void* dense = nullptr;
+    void *U = nullptr, *V = nullptr, *A = nullptr;
     int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
     int32_t* d_rank = nullptr;
     // two-phase low-rank MVM: (block, l) and (block, row chunk) work lists
@@ -1005,6 +1118,18 @@ DevFreq dev_freq(const bem_ctx* c)
     return F;
 }
 
+DevFreq32 dev_freq32(const bem_ctx* c)
+{
+    DevFreq32 F;
+    F.k = (float)c->k;
+    F.krev = c->k / (2.0 * BEM_PI);
+    F.cc.re = (float)c->cc.re;
+    F.cc.im = (float)c->cc.im;
+    F.cphi = c->d_cphi32;
+    F.cd = c->d_cd32;
+    return F;
+}
+
 int launch_check(bem_ctx* c, const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -1051,7 +1176,7 @@ int bem_destroy(bem_ctx_t c)
     free_list(c, c->allocs_part);
     free_list(c, c->allocs_piv);
     for (void* p : {(void*)c->d_alpha, (void*)c->d_beta, (void*)c->d_f, (void*)c->d_subst, (void*)c->d_cphi,
-                    (void*)c->d_cd, (void*)c->d_gm})
+                    (void*)c->d_cd, (void*)c->d_cphi32, (void*)c->d_cd32, (void*)c->d_gm})
         pfree(c, p);
     for (auto& kv : c->live) (void)hipFree(kv.first);
     c->live.clear();
@@ -1076,7 +1201,9 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
     free_list(c, c->allocs_mesh);
-    c->have_part = false;
+    c->have_part = false;   // partition, boundary conditions and frequency belong to the old mesh
+    c->have_bc = false;
+    c->have_freq = false;
     c->n = n;
     c->hbar = hbar;
     c->h_nodes.assign(nodes9, nodes9 + 9 * n);
@@ -1193,12 +1320,28 @@ int bem_set_frequency(bem_ctx_t c, double k, double cc_re, double cc_im)
         }
     }
     c->h_subst = subst;
-    if (!c->d_subst) {
+    if (c->freq_n != n) {   // (re)size for the current mesh
+        for (void** p : {(void**)&c->d_subst, (void**)&c->d_cphi, (void**)&c->d_cd, (void**)&c->d_cphi32,
+                         (void**)&c->d_cd32}) {
+            pfree(c, *p);
+            *p = nullptr;
+        }
+        c->freq_n = 0;
         int rc;
         if ((rc = dalloc(c, &c->d_subst, n, nullptr))) return rc;
         if ((rc = dalloc(c, &c->d_cphi, n, nullptr))) return rc;
         if ((rc = dalloc(c, &c->d_cd, n, nullptr))) return rc;
+        if ((rc = dalloc(c, &c->d_cphi32, n, nullptr))) return rc;
+        if ((rc = dalloc(c, &c->d_cd32, n, nullptr))) return rc;
+        c->freq_n = n;
     }
+    std::vector<bem_c32> cphi32(n), cd32(n);
+    for (int64_t j = 0; j < n; j++) {
+        cphi32[j].re = (float)cphi[j].re; cphi32[j].im = (float)cphi[j].im;
+        cd32[j].re = (float)cd[j].re; cd32[j].im = (float)cd[j].im;
+    }
+    HIPC(c, hipMemcpyAsync(c->d_cphi32, cphi32.data(), n * sizeof(bem_c32), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->d_cd32, cd32.data(), n * sizeof(bem_c32), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->d_subst, subst.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->d_cphi, cphi.data(), n * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->d_cd, cd.data(), n * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
@@ -1332,7 +1475,7 @@ namespace {
 // ACA over the blocks in `list` with per-block capacity `capf(b)` into freshly
 // allocated scratch; returns ranks/flags/evals per partition block
 struct AcaScratch {
-    bem_c64 *U = nullptr, *V = nullptr, *A = nullptr;
+    void *U = nullptr, *V = nullptr, *A = nullptr;
     int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
     int32_t *d_cap = nullptr, *d_rank = nullptr, *d_flag = nullptr, *d_list = nullptr;
     int64_t* d_ev = nullptr;
@@ -1342,7 +1485,24 @@ struct AcaScratch {
     ~AcaScratch() { if (ctx) free_list(ctx, owned); }
 };
 
-int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
+// one size class of blocks on stream st, storage type C
+template <class C>
+void launch_aca(int q, bool extra, unsigned nblocks, size_t shm, hipStream_t st, const AcaArgs& aq)
+{
+    dim3 g(nblocks);
+    if (q == 0) {
+        if (extra) hipLaunchKernelGGL((k_aca<true, 512, C>), g, dim3(512), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, 512, C>), g, dim3(512), shm, st, aq);
+    } else if (q == 1) {
+        if (extra) hipLaunchKernelGGL((k_aca<true, 256, C>), g, dim3(256), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, 256, C>), g, dim3(256), shm, st, aq);
+    } else {
+        if (extra) hipLaunchKernelGGL((k_aca<true, 64, C>), g, dim3(64), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, 64, C>), g, dim3(64), shm, st, aq);
+    }
+}
+
+int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
             const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv,
             const std::function<int()>& main_work, AcaScratch& S,
             std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev)
@@ -1359,9 +1519,13 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
         maxcap = std::max(maxcap, cap[b]);
     }
     int rc;
-    if ((rc = dalloc(c, &S.U, tu, &S.owned))) return rc;
-    if ((rc = dalloc(c, &S.V, tv, &S.owned))) return rc;
-    if ((rc = dalloc(c, &S.A, ta, &S.owned))) return rc;
+    const size_t esz = prec == BEM_PREC_FP32 ? sizeof(bem_c32) : sizeof(bem_c64);
+    if ((rc = dalloc_bytes(c, &S.U, tu * esz))) return rc;
+    S.owned.push_back(S.U);
+    if ((rc = dalloc_bytes(c, &S.V, tv * esz))) return rc;
+    S.owned.push_back(S.V);
+    if ((rc = dalloc_bytes(c, &S.A, ta * esz))) return rc;
+    S.owned.push_back(S.A);
     if ((rc = dupload(c, &S.d_uoff, S.uoff.data(), nb, &S.owned))) return rc;
     if ((rc = dupload(c, &S.d_voff, S.voff.data(), nb, &S.owned))) return rc;
     if ((rc = dupload(c, &S.d_aoff, S.aoff.data(), nb, &S.owned))) return rc;
@@ -1376,6 +1540,7 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
     AcaArgs a;
     a.M = c->mesh;
     a.F = dev_freq(c);
+    a.F32 = dev_freq32(c);
     a.kind = kind;
     a.blist = S.d_list;
     a.row0 = c->d_row0; a.rows = c->d_rows; a.col0 = c->d_col0; a.cols = c->d_cols;
@@ -1385,7 +1550,9 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
     a.tol = tol; a.maxrank = maxrank; a.seed = seed;
     a.piv_off = d_pivoff; a.piv = d_piv;
     a.sing = c->d_sing; a.nnz = c->nnz;
-    size_t shm = (size_t)(3 * maxcap + 2) * sizeof(bem_c64);
+    // dynamic LDS of aca_block: sw[cap] of the storage type, then the FP64
+    // dot products sdots[2 cap + 2]
+    size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
     // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock four,
     // larger eight.  The huge and large classes run on their own streams so
     // that their long tails overlap the small blocks (and the caller's dense
@@ -1397,8 +1564,9 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
     }
     std::vector<int32_t> ordered;
     for (auto& v : cls) ordered.insert(ordered.end(), v.begin(), v.end());
-    HIPC(c, hipMemcpyAsync(S.d_list, ordered.data(), ordered.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                           c->stream));
+    if (!ordered.empty())
+        HIPC(c, hipMemcpyAsync(S.d_list, ordered.data(), ordered.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                               c->stream));
     HIPC(c, hipEventRecord(c->ev_fork, c->stream));
     size_t off = 0;
     for (int q = 0; q < 3; q++) {
@@ -1410,18 +1578,9 @@ int run_aca(bem_ctx* c, int kind, double tol, int64_t maxrank, uint64_t seed, co
         hipStream_t st = q < 2 ? c->side[q] : c->stream;
         if (q < 2) HIPC(c, hipStreamWaitEvent(st, c->ev_fork, 0));
         Timer t(c, 2, st);
-        dim3 g((unsigned)cls[q].size());
         bool x = kind == BEM_KIND_DPHI;
-        if (q == 0) {
-            if (x) hipLaunchKernelGGL((k_aca<true, 512>), g, dim3(512), shm, st, aq);
-            else hipLaunchKernelGGL((k_aca<false, 512>), g, dim3(512), shm, st, aq);
-        } else if (q == 1) {
-            if (x) hipLaunchKernelGGL((k_aca<true, 256>), g, dim3(256), shm, st, aq);
-            else hipLaunchKernelGGL((k_aca<false, 256>), g, dim3(256), shm, st, aq);
-        } else {
-            if (x) hipLaunchKernelGGL((k_aca<true, 64>), g, dim3(64), shm, st, aq);
-            else hipLaunchKernelGGL((k_aca<false, 64>), g, dim3(64), shm, st, aq);
-        }
+        if (prec == BEM_PREC_FP32) launch_aca<bem_c32>(q, x, (unsigned)cls[q].size(), shm, st, aq);
+        else launch_aca<bem_c64>(q, x, (unsigned)cls[q].size(), shm, st, aq);
         if ((rc = launch_check(c, "k_aca"))) return rc;
     }
     for (int q = 0; q < 2; q++) {
@@ -1451,7 +1610,9 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     *out = nullptr;
     if (!c->have_part || !c->have_freq) return set_err(c, BEM_ERR_STATE, "bem_assemble: partition or frequency missing");
     if (kind != BEM_KIND_PHI && kind != BEM_KIND_DPHI) return set_err(c, BEM_ERR_ARG, "bem_assemble: bad kind");
-    if (precision != BEM_PREC_FP64) return set_err(c, BEM_ERR_ARG, "bem_assemble: precision not available in this entry point");
+    if (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)
+        return set_err(c, BEM_ERR_ARG, "bem_assemble: precision must be BEM_PREC_FP64 or BEM_PREC_FP32");
+    const bool f32 = precision == BEM_PREC_FP32;
     if (!(tol >= 0)) return set_err(c, BEM_ERR_ARG, "bem_assemble: negative tolerance");
     HIPC(c, hipSetDevice(c->device));
     double t0 = now_ms();
@@ -1459,14 +1620,16 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->ctx = c;
     h->kind = kind;
     h->prec = precision;
+    h->esz = f32 ? sizeof(bem_c32) : sizeof(bem_c64);
     int64_t nb = (int64_t)c->blocks.size();
     int rc;
     auto fail = [&](int code) { bem_hmat_destroy(h); return code; };
     // dense near field
-    if ((rc = dalloc(c, &h->dense, c->dense_total, nullptr))) return fail(rc);
-    bem_c64* outPhi = kind == BEM_KIND_PHI ? h->dense : nullptr;
-    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? h->dense : nullptr;
+    if ((rc = dalloc_bytes(c, &h->dense, (size_t)c->dense_total * h->esz))) return fail(rc);
+    bem_c64* outPhi = kind == BEM_KIND_PHI ? (bem_c64*)h->dense : nullptr;
+    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? (bem_c64*)h->dense : nullptr;
     DevFreq F = dev_freq(c);
+    DevFreq32 F32 = dev_freq32(c);
     hipEvent_t ev0, ev1, ev2;
     HIPC(c, hipEventCreate(&ev0)); HIPC(c, hipEventCreate(&ev1)); HIPC(c, hipEventCreate(&ev2));
     HIPC(c, hipEventRecord(ev0, c->stream));
@@ -1483,13 +1646,22 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         int rc2;
         if (!c->tiles.empty()) {
             Timer t(c, 0);
-            hipLaunchKernelGGL(k_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F,
-                               c->d_tiles, outPhi, outDPhi);
+            if (f32)
+                hipLaunchKernelGGL(k_dense32, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F32,
+                                   c->d_subst, kind, c->d_tiles, (bem_c32*)h->dense);
+            else
+                hipLaunchKernelGGL(k_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F,
+                                   c->d_tiles, outPhi, outDPhi);
             if ((rc2 = launch_check(c, "k_dense"))) return rc2;
         }
         if (!c->fixes.empty()) {
-            hipLaunchKernelGGL(k_scatter, dim3(nblk((int64_t)c->fixes.size(), kWG)), dim3(kWG), 0, c->stream,
-                               c->d_fixes, (int64_t)c->fixes.size(), c->nnz, c->d_sing, h->dense);
+            dim3 g(nblk((int64_t)c->fixes.size(), kWG));
+            if (f32)
+                hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                   (int64_t)c->fixes.size(), c->nnz, c->d_sing, (bem_c32*)h->dense);
+            else
+                hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                   (int64_t)c->fixes.size(), c->nnz, c->d_sing, (bem_c64*)h->dense);
             if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
         }
         return BEM_OK;
@@ -1530,7 +1702,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     AcaScratch S1, S2;
     std::vector<int32_t> rank, flag, rank2, flag2;
     std::vector<int64_t> evs, evs2;
-    if ((rc = run_aca(c, kind, tol, maxrank, seed, c->adm_list, cap, d_pivoff, d_piv, dense_work, S1, rank, flag,
+    if ((rc = run_aca(c, kind, precision, tol, maxrank, seed, c->adm_list, cap, d_pivoff, d_piv, dense_work, S1, rank, flag,
                       evs)))
         return fail(rc);
     HIPC(c, hipEventRecord(ev1, c->stream));
@@ -1546,7 +1718,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     if (!over.empty()) {
         std::vector<int32_t> cap2(nb, 0);
         for (int32_t b : over) cap2[b] = std::min<int32_t>(full_cap(b), kCapMax);
-        if ((rc = run_aca(c, kind, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, [] { return BEM_OK; }, S2, rank2,
+        if ((rc = run_aca(c, kind, precision, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, [] { return BEM_OK; }, S2, rank2,
                           flag2, evs2)))
             return fail(rc);
         for (int32_t b : over) {
@@ -1578,9 +1750,9 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         sumr += r;
         evaluated += evs[b];
     }
-    if ((rc = dalloc(c, &h->U, tu, nullptr))) return fail(rc);
-    if ((rc = dalloc(c, &h->V, tv, nullptr))) return fail(rc);
-    if ((rc = dalloc(c, &h->A, ta, nullptr))) return fail(rc);
+    if ((rc = dalloc_bytes(c, &h->U, (size_t)tu * h->esz))) return fail(rc);
+    if ((rc = dalloc_bytes(c, &h->V, (size_t)tv * h->esz))) return fail(rc);
+    if ((rc = dalloc_bytes(c, &h->A, (size_t)ta * h->esz))) return fail(rc);
     if ((rc = dupload(c, &h->d_uoff, h->uoff.data(), nb, nullptr))) return fail(rc);
     if ((rc = dupload(c, &h->d_voff, h->voff.data(), nb, nullptr))) return fail(rc);
     if ((rc = dupload(c, &h->d_aoff, h->aoff.data(), nb, nullptr))) return fail(rc);
@@ -1609,30 +1781,40 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         if ((rc = dupload(c, &h->d_toff, toff.data(), nb, nullptr))) return fail(rc);
         if ((rc = dalloc(c, &h->d_t, tot, nullptr))) return fail(rc);
     }
-    int32_t *d_keep = nullptr, *d_over = nullptr;
-    if (!keep.empty()) {
-        if ((rc = dupload(c, &d_keep, keep.data(), keep.size(), &own2))) return fail(rc);
-        hipLaunchKernelGGL(k_compact, dim3((unsigned)keep.size()), dim3(kWG), 0, c->stream, d_keep, c->d_rows,
-                           c->d_cols, S1.d_uoff, S1.d_voff, S1.d_aoff, S1.U, S1.V, S1.A, d_srank, h->d_uoff,
-                           h->d_voff, h->d_aoff, h->U, h->V, h->A);
-        if ((rc = launch_check(c, "k_compact"))) return fail(rc);
-    }
-    if (!over.empty()) {
-        if ((rc = dupload(c, &d_over, over.data(), over.size(), &own2))) return fail(rc);
-        hipLaunchKernelGGL(k_compact, dim3((unsigned)over.size()), dim3(kWG), 0, c->stream, d_over, c->d_rows,
-                           c->d_cols, S2.d_uoff, S2.d_voff, S2.d_aoff, S2.U, S2.V, S2.A, d_srank, h->d_uoff,
-                           h->d_voff, h->d_aoff, h->U, h->V, h->A);
-        if ((rc = launch_check(c, "k_compact"))) return fail(rc);
-    }
+    auto compact = [&](const std::vector<int32_t>& lst, const AcaScratch& Sx) -> int {
+        if (lst.empty()) return BEM_OK;
+        int32_t* d_lst = nullptr;
+        int rc2;
+        if ((rc2 = dupload(c, &d_lst, lst.data(), lst.size(), &own2))) return rc2;
+        dim3 g((unsigned)lst.size());
+        if (f32)
+            hipLaunchKernelGGL(k_compact<bem_c32>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
+                               Sx.d_uoff, Sx.d_voff, Sx.d_aoff, (const bem_c32*)Sx.U, (const bem_c32*)Sx.V,
+                               (const bem_c32*)Sx.A, d_srank, h->d_uoff, h->d_voff, h->d_aoff, (bem_c32*)h->U,
+                               (bem_c32*)h->V, (bem_c32*)h->A);
+        else
+            hipLaunchKernelGGL(k_compact<bem_c64>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
+                               Sx.d_uoff, Sx.d_voff, Sx.d_aoff, (const bem_c64*)Sx.U, (const bem_c64*)Sx.V,
+                               (const bem_c64*)Sx.A, d_srank, h->d_uoff, h->d_voff, h->d_aoff, (bem_c64*)h->U,
+                               (bem_c64*)h->V, (bem_c64*)h->A);
+        return launch_check(c, "k_compact");
+    };
+    if ((rc = compact(keep, S1))) return fail(rc);
+    if ((rc = compact(over, S2))) return fail(rc);
     HIPC(c, hipEventRecord(ev2, c->stream));
     // NaN check of the dense part (the reference reports NaN blocks,
     // hmatrix.cpp:812-820)
     int32_t* d_flag;
     if ((rc = dalloc(c, &d_flag, 1, &own2))) return fail(rc);
     HIPC(c, hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
-    if (c->dense_total)
-        hipLaunchKernelGGL(k_has_nonfinite, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream, c->dense_total,
-                           h->dense, d_flag);
+    if (c->dense_total) {
+        if (f32)
+            hipLaunchKernelGGL(k_has_nonfinite<bem_c32>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
+                               c->dense_total, (const bem_c32*)h->dense, d_flag);
+        else
+            hipLaunchKernelGGL(k_has_nonfinite<bem_c64>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
+                               c->dense_total, (const bem_c64*)h->dense, d_flag);
+    }
     int32_t hflag = 0;
     HIPC(c, hipMemcpyAsync(&hflag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -1649,7 +1831,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->stats[0] = (double)evaluated;
     h->stats[1] = maxr;
     h->stats[2] = nadm ? sumr / (double)nadm : 0;
-    h->stats[3] = (double)((c->dense_total + tu + tv + ta) * sizeof(bem_c64));
+    h->stats[3] = (double)((c->dense_total + tu + tv + ta) * (int64_t)h->esz);
     h->stats[4] = now_ms() - t0;
     h->stats[5] = (double)over.size();
     h->stats[6] = ms_dense;
@@ -1658,6 +1840,54 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     return BEM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// near-field tiles and low-rank blocks of one H-MVM into the partial vectors
+template <class S>
+int hmvm_blocks(bem_ctx* c, bem_hmat* h, const bem_c64* xd)
+{
+    int rc;
+    if (!c->tiles.empty()) {
+        Timer t(c, 3);
+        hipLaunchKernelGGL(k_mvm_dense<S>, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->d_tiles,
+                           (const S*)h->dense, xd, c->d_part);
+        if ((rc = launch_check(c, "k_mvm_dense"))) return rc;
+    }
+    if (h->nttask > 0) {
+        Timer t(c, 4);
+        hipLaunchKernelGGL(k_lr_t<S>, dim3(nblk(h->nttask, kWG / 64)), dim3(kWG), 0, c->stream, h->d_ttask,
+                           h->nttask, c->d_col0, c->d_cols, h->d_voff, h->d_aoff, h->d_toff, (const S*)h->V,
+                           (const S*)h->A, xd, h->d_t);
+        hipLaunchKernelGGL(k_lr_u<S>, dim3((unsigned)h->nutask), dim3(kWG), h->maxrank * sizeof(bem_c64), c->stream,
+                           h->d_utask, c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part, (const S*)h->U,
+                           h->d_t, c->d_part);
+        if ((rc = launch_check(c, "k_lr"))) return rc;
+    }
+    return BEM_OK;
+}
+
+// count complex entries at src (FP64 or FP32 on the device) into FP64 host memory
+int download_c64(bem_hmat* h, bem_c64* dst, const void* src, int64_t count)
+{
+    bem_ctx* c = h->ctx;
+    if (h->prec != BEM_PREC_FP32) {
+        HIPC(c, hipMemcpyAsync(dst, src, count * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return BEM_OK;
+    }
+    std::vector<bem_c32> tmp(count);
+    HIPC(c, hipMemcpyAsync(tmp.data(), src, count * sizeof(bem_c32), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int64_t t = 0; t < count; t++) dst[t] = c64(tmp[t].re, tmp[t].im);
+    return BEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bem_hmvm(bem_ctx_t c, bem_hmat_t h, const void* x, void* y, int accumulate)
 {
     if (!c || !h || !x || !y || h->ctx != c) return set_err(c, BEM_ERR_ARG, "bem_hmvm: bad arguments");
@@ -1665,21 +1895,9 @@ int bem_hmvm(bem_ctx_t c, bem_hmat_t h, const void* x, void* y, int accumulate)
     const bem_c64* xd = (const bem_c64*)x;
     bem_c64* yd = (bem_c64*)y;
     int rc;
-    if (!c->tiles.empty()) {
-        Timer t(c, 3);
-        hipLaunchKernelGGL(k_mvm_dense<bem_c64>, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->d_tiles,
-                           h->dense, xd, c->d_part);
-        if ((rc = launch_check(c, "k_mvm_dense"))) return rc;
-    }
-    if (h->nttask > 0) {
-        Timer t(c, 4);
-        hipLaunchKernelGGL(k_lr_t<bem_c64>, dim3(nblk(h->nttask, kWG / 64)), dim3(kWG), 0, c->stream, h->d_ttask,
-                           h->nttask, c->d_col0, c->d_cols, h->d_voff, h->d_aoff, h->d_toff, h->V, h->A, xd, h->d_t);
-        hipLaunchKernelGGL(k_lr_u<bem_c64>, dim3((unsigned)h->nutask), dim3(kWG), h->maxrank * sizeof(bem_c64),
-                           c->stream, h->d_utask, c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part, h->U,
-                           h->d_t, c->d_part);
-        if ((rc = launch_check(c, "k_lr"))) return rc;
-    }
+    if (h->prec == BEM_PREC_FP32) rc = hmvm_blocks<bem_c32>(c, h, xd);
+    else rc = hmvm_blocks<bem_c64>(c, h, xd);
+    if (rc) return rc;
     {
         Timer t(c, 5);
         hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(16 * c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr,
@@ -1702,26 +1920,20 @@ int bem_block_download(bem_hmat_t h, int64_t b, double* full_or_U, double* alpha
     bem_ctx* c = h->ctx;
     HIPC(c, hipSetDevice(c->device));
     const auto& B = c->blocks[b];
-    if (!B.adm) {
-        HIPC(c, hipMemcpyAsync(full_or_U, h->dense + c->dense_off[b], B.rows * B.cols * sizeof(bem_c64),
-                               hipMemcpyDeviceToHost, c->stream));
-    } else {
-        int64_t r = h->rank[b];
-        HIPC(c, hipMemcpyAsync(full_or_U, h->U + h->uoff[b], B.rows * r * sizeof(bem_c64), hipMemcpyDeviceToHost,
-                               c->stream));
-        if (alpha)
-            HIPC(c, hipMemcpyAsync(alpha, h->A + h->aoff[b], r * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
-        if (V) {
-            std::vector<bem_c64> rowmajor(r * B.cols);
-            HIPC(c, hipMemcpyAsync(rowmajor.data(), h->V + h->voff[b], r * B.cols * sizeof(bem_c64),
-                                   hipMemcpyDeviceToHost, c->stream));
-            HIPC(c, hipStreamSynchronize(c->stream));
-            bem_c64* Vo = (bem_c64*)V;  // reference layout: k x n column-major
-            for (int64_t l = 0; l < r; l++)
-                for (int64_t j = 0; j < B.cols; j++) Vo[j * r + l] = rowmajor[l * B.cols + j];
-        }
+    const char* dense = (const char*)h->dense;
+    const char *U = (const char*)h->U, *A = (const char*)h->A, *Vs = (const char*)h->V;
+    int rc;
+    if (!B.adm) return download_c64(h, (bem_c64*)full_or_U, dense + c->dense_off[b] * h->esz, B.rows * B.cols);
+    int64_t r = h->rank[b];
+    if ((rc = download_c64(h, (bem_c64*)full_or_U, U + h->uoff[b] * h->esz, B.rows * r))) return rc;
+    if (alpha && (rc = download_c64(h, (bem_c64*)alpha, A + h->aoff[b] * h->esz, r))) return rc;
+    if (V) {
+        std::vector<bem_c64> rowmajor(r * B.cols);
+        if ((rc = download_c64(h, rowmajor.data(), Vs + h->voff[b] * h->esz, r * B.cols))) return rc;
+        bem_c64* Vo = (bem_c64*)V;  // reference layout: k x n column-major
+        for (int64_t l = 0; l < r; l++)
+            for (int64_t j = 0; j < B.cols; j++) Vo[j * r + l] = rowmajor[l * B.cols + j];
     }
-    HIPC(c, hipStreamSynchronize(c->stream));
     return BEM_OK;
 }
 

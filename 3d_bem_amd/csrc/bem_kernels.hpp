@@ -191,4 +191,122 @@ __device__ __forceinline__ bem_c64 entry_general(const DevMesh& M, const DevFreq
     return bc_entry(F, kind, o, i, j);
 }
 
+// ---------------------------------------------------------------- FP32 path
+// Regular (far and near-field, non-singular) quadrature in complex FP32.
+// Only the geometry that decides the phase stays in FP64: R = p - q, |R|^2,
+// one Newton correction of the FP32 reciprocal square root, and the
+// reduction of k r / 2 pi to a fraction of a revolution.  Everything after
+// (hardware sin / cos of the revolution fraction, the kernel values and the
+// 16-point sums) is FP32 with explicit fmaf.
+
+struct bem_c32f4 { bem_c32 L, M, Mt, N; };
+
+// FP32 copies of the frequency data; krev = k / (2 pi) in FP64
+struct DevFreq32 {
+    float k;
+    double krev;
+    bem_c32 cc;
+    const bem_c32* cphi;
+    const bem_c32* cd;
+};
+
+__device__ __forceinline__ bem_c32 c32(float re, float im)
+{
+    bem_c32 z;
+    z.re = re;
+    z.im = im;
+    return z;
+}
+
+__device__ __forceinline__ bem_c32 c32_mul(bem_c32 a, bem_c32 b)
+{
+    return c32(fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re));
+}
+
+__device__ __forceinline__ bem_c32 c32_fma(bem_c32 a, bem_c32 b, bem_c32 acc)
+{
+    return c32(fmaf(a.re, b.re, fmaf(-a.im, b.im, acc.re)), fmaf(a.re, b.im, fmaf(a.im, b.re, acc.im)));
+}
+
+// one quadrature point, same formulas as bem_point_accumulate
+__device__ __forceinline__ void point_accumulate_f32(bem_c32f4* acc, const DevFreq32& F, double px, double py,
+                                                     double pz, float npx, float npy, float npz, float nqx, float nqy,
+                                                     float nqz, float upnq, double qx, double qy, double qz, float w)
+{
+    double Rx = px - qx, Ry = py - qy, Rz = pz - qz;
+    double rr = fma(Rz, Rz, fma(Ry, Ry, Rx * Rx));
+    float inv_r = __builtin_amdgcn_rsqf((float)rr);
+    double r0 = (double)((float)rr * inv_r);
+    double r = fma(0.5 * fma(-r0, r0, rr), (double)inv_r, r0);
+    double t = r * F.krev;
+    float rev = (float)(t - rint(t));     // in [-1/2, 1/2]
+    float s = __builtin_amdgcn_sinf(rev), c = __builtin_amdgcn_cosf(rev);
+    float kr = F.k * (float)r;
+    float fx = (float)Rx, fy = (float)Ry, fz = (float)Rz;
+    float rnq = -fmaf(fz, nqz, fmaf(fy, nqy, fx * nqx)) * inv_r;
+    float rnp = fmaf(fz, npz, fmaf(fy, npy, fx * npx)) * inv_r;
+    float g = inv_r * (float)BEM_INV_4PI;
+    float Gre = c * g, Gim = s * g;
+    float Grre = Gre * inv_r, Grim = Gim * inv_r;
+    float dGre = fmaf(-Grim, kr, -Grre);
+    float dGim = fmaf(Grre, kr, -Grim);
+    float Grrre = Grre * inv_r, Grrim = Grim * inv_r;
+    float a = fmaf(-kr, kr, 2.0f);
+    float b = -2.0f * kr;
+    float d2Gre = fmaf(Grrre, a, -(Grrim * b));
+    float d2Gim = fmaf(Grrre, b, Grrim * a);
+    float pp = rnp * rnq;
+    float tt = -(upnq + pp) * inv_r;
+    float Nre = fmaf(dGre, tt, d2Gre * pp);
+    float Nim = fmaf(dGim, tt, d2Gim * pp);
+    acc->L.re = fmaf(w, Gre, acc->L.re);
+    acc->L.im = fmaf(w, Gim, acc->L.im);
+    acc->M.re = fmaf(w, dGre * rnq, acc->M.re);
+    acc->M.im = fmaf(w, dGim * rnq, acc->M.im);
+    acc->Mt.re = fmaf(w, dGre * rnp, acc->Mt.re);
+    acc->Mt.im = fmaf(w, dGim * rnp, acc->Mt.im);
+    acc->N.re = fmaf(w, Nre, acc->N.re);
+    acc->N.im = fmaf(w, Nim, acc->N.im);
+}
+
+// order-8 16-point rule weights in FP32
+__constant__ static const float kTriWf[16] = {
+    0.14431560767779f, 0.09509163426728f, 0.09509163426728f, 0.09509163426728f, 0.10321737053472f, 0.10321737053472f,
+    0.10321737053472f, 0.03245849762320f, 0.03245849762320f, 0.03245849762320f, 0.02723031417443f, 0.02723031417443f,
+    0.02723031417443f, 0.02723031417443f, 0.02723031417443f, 0.02723031417443f};
+
+template <int UNROLL = 2>
+__device__ __forceinline__ bem_c32f4 regular_ops_f32(const DevFreq32& F, double px, double py, double pz, float npx,
+                                                     float npy, float npz, float nqx, float nqy, float nqz, float area,
+                                                     const double* qx, const double* qy, const double* qz, int64_t qs)
+{
+    float upnq = fmaf(npz, nqz, fmaf(npy, nqy, npx * nqx));
+    bem_c32f4 o;
+    o.L = o.M = o.Mt = o.N = c32(0.0f, 0.0f);
+#pragma unroll UNROLL
+    for (int q = 0; q < 16; q++)
+        point_accumulate_f32(&o, F, px, py, pz, npx, npy, npz, nqx, nqy, nqz, upnq, qx[q * qs], qy[q * qs],
+                             qz[q * qs], kTriWf[q]);
+    o.L = c32(o.L.re * area, o.L.im * area);
+    o.M = c32(o.M.re * area, o.M.im * area);
+    o.Mt = c32(o.Mt.re * area, o.Mt.im * area);
+    o.N = c32(o.N.re * area, o.N.im * area);
+    return o;
+}
+
+// off-diagonal Phi / DPhi entry from the FP32 operators (bem_phi_entry,
+// bem_dphi_entry)
+__device__ __forceinline__ bem_c32 bc_entry_f32(const DevFreq32& F, const int32_t* subst, int kind,
+                                                const bem_c32f4& o, int64_t j)
+{
+    bem_c32 X = c32_fma(F.cc, o.Mt, o.L);
+    bem_c32 Y = c32_fma(F.cc, o.N, o.M);
+    if (kind == 0) return c32_mul(F.cphi[j], subst[j] ? X : Y);
+    if (subst[j]) {
+        bem_c32 t = c32_mul(F.cd[j], X);
+        return c32(t.re - Y.re, t.im - Y.im);
+    }
+    return c32_fma(F.cd[j], Y, X);
+}
+
 }  // namespace bemdev

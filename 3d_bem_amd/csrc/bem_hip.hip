@@ -423,8 +423,8 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
 {
     const DevMesh& M = a.M;
     if (near_or_self(M, i, j)) return false;
-    bem_ops4 o = regular_ops<1>(a.F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j],
-                                M.nz[j], M.area[j], M.qx + j, M.qy + j, M.qz + j, M.n);
+    bem_ops4 o = regular_ops_vtx<1>(a.F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j],
+                                    M.nz[j], M.area[j], load_vtx(M, j));
     *e = bc_entry(a.F, a.kind, o, i, j);
     return true;
 }
@@ -432,9 +432,9 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
 {
     const DevMesh& M = a.M;
     if (near_or_self(M, i, j)) return false;
-    bem_c32f4 o = regular_ops_f32<1>(a.F32, M.px[i], M.py[i], M.pz[i], (float)M.nx[i], (float)M.ny[i],
-                                     (float)M.nz[i], (float)M.nx[j], (float)M.ny[j], (float)M.nz[j], (float)M.area[j],
-                                     M.qx + j, M.qy + j, M.qz + j, M.n);
+    bem_c32f4 o = regular_ops_f32_vtx<1>(a.F32, M.px[i], M.py[i], M.pz[i], (float)M.nx[i], (float)M.ny[i],
+                                         (float)M.nz[i], (float)M.nx[j], (float)M.ny[j], (float)M.nz[j],
+                                         (float)M.area[j], load_vtx(M, j));
     *e = bc_entry_f32(a.F32, a.F.subst, a.kind, o, j);
     return true;
 }
@@ -666,26 +666,42 @@ __global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tile
 }
 
 // low-rank blocks, phase 1 (K5a): t_b[l] = alpha_l (V_b[l, :] . x_J), one
-// wavefront per (block, l) task
+// wavefront per (block, l0) task covering rows l0 .. l0 + kLrRows - 1 of V_b:
+// each x element is loaded once for the group and the rows' loads are in
+// flight together
+constexpr int kLrRows = 4;
 template <class S>
 __global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks, int64_t ntask,
                                               const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
-                                              const int64_t* __restrict__ voff, const int64_t* __restrict__ aoff,
-                                              const int64_t* __restrict__ toff, const S* __restrict__ V,
-                                              const S* __restrict__ A, const bem_c64* __restrict__ x,
-                                              bem_c64* __restrict__ t)
+                                              const int32_t* __restrict__ rank, const int64_t* __restrict__ voff,
+                                              const int64_t* __restrict__ aoff, const int64_t* __restrict__ toff,
+                                              const S* __restrict__ V, const S* __restrict__ A,
+                                              const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
 {
     const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
     if (task >= ntask) return;
     const int lane = threadIdx.x & 63;
-    const int b = tasks[2 * task], l = tasks[2 * task + 1];
+    const int b = tasks[2 * task], l0 = tasks[2 * task + 1];
     const int64_t n = cols[b];
-    const S* v = V + voff[b] + (int64_t)l * n;
+    const int nl = min(kLrRows, rank[b] - l0);
+    const S* v = V + voff[b] + (int64_t)l0 * n;
     const bem_c64* xs = x + col0[b];
-    bem_c64 p = c64(0, 0);
-    for (int64_t j = lane; j < n; j += 64) p = c64_fma(ld64(v + j), xs[j], p);
-    p = wave_sum(p);
-    if (lane == 0) t[toff[b] + l] = c64_mul(ld64(A + aoff[b] + l), p);
+    bem_c64 p[kLrRows];
+#pragma unroll
+    for (int u = 0; u < kLrRows; u++) p[u] = c64(0, 0);
+    for (int64_t j = lane; j < n; j += 64) {
+        const bem_c64 xv = xs[j];
+#pragma unroll
+        for (int u = 0; u < kLrRows; u++)
+            if (u < nl) p[u] = c64_fma(ld64(v + (int64_t)u * n + j), xv, p[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kLrRows; u++) p[u] = wave_sum(p[u]);
+    if (lane == 0) {   // wave_sum leaves the full sums in lane 0
+#pragma unroll
+        for (int u = 0; u < kLrRows; u++)
+            if (u < nl) t[toff[b] + l0 + u] = c64_mul(ld64(A + aoff[b] + l0 + u), p[u]);
+    }
 }
 
 // low-rank blocks, phase 2 (K5b): partial_b[i] = sum_l U_b[i, l] t_b[l] over a
@@ -882,6 +898,39 @@ struct DBuf {
     size_t n = 0;
 };
 
+// Recycled host vectors: the per-assembly offset, rank and task arrays are
+// taken from here and given back, so repeated assemblies touch memory that is
+// already mapped instead of faulting in fresh pages every time.
+struct VecPool {
+    std::vector<std::vector<int64_t>> v64;
+    std::vector<std::vector<int32_t>> v32;
+    template <class T>
+    std::vector<std::vector<T>>& list();
+    template <class T>
+    std::vector<T> take(size_t n, T fill)
+    {
+        auto& L = list<T>();
+        std::vector<T> v;
+        if (!L.empty()) {
+            v = std::move(L.back());
+            L.pop_back();
+        }
+        v.assign(n, fill);
+        return v;
+    }
+    template <class T>
+    void give(std::vector<T>& v)
+    {
+        auto& L = list<T>();
+        if (L.size() < 64 && v.capacity()) L.push_back(std::move(v));
+        v = std::vector<T>();
+    }
+};
+template <>
+std::vector<std::vector<int64_t>>& VecPool::list<int64_t>() { return v64; }
+template <>
+std::vector<std::vector<int32_t>>& VecPool::list<int32_t>() { return v32; }
+
 }  // namespace
 
 struct bem_ctx {
@@ -950,6 +999,16 @@ struct bem_ctx {
     // calling hipMalloc/hipFree every time
     std::multimap<size_t, void*> pool;
     std::unordered_map<void*, size_t> live;
+    // pinned host staging for the per-assembly metadata uploads and the
+    // rank / flag downloads (one copy each instead of one per array)
+    char* stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t stage_done = nullptr;
+    bool stage_busy = false;
+    VecPool vp;
+    // GMRES: residual estimates of the last two Arnoldi columns
+    double* gm_res = nullptr;
+    hipEvent_t gm_ev[2] = {nullptr, nullptr};
 };
 
 struct bem_hmat {
@@ -958,6 +1017,7 @@ struct bem_hmat {
     size_t esz = sizeof(bem_c64);   // bytes per stored entry (FP64 or FP32 complex)
     void* dense = nullptr;
     void *U = nullptr, *V = nullptr, *A = nullptr;
+    void* meta = nullptr;    // one allocation holding the offset, rank and task arrays below
     int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
     int32_t* d_rank = nullptr;
     // two-phase low-rank MVM: (block, l) and (block, row chunk) work lists
@@ -1107,6 +1167,63 @@ void free_list(bem_ctx* c, std::vector<void*>& v)
     v.clear();
 }
 
+// pinned staging buffer of at least `bytes`, free for writing (the previous
+// transfer through it has completed)
+int stage_get(bem_ctx* c, size_t bytes, char** out)
+{
+    if (c->stage_busy) {
+        HIPC(c, hipEventSynchronize(c->stage_done));
+        c->stage_busy = false;
+    }
+    if (c->stage_cap < bytes) {
+        if (c->stage) (void)hipHostFree(c->stage);
+        c->stage = nullptr;
+        c->stage_cap = 0;
+        size_t cap = std::max(bytes, (size_t)1 << 20);
+        HIPC(c, hipHostMalloc((void**)&c->stage, cap, hipHostMallocDefault));
+        c->stage_cap = cap;
+    }
+    *out = c->stage;
+    return BEM_OK;
+}
+
+// Several host arrays uploaded into one device allocation with one copy
+// through the pinned staging buffer.  Pieces are 256-byte aligned.
+struct Packer {
+    std::vector<std::pair<const void*, size_t>> parts;
+    std::vector<size_t> offs;
+    size_t total = 0;
+    char* dbase = nullptr;
+    template <class T>
+    size_t add(const std::vector<T>& v)
+    {
+        size_t o = total;
+        parts.push_back({v.data(), v.size() * sizeof(T)});
+        offs.push_back(o);
+        total = (o + v.size() * sizeof(T) + 255) & ~(size_t)255;
+        return parts.size() - 1;
+    }
+    template <class T>
+    T* ptr(size_t k) const { return (T*)(dbase + offs[k]); }
+    int upload(bem_ctx* c, std::vector<void*>* owner)
+    {
+        int rc;
+        void* d = nullptr;
+        if ((rc = dalloc_bytes(c, &d, total ? total : 1))) return rc;
+        dbase = (char*)d;
+        if (owner) owner->push_back(d);
+        if (!total) return BEM_OK;
+        char* h = nullptr;
+        if ((rc = stage_get(c, total, &h))) return rc;
+        for (size_t k = 0; k < parts.size(); k++)
+            if (parts[k].second) std::memcpy(h + offs[k], parts[k].first, parts[k].second);
+        HIPC(c, hipMemcpyAsync(dbase, h, total, hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipEventRecord(c->stage_done, c->stream));
+        c->stage_busy = true;
+        return BEM_OK;
+    }
+};
+
 DevFreq dev_freq(const bem_ctx* c)
 {
     DevFreq F;
@@ -1159,7 +1276,11 @@ int bem_create(bem_ctx_t* out, int device)
         hipStreamCreateWithFlags(&c->side[1], hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->gm_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->gm_ev[1], hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&c->gm_res, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return BEM_ERR_HIP;
     }
@@ -1186,6 +1307,10 @@ int bem_destroy(bem_ctx_t c)
     for (hipStream_t t : c->side) { (void)hipStreamSynchronize(t); (void)hipStreamDestroy(t); }
     (void)hipEventDestroy(c->ev_fork);
     for (hipEvent_t e : c->ev_join) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->gm_ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c->stage_done);
+    if (c->stage) (void)hipHostFree(c->stage);
+    if (c->gm_res) (void)hipHostFree(c->gm_res);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return BEM_OK;
@@ -1230,9 +1355,13 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
         soa[3 * n + e] = normal3[3 * e]; soa[4 * n + e] = normal3[3 * e + 1]; soa[5 * n + e] = normal3[3 * e + 2];
         soa[6 * n + e] = area[e];
     }
-    double *d_soa, *d_nodes, *d_q;
+    std::vector<double> vtx(9 * n);
+    for (int64_t e = 0; e < n; e++)
+        for (int d = 0; d < 9; d++) vtx[d * n + e] = nodes9[9 * e + d];
+    double *d_soa, *d_nodes, *d_q, *d_vtx;
     int rc;
     if ((rc = dupload(c, &d_soa, soa.data(), soa.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_vtx, vtx.data(), vtx.size(), &c->allocs_mesh))) return rc;
     if ((rc = dupload(c, &d_nodes, c->h_nodes.data(), c->h_nodes.size(), &c->allocs_mesh))) return rc;
     if ((rc = dalloc(c, &d_q, (size_t)48 * n, &c->allocs_mesh))) return rc;
     DevMesh& M = c->mesh;
@@ -1241,6 +1370,7 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     M.nx = d_soa + 3 * n; M.ny = d_soa + 4 * n; M.nz = d_soa + 5 * n;
     M.area = d_soa + 6 * n;
     M.qx = d_q; M.qy = d_q + 16 * n; M.qz = d_q + 32 * n;
+    M.vtx = d_vtx;
     hipLaunchKernelGGL(k_quadpoints, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, d_nodes, d_q, d_q + 16 * n,
                        d_q + 32 * n);
     if ((rc = launch_check(c, "k_quadpoints"))) return rc;
@@ -1460,10 +1590,10 @@ int bem_hmat_destroy(bem_hmat_t h)
     (void)hipSetDevice(h->ctx->device);
     (void)hipStreamSynchronize(h->ctx->stream);
     for (hipStream_t t : h->ctx->side) (void)hipStreamSynchronize(t);
-    for (void* p : {(void*)h->dense, (void*)h->U, (void*)h->V, (void*)h->A, (void*)h->d_uoff, (void*)h->d_voff,
-                    (void*)h->d_aoff, (void*)h->d_rank, (void*)h->d_ttask, (void*)h->d_utask, (void*)h->d_toff,
-                    (void*)h->d_t})
+    for (void* p : {(void*)h->dense, (void*)h->U, (void*)h->V, (void*)h->A, h->meta, (void*)h->d_t})
         pfree(h->ctx, p);
+    for (auto* v : {&h->uoff, &h->voff, &h->aoff}) h->ctx->vp.give(*v);
+    h->ctx->vp.give(h->rank);
     delete h;
     return BEM_OK;
 }
@@ -1482,7 +1612,12 @@ struct AcaScratch {
     std::vector<int64_t> uoff, voff, aoff;
     std::vector<void*> owned;
     bem_ctx* ctx = nullptr;
-    ~AcaScratch() { if (ctx) free_list(ctx, owned); }
+    ~AcaScratch()
+    {
+        if (!ctx) return;
+        free_list(ctx, owned);
+        ctx->vp.give(uoff); ctx->vp.give(voff); ctx->vp.give(aoff);
+    }
 };
 
 // one size class of blocks on stream st, storage type C
@@ -1509,7 +1644,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
 {
     int64_t nb = (int64_t)c->blocks.size();
     S.ctx = c;
-    S.uoff.assign(nb, 0); S.voff.assign(nb, 0); S.aoff.assign(nb, 0);
+    S.uoff = c->vp.take<int64_t>(nb, 0); S.voff = c->vp.take<int64_t>(nb, 0); S.aoff = c->vp.take<int64_t>(nb, 0);
     int64_t tu = 0, tv = 0, ta = 0;
     int maxcap = 1;
     for (int32_t b : list) {
@@ -1526,17 +1661,32 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     S.owned.push_back(S.V);
     if ((rc = dalloc_bytes(c, &S.A, ta * esz))) return rc;
     S.owned.push_back(S.A);
-    if ((rc = dupload(c, &S.d_uoff, S.uoff.data(), nb, &S.owned))) return rc;
-    if ((rc = dupload(c, &S.d_voff, S.voff.data(), nb, &S.owned))) return rc;
-    if ((rc = dupload(c, &S.d_aoff, S.aoff.data(), nb, &S.owned))) return rc;
-    if ((rc = dupload(c, &S.d_cap, cap.data(), nb, &S.owned))) return rc;
-    if ((rc = dupload(c, &S.d_list, list.data(), list.size(), &S.owned))) return rc;
-    if ((rc = dalloc(c, &S.d_rank, nb, &S.owned))) return rc;
-    if ((rc = dalloc(c, &S.d_flag, nb, &S.owned))) return rc;
-    if ((rc = dalloc(c, &S.d_ev, nb, &S.owned))) return rc;
-    HIPC(c, hipMemsetAsync(S.d_rank, 0, nb * sizeof(int32_t), c->stream));
-    HIPC(c, hipMemsetAsync(S.d_flag, 0, nb * sizeof(int32_t), c->stream));
-    HIPC(c, hipMemsetAsync(S.d_ev, 0, nb * sizeof(int64_t), c->stream));
+    // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock four,
+    // larger eight.  The huge and large classes run on their own streams so
+    // that their long tails overlap the small blocks (and the caller's dense
+    // tiles); within each class the largest blocks start first.
+    std::vector<int32_t> cls[3];
+    for (int32_t b : list) {
+        int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
+        cls[s2 > kHugeBlock ? 0 : (s2 > kSmallBlock ? 1 : 2)].push_back(b);
+    }
+    std::vector<int32_t> ordered;
+    for (auto& v : cls) ordered.insert(ordered.end(), v.begin(), v.end());
+    // offsets, capacities and the block list in one upload; ranks, flags and
+    // evaluation counts in one allocation written by the kernel for every
+    // listed block (the host reads listed blocks only)
+    Packer pk;
+    size_t k_uoff = pk.add(S.uoff), k_voff = pk.add(S.voff), k_aoff = pk.add(S.aoff);
+    size_t k_cap = pk.add(cap), k_list = pk.add(ordered);
+    if ((rc = pk.upload(c, &S.owned))) return rc;
+    S.d_uoff = pk.ptr<int64_t>(k_uoff); S.d_voff = pk.ptr<int64_t>(k_voff); S.d_aoff = pk.ptr<int64_t>(k_aoff);
+    S.d_cap = pk.ptr<int32_t>(k_cap); S.d_list = pk.ptr<int32_t>(k_list);
+    void* res = nullptr;
+    if ((rc = dalloc_bytes(c, &res, (size_t)nb * 16))) return rc;
+    S.owned.push_back(res);
+    S.d_ev = (int64_t*)res;
+    S.d_rank = (int32_t*)((char*)res + (size_t)nb * 8);
+    S.d_flag = S.d_rank + nb;
     AcaArgs a;
     a.M = c->mesh;
     a.F = dev_freq(c);
@@ -1553,20 +1703,6 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     // dynamic LDS of aca_block: sw[cap] of the storage type, then the FP64
     // dot products sdots[2 cap + 2]
     size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
-    // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock four,
-    // larger eight.  The huge and large classes run on their own streams so
-    // that their long tails overlap the small blocks (and the caller's dense
-    // tiles); within each class the largest blocks start first.
-    std::vector<int32_t> cls[3];
-    for (int32_t b : list) {
-        int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
-        cls[s2 > kHugeBlock ? 0 : (s2 > kSmallBlock ? 1 : 2)].push_back(b);
-    }
-    std::vector<int32_t> ordered;
-    for (auto& v : cls) ordered.insert(ordered.end(), v.begin(), v.end());
-    if (!ordered.empty())
-        HIPC(c, hipMemcpyAsync(S.d_list, ordered.data(), ordered.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                               c->stream));
     HIPC(c, hipEventRecord(c->ev_fork, c->stream));
     size_t off = 0;
     for (int q = 0; q < 3; q++) {
@@ -1587,11 +1723,14 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
         HIPC(c, hipEventRecord(c->ev_join[q], c->side[q]));
         HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join[q], 0));
     }
-    rank.assign(nb, 0); flag.assign(nb, 0); ev.assign(nb, 0);
-    HIPC(c, hipMemcpyAsync(rank.data(), S.d_rank, nb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipMemcpyAsync(flag.data(), S.d_flag, nb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipMemcpyAsync(ev.data(), S.d_ev, nb * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    rank = c->vp.take<int32_t>(nb, 0); flag = c->vp.take<int32_t>(nb, 0); ev = c->vp.take<int64_t>(nb, 0);
+    char* h = nullptr;
+    if ((rc = stage_get(c, (size_t)nb * 16, &h))) return rc;
+    HIPC(c, hipMemcpyAsync(h, res, (size_t)nb * 16, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
+    std::memcpy(ev.data(), h, (size_t)nb * 8);
+    std::memcpy(rank.data(), h + (size_t)nb * 8, (size_t)nb * 4);
+    std::memcpy(flag.data(), h + (size_t)nb * 12, (size_t)nb * 4);
     return BEM_OK;
 }
 
@@ -1692,7 +1831,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     }
     // ACA, first pass with a modest rank capacity, then the overflowing
     // blocks again with their full capacity
-    std::vector<int32_t> cap(nb, 0);
+    std::vector<int32_t> cap = c->vp.take<int32_t>(nb, 0);
     auto full_cap = [&](int32_t b) {
         int64_t m = std::min(c->blocks[b].rows, c->blocks[b].cols);
         if (maxrank > 0) m = std::min(m, maxrank);
@@ -1716,7 +1855,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         (flag[b] == 1 ? over : keep).push_back(b);
     }
     if (!over.empty()) {
-        std::vector<int32_t> cap2(nb, 0);
+        std::vector<int32_t> cap2 = c->vp.take<int32_t>(nb, 0);
         for (int32_t b : over) cap2[b] = std::min<int32_t>(full_cap(b), kCapMax);
         if ((rc = run_aca(c, kind, precision, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, [] { return BEM_OK; }, S2, rank2,
                           flag2, evs2)))
@@ -1732,10 +1871,12 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
                                    (flag2[b] == 2 ? " non-finite" : " needs rank > 1024; lower the tolerance or set max_rank"));
             }
         }
+        c->vp.give(cap2);
     }
     free_list(c, tmp_owned);
     // packed storage at the final ranks
-    h->uoff.assign(nb, 0); h->voff.assign(nb, 0); h->aoff.assign(nb, 0); h->rank.assign(nb, -1);
+    h->uoff = c->vp.take<int64_t>(nb, 0); h->voff = c->vp.take<int64_t>(nb, 0);
+    h->aoff = c->vp.take<int64_t>(nb, 0); h->rank = c->vp.take<int32_t>(nb, -1);
     int64_t tu = 0, tv = 0, ta = 0;
     double maxr = 0, sumr = 0;
     int64_t evaluated = 0;
@@ -1753,39 +1894,47 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     if ((rc = dalloc_bytes(c, &h->U, (size_t)tu * h->esz))) return fail(rc);
     if ((rc = dalloc_bytes(c, &h->V, (size_t)tv * h->esz))) return fail(rc);
     if ((rc = dalloc_bytes(c, &h->A, (size_t)ta * h->esz))) return fail(rc);
-    if ((rc = dupload(c, &h->d_uoff, h->uoff.data(), nb, nullptr))) return fail(rc);
-    if ((rc = dupload(c, &h->d_voff, h->voff.data(), nb, nullptr))) return fail(rc);
-    if ((rc = dupload(c, &h->d_aoff, h->aoff.data(), nb, nullptr))) return fail(rc);
-    std::vector<int32_t> rk(nb);
-    for (int64_t b = 0; b < nb; b++) rk[b] = rank[b];
-    int32_t* d_srank;
+    // offsets, ranks, MVM task lists and the packing lists: one upload
+    std::vector<int32_t>& rk = rank;   // first-pass ranks with the re-run ranks merged in
     std::vector<void*> own2;
-    if ((rc = dupload(c, &d_srank, rk.data(), nb, &own2))) return fail(rc);
-    if ((rc = dupload(c, &h->d_rank, h->rank.data(), nb, nullptr))) return fail(rc);
-    {
-        std::vector<int32_t> tt, ut;
-        std::vector<int64_t> toff(nb, 0);
-        int64_t tot = 0;
-        for (int32_t b : c->adm_list) {
-            int r = h->rank[b];
-            toff[b] = tot;
-            tot += r;
-            h->maxrank = std::max(h->maxrank, r);
-            for (int l = 0; l < r; l++) { tt.push_back(b); tt.push_back(l); }
-            for (int64_t q = 0; q * kWG < c->blocks[b].rows; q++) { ut.push_back(b); ut.push_back((int32_t)q); }
-        }
-        h->nttask = (int64_t)tt.size() / 2;
-        h->nutask = (int64_t)ut.size() / 2;
-        if ((rc = dupload(c, &h->d_ttask, tt.data(), tt.size(), nullptr))) return fail(rc);
-        if ((rc = dupload(c, &h->d_utask, ut.data(), ut.size(), nullptr))) return fail(rc);
-        if ((rc = dupload(c, &h->d_toff, toff.data(), nb, nullptr))) return fail(rc);
-        if ((rc = dalloc(c, &h->d_t, tot, nullptr))) return fail(rc);
+    // (block, l) tasks of the first MVM phase, (block, row chunk) tasks of
+    // the second; sized first, then filled by index
+    std::vector<int64_t> toff = c->vp.take<int64_t>(nb, 0);
+    int64_t tot = 0, ntt = 0, nchunks = 0;
+    for (int32_t b : c->adm_list) {
+        tot += h->rank[b];
+        ntt += (h->rank[b] + kLrRows - 1) / kLrRows;
+        nchunks += (c->blocks[b].rows + kWG - 1) / kWG;
     }
-    auto compact = [&](const std::vector<int32_t>& lst, const AcaScratch& Sx) -> int {
+    std::vector<int32_t> tt = c->vp.take<int32_t>((size_t)2 * ntt, 0), ut = c->vp.take<int32_t>((size_t)2 * nchunks, 0);
+    {
+        int64_t pt = 0, pu = 0, acc = 0;
+        for (int32_t b : c->adm_list) {
+            const int r = h->rank[b];
+            toff[b] = acc;
+            acc += r;
+            h->maxrank = std::max(h->maxrank, r);
+            for (int l = 0; l < r; l += kLrRows, pt++) { tt[2 * pt] = b; tt[2 * pt + 1] = l; }
+            const int64_t nq = (c->blocks[b].rows + kWG - 1) / kWG;
+            for (int64_t q = 0; q < nq; q++, pu++) { ut[2 * pu] = b; ut[2 * pu + 1] = (int32_t)q; }
+        }
+    }
+    h->nttask = ntt;
+    h->nutask = nchunks;
+    Packer pk;
+    size_t k_uoff = pk.add(h->uoff), k_voff = pk.add(h->voff), k_aoff = pk.add(h->aoff);
+    size_t k_rank = pk.add(h->rank), k_srank = pk.add(rk), k_tt = pk.add(tt), k_ut = pk.add(ut);
+    size_t k_toff = pk.add(toff), k_keep = pk.add(keep), k_over = pk.add(over);
+    rc = pk.upload(c, nullptr);
+    h->meta = pk.dbase;   // owned by h from here on, also on failure
+    if (rc) return fail(rc);
+    h->d_uoff = pk.ptr<int64_t>(k_uoff); h->d_voff = pk.ptr<int64_t>(k_voff); h->d_aoff = pk.ptr<int64_t>(k_aoff);
+    h->d_rank = pk.ptr<int32_t>(k_rank);
+    int32_t* d_srank = pk.ptr<int32_t>(k_srank);
+    h->d_ttask = pk.ptr<int32_t>(k_tt); h->d_utask = pk.ptr<int32_t>(k_ut); h->d_toff = pk.ptr<int64_t>(k_toff);
+    if ((rc = dalloc(c, &h->d_t, tot, nullptr))) return fail(rc);
+    auto compact = [&](const std::vector<int32_t>& lst, const int32_t* d_lst, const AcaScratch& Sx) -> int {
         if (lst.empty()) return BEM_OK;
-        int32_t* d_lst = nullptr;
-        int rc2;
-        if ((rc2 = dupload(c, &d_lst, lst.data(), lst.size(), &own2))) return rc2;
         dim3 g((unsigned)lst.size());
         if (f32)
             hipLaunchKernelGGL(k_compact<bem_c32>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
@@ -1799,8 +1948,8 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
                                (bem_c64*)h->V, (bem_c64*)h->A);
         return launch_check(c, "k_compact");
     };
-    if ((rc = compact(keep, S1))) return fail(rc);
-    if ((rc = compact(over, S2))) return fail(rc);
+    if ((rc = compact(keep, pk.ptr<int32_t>(k_keep), S1))) return fail(rc);
+    if ((rc = compact(over, pk.ptr<int32_t>(k_over), S2))) return fail(rc);
     HIPC(c, hipEventRecord(ev2, c->stream));
     // NaN check of the dense part (the reference reports NaN blocks,
     // hmatrix.cpp:812-820)
@@ -1836,6 +1985,8 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->stats[5] = (double)over.size();
     h->stats[6] = ms_dense;
     h->stats[7] = ms_aca;
+    for (auto* v : {&cap, &rank, &flag, &rank2, &flag2, &tt, &ut}) c->vp.give(*v);
+    for (auto* v : {&evs, &evs2, &toff}) c->vp.give(*v);
     *out = h;
     return BEM_OK;
 }
@@ -1858,8 +2009,8 @@ int hmvm_blocks(bem_ctx* c, bem_hmat* h, const bem_c64* xd)
     if (h->nttask > 0) {
         Timer t(c, 4);
         hipLaunchKernelGGL(k_lr_t<S>, dim3(nblk(h->nttask, kWG / 64)), dim3(kWG), 0, c->stream, h->d_ttask,
-                           h->nttask, c->d_col0, c->d_cols, h->d_voff, h->d_aoff, h->d_toff, (const S*)h->V,
-                           (const S*)h->A, xd, h->d_t);
+                           h->nttask, c->d_col0, c->d_cols, h->d_rank, h->d_voff, h->d_aoff, h->d_toff,
+                           (const S*)h->V, (const S*)h->A, xd, h->d_t);
         hipLaunchKernelGGL(k_lr_u<S>, dim3((unsigned)h->nutask), dim3(kWG), h->maxrank * sizeof(bem_c64), c->stream,
                            h->d_utask, c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part, (const S*)h->U,
                            h->d_t, c->d_part);
@@ -2017,10 +2168,12 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
         std::fill(g.begin(), g.end(), c64(0, 0));
         g[0] = c64(beta, 0);
         HIPC(c, hipMemcpyAsync(dg, g.data(), (m + 2) * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
-        int64_t j = 0;
-        for (; j < m && it < maxit;) {
-            if ((rc = bem_hmvm(c, h, V + j * n, w, 0))) return rc;
-            const int cnt = (int)(j + 1);
+        // Arnoldi column jj (basis vector jj + 1, Givens rotation jj); its
+        // residual estimate is copied to the pinned ring slot jj & 1
+        auto launch_col = [&](int64_t jj) -> int {
+            int rc2;
+            if ((rc2 = bem_hmvm(c, h, V + jj * n, w, 0))) return rc2;
+            const int cnt = (int)(jj + 1);
             for (int pass = 0; pass < 2; pass++) {   // classical Gram-Schmidt, twice
                 hipLaunchKernelGGL(k_cgs_dots, dim3(nchunk, (cnt + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0,
                                    c->stream, n, cnt, V, w, dpart);
@@ -2030,14 +2183,36 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
             }
             norm_dev(w);
             hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, w, dnorm,
-                               V + (j + 1) * n);
-            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, c->stream, (int)j, (int)m, dhacc, dnorm, dH, dcs_c,
+                               V + (jj + 1) * n);
+            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, c->stream, (int)jj, (int)m, dhacc, dnorm, dH, dcs_c,
                                dcs_s, dg, bn, dres);
-            if ((rc = launch_check(c, "arnoldi")) || (rc = fetch(&rel, dres))) return rc;
+            if ((rc2 = launch_check(c, "arnoldi"))) return rc2;
+            HIPC(c, hipMemcpyAsync(&c->gm_res[jj & 1], dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPC(c, hipEventRecord(c->gm_ev[jj & 1], c->stream));
+            return BEM_OK;
+        };
+        // One column is kept in flight ahead of the convergence check, so the
+        // host round trip overlaps device work.  A column launched after the
+        // converged one only writes basis vector jj + 1, column jj of H and
+        // entries jj, jj + 1 of g, none of which the solution of the first
+        // jj columns reads: the result is the same as without look-ahead.
+        const int64_t cap = std::min<int64_t>(m, maxit - it);
+        int64_t j = 0, nl = 0;
+        if (cap > 0) {
+            if ((rc = launch_col(0))) return rc;
+            nl = 1;
+        }
+        while (j < nl) {
+            if (nl < cap) {
+                if ((rc = launch_col(nl))) return rc;
+                nl++;
+            }
+            HIPC(c, hipEventSynchronize(c->gm_ev[j & 1]));
+            rel = c->gm_res[j & 1];
             j++;
-            it++;
             if (rel < tol) break;
         }
+        it += j;
         // y = R^-1 g on the host, x += V y
         HIPC(c, hipMemcpyAsync(H.data(), dH, (m + 1) * j * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipMemcpyAsync(g.data(), dg, (j + 1) * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));

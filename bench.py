@@ -8,10 +8,18 @@ GMRES to tolerance, boundary-condition reconstruction and sound pressure
 partition and near-singular geometry are set up once, before the timed
 region, and reported as setup.
 
-Prints one JSON line (rank 0).  --gpus N > 1 is refused: this build runs on
-one GPU.
+The headline is the mixed-precision form (--precision fp32, default):
+regular quadrature, ACA and block storage in complex FP32, self and
+near-singular entries in FP64, GMRES in FP64 over the FP32 operator with
+the restart residual recomputed with FP64 accumulation.  The plain FP64 path
+is timed beside it on the same workload (fp64_value, speedup_vs_fp64) and
+the surface pressure of both is compared with the FP64 CPU restatement run
+on the host (cpu_baseline leg).
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 import argparse
+import ctypes
 import importlib
 import json
 import os
@@ -23,16 +31,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-PEAK_FP32_TFLOPS = 157.3
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}   # MI355X vector peaks
 PEAK_HBM_GBS = 8000.0
-# algorithmic flop counts per entry (each sqrt, sin, cos, div counts as one)
+# algorithmic flop count of one regular entry (16 points x 95 + 40; sqrt,
+# sin, cos and div count as one flop each)
 F_REG = 16 * 95 + 40
-F_SELF = 3 * 32 * 12 + 60
-
-
-def f_near(nsub):
-    return nsub * 25 * 95 + 150
+FAMILIES = ["dense_near_field", "singular_table", "aca", "mvm_dense", "mvm_lowrank", "mvm_gather"]
 
 
 def workload(level):
@@ -43,7 +47,8 @@ def workload(level):
 
 
 def run_cpu_baseline(tri, bc, args):
-    """The CPU restatement (oracle/) on the same workload, all host threads."""
+    """The CPU restatement (oracle/) on the same workload, all host threads:
+    one full frequency point."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -56,35 +61,9 @@ def run_cpu_baseline(tri, bc, args):
     return pr, t, threads, st
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--level", type=int, default=5)
-    ap.add_argument("--freq", type=float, default=1000.0)
-    ap.add_argument("--leaf", type=int, default=64)
-    ap.add_argument("--eta", type=float, default=2.0)
-    ap.add_argument("--aca-tol", type=float, default=1e-5)
-    ap.add_argument("--gmres-tol", type=float, default=1e-8)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
-        sys.exit(2)
-
-    bem, tri, bc = workload(args.level)
-    n = len(tri)
-    prec = bem.FP64 if args.precision == "fp64" else bem.FP32
-    t0 = time.perf_counter()
-    S = bem.Solver(tri, bc, leaf=args.leaf, eta=args.eta)
-    setup_s = time.perf_counter() - t0
-    _, blocks = S.layout()
-    ctx_h = bem.lib().bem_solver_context(S.h)
-
+def timed_run(bem, S, ctx_h, args, prec):
+    """warmup + timed steps of one precision; kernel family times from HIP
+    events recorded on the launching streams"""
     def solve():
         return S.solve(args.freq, c=343.0, rho=1.21, coupling=1, aca_tol=args.aca_tol, gmres_tol=args.gmres_tol,
                        seed=args.seed, precision=prec)
@@ -92,51 +71,120 @@ def main():
     for _ in range(args.warmup):
         solve()
     prof = np.zeros(12)
-    bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(__import__("ctypes").c_void_p))
+    bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(ctypes.c_void_p))
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         _, _, pr, st = solve()
         stats.append(st)
     elapsed = time.perf_counter() - t0
-    bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(__import__("ctypes").c_void_p))
-    avg_ms, cnt = prof[:6], prof[6:]
-    families = ["dense_assembly", "near_singular_fixup", "aca", "mvm_dense", "mvm_lowrank", "mvm_gather"]
+    bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(ctypes.c_void_p))
+    return pr, stats, elapsed, prof[:6].copy(), prof[6:].copy()
+
+
+def roofline(avg_ms, cnt, stats, blocks, n, prec_name, steps):
+    """Dominant kernel family (largest total device time) against its bound.
+    Assembly families: algorithmic flops (F_REG per regular entry) over the
+    family's mean launch duration, vector-ALU peak of the precision.  MVM
+    families: algorithmic bytes of the stored operator over the duration."""
     tot = avg_ms * cnt
     dom = int(np.argmax(tot))
     st = stats[-1]
-
-    # algorithmic work per launch of each family
+    s = 16 if prec_name == "fp64" else 8
     dense = blocks[blocks[:, 4] == 0]
     n_dense = int((dense[:, 1] * dense[:, 3]).sum())
-    nb = [len(blocks)]
-    rl = {}
-    if dom in (0, 1, 2):
+    launches = cnt[dom] / steps          # launches of this family per frequency point
+    if dom in (0, 2):
         if dom == 0:
-            flops = F_REG * (n_dense - n)  # regular entries of the dense blocks (self/near done by the fix-up)
-        elif dom == 2:
-            ev = (st["evaluated_phi"] + st["evaluated_dphi"]) / 2 - n_dense
-            flops = F_REG * ev
+            entries = 2 * (n_dense - n)  # regular entries of the Phi and DPhi near field
         else:
-            flops = F_SELF * n + f_near(2.5) * 12 * n
-        peak = PEAK_FP64_TFLOPS if prec == bem.FP64 else PEAK_FP32_TFLOPS
+            entries = st["evaluated_phi"] + st["evaluated_dphi"] - 2 * n_dense
+        flops = F_REG * entries / launches
         ach = flops / (avg_ms[dom] * 1e-3) / 1e12
+        peak = PEAK_TFLOPS[prec_name]
         rl = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-              "frac": round(ach / peak, 4), "traffic": None}
+              "frac": round(ach / peak, 4), "traffic": None,
+              "algorithmic_per_launch": f"{flops:.4g} flop ({entries / launches:.4g} regular entries x {F_REG})"}
+    elif dom == 1:
+        rl = {"bound": "mfma", "achieved": None, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": None,
+              "traffic": None}
     else:
-        s = 16 if prec == bem.FP64 else 8
         if dom == 3:
-            byts = s * n_dense + 16 * n_dense / 64
+            byts = s * n_dense
         elif dom == 4:
             byts = st["dphi_bytes"] - s * n_dense
         else:
             byts = 16 * 2 * n
         ach = byts / (avg_ms[dom] * 1e-3) / 1e9
         rl = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-              "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None}
-    rl["kernel"] = families[dom]
+              "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+              "algorithmic_per_launch": f"{byts:.4g} B"}
+    rl["kernel"] = FAMILIES[dom]
     rl["avg_ms"] = round(float(avg_ms[dom]), 4)
+    rl["launches_per_point"] = round(float(launches), 2)
+    return rl
 
+
+def copy_calibration(mib):
+    """hipMemcpy device-to-device of mib MiB, five times: the runtime's plain
+    copy kernel reads and writes exactly mib MiB per launch"""
+    bem = importlib.import_module("3d_bem_amd")
+    hip = ctypes.CDLL("libamdhip64.so")
+    ctx = bem.Context(0)
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if bem.lib().bem_malloc(ctx.h, ctypes.c_size_t(nbytes), ctypes.byref(p)) != 0:
+            raise RuntimeError("bem_malloc failed")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    for _ in range(5):
+        rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    for p in (src, dst):
+        bem.lib().bem_free(ctx.h, p)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--freq", type=float, default=1000.0)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--eta", type=float, default=2.0)
+    ap.add_argument("--aca-tol", type=float, default=1e-5)
+    ap.add_argument("--gmres-tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (the plain copy kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
+        sys.exit(2)
+
+    bem, tri, bc = workload(args.level)
+    n = len(tri)
+    t0 = time.perf_counter()
+    S = bem.Solver(tri, bc, leaf=args.leaf, eta=args.eta)
+    setup_s = time.perf_counter() - t0
+    _, blocks = S.layout()
+    ctx_h = bem.lib().bem_solver_context(S.h)
+    prec = bem.FP32 if args.precision == "fp32" else bem.FP64
+
+    pr, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec)
+    print(f"# {args.precision}: {1e3 * elapsed / args.steps:.2f} ms/point", file=sys.stderr, flush=True)
+    st = stats[-1]
     out = {
         "metric": "freq-points solved/s at 20k tris (FP32+refine vs FP64); rel-L2 err vs FP64 oracle",
         "value": round(args.steps / elapsed, 4),
@@ -148,7 +196,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == bem.FP64 else "f32+f64",
+        "dtype": "c64+f64" if prec == bem.FP32 else "c128",
         "data": "synthetic",
         "config": {"workload": f"pulsating sphere, icosphere level {args.level} ({n} triangles), r=0.5 m, "
                                f"{args.freq:g} Hz, Burton-Miller, leaf {args.leaf}, eta {args.eta:g}, "
@@ -156,17 +204,28 @@ def main():
                    "triangles": n, "setup_s": round(setup_s, 3),
                    "gmres_iters": st["gmres_iters"], "residual": st["residual"],
                    "max_rank_phi": st["max_rank_phi"], "max_rank_dphi": st["max_rank_dphi"],
+                   "evaluated_entries": [st["evaluated_phi"], st["evaluated_dphi"]],
                    "phase_ms": {k: round(st[k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
-                   "kernel_ms": {f: [round(float(a), 4), int(c)] for f, a, c in zip(families, avg_ms, cnt)}},
-        "roofline": rl,
+                   "kernel_ms": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
+        "roofline": roofline(avg_ms, cnt, stats, blocks, n, args.precision, args.steps),
     }
+    pr64 = None
+    if not args.no_fp64_leg and prec != bem.FP64:
+        pr64, stats64, el64, _, _ = timed_run(bem, S, ctx_h, args, bem.FP64)
+        print(f"# fp64: {1e3 * el64 / args.steps:.2f} ms/point", file=sys.stderr, flush=True)
+        out["fp64_value"] = round(args.steps / el64, 4)
+        out["fp64_ms_per_step"] = round(1e3 * el64 / args.steps, 3)
+        out["speedup_vs_fp64"] = round(el64 / elapsed, 3)
+        out["rel_l2_vs_fp64_gpu"] = float(np.linalg.norm(pr - pr64) / np.linalg.norm(pr64))
     if not args.no_cpu_baseline:
         pr_cpu, t_cpu, threads, st_cpu = run_cpu_baseline(tri, bc, args)
         out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 5), "unit": "freq-points/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"one full frequency point of the same workload ({n} triangles) with the "
-                                         f"CPU restatement, {t_cpu:.1f} s"}
-        out["rel_l2_vs_cpu"] = float(np.linalg.norm(pr - pr_cpu) / np.linalg.norm(pr_cpu))
+                               "sample": f"one full frequency point of the same workload ({n} triangles, FP64, "
+                                         f"set-up included) with the CPU restatement: {t_cpu:.1f} s"}
+        out["rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(pr - pr_cpu) / np.linalg.norm(pr_cpu))
+        if pr64 is not None:
+            out["fp64_rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(pr64 - pr_cpu) / np.linalg.norm(pr_cpu))
     print(json.dumps(out))
 
 

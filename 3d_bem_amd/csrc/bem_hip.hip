@@ -35,8 +35,10 @@ constexpr int kWG = 256;        // workgroup size of the block kernels (4 wavefr
 constexpr int kTile = 64;       // dense tile edge
 constexpr int kCapFirst = 48;   // first-pass ACA rank capacity per block
 constexpr int kCapMax = 1024;   // second-pass capacity limit (LDS: (3 cap + 2) x 16 B <= 48 KiB)
-constexpr int64_t kSmallBlock = 384;   // m + n up to which an ACA block gets one wavefront
-constexpr int64_t kHugeBlock = 1024;   // m + n above which an ACA block gets eight wavefronts
+// ACA workgroup size classes by m + n, per storage precision [FP64, FP32]:
+// up to kSmallBlock one wavefront, up to kHugeBlock four, above eight
+constexpr int64_t kSmallBlock[2] = {384, 384};
+constexpr int64_t kHugeBlock[2] = {(int64_t)1 << 40, 2048};
 constexpr double kTiny = 10.0 * 2.220446049250313e-16;  // global.h:53
 
 struct Tile {
@@ -747,34 +749,51 @@ __global__ void k_mvm_gather(int64_t n, const int64_t* __restrict__ gptr, const 
 
 // ------------------------------------------------------------- GMRES
 
-constexpr int64_t kDotChunk = 4096;   // rows per workgroup of the Arnoldi dot products
+constexpr int64_t kDotChunk = 128;   // rows per wavefront of the Arnoldi dot products
+constexpr int kDotVecs = 8;          // basis vectors per wavefront (w loaded once for them)
 
 // part[l * nchunk + chunk] = sum over the chunk's rows of conj(V_l) w; one
-// wavefront per (chunk, l)
+// wavefront per (chunk, group of kDotVecs basis vectors)
 __global__ __launch_bounds__(kWG) void k_cgs_dots(int64_t n, int cnt, const bem_c64* __restrict__ V,
                                                   const bem_c64* __restrict__ w, bem_c64* __restrict__ part)
 {
-    const int l = blockIdx.y * (kWG / 64) + (threadIdx.x >> 6);
-    if (l >= cnt) return;
+    const int l0 = (blockIdx.y * (kWG / 64) + (threadIdx.x >> 6)) * kDotVecs;
+    if (l0 >= cnt) return;
     const int lane = threadIdx.x & 63;
+    const int nv = min(kDotVecs, cnt - l0);
     const int64_t i0 = (int64_t)blockIdx.x * kDotChunk, i1 = i0 + kDotChunk < n ? i0 + kDotChunk : n;
-    const bem_c64* v = V + (int64_t)l * n;
-    bem_c64 p = c64(0, 0);
-    for (int64_t i = i0 + lane; i < i1; i += 64) p = c64_fma(c64_conj(v[i]), w[i], p);
-    p = wave_sum(p);
-    if (lane == 0) part[(int64_t)l * gridDim.x + blockIdx.x] = p;
+    bem_c64 p[kDotVecs];
+#pragma unroll
+    for (int u = 0; u < kDotVecs; u++) p[u] = c64(0, 0);
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
+        const bem_c64 wi = w[i];
+#pragma unroll
+        for (int u = 0; u < kDotVecs; u++)
+            if (u < nv) p[u] = c64_fma(c64_conj(V[(int64_t)(l0 + u) * n + i]), wi, p[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kDotVecs; u++) p[u] = wave_sum(p[u]);
+    if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < kDotVecs; u++)
+            if (u < nv) part[(int64_t)(l0 + u) * gridDim.x + blockIdx.x] = p[u];
+    }
 }
 
-// h[l] = sum over chunks (fixed order); hacc[l] = h[l] (first pass) or += h[l]
-__global__ void k_cgs_reduce(int cnt, int nchunk, const bem_c64* __restrict__ part, bem_c64* h, bem_c64* hacc,
-                             int first)
+// h[l] = sum over chunks (lane-strided, then the wavefront tree: a fixed
+// order); hacc[l] = h[l] (first pass) or += h[l].  One wavefront per l.
+__global__ __launch_bounds__(64) void k_cgs_reduce(int cnt, int nchunk, const bem_c64* __restrict__ part, bem_c64* h,
+                                                   bem_c64* hacc, int first)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.x;
     if (l >= cnt) return;
-    bem_c64 s = part[(int64_t)l * nchunk];
-    for (int q = 1; q < nchunk; q++) s = c64_add(s, part[(int64_t)l * nchunk + q]);
-    h[l] = s;
-    hacc[l] = first ? s : c64_add(hacc[l], s);
+    bem_c64 s = c64(0, 0);
+    for (int q = threadIdx.x; q < nchunk; q += 64) s = c64_add(s, part[(int64_t)l * nchunk + q]);
+    s = wave_sum(s);
+    if (threadIdx.x == 0) {
+        h[l] = s;
+        hacc[l] = first ? s : c64_add(hacc[l], s);
+    }
 }
 
 // One Arnoldi column on the device: h[0..j] = hacc, h[j+1] = |w|; apply the
@@ -817,7 +836,15 @@ __global__ void k_orth_update(int64_t n, int cnt, const bem_c64* V, const bem_c6
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     bem_c64 acc = w[i];
-    for (int l = 0; l < cnt; l++) acc = c64_fms(h[l], V[(int64_t)l * n + i], acc);
+    int l = 0;
+    for (; l + 8 <= cnt; l += 8) {   // loads first, then the products in order
+        bem_c64 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = V[(int64_t)(l + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = c64_fms(h[l + u], v[u], acc);
+    }
+    for (; l < cnt; l++) acc = c64_fms(h[l], V[(int64_t)l * n + i], acc);
     w[i] = acc;
 }
 
@@ -1668,7 +1695,8 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     std::vector<int32_t> cls[3];
     for (int32_t b : list) {
         int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
-        cls[s2 > kHugeBlock ? 0 : (s2 > kSmallBlock ? 1 : 2)].push_back(b);
+        const int pi = prec == BEM_PREC_FP32 ? 1 : 0;
+        cls[s2 > kHugeBlock[pi] ? 0 : (s2 > kSmallBlock[pi] ? 1 : 2)].push_back(b);
     }
     std::vector<int32_t> ordered;
     for (auto& v : cls) ordered.insert(ordered.end(), v.begin(), v.end());
@@ -2175,9 +2203,10 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
             if ((rc2 = bem_hmvm(c, h, V + jj * n, w, 0))) return rc2;
             const int cnt = (int)(jj + 1);
             for (int pass = 0; pass < 2; pass++) {   // classical Gram-Schmidt, twice
-                hipLaunchKernelGGL(k_cgs_dots, dim3(nchunk, (cnt + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0,
+                const int ngroups = (cnt + kDotVecs - 1) / kDotVecs;
+                hipLaunchKernelGGL(k_cgs_dots, dim3(nchunk, (ngroups + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0,
                                    c->stream, n, cnt, V, w, dpart);
-                hipLaunchKernelGGL(k_cgs_reduce, dim3(nblk(cnt, 64)), dim3(64), 0, c->stream, cnt, nchunk, dpart, dh,
+                hipLaunchKernelGGL(k_cgs_reduce, dim3(cnt), dim3(64), 0, c->stream, cnt, nchunk, dpart, dh,
                                    dhacc, pass == 0 ? 1 : 0);
                 hipLaunchKernelGGL(k_orth_update, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, cnt, V, dh, w);
             }

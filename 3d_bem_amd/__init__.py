@@ -38,6 +38,7 @@ _SIGS = {
     "bem_hmat_stats": (_int, [_p, _p]),
     "bem_gmres": (_int, [_p, _p, _p, _p, _dbl, _i64, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)]),
     "bem_reconstruct": (_int, [_p, _p, _dbl, _dbl, _p, _p, _p]),
+    "bem_reconstruct_g": (_int, [_p, _p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_entries4": (_int, [_p, _i64, _p, _p]),
     "bem_entries": (_int, [_p, _int, _i64, _p, _p]),
     "bem_malloc": (_int, [_p, ctypes.c_size_t, ctypes.POINTER(_p)]),
@@ -53,6 +54,8 @@ _SIGS = {
     "bem_solver_layout": (_int, [_p, _p, ctypes.POINTER(_i64), _p, _i64]),
     "bem_solver_set_sources": (_int, [_p, _i64, _p, _int, _p]),
     "bem_solver_solve": (_int, [_p, _dbl, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _p, _p, _p, _p]),
+    "bem_solver_sweep": (_int, [_p, _i64, _p, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _i64, _p, _p,
+                                _int, _p, _p, _p]),
 }
 EXPORTS = tuple(_SIGS)
 
@@ -337,3 +340,31 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def sweep(self, freqs, g=None, planes=None, c=343.0, rho=1.21, coupling=1, aca_tol=1e-6, maxrank=0,
+              gmres_tol=None, restart=300, seed=1, precision=FP64, lanes=1):
+        """Frequency sweep (bem_solver_sweep): pressure (nf, nrhs, n) in
+        cluster order, GMRES iterations (nf, nrhs), stats per frequency.
+        g: (nrhs, n) complex boundary data in input order (None: the g given
+        at creation, one right-hand side); planes: (nrhs, 5) plane waves."""
+        freqs = _c(freqs, np.float64)
+        nf = len(freqs)
+        if g is not None:
+            g = _c(np.atleast_2d(g), np.complex128)
+            nrhs = len(g)
+        else:
+            nrhs = 1 if planes is None else len(planes)
+        pl = None if planes is None else _c(np.atleast_2d(planes), np.float64)
+        if pl is not None and len(pl) != nrhs:
+            raise ValueError("planes must have one row per right-hand side")
+        pr = np.empty((nf, nrhs, self.n), np.complex128)
+        its = np.zeros((nf, nrhs), np.int64)
+        st = np.zeros((nf, 16))
+        gt = -1.0 if gmres_tol is None else float(gmres_tol)
+        rc = lib().bem_solver_sweep(self.h, nf, _ptr(freqs), float(c), float(rho), int(coupling), float(aca_tol),
+                                    int(maxrank), gt, int(restart), int(seed), int(precision), nrhs,
+                                    None if g is None else _ptr(g), None if pl is None else _ptr(pl), int(lanes),
+                                    _ptr(pr), _ptr(its), _ptr(st))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
+        return pr, its, [dict(zip(STAT_KEYS, r.tolist())) for r in st]

@@ -617,8 +617,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
 }
 
 // ACA (K3): one workgroup of NT threads per listed admissible block
+// FP32 ACA limited to 128 VGPRs: four wavefronts per SIMD at every size
+template <int NT, class C> constexpr int kAcaWaves = 1;
+template <> constexpr int kAcaWaves<64, bem_c32> = 4;
+
 template <bool EXTRA, int NT, class C>
-__global__ __launch_bounds__(NT) void k_aca(AcaArgs a)
+__global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) char dyn[];
     __shared__ ArgMax sred[NT / 64];
@@ -1647,14 +1651,18 @@ struct AcaScratch {
     }
 };
 
+// workgroup of the largest class (FP64 never forms it: kHugeBlock[0])
+template <class C> constexpr int kHugeThreads = 512;
+
 // one size class of blocks on stream st, storage type C
 template <class C>
 void launch_aca(int q, bool extra, unsigned nblocks, size_t shm, hipStream_t st, const AcaArgs& aq)
 {
     dim3 g(nblocks);
     if (q == 0) {
-        if (extra) hipLaunchKernelGGL((k_aca<true, 512, C>), g, dim3(512), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<false, 512, C>), g, dim3(512), shm, st, aq);
+        constexpr int NT = kHugeThreads<C>;
+        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C>), g, dim3(NT), shm, st, aq);
     } else if (q == 1) {
         if (extra) hipLaunchKernelGGL((k_aca<true, 256, C>), g, dim3(256), shm, st, aq);
         else hipLaunchKernelGGL((k_aca<false, 256, C>), g, dim3(256), shm, st, aq);
@@ -2265,17 +2273,25 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
     return BEM_OK;
 }
 
-int bem_reconstruct(bem_ctx_t c, const void* x, double rho, double freq, void* phi, void* dphi, void* pr)
+int bem_reconstruct_g(bem_ctx_t c, const void* x, const void* g, double rho, double freq, void* phi, void* dphi,
+                      void* pr)
 {
-    if (!c || !x || !c->have_freq) return set_err(c, BEM_ERR_ARG, "bem_reconstruct: bad arguments");
+    if (!c || !x || !g || !c->have_freq) return set_err(c, BEM_ERR_ARG, "bem_reconstruct: bad arguments");
     HIPC(c, hipSetDevice(c->device));
     double s = freq != 0 ? (rho * (2.0 * BEM_PI)) * freq : rho * (2.0 * BEM_PI);
     hipLaunchKernelGGL(k_reconstruct, dim3(nblk(c->n, 256)), dim3(256), 0, c->stream, c->n, (const bem_c64*)x,
-                       c->d_subst, c->d_alpha, c->d_beta, c->d_f, s, (bem_c64*)phi, (bem_c64*)dphi, (bem_c64*)pr);
+                       c->d_subst, c->d_alpha, c->d_beta, (const bem_c64*)g, s, (bem_c64*)phi, (bem_c64*)dphi,
+                       (bem_c64*)pr);
     int rc = launch_check(c, "k_reconstruct");
     if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     return BEM_OK;
+}
+
+int bem_reconstruct(bem_ctx_t c, const void* x, double rho, double freq, void* phi, void* dphi, void* pr)
+{
+    if (!c || !x || !c->have_freq) return set_err(c, BEM_ERR_ARG, "bem_reconstruct: bad arguments");
+    return bem_reconstruct_g(c, x, c->d_f, rho, freq, phi, dphi, pr);
 }
 
 static int entries_impl(bem_ctx_t c, int mode, int64_t np, const int64_t* ij, double* out)

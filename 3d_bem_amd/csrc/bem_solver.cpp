@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "../../include/bem_detmath.h"
 
@@ -213,6 +214,13 @@ void BoundaryElementSolver::calculateBoundaryConditionAndSourceTermVector()
 
 void BoundaryElementSolver::calculateBoundarySolution()
 {
+    setWavenumberAndCoupling();
+    hMatrixSolve();
+}
+
+// boundaryelementsolver.cpp:12-59: k = 2 pi f / c and the coupling parameter
+void BoundaryElementSolver::setWavenumberAndCoupling()
+{
     if (!(frequency > 0)) throw Error(BEM_ERR_ARG, "frequency must be > 0 (the f = 0 Laplace branch is not on this path)");
     wavenumber = (2.0 * frequency * BEM_PI) / waveSpeed;
     switch (bemSolverParameters.coupling) {
@@ -226,7 +234,75 @@ void BoundaryElementSolver::calculateBoundarySolution()
     }
     check(ctx_, bem_set_frequency(ctx_, wavenumber, couplingParameter.real(), couplingParameter.imag()),
           "bem_set_frequency");
-    hMatrixSolve();
+}
+
+void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHandSide>& rhs,
+                                                       std::vector<std::vector<cplx>>& pressure,
+                                                       std::vector<int64_t>& iterations)
+{
+    setWavenumberAndCoupling();
+    auto& E = boundaryElements;
+    const auto& P = bemSolverParameters;
+    const size_t nr = rhs.size();
+    std::memset(stats, 0, sizeof stats);
+    double t0 = now_ms();
+    // source term of every right-hand side
+    std::vector<std::vector<cplx>> s(nr);
+    auto keepPoints = pointSources;
+    auto keepPlanes = planeWaves;
+    bool anyG = false;
+    for (size_t k = 0; k < nr; k++) {
+        if ((int64_t)rhs[k].g.size() != E.n) throw Error(BEM_ERR_ARG, "right-hand side of the wrong length");
+        pointSources = rhs[k].pointSources;
+        planeWaves = rhs[k].planeWaves;
+        calculateBoundaryConditionAndSourceTermVector();
+        s[k] = sourceTermVector;
+        for (const cplx& v : rhs[k].g) anyG = anyG || v != cplx(0, 0);
+    }
+    pointSources = keepPoints;
+    planeWaves = keepPlanes;
+    std::vector<std::unique_ptr<DeviceVector>> b, g;
+    for (size_t k = 0; k < nr; k++) {
+        b.push_back(std::make_unique<DeviceVector>(ctx_, E.n));
+        g.push_back(std::make_unique<DeviceVector>(ctx_, E.n));
+        b[k]->upload(s[k]);
+        g[k]->upload(rhs[k].g);
+    }
+    double t1 = t0;
+    if (anyG) {   // b_k = s_k + Phi g_k (:578-579)
+        phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+        stats[7] = phiHMat->stat(1);
+        stats[9] = phiHMat->stat(0);
+        t1 = now_ms();
+        for (size_t k = 0; k < nr; k++) HArithm::MVM(*b[k], *phiHMat, *g[k]);
+        phiHMat->clear();
+    }
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    double t2 = now_ms();
+    dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    double t3 = now_ms();
+    stats[8] = dPhiHMat->stat(1);
+    stats[10] = dPhiHMat->stat(0);
+    stats[15] = dPhiHMat->stat(3);
+    const double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
+    pressure.assign(nr, {});
+    iterations.assign(nr, 0);
+    DeviceVector x(ctx_, E.n), phi(ctx_, E.n), dphi(ctx_, E.n), p(ctx_, E.n);
+    double est = 0;
+    for (size_t k = 0; k < nr; k++) {
+        x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
+        iterations[k] = GMRES::gmresSolve(*dPhiHMat, x, *b[k], tol, P.gmresRestart, &est);
+        check(ctx_, bem_reconstruct_g(ctx_, x.data(), g[k]->data(), airDensity, frequency, phi.data(), dphi.data(),
+                                      p.data()),
+              "bem_reconstruct_g");
+        pressure[k] = p.download();
+    }
+    double t4 = now_ms();
+    dPhiHMat->clear();
+    stats[0] = t1 - t0; stats[1] = t2 - t1; stats[2] = t3 - t2; stats[3] = t4 - t3; stats[4] = t4 - t0;
+    stats[5] = nr ? (double)iterations[nr - 1] : 0;
+    stats[6] = est;
 }
 
 // BoundaryElementSolver::hMatrixSolve (boundaryelementsolver.cpp:526-744)
@@ -296,6 +372,12 @@ void BoundaryElementSolver::hMatrixSolve()
 struct bem_solver {
     std::unique_ptr<bem::BoundaryElementSolver> s;
     std::string err;
+    // creation arguments, for the extra sweep lanes
+    int device = 0;
+    int64_t leaf = 0;
+    double eta = 0;
+    bem::BoundaryElements input;   // input order
+    std::vector<std::unique_ptr<bem::BoundaryElementSolver>> lanes;
 };
 
 extern "C" {
@@ -318,6 +400,10 @@ int bem_solver_create(bem_solver_t* out, int device, int64_t n, const double* no
             E.bc[e].b = bem::cplx(q[2], q[3]);
             E.bc[e].g = bem::cplx(q[4], q[5]);
         }
+        S->device = device;
+        S->leaf = leaf;
+        S->eta = eta;
+        S->input = E;
         S->s = std::make_unique<bem::BoundaryElementSolver>(device, std::move(E), leaf, eta);
     } catch (const bem::Error& e) {
         delete S;
@@ -395,6 +481,88 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         if (dphi) std::memcpy(dphi, S.dPhiSolution.data(), n * sizeof(bem::cplx));
         if (pressure) std::memcpy(pressure, S.soundPressure.data(), n * sizeof(bem::cplx));
         if (stats16) std::memcpy(stats16, S.stats, sizeof S.stats);
+    } catch (const bem::Error& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return BEM_ERR_HIP;
+    }
+    return BEM_OK;
+}
+
+int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, double rho, int coupling,
+                     double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
+                     int64_t nrhs, const double* g, const double* planes5, int lanes, double* pressure,
+                     int64_t* iters, double* stats16)
+{
+    if (!s || !s->s || nf < 0 || (nf && !freqs) || nrhs < 1 || lanes < 1 || coupling < 0 || coupling > 2 ||
+        max_rank < 0 || restart < 1 || !pressure)
+        return BEM_ERR_ARG;
+    try {
+        const int64_t n = s->s->boundaryElements.n;
+        const std::vector<int64_t>& perm = s->s->boundaryElements.elementIndex;   // perm[new] = old
+        std::vector<bem::BoundaryElementSolver::RightHandSide> rhs(nrhs);
+        for (int64_t k = 0; k < nrhs; k++) {
+            rhs[k].g.resize(n);
+            for (int64_t e = 0; e < n; e++) {
+                const int64_t o = perm[e];
+                rhs[k].g[e] = g ? bem::cplx(g[2 * (k * n + o)], g[2 * (k * n + o) + 1]) : s->input.bc[o].g;
+            }
+            if (planes5) {
+                const double* q = planes5 + 5 * k;
+                if (q[3] != 0 || q[4] != 0) rhs[k].planeWaves.push_back(bem::PlaneWave{q[0], q[1], q[2], bem::cplx(q[3], q[4])});
+            }
+        }
+        const int L = (int)std::min<int64_t>(lanes, std::max<int64_t>(nf, 1));
+        while ((int)s->lanes.size() < L - 1)
+            s->lanes.push_back(std::make_unique<bem::BoundaryElementSolver>(s->device, s->input, s->leaf, s->eta));
+        std::vector<bem::BoundaryElementSolver*> solver(L);
+        solver[0] = s->s.get();
+        for (int q = 1; q < L; q++) solver[q] = s->lanes[q - 1].get();
+        std::vector<std::string> errs(L);
+        std::vector<int> codes(L, BEM_OK);
+        auto work = [&](int q) {
+            try {
+                auto& S = *solver[q];
+                S.setWaveSpeed(c);
+                S.setAirDensity(rho);
+                auto& P = S.bemSolverParameters;
+                P.acaRelativeError = aca_tol;
+                P.acaMaxRank = (unsigned long)max_rank;
+                P.coupling = (bem::BemCoupling)coupling;
+                P.gmresTolerance = gmres_tol;
+                P.gmresRestart = (long)restart;
+                P.pivotSeed = seed;
+                P.precision = precision;
+                std::vector<std::vector<bem::cplx>> pr;
+                std::vector<int64_t> its;
+                for (int64_t f = q; f < nf; f += L) {
+                    S.setFrequency(freqs[f]);
+                    S.calculateBoundarySolutions(rhs, pr, its);
+                    for (int64_t k = 0; k < nrhs; k++) {
+                        std::memcpy(pressure + 2 * ((f * nrhs + k) * n), pr[k].data(), n * sizeof(bem::cplx));
+                        if (iters) iters[f * nrhs + k] = its[k];
+                    }
+                    if (stats16) std::memcpy(stats16 + 16 * f, S.stats, sizeof S.stats);
+                }
+            } catch (const bem::Error& e) {
+                codes[q] = e.code;
+                errs[q] = e.msg;
+            } catch (const std::exception& e) {
+                codes[q] = BEM_ERR_HIP;
+                errs[q] = e.what();
+            }
+        };
+        std::vector<std::thread> th;
+        for (int q = 1; q < L; q++) th.emplace_back(work, q);
+        work(0);
+        for (auto& t : th) t.join();
+        for (int q = 0; q < L; q++)
+            if (codes[q] != BEM_OK) {
+                s->err = errs[q];
+                return codes[q];
+            }
     } catch (const bem::Error& e) {
         s->err = e.msg;
         return e.code;

@@ -124,6 +124,19 @@ public:
     void setWaveSpeed(double c) { waveSpeed = c; }
     void setAirDensity(double rho) { airDensity = rho; }
     void calculateBoundarySolution();      // boundaryelementsolver.cpp:12-138 (H-matrix branch)
+    // One right-hand side of a multi-right-hand-side solve on the same a, b:
+    // boundary data g (cluster order) and its own sources.
+    struct RightHandSide {
+        std::vector<cplx> g;
+        std::vector<PointSource> pointSources;
+        std::vector<PlaneWave> planeWaves;
+    };
+    // calculateBoundarySolution for several right-hand sides: Phi and DPhi
+    // assembled once, b_k = s_k + Phi g_k, GMRES and reconstruction per k;
+    // each right-hand side's pressure is the same as calculateBoundarySolution
+    // gives for it alone
+    void calculateBoundarySolutions(const std::vector<RightHandSide>& rhs, std::vector<std::vector<cplx>>& pressure,
+                                    std::vector<int64_t>& iterations);
     BoundaryElements boundaryElements;     // cluster order after construction
     BemSolverParameters bemSolverParameters;
     std::vector<PointSource> pointSources;
@@ -136,6 +149,7 @@ public:
     std::shared_ptr<ClusterTree> clusterTree;
 private:
     void hMatrixSolve();
+    void setWavenumberAndCoupling();
     void calculateBoundaryConditionAndSourceTermVector();
     bem_ctx_t ctx_ = nullptr;
     double wavenumber = 0;

@@ -102,6 +102,12 @@ int bem_gmres(bem_ctx_t ctx, bem_hmat_t h, const void* b_dev, void* x_dev, doubl
    p = i rho 2 pi f phi (phiSolutionToSoundPressure, :2251-2261) */
 int bem_reconstruct(bem_ctx_t ctx, const void* x_dev, double rho, double freq, void* phi_dev, void* dphi_dev,
                     void* pressure_dev);
+/* bem_reconstruct with the boundary-condition right-hand side g taken from
+   g_dev (n complex, device) instead of the one set by bem_set_bc: the
+   reconstruction of a further right-hand side on the same a, b
+   (boundaryelementsolver.cpp:729-740 with f = g_dev). */
+int bem_reconstruct_g(bem_ctx_t ctx, const void* x_dev, const void* g_dev, double rho, double freq, void* phi_dev,
+                      void* dphi_dev, void* pressure_dev);
 
 /* L, M, Mt, N (out npairs x 8) or Phi/DPhi entries (out npairs x 2) of
    (row, col) pairs given in host memory, evaluated on the device
@@ -143,6 +149,24 @@ int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* points6
 int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);
+
+/* Frequency sweep with several right-hand sides per frequency
+   (BemSolverController::run's loop, bemsolvercontroller.cpp:59-79, with the
+   per-frequency hMatrixSolve of boundaryelementsolver.cpp:526-744).  For
+   every frequency Phi and DPhi are assembled once and GMRES runs once per
+   right-hand side; right-hand side k has the boundary data g_k (nrhs x n
+   complex, input element order; NULL: the g given at creation for every k)
+   and the plane wave planes5[k] (dir x, y, z, amplitude re, im; NULL or zero
+   amplitude: none) on the a, b given at creation.  `lanes` solver copies on
+   their own streams and host threads process the frequencies concurrently
+   (frequency f on lane f % lanes); every frequency point is computed exactly
+   as by bem_solver_solve.  Outputs: pressure nf x nrhs x n complex (cluster
+   order), iters nf x nrhs, stats nf x 16 (bem_solver_solve's layout, for the
+   last right-hand side). */
+int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, double rho, int coupling,
+                     double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
+                     int64_t nrhs, const double* g, const double* planes5, int lanes, double* pressure,
+                     int64_t* iters, double* stats16);
 
 #ifdef __cplusplus
 }

@@ -1,20 +1,22 @@
 """Frequency points solved per second on the 20480-triangle pulsating sphere
-(icosphere level 5, radius 0.5 m, 1000 Hz, ka = 9.16; leaf 64, eta 2, ACA
+(icosphere level 5, radius 0.5 m, ka = 9.16 at 1000 Hz; leaf 64, eta 2, ACA
 1e-5, GMRES 1e-8, Burton-Miller), one MI355X.
 
-A step is one frequency point: Phi assembly, b = s + Phi f, DPhi assembly,
-GMRES to tolerance, boundary-condition reconstruction and sound pressure
-(boundaryelementsolver.cpp:526-744, 2251-2261).  The per-mesh cluster tree,
-partition and near-singular geometry are set up once, before the timed
-region, and reported as setup.
+A step is one batch of --batch frequency points spread over 960-1040 Hz
+(1000 Hz included).  Each point is the full per-frequency path: Phi
+assembly, b = s + Phi f, DPhi assembly, GMRES to tolerance, boundary-
+condition reconstruction and sound pressure (boundaryelementsolver.cpp:
+526-744, 2251-2261).  The per-mesh cluster tree, partition and near-singular
+geometry are set up once, before the timed region, and reported as setup.
 
-The headline is the mixed-precision form (--precision fp32, default):
-regular quadrature, ACA and block storage in complex FP32, self and
-near-singular entries in FP64, GMRES in FP64 over the FP32 operator with
-the restart residual recomputed with FP64 accumulation.  The plain FP64 path
-is timed beside it on the same workload (fp64_value, speedup_vs_fp64) and
-the surface pressure of both is compared with the FP64 CPU restatement run
-on the host (cpu_baseline leg).
+The headline is the mixed-precision batched form (--precision fp32,
+default): regular quadrature, ACA and block storage in complex FP32, self and
+near-singular entries in FP64, GMRES in FP64 over the FP32 operator with the
+restart residual recomputed with FP64 accumulation, and the batch's points
+solved concurrently on --lanes solver lanes.  The plain FP64 path (one point
+after the other) is timed beside it on the same points (fp64_value,
+speedup_vs_fp64), and the 1000 Hz surface pressure of both is compared with
+the FP64 CPU restatement run on the host (cpu_baseline leg).
 
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -61,39 +63,45 @@ def run_cpu_baseline(tri, bc, args):
     return pr, t, threads, st
 
 
-def timed_run(bem, S, ctx_h, args, prec):
-    """warmup + timed steps of one precision; kernel family times from HIP
-    events recorded on the launching streams"""
-    def solve():
-        return S.solve(args.freq, c=343.0, rho=1.21, coupling=1, aca_tol=args.aca_tol, gmres_tol=args.gmres_tol,
-                       seed=args.seed, precision=prec)
+def batch_freqs(center, batch):
+    """batch points over [0.96, 1.04] x center with center itself included"""
+    if batch == 1:
+        return np.array([center])
+    f = center * (0.96 + 0.08 * np.arange(batch) / (batch - 1))
+    f[np.argmin(np.abs(f - center))] = center
+    return f
+
+
+def timed_run(bem, S, ctx_h, args, prec, freqs, lanes):
+    """warmup + timed steps (one sweep of the batch each); kernel family
+    times from HIP events recorded on lane 0's launching streams"""
+    def step():
+        return S.sweep(freqs, c=343.0, rho=1.21, coupling=1, aca_tol=args.aca_tol, gmres_tol=args.gmres_tol,
+                       seed=args.seed, precision=prec, lanes=lanes)
 
     for _ in range(args.warmup):
-        solve()
+        step()
     prof = np.zeros(12)
     bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(ctypes.c_void_p))
-    stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        _, _, pr, st = solve()
-        stats.append(st)
+        pr, its, stats = step()
     elapsed = time.perf_counter() - t0
     bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(ctypes.c_void_p))
-    return pr, stats, elapsed, prof[:6].copy(), prof[6:].copy()
+    return pr, its, stats, elapsed, prof[:6].copy(), prof[6:].copy()
 
 
-def roofline(avg_ms, cnt, stats, blocks, n, prec_name, steps):
+def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0):
     """Dominant kernel family (largest total device time) against its bound.
     Assembly families: algorithmic flops (F_REG per regular entry) over the
     family's mean launch duration, vector-ALU peak of the precision.  MVM
     families: algorithmic bytes of the stored operator over the duration."""
     tot = avg_ms * cnt
     dom = int(np.argmax(tot))
-    st = stats[-1]
     s = 16 if prec_name == "fp64" else 8
     dense = blocks[blocks[:, 4] == 0]
     n_dense = int((dense[:, 1] * dense[:, 3]).sum())
-    launches = cnt[dom] / steps          # launches of this family per frequency point
+    launches = cnt[dom] / points0        # launches of this family per frequency point (lane 0)
     if dom in (0, 2):
         if dom == 0:
             entries = 2 * (n_dense - n)  # regular entries of the Phi and DPhi near field
@@ -149,8 +157,10 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8, help="frequency points per step")
+    ap.add_argument("--lanes", type=int, default=4, help="concurrent solver lanes of the FP32 form")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--freq", type=float, default=1000.0)
@@ -181,13 +191,18 @@ def main():
     _, blocks = S.layout()
     ctx_h = bem.lib().bem_solver_context(S.h)
     prec = bem.FP32 if args.precision == "fp32" else bem.FP64
+    freqs = batch_freqs(args.freq, args.batch)
+    ic = int(np.argmin(np.abs(freqs - args.freq)))
+    lanes = args.lanes if prec == bem.FP32 else 1
 
-    pr, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec)
-    print(f"# {args.precision}: {1e3 * elapsed / args.steps:.2f} ms/point", file=sys.stderr, flush=True)
-    st = stats[-1]
+    pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes)
+    points = args.steps * len(freqs)
+    print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, lanes {lanes}", file=sys.stderr, flush=True)
+    st = stats[ic]
+    points0 = args.steps * len(range(0, len(freqs), max(lanes, 1)))
     out = {
         "metric": "freq-points solved/s at 20k tris (FP32+refine vs FP64); rel-L2 err vs FP64 oracle",
-        "value": round(args.steps / elapsed, 4),
+        "value": round(points / elapsed, 4),
         "unit": "freq-points/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -199,33 +214,37 @@ def main():
         "dtype": "c64+f64" if prec == bem.FP32 else "c128",
         "data": "synthetic",
         "config": {"workload": f"pulsating sphere, icosphere level {args.level} ({n} triangles), r=0.5 m, "
-                               f"{args.freq:g} Hz, Burton-Miller, leaf {args.leaf}, eta {args.eta:g}, "
-                               f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}",
-                   "triangles": n, "setup_s": round(setup_s, 3),
-                   "gmres_iters": st["gmres_iters"], "residual": st["residual"],
+                               f"{len(freqs)} frequency points {freqs[0]:.0f}-{freqs[-1]:.0f} Hz per step "
+                               f"(incl. {args.freq:g} Hz), Burton-Miller, leaf {args.leaf}, eta {args.eta:g}, "
+                               f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}, {lanes} lanes",
+                   "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "setup_s": round(setup_s, 3),
+                   "ms_per_point": round(1e3 * elapsed / points, 3),
+                   "gmres_iters": int(its[ic, 0]), "residual_estimate": st["residual"],
                    "max_rank_phi": st["max_rank_phi"], "max_rank_dphi": st["max_rank_dphi"],
                    "evaluated_entries": [st["evaluated_phi"], st["evaluated_dphi"]],
                    "phase_ms": {k: round(st[k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
-                   "kernel_ms": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
-        "roofline": roofline(avg_ms, cnt, stats, blocks, n, args.precision, args.steps),
+                   "kernel_ms_lane0": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
+        "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0),
     }
-    pr64 = None
+    p_head = pr[ic, 0]
+    p64 = None
     if not args.no_fp64_leg and prec != bem.FP64:
-        pr64, stats64, el64, _, _ = timed_run(bem, S, ctx_h, args, bem.FP64)
-        print(f"# fp64: {1e3 * el64 / args.steps:.2f} ms/point", file=sys.stderr, flush=True)
-        out["fp64_value"] = round(args.steps / el64, 4)
-        out["fp64_ms_per_step"] = round(1e3 * el64 / args.steps, 3)
+        pr64, _, _, el64, _, _ = timed_run(bem, S, ctx_h, args, bem.FP64, freqs, 1)
+        print(f"# fp64: {1e3 * el64 / points:.2f} ms/point", file=sys.stderr, flush=True)
+        p64 = pr64[ic, 0]
+        out["fp64_value"] = round(points / el64, 4)
+        out["fp64_ms_per_point"] = round(1e3 * el64 / points, 3)
         out["speedup_vs_fp64"] = round(el64 / elapsed, 3)
-        out["rel_l2_vs_fp64_gpu"] = float(np.linalg.norm(pr - pr64) / np.linalg.norm(pr64))
+        out["rel_l2_vs_fp64_gpu"] = float(np.linalg.norm(p_head - p64) / np.linalg.norm(p64))
     if not args.no_cpu_baseline:
         pr_cpu, t_cpu, threads, st_cpu = run_cpu_baseline(tri, bc, args)
         out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 5), "unit": "freq-points/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"one full frequency point of the same workload ({n} triangles, FP64, "
-                                         f"set-up included) with the CPU restatement: {t_cpu:.1f} s"}
-        out["rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(pr - pr_cpu) / np.linalg.norm(pr_cpu))
-        if pr64 is not None:
-            out["fp64_rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(pr64 - pr_cpu) / np.linalg.norm(pr_cpu))
+                               "sample": f"one full frequency point ({args.freq:g} Hz) of the same workload ({n} "
+                                         f"triangles, FP64, set-up included) with the CPU restatement: {t_cpu:.1f} s"}
+        out["rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(p_head - pr_cpu) / np.linalg.norm(pr_cpu))
+        if p64 is not None:
+            out["fp64_rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(p64 - pr_cpu) / np.linalg.norm(pr_cpu))
     print(json.dumps(out))
 
 

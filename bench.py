@@ -91,7 +91,7 @@ def timed_run(bem, S, ctx_h, args, prec, freqs, lanes):
     return pr, its, stats, elapsed, prof[:6].copy(), prof[6:].copy()
 
 
-def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0):
+def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload):
     """Dominant kernel family (largest total device time) against its bound.
     Assembly families: algorithmic flops (F_REG per regular entry) over the
     family's mean launch duration, vector-ALU peak of the precision.  MVM
@@ -127,10 +127,26 @@ def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0):
         rl = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
               "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
               "algorithmic_per_launch": f"{byts:.4g} B"}
+    if dom == 2 and prec_name == "fp32" and measured_workload:
+        rl["traffic"] = measured_traffic("aca", "fp32")
     rl["kernel"] = FAMILIES[dom]
     rl["avg_ms"] = round(float(avg_ms[dom]), 4)
     rl["launches_per_point"] = round(float(launches), 2)
     return rl
+
+
+def measured_traffic(family, prec_name):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration), or None"""
+    key = f"traffic_bytes_per_launch {family} {prec_name} ="
+    path = os.path.join(ROOT, "profiles", f"{family}_{prec_name}_sphere20480_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(key):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(mib):
@@ -224,7 +240,8 @@ def main():
                    "evaluated_entries": [st["evaluated_phi"], st["evaluated_dphi"]],
                    "phase_ms": {k: round(st[k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
                    "kernel_ms_lane0": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
-        "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0),
+        "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0,
+                             (args.level, args.leaf, args.eta, args.aca_tol) == (5, 64, 2.0, 1e-5)),
     }
     p_head = pr[ic, 0]
     p64 = None

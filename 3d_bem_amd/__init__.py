@@ -39,6 +39,8 @@ _SIGS = {
     "bem_gmres": (_int, [_p, _p, _p, _p, _dbl, _i64, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)]),
     "bem_reconstruct": (_int, [_p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_reconstruct_g": (_int, [_p, _p, _p, _dbl, _dbl, _p, _p, _p]),
+    "bem_field": (_int, [_p, _i64, _p, _p, _p, _p]),
+    "bem_solver_field": (_int, [_p, _i64, _p, _p]),
     "bem_entries4": (_int, [_p, _i64, _p, _p]),
     "bem_entries": (_int, [_p, _int, _i64, _p, _p]),
     "bem_malloc": (_int, [_p, ctypes.c_size_t, ctypes.POINTER(_p)]),
@@ -201,6 +203,13 @@ class Context:
         self._check(lib().bem_reconstruct(self.h, x_dev.ptr, float(rho), float(freq), phi.ptr, dphi.ptr, p.ptr))
         return phi.download(), dphi.download(), p.download()
 
+    def field(self, obs, phi_dev, dphi_dev):
+        """bem_field: boundary term of the field potential at obs (nobs, 3)"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        out = np.empty(len(obs), np.complex128)
+        self._check(lib().bem_field(self.h, len(obs), _ptr(obs), phi_dev.ptr, dphi_dev.ptr, _ptr(out)))
+        return out
+
     def synchronize(self):
         self._check(lib().bem_synchronize(self.h))
 
@@ -340,6 +349,15 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def field(self, obs):
+        """sound pressure at observation points (nobs, 3) from the last solve"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        out = np.empty(len(obs), np.complex128)
+        rc = lib().bem_solver_field(self.h, len(obs), _ptr(obs), _ptr(out))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
+        return out
 
     def sweep(self, freqs, g=None, planes=None, c=343.0, rho=1.21, coupling=1, aca_tol=1e-6, maxrank=0,
               gmres_tol=None, restart=300, seed=1, precision=FP64, lanes=1):

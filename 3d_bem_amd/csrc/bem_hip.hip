@@ -214,6 +214,80 @@ __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int6
     }
 }
 
+// ------------------------------------------------------ field evaluation
+
+__constant__ static const double kTri7[7][3] = BEM_TRI7;
+constexpr int64_t kFieldChunk = 1024;   // boundary elements per partial sum of the field
+
+// Field potential of the boundary solution at observation points
+// (calculateFieldSolutionRegular / BemOperatorField,
+// boundaryelementsolver.cpp:2262-2353), one 64-point tile x one chunk of
+// kFieldChunk elements per workgroup.  Canonical order: inside a chunk four
+// partial sums over element index mod 4 (ascending), combined
+// (s0 + s1) + (s2 + s3); k_field_reduce adds the chunks in order.
+__global__ __launch_bounds__(kWG) void k_field(DevMesh M, double k, int64_t nobs, const double* __restrict__ obs,
+                                               const bem_c64* __restrict__ phi, const bem_c64* __restrict__ dphi,
+                                               int nchunk, bem_c64* __restrict__ part)
+{
+    __shared__ double sv[9][kTile];
+    __shared__ double sn[3][kTile], sa[kTile];
+    __shared__ bem_c64 sphi[kTile], sdphi[kTile];
+    __shared__ bem_c64 red[4][kTile];
+    const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
+    const int64_t n = M.n;
+    const int64_t i = (int64_t)blockIdx.x * kTile + r;
+    const bool live = i < nobs;
+    const double px = live ? obs[i] : 0.0, py = live ? obs[nobs + i] : 0.0, pz = live ? obs[2 * nobs + i] : 0.0;
+    const int64_t j0 = (int64_t)blockIdx.y * kFieldChunk;
+    const int64_t j1 = j0 + kFieldChunk < n ? j0 + kFieldChunk : n;
+    bem_c64 acc = c64(0, 0);
+    for (int64_t t0 = j0; t0 < j1; t0 += kTile) {
+        const int cnt = (int)(j1 - t0 < kTile ? j1 - t0 : kTile);
+        __syncthreads();
+        for (int idx = tid; idx < 9 * kTile; idx += kWG) {
+            int d = idx / kTile, c = idx % kTile;
+            if (c < cnt) sv[d][c] = M.vtx[(int64_t)d * n + t0 + c];
+        }
+        for (int c = tid; c < cnt; c += kWG) {
+            sn[0][c] = M.nx[t0 + c]; sn[1][c] = M.ny[t0 + c]; sn[2][c] = M.nz[t0 + c];
+            sa[c] = M.area[t0 + c];
+            sphi[c] = phi[t0 + c];
+            sdphi[c] = dphi[t0 + c];
+        }
+        __syncthreads();
+        if (!live) continue;
+        for (int c = cg; c < cnt; c += 4) {
+            bem_c64 L = c64(0, 0), Mk = c64(0, 0);
+#pragma unroll
+            for (int q = 0; q < 7; q++) {
+                double x1 = kTri7[q][0], x2 = kTri7[q][1];
+                double x3 = (1.0 - x2) - x1;
+                double qx = (x1 * sv[0][c] + x2 * sv[3][c]) + x3 * sv[6][c];
+                double qy = (x1 * sv[1][c] + x2 * sv[4][c]) + x3 * sv[7][c];
+                double qz = (x1 * sv[2][c] + x2 * sv[5][c]) + x3 * sv[8][c];
+                bem_field_point_accumulate(&L, &Mk, k, px, py, pz, sn[0][c], sn[1][c], sn[2][c], qx, qy, qz,
+                                           kTri7[q][2]);
+            }
+            L = c64_scale(L, sa[c]);
+            Mk = c64_scale(Mk, sa[c]);
+            acc = c64_add(acc, bem_field_term(L, Mk, sphi[c], sdphi[c]));
+        }
+    }
+    red[cg][r] = acc;
+    __syncthreads();
+    if (cg == 0 && live)
+        part[i * nchunk + blockIdx.y] = c64_add(c64_add(red[0][r], red[1][r]), c64_add(red[2][r], red[3][r]));
+}
+
+__global__ void k_field_reduce(int64_t nobs, int nchunk, const bem_c64* __restrict__ part, bem_c64* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nobs) return;
+    bem_c64 acc = part[i * nchunk];
+    for (int q = 1; q < nchunk; q++) acc = c64_add(acc, part[i * nchunk + q]);
+    out[i] = acc;
+}
+
 // -------------------------------------------------- workgroup reductions
 
 __device__ __forceinline__ double wave_sum(double x)
@@ -2285,6 +2359,36 @@ int bem_reconstruct_g(bem_ctx_t c, const void* x, const void* g, double rho, dou
     int rc = launch_check(c, "k_reconstruct");
     if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+int bem_field(bem_ctx_t c, int64_t nobs, const double* obs3, const void* phi, const void* dphi, double* out2)
+{
+    if (!c || nobs < 0 || (nobs && (!obs3 || !out2)) || !phi || !dphi)
+        return set_err(c, BEM_ERR_ARG, "bem_field: bad arguments");
+    if (!c->have_freq || c->n <= 0) return set_err(c, BEM_ERR_STATE, "bem_field: mesh or frequency missing");
+    if (nobs == 0) return BEM_OK;
+    HIPC(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    const int nchunk = (int)((n + kFieldChunk - 1) / kFieldChunk);
+    std::vector<double> soa(3 * nobs);
+    for (int64_t i = 0; i < nobs; i++)
+        for (int d = 0; d < 3; d++) soa[d * nobs + i] = obs3[3 * i + d];
+    std::vector<void*> own;
+    double* d_obs;
+    bem_c64 *d_part, *d_out;
+    int rc;
+    if ((rc = dupload(c, &d_obs, soa.data(), soa.size(), &own))) { free_list(c, own); return rc; }
+    if ((rc = dalloc(c, &d_part, (size_t)nobs * nchunk, &own))) { free_list(c, own); return rc; }
+    if ((rc = dalloc(c, &d_out, (size_t)nobs, &own))) { free_list(c, own); return rc; }
+    hipLaunchKernelGGL(k_field, dim3(nblk(nobs, kTile), nchunk), dim3(kWG), 0, c->stream, c->mesh, c->k, nobs, d_obs,
+                       (const bem_c64*)phi, (const bem_c64*)dphi, nchunk, d_part);
+    hipLaunchKernelGGL(k_field_reduce, dim3(nblk(nobs, 256)), dim3(256), 0, c->stream, nobs, nchunk, d_part, d_out);
+    if ((rc = launch_check(c, "k_field"))) { free_list(c, own); return rc; }
+    hipError_t e = hipMemcpyAsync(out2, d_out, (size_t)nobs * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    free_list(c, own);
+    if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, std::string("bem_field: ") + hipGetErrorString(e));
     return BEM_OK;
 }
 

@@ -236,6 +236,41 @@ void BoundaryElementSolver::setWavenumberAndCoupling()
           "bem_set_frequency");
 }
 
+std::vector<cplx> BoundaryElementSolver::calculateFieldSolution(const std::vector<double>& obs3)
+{
+    auto& E = boundaryElements;
+    if ((int64_t)phiSolution.size() != E.n || (int64_t)dPhiSolution.size() != E.n)
+        throw Error(BEM_ERR_STATE, "calculateFieldSolution: no boundary solution yet");
+    const int64_t nobs = (int64_t)obs3.size() / 3;
+    DeviceVector phi(ctx_, E.n), dphi(ctx_, E.n);
+    phi.upload(phiSolution);
+    dphi.upload(dPhiSolution);
+    std::vector<cplx> field(nobs);
+    check(ctx_, bem_field(ctx_, nobs, obs3.data(), phi.data(), dphi.data(), (double*)field.data()), "bem_field");
+    // incident field of the sources (sourcePhiTerm, :3443-3450; plane waves A e^{ik d.x})
+    const double k = wavenumber;
+    const bem_c64 s = c64(0.0, airDensity * (2.0 * BEM_PI) * frequency);   // p = i rho 2 pi f phi (:2234-2240)
+    std::vector<cplx> pressure(nobs);
+    for (int64_t i = 0; i < nobs; i++) {
+        const double x = obs3[3 * i], y = obs3[3 * i + 1], z = obs3[3 * i + 2];
+        bem_c64 acc = c64(field[i].real(), field[i].imag());
+        for (auto& src : pointSources) {
+            double Rx = src.x - x, Ry = src.y - y, Rz = src.z - z;
+            double r = std::sqrt(fma(Rz, Rz, fma(Ry, Ry, Rx * Rx)));
+            bem_c64 G = c64_mul(c64(src.weight.real(), src.weight.imag()),
+                                c64_scale(det_expi(k * r), 1.0 / (4.0 * BEM_PI * r)));
+            acc = c64_add(acc, G);
+        }
+        for (auto& w : planeWaves) {
+            double dx = fma(w.dz, z, fma(w.dy, y, w.dx * x));
+            acc = c64_add(acc, c64_mul(c64(w.amplitude.real(), w.amplitude.imag()), det_expi(k * dx)));
+        }
+        bem_c64 pr = c64_mul(s, acc);
+        pressure[i] = cplx(pr.re, pr.im);
+    }
+    return pressure;
+}
+
 void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHandSide>& rhs,
                                                        std::vector<std::vector<cplx>>& pressure,
                                                        std::vector<int64_t>& iterations)
@@ -481,6 +516,23 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         if (dphi) std::memcpy(dphi, S.dPhiSolution.data(), n * sizeof(bem::cplx));
         if (pressure) std::memcpy(pressure, S.soundPressure.data(), n * sizeof(bem::cplx));
         if (stats16) std::memcpy(stats16, S.stats, sizeof S.stats);
+    } catch (const bem::Error& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return BEM_ERR_HIP;
+    }
+    return BEM_OK;
+}
+
+int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* pressure2)
+{
+    if (!s || !s->s || nobs < 0 || (nobs && (!obs3 || !pressure2))) return BEM_ERR_ARG;
+    try {
+        std::vector<double> obs(obs3, obs3 + 3 * nobs);
+        std::vector<bem::cplx> p = s->s->calculateFieldSolution(obs);
+        if (nobs) std::memcpy(pressure2, p.data(), nobs * sizeof(bem::cplx));
     } catch (const bem::Error& e) {
         s->err = e.msg;
         return e.code;

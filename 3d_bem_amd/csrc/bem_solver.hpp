@@ -137,6 +137,10 @@ public:
     // gives for it alone
     void calculateBoundarySolutions(const std::vector<RightHandSide>& rhs, std::vector<std::vector<cplx>>& pressure,
                                     std::vector<int64_t>& iterations);
+    // calculateFieldSolution with the direct sum (calculateFieldSolutionRegular,
+    // boundaryelementsolver.cpp:2192-2316): sound pressure at observation
+    // points from the last boundary solution, incident sources included
+    std::vector<cplx> calculateFieldSolution(const std::vector<double>& observationPoints3);
     BoundaryElements boundaryElements;     // cluster order after construction
     BemSolverParameters bemSolverParameters;
     std::vector<PointSource> pointSources;

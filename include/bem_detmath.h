@@ -187,6 +187,52 @@ BEM_HD void bem_point_accumulate(bem_ops4* acc, double k,
     acc->N.im = fma(w, Nim, acc->N.im);
 }
 
+/* The order-5 7-point triangle rule (trianglequadraturerules.cpp:47-56):
+   rows (xi1, xi2, w) with the reference's 14-digit literals; the point is
+   (xi1 a + xi2 b) + xi3 c with xi3 = (1 - xi2) - xi1. */
+#define BEM_TRI7 { \
+    {0.33333333333333, 0.33333333333333, 0.22500000000000}, \
+    {0.47014206410511, 0.47014206410511, 0.13239415278851}, \
+    {0.47014206410511, 0.05971587178977, 0.13239415278851}, \
+    {0.05971587178977, 0.47014206410511, 0.13239415278851}, \
+    {0.10128650732346, 0.10128650732346, 0.12593918054483}, \
+    {0.10128650732346, 0.79742698535309, 0.12593918054483}, \
+    {0.79742698535309, 0.10128650732346, 0.12593918054483}}
+
+/* One quadrature point of the field operators at an observation point x
+   (BemOperatorField, boundaryelementsolver.cpp:2318-2353): with R = x - q,
+   L += w G and M += w G' rnq, rnq = -(R.nq)/r; the same operation order as
+   bem_point_accumulate. */
+BEM_HD void bem_field_point_accumulate(bem_c64* L, bem_c64* M, double k, double px, double py, double pz,
+                                       double nqx, double nqy, double nqz, double qx, double qy, double qz,
+                                       double w)
+{
+    double Rx = px - qx, Ry = py - qy, Rz = pz - qz;
+    double rr = fma(Rz, Rz, fma(Ry, Ry, Rx * Rx));
+    double r = sqrt(rr);
+    double inv_r = 1.0 / r;
+    double rnq = -fma(Rz, nqz, fma(Ry, nqy, Rx * nqx)) * inv_r;
+    double kr = k * r;
+    double s, c;
+    det_sincos(kr, &s, &c);
+    double g = inv_r * BEM_INV_4PI;
+    double Gre = c * g, Gim = s * g;
+    double Grre = Gre * inv_r, Grim = Gim * inv_r;
+    double dGre = fma(-Grim, kr, -Grre);
+    double dGim = fma(Grre, kr, -Grim);
+    L->re = fma(w, Gre, L->re);
+    L->im = fma(w, Gim, L->im);
+    M->re = fma(w, dGre * rnq, M->re);
+    M->im = fma(w, dGim * rnq, M->im);
+}
+
+/* Contribution M phi_j - L dphi_j of element j to the field potential
+   (calculateFieldSolutionRegular, boundaryelementsolver.cpp:2299-2304) */
+BEM_HD bem_c64 bem_field_term(bem_c64 L, bem_c64 M, bem_c64 phi, bem_c64 dphi)
+{
+    return c64_sub(c64_mul(M, phi), c64_mul(L, dphi));
+}
+
 /* Boundary-condition substitution of one entry (reference
    implicitPhiMatrix2 / implicitDPhiMatrix2, boundaryelementsolver.cpp:2004-2040).
    X = L + cc Mt + delta cc/2,  Y = M + cc N - delta/2.

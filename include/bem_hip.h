@@ -109,6 +109,16 @@ int bem_reconstruct(bem_ctx_t ctx, const void* x_dev, double rho, double freq, v
 int bem_reconstruct_g(bem_ctx_t ctx, const void* x_dev, const void* g_dev, double rho, double freq, void* phi_dev,
                       void* dphi_dev, void* pressure_dev);
 
+/* Field potential of the boundary solution at observation points:
+   out[i] = sum_j M_j(x_i) phi_j - L_j(x_i) dphi_j with the order-5 7-point
+   rule (BemOperatorField and calculateFieldSolutionRegular,
+   boundaryelementsolver.cpp:2262-2353; the H-matrix variant
+   calculateFieldSolutionFast, :2397-2570, approximates this sum).  obs3:
+   nobs x 3 host; phi_dev, dphi_dev: n complex on the device (cluster order);
+   out2: nobs complex host.  Sources are not included. */
+int bem_field(bem_ctx_t ctx, int64_t nobs, const double* obs3, const void* phi_dev, const void* dphi_dev,
+              double* out2);
+
 /* L, M, Mt, N (out npairs x 8) or Phi/DPhi entries (out npairs x 2) of
    (row, col) pairs given in host memory, evaluated on the device
    (BemOperatorsConst / implicitPhiMatrix2 / implicitDPhiMatrix2). */
@@ -149,6 +159,13 @@ int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* points6
 int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);
+
+/* Sound pressure at observation points (nobs x 3, host) from the last
+   bem_solver_solve: the boundary term of bem_field plus the incident point
+   sources and plane wave, times i rho 2 pi f (calculateFieldSolution with
+   the direct sum, boundaryelementsolver.cpp:2192-2316).  pressure2: nobs
+   complex. */
+int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* pressure2);
 
 /* Frequency sweep with several right-hand sides per frequency
    (BemSolverController::run's loop, bemsolvercontroller.cpp:59-79, with the

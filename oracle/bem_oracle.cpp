@@ -1005,6 +1005,44 @@ void orc_get_rhs_parts(const orc_problem* p, double* s2, double* a2, double* b2,
 }
 void orc_get_params(const orc_problem* p, double* o) { o[0] = p->k; o[1] = p->cc.re; o[2] = p->cc.im; o[3] = p->hbar; }
 
+void orc_field(const orc_problem* p, int64_t nobs, const double* obs3, const double* phi2, const double* dphi2,
+               double* out2)
+{
+    static const double rule[7][3] = BEM_TRI7;
+    const int64_t n = p->n, chunk = 1024;
+    const cplx* phi = (const cplx*)phi2;
+    const cplx* dphi = (const cplx*)dphi2;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nobs; i++) {
+        const double px = obs3[3 * i], py = obs3[3 * i + 1], pz = obs3[3 * i + 2];
+        cplx total = c64(0, 0);
+        for (int64_t j0 = 0; j0 < n; j0 += chunk) {
+            const int64_t j1 = std::min(n, j0 + chunk);
+            cplx part[4] = {c64(0, 0), c64(0, 0), c64(0, 0), c64(0, 0)};
+            for (int64_t j = j0; j < j1; j++) {
+                const Tri& t = p->tri[j];
+                const V3& nq = p->normal[j];
+                cplx L = c64(0, 0), M = c64(0, 0);
+                for (int q = 0; q < 7; q++) {
+                    double x1 = rule[q][0], x2 = rule[q][1];
+                    double x3 = (1.0 - x2) - x1;
+                    double qx = (x1 * t.a.x + x2 * t.b.x) + x3 * t.c.x;
+                    double qy = (x1 * t.a.y + x2 * t.b.y) + x3 * t.c.y;
+                    double qz = (x1 * t.a.z + x2 * t.b.z) + x3 * t.c.z;
+                    bem_field_point_accumulate(&L, &M, p->k, px, py, pz, nq.x, nq.y, nq.z, qx, qy, qz, rule[q][2]);
+                }
+                L = c64_scale(L, p->area[j]);
+                M = c64_scale(M, p->area[j]);
+                part[j & 3] = c64_add(part[j & 3], bem_field_term(L, M, phi[j], dphi[j]));
+            }
+            cplx cs = c64_add(c64_add(part[0], part[1]), c64_add(part[2], part[3]));
+            total = j0 == 0 ? cs : c64_add(total, cs);
+        }
+        out2[2 * i] = total.re;
+        out2[2 * i + 1] = total.im;
+    }
+}
+
 void orc_entries4(const orc_problem* p, int64_t np, const int64_t* ij, double* out)
 {
 #pragma omp parallel for schedule(dynamic, 64)

@@ -100,6 +100,16 @@ int orc_solve(orc_problem* p, double acatol, int64_t maxrank, double gmres_tol, 
    pivoting LU; phi/dphi/pressure out */
 int orc_dense_solve(orc_problem* p, double rho, double freq, double* phi, double* dphi, double* pressure);
 
+/* Field potential of the boundary solution phi, dphi (n complex each,
+   cluster order) at nobs observation points (nobs x 3): out[i] =
+   sum_j M_j(x_i) phi_j - L_j(x_i) dphi_j with the order-5 7-point rule, as
+   calculateFieldSolutionRegular / BemOperatorField
+   (boundaryelementsolver.cpp:2262-2353).  Summation order (fixed, the GPU's):
+   chunks of 1024 elements in order; inside a chunk four partial sums over
+   element index mod 4 combined (s0 + s1) + (s2 + s3).  Sources excluded. */
+void orc_field(const orc_problem* p, int64_t nobs, const double* obs3, const double* phi2, const double* dphi2,
+               double* out2);
+
 /* quadrature tables used (for tests): order-8 triangle rule 16x3 and the
    5-point Gauss-Legendre rule 5x2, the 16-point half Gauss-Legendre 16x2 */
 void orc_tables(double* tri16x3, double* gl5x2, double* half16x2);

@@ -55,6 +55,7 @@ def lib():
             "orc_solve": (ctypes.c_int, [_p, _dbl, _i64, _dbl, _i64, ctypes.c_uint64, _dbl, _dbl, ctypes.c_int,
                                          _p, _p, _p, _p]),
             "orc_dense_solve": (ctypes.c_int, [_p, _dbl, _dbl, _p, _p, _p]),
+            "orc_field": (None, [_p, ctypes.c_int64, _p, _p, _p, _p]),
             "orc_tables": (None, [_p, _p, _p]),
             "orc_sincos": (None, [_i64, _p, _p, _p]),
             "orc_project": (ctypes.c_int, [_p, _p, _p]),
@@ -182,6 +183,14 @@ class Problem:
                 "max_rank_phi", "max_rank_dphi", "evaluated_phi", "evaluated_dphi", "near_blocks", "far_blocks",
                 "gmres_est"]
         return phi, dphi, pr, dict(zip(keys, st[:len(keys)].tolist()))
+
+    def field(self, obs, phi, dphi):
+        """boundary term of the field potential at observation points (nobs, 3)"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        phi, dphi = _c(phi, np.complex128), _c(dphi, np.complex128)
+        out = np.empty(len(obs), np.complex128)
+        lib().orc_field(self.h, len(obs), _ptr(obs), _ptr(phi), _ptr(dphi), _ptr(out))
+        return out
 
     def dense_solve(self, rho, freq):
         n = self.n

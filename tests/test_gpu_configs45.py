@@ -1,0 +1,82 @@
+"""The large mixed-precision cases: the 76440-triangle capsule with a seeded
+30 % impedance patchwork, and 16 seeded ellipsoids of uneven size each at
+its own frequency.  The capsule's excitation is a seeded plane wave (the
+case names none; without one the solution is zero)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_config4_capsule_impedance(bem, meshes):
+    """Capsule L = 2 m, r = 0.25 m, edge 1.07 cm (76440 triangles), 1500 Hz,
+    Z = rho c (1.5 + 0.5i) on a seeded 30 % of the elements, rigid elsewhere,
+    leaf 64, eta 1.5, ACA 1e-4; FP32 with the FP64-residual restarts to 1e-8.
+    The FP32 pressure agrees with the FP64 path at the ACA tolerance, both
+    converge (GMRES(300) without the reference's H-LU preconditioner needs
+    several restarts here), and the reconstructed phi, dphi satisfy the
+    impedance / rigid boundary conditions element by element."""
+    f, c, rho = 1500.0, 343.0, 1.21
+    k = 2 * np.pi * f / c
+    tri = meshes.capsule(2.0, 0.25, 0.0107)
+    n = len(tri)
+    assert 70000 < n < 90000
+    bc = meshes.impedance_bc(n, k, 0.3, 4)
+    d = meshes.unit_direction(4)
+    S = bem.Solver(tri, bc, leaf=64, eta=1.5)
+    S.set_sources(plane=np.array([d[0], d[1], d[2], 1.0, 0.0]))
+    perm, _ = S.layout()
+    phi, dphi, p32, st32 = S.solve(f, c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP32)
+    _, _, p64, st64 = S.solve(f, c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP64)
+    print("config-4 n", n, "FP32 vs FP64", rel(p32, p64), "iters", st32["gmres_iters"], st64["gmres_iters"],
+          "residual", st32["residual"], st64["residual"])
+    assert st32["residual"] < 1e-7 and st64["residual"] < 1e-7
+    assert rel(p32, p64) < 1e-4
+    a = (bc[:, 0] + 1j * bc[:, 1])[perm]                  # cluster order
+    imp = a != 0
+    assert 0.25 < imp.mean() < 0.35
+    # a phi + dphi/dn = 0 on every element (b = 1, g = 0)
+    assert np.abs(a * phi + dphi).max() < 1e-12 * np.abs(dphi).max()
+    assert np.all(np.abs(dphi[~imp]) < 1e-12 * np.abs(dphi).max())
+
+
+def test_config5_ellipsoid_batch(bem, O, meshes):
+    """16 ellipsoids, semi-axes uniform in [0.2, 1.0] m (seeded), 5120 or
+    20480 triangles (250880 in all), each at a frequency uniform in
+    [200, 2000] Hz with k max(axes) <= 20, normal velocity N(0, 1) per
+    element, Burton-Miller, FP32: every solve converges, FP32 agrees with
+    FP64 on four of them, and the FP64 path equals the CPU restatement on the
+    smallest one."""
+    rng = np.random.default_rng(5)
+    c, rho = 343.0, 1.21
+    levels = rng.permutation([5] * 11 + [4] * 5)
+    total = 0
+    checked = 0
+    for q, level in enumerate(levels):
+        axes = rng.uniform(0.2, 1.0, 3)
+        f = rng.uniform(200.0, 2000.0)
+        f = min(f, 20.0 * c / (2 * np.pi * axes.max()))
+        tri = meshes.ellipsoid(int(level), axes)
+        n = len(tri)
+        total += n
+        bc = meshes.robin_bc(n, 0.0, 1.0, rng.standard_normal(n))
+        S = bem.Solver(tri, bc, leaf=64, eta=2.0)
+        _, _, p32, st = S.solve(f, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=q, precision=bem.FP32)
+        assert st["residual"] < 1e-7, (q, st["residual"])
+        if q % 4 == 0:
+            _, _, p64, _ = S.solve(f, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=q, precision=bem.FP64)
+            e = rel(p32, p64)
+            print("ellipsoid", q, "n", n, "f", round(f, 1), "axes", np.round(axes, 3), "FP32 vs FP64", e)
+            assert e < 1e-4
+            if level == 4 and checked == 0:
+                P = O.Problem(tri, bc, leaf=64, eta=2.0)
+                P.set_frequency(f, c, 1)
+                _, _, pc, _ = P.solve(1e-5, gmres_tol=1e-8, seed=q, rho=rho, freq=f)
+                assert rel(p64, pc) < 1e-10
+                checked += 1
+        S.close()
+    assert total == 11 * 20480 + 5 * 5120

@@ -40,6 +40,8 @@ _SIGS = {
     "bem_reconstruct": (_int, [_p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_reconstruct_g": (_int, [_p, _p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_field": (_int, [_p, _i64, _p, _p, _p, _p]),
+    "bem_read_msh": (_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i64, _p, ctypes.POINTER(_i64)]),
+    "bem_read_msh_error": (ctypes.c_char_p, []),
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),
     "bem_entries4": (_int, [_p, _i64, _p, _p]),
     "bem_entries": (_int, [_p, _int, _i64, _p, _p]),
@@ -294,6 +296,22 @@ class HMatrix:
 STAT_KEYS = ["t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres", "t_total", "gmres_iters", "residual",
              "max_rank_phi", "max_rank_dphi", "evaluated_phi", "evaluated_dphi", "near_blocks", "far_blocks",
              "refine_steps", "fp32_gmres_iters", "dphi_bytes"]
+
+
+def read_msh(path, include=None, exclude=None):
+    """Triangles (n, 9) of a GMSH .msh file (bem_read_msh); include / exclude
+    are lists of group ranges "a...b", group numbers or physical names."""
+    enc = lambda v: None if not v else ",".join(str(x) for x in v).encode()
+    cnt = _i64()
+    rc = lib().bem_read_msh(str(path).encode(), enc(include), enc(exclude), 0, None, ctypes.byref(cnt))
+    if rc != BEM_OK:
+        raise BemError(rc, lib().bem_read_msh_error().decode())
+    tri = np.empty((cnt.value, 9), np.float64)
+    rc = lib().bem_read_msh(str(path).encode(), enc(include), enc(exclude), cnt.value, _ptr(tri),
+                            ctypes.byref(cnt))
+    if rc != BEM_OK:
+        raise BemError(rc, lib().bem_read_msh_error().decode())
+    return tri
 
 
 class Solver:

@@ -164,3 +164,49 @@ def unit_direction(seed):
     """Direction uniform on the unit sphere (normalised N(0, I3))."""
     d = np.random.default_rng(seed).standard_normal(3)
     return d / np.linalg.norm(d)
+
+
+def write_msh(path, tri, groups=None, names=None, version="4.1"):
+    """Write triangles (n, 9) as an ASCII GMSH mesh (format "4.1" or "2.2").
+    groups: geometrical group (surface entity tag) per triangle, default 1;
+    each group is also a physical group of the same number, named by
+    names[group] if given.  Coincident vertices share one node."""
+    tri = np.asarray(tri, dtype=np.float64).reshape(-1, 9)
+    n = len(tri)
+    groups = np.ones(n, np.int64) if groups is None else np.asarray(groups, np.int64)
+    names = names or {}
+    pts = tri.reshape(-1, 3)
+    uniq, inv = np.unique(pts, axis=0, return_inverse=True)
+    conn = inv.reshape(-1, 3) + 1
+    tags = sorted(set(groups.tolist()))
+    out = []
+    out += ["$MeshFormat", f"{version} 0 8", "$EndMeshFormat"]
+    if names:
+        out += ["$PhysicalNames", str(len(names))]
+        out += [f'2 {g} "{names[g]}"' for g in sorted(names)]
+        out += ["$EndPhysicalNames"]
+    if version.startswith("4"):
+        out += ["$Entities", f"0 0 {len(tags)} 0"]
+        lo, hi = uniq.min(0), uniq.max(0)
+        for g in tags:
+            out.append(f"{g} {lo[0]} {lo[1]} {lo[2]} {hi[0]} {hi[1]} {hi[2]} 1 {g} 0")
+        out += ["$EndEntities", "$Nodes", f"1 {len(uniq)} 1 {len(uniq)}", f"2 {tags[0]} 0 {len(uniq)}"]
+        out += [str(i + 1) for i in range(len(uniq))]
+        out += [f"{x!r} {y!r} {z!r}" for x, y, z in uniq.tolist()]
+        out += ["$EndNodes", "$Elements", f"{len(tags)} {n} 1 {n}"]
+        eid = 1
+        for g in tags:
+            sel = np.nonzero(groups == g)[0]
+            out.append(f"2 {g} 2 {len(sel)}")
+            for t in sel:
+                out.append(f"{eid} {conn[t, 0]} {conn[t, 1]} {conn[t, 2]}")
+                eid += 1
+        out += ["$EndElements"]
+    else:
+        out += ["$Nodes", str(len(uniq))]
+        out += [f"{i + 1} {x!r} {y!r} {z!r}" for i, (x, y, z) in enumerate(uniq.tolist())]
+        out += ["$EndNodes", "$Elements", str(n)]
+        out += [f"{t + 1} 2 2 {groups[t]} {groups[t]} {conn[t, 0]} {conn[t, 1]} {conn[t, 2]}" for t in range(n)]
+        out += ["$EndElements"]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")

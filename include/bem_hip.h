@@ -167,6 +167,20 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
    complex. */
 int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* pressure2);
 
+/* Boundary mesh from a GMSH .msh file (mshReader::readMsh,
+   mshreader.cpp:10-428): ASCII format 2 (as the reference parses) or 4.1.
+   Triangles, and quadrilaterals split into (1, 2, 3) and (1, 3, 4), of the
+   included groups: include_csv / exclude_csv are comma-separated entries
+   "a...b" (geometrical group range, either end may be empty), a number
+   (geometrical group) or a physical group name; "all" or an empty include
+   list includes everything except the excluded groups.  With nodes9 NULL
+   only *ntri is set; otherwise nodes9 (cap x 9) receives the triangles in
+   file order.  Errors: BEM_ERR_ARG (file, format, cap too small; the reason
+   in bem_read_msh_error). */
+int bem_read_msh(const char* path, const char* include_csv, const char* exclude_csv, int64_t cap, double* nodes9,
+                 int64_t* ntri);
+const char* bem_read_msh_error(void);
+
 /* Frequency sweep with several right-hand sides per frequency
    (BemSolverController::run's loop, bemsolvercontroller.cpp:59-79, with the
    per-frequency hMatrixSolve of boundaryelementsolver.cpp:526-744).  For

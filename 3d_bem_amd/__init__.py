@@ -43,6 +43,11 @@ _SIGS = {
     "bem_read_msh": (_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i64, _p, ctypes.POINTER(_i64)]),
     "bem_read_msh_error": (ctypes.c_char_p, []),
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),
+    "bem_solver_set_preconditioner": (_int, [_p, _int]),
+    "bem_precond_create": (_int, [_p, _p, ctypes.POINTER(_p)]),
+    "bem_precond_destroy": (_int, [_p]),
+    "bem_precond_apply": (_int, [_p, _p, _p, _p]),
+    "bem_gmres_prec": (_int, [_p, _p, _p, _p, _p, _dbl, _i64, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)]),
     "bem_entries4": (_int, [_p, _i64, _p, _p]),
     "bem_entries": (_int, [_p, _int, _i64, _p, _p]),
     "bem_malloc": (_int, [_p, ctypes.c_size_t, ctypes.POINTER(_p)]),
@@ -282,15 +287,44 @@ class HMatrix:
     def mvm_dev(self, xd, yd, accumulate=False):
         self.ctx._check(lib().bem_hmvm(self.ctx.h, self.h, xd.ptr, yd.ptr, 1 if accumulate else 0))
 
-    def gmres(self, b, tol, restart=300, maxit=100000, x0=None):
+    def preconditioner(self):
+        """leaf-block LU preconditioner of this operator (bem_precond_create)"""
+        p = _p()
+        self.ctx._check(lib().bem_precond_create(self.ctx.h, self.h, ctypes.byref(p)))
+        return Preconditioner(self.ctx, p)
+
+    def gmres(self, b, tol, restart=300, maxit=100000, x0=None, precond=None):
         bd = self.ctx.array(b)
         xd = self.ctx.array(np.zeros(self.ctx.n, np.complex128) if x0 is None else x0)
         its, rel = _i64(), _dbl()
-        rc = lib().bem_gmres(self.ctx.h, self.h, bd.ptr, xd.ptr, float(tol), int(restart), int(maxit),
-                             ctypes.byref(its), ctypes.byref(rel))
+        rc = lib().bem_gmres_prec(self.ctx.h, self.h, None if precond is None else precond.h, bd.ptr, xd.ptr,
+                                  float(tol), int(restart), int(maxit), ctypes.byref(its), ctypes.byref(rel))
         if rc not in (BEM_OK, BEM_ERR_NOCONV):
             self.ctx._check(rc)
         return xd.download(), its.value, rel.value, rc
+
+
+class Preconditioner:
+    """bem_prec_t: z = M^-1 v with M the diagonal leaf blocks"""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def apply(self, v):
+        vd, zd = self.ctx.array(v), self.ctx.array()
+        self.ctx._check(lib().bem_precond_apply(self.ctx.h, self.h, vd.ptr, zd.ptr))
+        return zd.download()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bem_precond_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 STAT_KEYS = ["t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres", "t_total", "gmres_iters", "residual",
@@ -367,6 +401,12 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def set_preconditioner(self, kind):
+        """0: none, 1: leaf-block LU (bem_solver_set_preconditioner)"""
+        rc = lib().bem_solver_set_preconditioner(self.h, int(kind))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_preconditioner")
 
     def field(self, obs):
         """sound pressure at observation points (nobs, 3) from the last solve"""

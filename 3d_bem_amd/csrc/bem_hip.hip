@@ -958,13 +958,105 @@ __global__ void k_scale_by_norm(int64_t n, const bem_c64* src, const double* nrm
 }
 
 // x += sum_l V_l y_l
-__global__ void k_combine(int64_t n, int cnt, const bem_c64* V, const bem_c64* y, bem_c64* x)
+__global__ void k_combine(int64_t n, int cnt, const bem_c64* V, const bem_c64* y, bem_c64* x, int accumulate)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bem_c64 acc = x[i];
+    bem_c64 acc = accumulate ? x[i] : c64(0, 0);
     for (int l = 0; l < cnt; l++) acc = c64_fma(V[(int64_t)l * n + i], y[l], acc);
     x[i] = acc;
+}
+
+// ------------------------------------------- leaf-block preconditioner
+// M = block diagonal of the operator's diagonal leaf blocks (the leaf level
+// of the reference's H-LU, HArithm::LUDecomposition harithm.cpp:1192-1296),
+// each factored by LU with partial pivoting in FP64.
+
+constexpr int kPrecMax = 64;   // largest diagonal block
+
+struct PrecBlock { int64_t row0, lu; int32_t m, pad; };
+
+// one workgroup per diagonal block: copy to LDS (column-major), factor,
+// write L\U and the pivot rows back; flag[0] = 1 on a zero pivot
+template <class S>
+__global__ __launch_bounds__(kWG) void k_blu_factor(const PrecBlock* __restrict__ blk, const int64_t* __restrict__ src,
+                                                    const S* __restrict__ dense, bem_c64* __restrict__ lu,
+                                                    int32_t* __restrict__ piv, int32_t* flag)
+{
+    __shared__ bem_c64 a[kPrecMax * kPrecMax];
+    __shared__ ArgMax sred[kWG / 64];
+    __shared__ int prow;
+    const PrecBlock B = blk[blockIdx.x];
+    const int m = B.m, tid = threadIdx.x;
+    const S* d = dense + src[blockIdx.x];
+    for (int t = tid; t < m * m; t += kWG) a[t] = ld64(d + t);
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        ArgMax am = block_argmax<kWG>(a + k * m + k, m - k, -1, sred);
+        if (tid == 0) {
+            prow = k + (int)am.i;
+            piv[B.row0 + k] = prow;
+            if (!(am.v > 0)) flag[0] = 1;
+        }
+        __syncthreads();
+        const int p = prow;
+        if (p != k)
+            for (int j = tid; j < m; j += kWG) {
+                bem_c64 t = a[j * m + k];
+                a[j * m + k] = a[j * m + p];
+                a[j * m + p] = t;
+            }
+        __syncthreads();
+        const bem_c64 inv = c64_div(c64(1.0, 0.0), a[k * m + k]);
+        for (int i = k + 1 + tid; i < m; i += kWG) a[k * m + i] = c64_mul(a[k * m + i], inv);
+        __syncthreads();
+        const int w = m - k - 1;
+        for (int t = tid; t < w * w; t += kWG) {
+            const int i = k + 1 + t % w, j = k + 1 + t / w;
+            a[j * m + i] = c64_fms(a[k * m + i], a[j * m + k], a[j * m + i]);
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < m * m; t += kWG) lu[B.lu + t] = a[t];
+}
+
+// z = M^-1 v on every diagonal block: one wavefront per block, z in LDS
+__global__ __launch_bounds__(64) void k_blu_apply(const PrecBlock* __restrict__ blk, const bem_c64* __restrict__ lu,
+                                                  const int32_t* __restrict__ piv, const bem_c64* __restrict__ v,
+                                                  bem_c64* __restrict__ z)
+{
+    __shared__ bem_c64 y[kPrecMax];
+    const PrecBlock B = blk[blockIdx.x];
+    const int m = B.m, i = threadIdx.x;
+    const bem_c64* a = lu + B.lu;
+    if (i < m) y[i] = v[B.row0 + i];
+    __syncthreads();
+    if (i == 0)
+        for (int k = 0; k < m; k++) {
+            const int p = piv[B.row0 + k];
+            if (p != k) { bem_c64 t = y[k]; y[k] = y[p]; y[p] = t; }
+        }
+    __syncthreads();
+    for (int k = 0; k < m; k++) {          // unit lower triangle
+        const bem_c64 yk = y[k];
+        if (i > k && i < m) y[i] = c64_fms(a[k * m + i], yk, y[i]);
+        __syncthreads();
+    }
+    for (int k = m - 1; k >= 0; k--) {     // upper triangle
+        if (i == k) y[k] = c64_div(y[k], a[k * m + k]);
+        __syncthreads();
+        const bem_c64 yk = y[k];
+        if (i < k) y[i] = c64_fms(a[k * m + i], yk, y[i]);
+        __syncthreads();
+    }
+    if (i < m) z[B.row0 + i] = y[i];
+}
+
+// x += z
+__global__ void k_axpy1(int64_t n, const bem_c64* z, bem_c64* x)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = c64_add(x[i], z[i]);
 }
 
 // r = b - r
@@ -1134,6 +1226,14 @@ struct bem_hmat {
     std::vector<int64_t> uoff, voff, aoff;
     std::vector<int32_t> rank;
     double stats[8] = {0};
+};
+
+struct bem_prec {
+    bem_ctx* ctx = nullptr;
+    int64_t nblk = 0;
+    PrecBlock* d_blk = nullptr;
+    bem_c64* d_lu = nullptr;
+    int32_t* d_piv = nullptr;
 };
 
 namespace {
@@ -2205,10 +2305,95 @@ int bem_hmat_stats(bem_hmat_t h, double* s)
     return BEM_OK;
 }
 
+int bem_precond_create(bem_ctx_t c, bem_hmat_t h, bem_prec_t* out)
+{
+    if (!c || !h || !out || h->ctx != c) return set_err(c, BEM_ERR_ARG, "bem_precond_create: bad arguments");
+    *out = nullptr;
+    HIPC(c, hipSetDevice(c->device));
+    // the diagonal leaf blocks must tile [0, n)
+    std::vector<PrecBlock> blk;
+    std::vector<int64_t> src;
+    int64_t covered = 0, lu = 0;
+    for (size_t b = 0; b < c->blocks.size(); b++) {
+        const auto& B = c->blocks[b];
+        if (B.adm || B.row0 != B.col0 || B.rows != B.cols) continue;
+        if (B.rows > kPrecMax)
+            return set_err(c, BEM_ERR_ARG, "bem_precond_create: diagonal block larger than 64 (leaf size above 64)");
+        blk.push_back(PrecBlock{B.row0, lu, (int32_t)B.rows, 0});
+        src.push_back(c->dense_off[b]);
+        lu += B.rows * B.rows;
+        covered += B.rows;
+    }
+    if (covered != c->n) return set_err(c, BEM_ERR_ARG, "bem_precond_create: diagonal leaf blocks do not cover the matrix");
+    auto p = new bem_prec();
+    p->ctx = c;
+    p->nblk = (int64_t)blk.size();
+    std::vector<void*> own;
+    int32_t* d_flag;
+    int64_t* d_src;
+    int rc;
+    auto fail = [&](int code) {
+        free_list(c, own);
+        pfree(c, p->d_blk); pfree(c, p->d_lu); pfree(c, p->d_piv);
+        delete p;
+        return code;
+    };
+    if ((rc = dupload(c, &p->d_blk, blk.data(), blk.size(), nullptr))) return fail(rc);
+    if ((rc = dalloc(c, &p->d_lu, (size_t)lu, nullptr))) return fail(rc);
+    if ((rc = dalloc(c, &p->d_piv, (size_t)c->n, nullptr))) return fail(rc);
+    if ((rc = dupload(c, &d_src, src.data(), src.size(), &own))) return fail(rc);
+    if ((rc = dalloc(c, &d_flag, 1, &own))) return fail(rc);
+    HIPC(c, hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
+    if (h->prec == BEM_PREC_FP32)
+        hipLaunchKernelGGL(k_blu_factor<bem_c32>, dim3((unsigned)p->nblk), dim3(kWG), 0, c->stream, p->d_blk, d_src,
+                           (const bem_c32*)h->dense, p->d_lu, p->d_piv, d_flag);
+    else
+        hipLaunchKernelGGL(k_blu_factor<bem_c64>, dim3((unsigned)p->nblk), dim3(kWG), 0, c->stream, p->d_blk, d_src,
+                           (const bem_c64*)h->dense, p->d_lu, p->d_piv, d_flag);
+    if ((rc = launch_check(c, "k_blu_factor"))) return fail(rc);
+    int32_t hflag = 0;
+    HIPC(c, hipMemcpyAsync(&hflag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    free_list(c, own);
+    if (hflag) {
+        own.clear();
+        return fail(set_err(c, BEM_ERR_NAN, "bem_precond_create: singular diagonal block"));
+    }
+    *out = p;
+    return BEM_OK;
+}
+
+int bem_precond_destroy(bem_prec_t p)
+{
+    if (!p) return BEM_ERR_ARG;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    pfree(p->ctx, p->d_blk);
+    pfree(p->ctx, p->d_lu);
+    pfree(p->ctx, p->d_piv);
+    delete p;
+    return BEM_OK;
+}
+
+int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
+{
+    if (!c || !p || !v || !z || p->ctx != c || v == z) return set_err(c, BEM_ERR_ARG, "bem_precond_apply: bad arguments");
+    HIPC(c, hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_blu_apply, dim3((unsigned)p->nblk), dim3(64), 0, c->stream, p->d_blk, p->d_lu, p->d_piv,
+                       (const bem_c64*)v, (bem_c64*)z);
+    return launch_check(c, "k_blu_apply");
+}
+
 int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int64_t restart, int64_t maxit,
               int64_t* iters, double* rel_resid)
 {
-    if (!c || !h || !b || !x || restart < 1 || maxit < 0 || h->ctx != c)
+    return bem_gmres_prec(c, h, nullptr, b, x, tol, restart, maxit, iters, rel_resid);
+}
+
+int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void* x, double tol, int64_t restart,
+                   int64_t maxit, int64_t* iters, double* rel_resid)
+{
+    if (!c || !h || !b || !x || restart < 1 || maxit < 0 || h->ctx != c || (pc && pc->ctx != c))
         return set_err(c, BEM_ERR_ARG, "bem_gmres: bad arguments");
     HIPC(c, hipSetDevice(c->device));
     const int64_t n = c->n;
@@ -2216,7 +2401,7 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
     const int nchunk = (int)((n + kDotChunk - 1) / kDotChunk);
     // workspace (complex slots): V (m+1) n, w n, r n, h, hacc, y, g, cs_s
     // (m+2 each), H (m+1) m, dot partials (m+1) nchunk, then doubles
-    int64_t need = (m + 1) * n + 2 * n + 5 * (m + 2) + (m + 1) * m + (m + 1) * nchunk + (m + 2) + 640;
+    int64_t need = (m + 1) * n + 3 * n + 5 * (m + 2) + (m + 1) * m + (m + 1) * nchunk + (m + 2) + 640;
     if (c->gm_cap < need) {
         pfree(c, c->d_gm);
         c->d_gm = nullptr;
@@ -2235,7 +2420,8 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
     bem_c64* dcs_s = dg + (m + 2);
     bem_c64* dH = dcs_s + (m + 2);
     bem_c64* dpart = dH + (m + 1) * m;
-    double* dcs_c = (double*)(dpart + (m + 1) * nchunk);
+    bem_c64* zp = dpart + (m + 1) * nchunk;   // preconditioned vector (right preconditioning)
+    double* dcs_c = (double*)(zp + n);
     double* dn = dcs_c + 2 * (m + 2);      // norm partials (256) + result + residual estimate
     double* dnorm = dn + 512;
     double* dres = dnorm + 1;
@@ -2282,7 +2468,10 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
         // residual estimate is copied to the pinned ring slot jj & 1
         auto launch_col = [&](int64_t jj) -> int {
             int rc2;
-            if ((rc2 = bem_hmvm(c, h, V + jj * n, w, 0))) return rc2;
+            if (pc) {   // w = A M^-1 v_jj
+                if ((rc2 = bem_precond_apply(c, pc, V + jj * n, zp))) return rc2;
+                if ((rc2 = bem_hmvm(c, h, zp, w, 0))) return rc2;
+            } else if ((rc2 = bem_hmvm(c, h, V + jj * n, w, 0))) return rc2;
             const int cnt = (int)(jj + 1);
             for (int pass = 0; pass < 2; pass++) {   // classical Gram-Schmidt, twice
                 const int ngroups = (cnt + kDotVecs - 1) / kDotVecs;
@@ -2336,7 +2525,13 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
         }
         if (j > 0) {
             HIPC(c, hipMemcpyAsync(dy, y.data(), j * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL(k_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)j, V, dy, xd);
+            if (pc) {   // x += M^-1 (V y); w is free here
+                hipLaunchKernelGGL(k_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)j, V, dy, w, 0);
+                if ((rc = bem_precond_apply(c, pc, w, zp))) return rc;
+                hipLaunchKernelGGL(k_axpy1, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, zp, xd);
+            } else {
+                hipLaunchKernelGGL(k_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)j, V, dy, xd, 1);
+            }
             if ((rc = launch_check(c, "k_combine"))) return rc;
         }
     }

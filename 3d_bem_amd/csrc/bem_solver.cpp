@@ -133,11 +133,12 @@ void HArithm::MVM(DeviceVector& y, HMatrix& h, const DeviceVector& x)
     check(h.context(), bem_synchronize(h.context()), "HArithm::MVM");
 }
 
-int64_t GMRES::gmresSolve(HMatrix& A, DeviceVector& x, const DeviceVector& b, double tol, long restart, double* rel)
+int64_t GMRES::gmresSolve(HMatrix& A, DeviceVector& x, const DeviceVector& b, double tol, long restart, double* rel,
+                          bem_prec_t precond)
 {
     int64_t its = 0;
     double r = 0;
-    int rc = bem_gmres(A.context(), A.handle(), b.data(), x.data(), tol, restart, 100000, &its, &r);
+    int rc = bem_gmres_prec(A.context(), A.handle(), precond, b.data(), x.data(), tol, restart, 100000, &its, &r);
     if (rc != BEM_OK && rc != BEM_ERR_NOCONV) check(A.context(), rc, "GMRES::gmresSolve");
     if (rel) *rel = r;
     return its;
@@ -325,14 +326,17 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     iterations.assign(nr, 0);
     DeviceVector x(ctx_, E.n), phi(ctx_, E.n), dphi(ctx_, E.n), p(ctx_, E.n);
     double est = 0;
+    bem_prec_t pc = nullptr;
+    if (P.preconditioner) check(ctx_, bem_precond_create(ctx_, dPhiHMat->handle(), &pc), "bem_precond_create");
     for (size_t k = 0; k < nr; k++) {
         x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
-        iterations[k] = GMRES::gmresSolve(*dPhiHMat, x, *b[k], tol, P.gmresRestart, &est);
+        iterations[k] = GMRES::gmresSolve(*dPhiHMat, x, *b[k], tol, P.gmresRestart, &est, pc);
         check(ctx_, bem_reconstruct_g(ctx_, x.data(), g[k]->data(), airDensity, frequency, phi.data(), dphi.data(),
                                       p.data()),
               "bem_reconstruct_g");
         pressure[k] = p.download();
     }
+    if (pc) bem_precond_destroy(pc);
     double t4 = now_ms();
     dPhiHMat->clear();
     stats[0] = t1 - t0; stats[1] = t2 - t1; stats[2] = t3 - t2; stats[3] = t4 - t3; stats[4] = t4 - t0;
@@ -373,7 +377,10 @@ void BoundaryElementSolver::hMatrixSolve()
     x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
     double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
     double est = 0;
-    int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est);
+    bem_prec_t pc = nullptr;
+    if (P.preconditioner) check(ctx_, bem_precond_create(ctx_, dPhiHMat->handle(), &pc), "bem_precond_create");
+    int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est, pc);
+    if (pc) bem_precond_destroy(pc);
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t4 = now_ms();
     // true residual (:682-684)
@@ -413,6 +420,7 @@ struct bem_solver {
     double eta = 0;
     bem::BoundaryElements input;   // input order
     std::vector<std::unique_ptr<bem::BoundaryElementSolver>> lanes;
+    int preconditioner = 0;
 };
 
 extern "C" {
@@ -510,6 +518,7 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         S.bemSolverParameters.gmresRestart = (long)restart;
         S.bemSolverParameters.pivotSeed = seed;
         S.bemSolverParameters.precision = precision;
+        S.bemSolverParameters.preconditioner = s->preconditioner;
         S.calculateBoundarySolution();
         int64_t n = S.boundaryElements.n;
         if (phi) std::memcpy(phi, S.phiSolution.data(), n * sizeof(bem::cplx));
@@ -523,6 +532,13 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         s->err = e.what();
         return BEM_ERR_HIP;
     }
+    return BEM_OK;
+}
+
+int bem_solver_set_preconditioner(bem_solver_t s, int kind)
+{
+    if (!s || !s->s || kind < 0 || kind > 1) return BEM_ERR_ARG;
+    s->preconditioner = kind;
     return BEM_OK;
 }
 
@@ -587,6 +603,7 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 P.gmresRestart = (long)restart;
                 P.pivotSeed = seed;
                 P.precision = precision;
+                P.preconditioner = s->preconditioner;
                 std::vector<std::vector<bem::cplx>> pr;
                 std::vector<int64_t> its;
                 for (int64_t f = q; f < nf; f += L) {

@@ -96,7 +96,7 @@ struct HArithm {
 // HMatrix/gmres.h:170-180
 struct GMRES {
     static int64_t gmresSolve(HMatrix& A, DeviceVector& x, const DeviceVector& b, double tolerance = 0.01,
-                              long iterations = 300, double* relResidual = nullptr);
+                              long iterations = 300, double* relResidual = nullptr, bem_prec_t precond = nullptr);
 };
 
 enum BemCoupling : int { NoCoupling = 0, BurtonMillerCoupling = 1, KirkupCoupling = 2 };
@@ -110,6 +110,7 @@ struct BemSolverParameters {
     long gmresRestart = 300;
     uint64_t pivotSeed = 1;
     int precision = BEM_PREC_FP64;
+    int preconditioner = 0;       // 0 none, 1 leaf-block LU (bem_precond_create)
 };
 
 struct PointSource { double x, y, z; cplx weight; };

@@ -98,6 +98,21 @@ int bem_hmat_stats(bem_hmat_t h, double* stats8);
    return.  Returns BEM_ERR_NOCONV when maxit is reached above tol. */
 int bem_gmres(bem_ctx_t ctx, bem_hmat_t h, const void* b_dev, void* x_dev, double tol, int64_t restart,
               int64_t maxit, int64_t* iters, double* rel_resid);
+/* Leaf-block preconditioner of an assembled operator: LU with partial
+   pivoting (FP64) of every diagonal leaf block, i.e. the leaf level of the
+   reference's H-LU preconditioner (HMatrix(const HMatrix&, rank, err) +
+   HArithm::LUDecomposition, hmatrix.cpp:770-776, harithm.cpp:1192-1296)
+   without its low-rank coupling.  Needs diagonal blocks of at most 64 rows
+   (leaf size <= 64).  bem_gmres_prec runs GMRES on A M^-1 (right
+   preconditioning; x is the solution of A x = b); pc = NULL is bem_gmres.
+   bem_precond_apply: z = M^-1 v (device vectors, n complex, z != v). */
+typedef struct bem_prec* bem_prec_t;
+int bem_precond_create(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t* out);
+int bem_precond_destroy(bem_prec_t p);
+int bem_precond_apply(bem_ctx_t ctx, bem_prec_t p, const void* v_dev, void* z_dev);
+int bem_gmres_prec(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t p, const void* b_dev, void* x_dev, double tol,
+                   int64_t restart, int64_t maxit, int64_t* iters, double* rel_resid);
+
 /* boundary-condition reconstruction (boundaryelementsolver.cpp:729-740) and
    p = i rho 2 pi f phi (phiSolutionToSoundPressure, :2251-2261) */
 int bem_reconstruct(bem_ctx_t ctx, const void* x_dev, double rho, double freq, void* phi_dev, void* dphi_dev,
@@ -159,6 +174,10 @@ int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* points6
 int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);
+
+/* GMRES preconditioner of bem_solver_solve / bem_solver_sweep: 0 none (the
+   default), 1 the leaf-block LU of bem_precond_create */
+int bem_solver_set_preconditioner(bem_solver_t s, int kind);
 
 /* Sound pressure at observation points (nobs x 3, host) from the last
    bem_solver_solve: the boundary term of bem_field plus the incident point

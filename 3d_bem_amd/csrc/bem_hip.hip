@@ -468,20 +468,23 @@ __device__ __forceinline__ double aca_update_norm(C alpha, const C* A, int r, co
 // issued kCrossUnroll at a time ahead of their products so that the L2 / HBM
 // latency is paid once per kCrossUnroll terms rather than once per term.
 constexpr int kCrossUnroll = 8;
+// Offsets are 32-bit from the block's (wavefront-uniform) base: a block's
+// factors hold at most kCapMax x (m + n) < 2^31 entries.
 template <bool W_FIRST, class C>
-__device__ __forceinline__ C cross_sum(const C* w, const C* B, int64_t ld, int64_t idx, int r)
+__device__ __forceinline__ C cross_sum(const C* w, const C* B, int64_t ld64, int64_t idx64, int r)
 {
+    const uint32_t ld = (uint32_t)ld64, idx = (uint32_t)idx64;
     C s = cx_zero<C>();
     int l = 0;
     for (; l + kCrossUnroll <= r; l += kCrossUnroll) {
         C v[kCrossUnroll];
 #pragma unroll
-        for (int u = 0; u < kCrossUnroll; u++) v[u] = B[(int64_t)(l + u) * ld + idx];
+        for (int u = 0; u < kCrossUnroll; u++) v[u] = B[(uint32_t)(l + u) * ld + idx];
 #pragma unroll
         for (int u = 0; u < kCrossUnroll; u++) s = W_FIRST ? cx_fma(w[l + u], v[u], s) : cx_fma(v[u], w[l + u], s);
     }
     for (; l < r; l++) {
-        C v = B[(int64_t)l * ld + idx];
+        C v = B[(uint32_t)l * ld + idx];
         s = W_FIRST ? cx_fma(w[l], v, s) : cx_fma(v, w[l], s);
     }
     return s;
@@ -508,9 +511,7 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
 {
     const DevMesh& M = a.M;
     if (near_or_self(M, i, j)) return false;
-    bem_c32f4 o = regular_ops_f32_vtx<1>(a.F32, M.px[i], M.py[i], M.pz[i], (float)M.nx[i], (float)M.ny[i],
-                                         (float)M.nz[i], (float)M.nx[j], (float)M.ny[j], (float)M.nz[j],
-                                         (float)M.area[j], load_vtx(M, j));
+    bem_c32f4 o = regular_ops_f32_edge<1>(a.F32, M, i, j);
     *e = bc_entry_f32(a.F32, a.F.subst, a.kind, o, j);
     return true;
 }
@@ -1560,13 +1561,21 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
         soa[3 * n + e] = normal3[3 * e]; soa[4 * n + e] = normal3[3 * e + 1]; soa[5 * n + e] = normal3[3 * e + 2];
         soa[6 * n + e] = area[e];
     }
-    std::vector<double> vtx(9 * n);
-    for (int64_t e = 0; e < n; e++)
+    std::vector<double> vtx(9 * n), edg(9 * n);
+    for (int64_t e = 0; e < n; e++) {
         for (int d = 0; d < 9; d++) vtx[d * n + e] = nodes9[9 * e + d];
-    double *d_soa, *d_nodes, *d_q, *d_vtx;
+        for (int d = 0; d < 3; d++) {
+            const double cv = nodes9[9 * e + 6 + d];
+            edg[d * n + e] = cv;
+            edg[(3 + d) * n + e] = nodes9[9 * e + d] - cv;
+            edg[(6 + d) * n + e] = nodes9[9 * e + 3 + d] - cv;
+        }
+    }
+    double *d_soa, *d_nodes, *d_q, *d_vtx, *d_edg;
     int rc;
     if ((rc = dupload(c, &d_soa, soa.data(), soa.size(), &c->allocs_mesh))) return rc;
     if ((rc = dupload(c, &d_vtx, vtx.data(), vtx.size(), &c->allocs_mesh))) return rc;
+    if ((rc = dupload(c, &d_edg, edg.data(), edg.size(), &c->allocs_mesh))) return rc;
     if ((rc = dupload(c, &d_nodes, c->h_nodes.data(), c->h_nodes.size(), &c->allocs_mesh))) return rc;
     if ((rc = dalloc(c, &d_q, (size_t)48 * n, &c->allocs_mesh))) return rc;
     DevMesh& M = c->mesh;
@@ -1576,6 +1585,7 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     M.area = d_soa + 6 * n;
     M.qx = d_q; M.qy = d_q + 16 * n; M.qz = d_q + 32 * n;
     M.vtx = d_vtx;
+    M.edg = d_edg;
     hipLaunchKernelGGL(k_quadpoints, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, d_nodes, d_q, d_q + 16 * n,
                        d_q + 32 * n);
     if ((rc = launch_check(c, "k_quadpoints"))) return rc;

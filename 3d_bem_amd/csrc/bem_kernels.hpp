@@ -48,6 +48,7 @@ struct DevMesh {
     const double* area;
     const double *qx, *qy, *qz;
     const double* vtx;            // vertices a, b, c: vtx[c * n + j], c = 0..8 (ax ay az bx .. cz)
+    const double* edg;            // c, a - c, b - c: edg[c * n + j] (the FP32 path's point form)
     const int64_t* nr_ptr;        // near-singular CSR (n + 1)
     const int32_t* nr_row;        // row of each pair
     const int32_t* nr_col;
@@ -272,12 +273,24 @@ __device__ __forceinline__ bem_c32 c32_fma(bem_c32 a, bem_c32 b, bem_c32 acc)
     return c32(fmaf(a.re, b.re, fmaf(-a.im, b.im, acc.re)), fmaf(a.re, b.im, fmaf(a.im, b.re, acc.im)));
 }
 
+// one quadrature point given R = p - q (FP64), same formulas as
+// bem_point_accumulate
+__device__ __forceinline__ void point_accumulate_f32_r(bem_c32f4* acc, const DevFreq32& F, double Rx, double Ry,
+                                                       double Rz, float npx, float npy, float npz, float nqx,
+                                                       float nqy, float nqz, float upnq, float w);
+
 // one quadrature point, same formulas as bem_point_accumulate
 __device__ __forceinline__ void point_accumulate_f32(bem_c32f4* acc, const DevFreq32& F, double px, double py,
                                                      double pz, float npx, float npy, float npz, float nqx, float nqy,
                                                      float nqz, float upnq, double qx, double qy, double qz, float w)
 {
-    double Rx = px - qx, Ry = py - qy, Rz = pz - qz;
+    point_accumulate_f32_r(acc, F, px - qx, py - qy, pz - qz, npx, npy, npz, nqx, nqy, nqz, upnq, w);
+}
+
+__device__ __forceinline__ void point_accumulate_f32_r(bem_c32f4* acc, const DevFreq32& F, double Rx, double Ry,
+                                                       double Rz, float npx, float npy, float npz, float nqx,
+                                                       float nqy, float nqz, float upnq, float w)
+{
     double rr = fma(Rz, Rz, fma(Ry, Ry, Rx * Rx));
     float inv_r = __builtin_amdgcn_rsqf((float)rr);
     double r0 = (double)((float)rr * inv_r);
@@ -352,6 +365,35 @@ __device__ __forceinline__ bem_c32f4 regular_ops_f32_vtx(const DevFreq32& F, dou
         quad_point(e, q, &qx, &qy, &qz);
         point_accumulate_f32(&o, F, px, py, pz, npx, npy, npz, nqx, nqy, nqz, upnq, qx, qy, qz, kTriWf[q]);
     }
+    o.L = c32(o.L.re * area, o.L.im * area);
+    o.M = c32(o.M.re * area, o.M.im * area);
+    o.Mt = c32(o.Mt.re * area, o.Mt.im * area);
+    o.N = c32(o.N.re * area, o.N.im * area);
+    return o;
+}
+
+// regular_ops_f32 with R = d - xi1 (a - c) - xi2 (b - c), d = p - c: the
+// point q = c + xi1 (a - c) + xi2 (b - c) (the rule's point, rounded
+// differently from the FP64 path's form) at two FP64 fma per coordinate
+template <int UNROLL = 1>
+__device__ __forceinline__ bem_c32f4 regular_ops_f32_edge(const DevFreq32& F, const DevMesh& M, int64_t i, int64_t j)
+{
+    const int64_t n = M.n;
+    const double* g = M.edg + j;
+    const double dx = M.px[i] - g[0], dy = M.py[i] - g[n], dz = M.pz[i] - g[2 * n];
+    const double ax = g[3 * n], ay = g[4 * n], az = g[5 * n], bx = g[6 * n], by = g[7 * n], bz = g[8 * n];
+    const float npx = (float)M.nx[i], npy = (float)M.ny[i], npz = (float)M.nz[i];
+    const float nqx = (float)M.nx[j], nqy = (float)M.ny[j], nqz = (float)M.nz[j];
+    const float upnq = fmaf(npz, nqz, fmaf(npy, nqy, npx * nqx));
+    bem_c32f4 o;
+    o.L = o.M = o.Mt = o.N = c32(0.0f, 0.0f);
+#pragma unroll UNROLL
+    for (int q = 0; q < 16; q++) {
+        const double x1 = kTriX[q][0], x2 = kTriX[q][1];
+        point_accumulate_f32_r(&o, F, fma(-x1, ax, fma(-x2, bx, dx)), fma(-x1, ay, fma(-x2, by, dy)),
+                               fma(-x1, az, fma(-x2, bz, dz)), npx, npy, npz, nqx, nqy, nqz, upnq, kTriWf[q]);
+    }
+    const float area = (float)M.area[j];
     o.L = c32(o.L.re * area, o.L.im * area);
     o.M = c32(o.M.re * area, o.M.im * area);
     o.Mt = c32(o.Mt.re * area, o.Mt.im * area);

@@ -28,6 +28,8 @@ _SIGS = {
     "bem_last_error": (ctypes.c_char_p, [_p]),
     "bem_upload_mesh": (_int, [_p, _i64, _p, _p, _p, _p, _dbl]),
     "bem_set_bc": (_int, [_p, _p, _p, _p]),
+    "bem_set_plane": (_int, [_p, _p, _p]),
+    "bem_solver_set_impedance_plane": (_int, [_p, _p, _p]),
     "bem_upload_partition": (_int, [_p, _i64, _p, _p, _p, _p, _p]),
     "bem_set_frequency": (_int, [_p, _dbl, _dbl, _dbl]),
     "bem_assemble": (_int, [_p, _int, _dbl, _i64, _int, _u64, ctypes.POINTER(_p)]),
@@ -170,6 +172,14 @@ class Context:
     def set_bc(self, alpha, beta, f):
         a, b, g = (_c(v, np.complex128) for v in (alpha, beta, f))
         self._check(lib().bem_set_bc(self.h, _ptr(a), _ptr(b), _ptr(g)))
+
+    def set_plane(self, point=None, normal=None):
+        """one impedance plane (rigid image) through point with normal; None removes it"""
+        if normal is None:
+            self._check(lib().bem_set_plane(self.h, None, None))
+            return
+        pt, nr = _c(point, np.float64), _c(normal, np.float64)
+        self._check(lib().bem_set_plane(self.h, _ptr(pt), _ptr(nr)))
 
     def upload_partition(self, blocks5):
         B = _c(blocks5, np.int64)
@@ -401,6 +411,17 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def set_impedance_plane(self, point=None, normal=None):
+        """the scene's impedance plane (a rigid image) through point with
+        normal; None removes it (bem_solver_set_impedance_plane)"""
+        if normal is None:
+            rc = lib().bem_solver_set_impedance_plane(self.h, None, None)
+        else:
+            pt, nr = _c(point, np.float64), _c(normal, np.float64)
+            rc = lib().bem_solver_set_impedance_plane(self.h, _ptr(pt), _ptr(nr))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
 
     def set_preconditioner(self, kind):
         """0: none, 1: leaf-block LU (bem_solver_set_preconditioner)"""

@@ -74,6 +74,7 @@ __global__ void k_quadpoints(int64_t n, const double* __restrict__ nodes, double
 // Dense near-field tiles (K1): regular quadrature of every (row, col) pair of
 // a 64 x 64 tile with the column data staged in LDS.  Self and
 // near-singular pairs are skipped here and written by k_fixup.
+template <bool PLANE>
 __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile* __restrict__ tiles,
                                                bem_c64* outPhi, bem_c64* outDPhi)
 {
@@ -113,8 +114,15 @@ __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile*
         bem_c64 X, Y;
         bem_xy(&o, F.cc, 0, &X, &Y);
         int64_t d = T.dest + r + (int64_t)c * T.ld;
-        if (outPhi) outPhi[d] = bem_phi_entry(X, Y, F.subst[j], F.cphi[j]);
-        if (outDPhi) outDPhi[d] = bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
+        bem_c64 ep = bem_phi_entry(X, Y, F.subst[j], F.cphi[j]);
+        bem_c64 ed = bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
+        if (PLANE) {   // impedance plane: + the mirror image's entry
+            refl_xy(M, F, i, j, &X, &Y);
+            ep = c64_add(ep, bem_phi_entry(X, Y, F.subst[j], F.cphi[j]));
+            ed = c64_add(ed, bem_dphi_entry(X, Y, F.subst[j], F.cd[j]));
+        }
+        if (outPhi) outPhi[d] = ep;
+        if (outDPhi) outDPhi[d] = ed;
     }
 }
 
@@ -140,11 +148,13 @@ __global__ __launch_bounds__(kWG) void k_singular(DevMesh M, DevFreq F, int kind
         o = near_ops(F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
                      M.nr_geo + M.nr_off[t], M.nr_nsub[t]);
     }
-    sing[t] = bc_entry(F, kind, o, i, j);
+    bem_c64 e = bc_entry(F, kind, o, i, j);
+    sing[t] = M.rvtx ? c64_add(e, refl_entry(M, F, kind, i, j)) : e;
 }
 
 // FP32 dense near-field tiles: as k_dense with the regular quadrature of
 // regular_ops_f32, written as complex FP32
+template <bool PLANE>
 __global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const int32_t* __restrict__ subst, int kind,
                                                  const Tile* __restrict__ tiles, bem_c32* out)
 {
@@ -182,7 +192,12 @@ __global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const i
         if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
         bem_c32f4 o = regular_ops_f32(F, px, py, pz, npx, npy, npz, sn[0][c], sn[1][c], sn[2][c], sa[c],
                                       &sq[0][0][c], &sq[1][0][c], &sq[2][0][c], kTile);
-        out[T.dest + r + (int64_t)c * T.ld] = bc_entry_f32(F, subst, kind, o, j);
+        bem_c32 e = bc_entry_f32(F, subst, kind, o, j);
+        if (PLANE) {   // impedance plane: + the mirror image's entry
+            bem_c32 m = bc_entry_f32(F, subst, kind, refl_ops_f32(F, M, i, j), j);
+            e = c32(e.re + m.re, e.im + m.im);
+        }
+        out[T.dest + r + (int64_t)c * T.ld] = e;
     }
 }
 
@@ -206,11 +221,11 @@ __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int6
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= np) return;
     int64_t i = ij[2 * t], j = ij[2 * t + 1];
-    bem_ops4 o = ops_general(M, F.k, i, j);
-    if (mode == 2) {
+    if (mode == 2) {   // the free-space operators
+        bem_ops4 o = ops_general(M, F.k, i, j);
         out[4 * t] = o.L; out[4 * t + 1] = o.M; out[4 * t + 2] = o.Mt; out[4 * t + 3] = o.N;
     } else {
-        out[t] = bc_entry(F, mode, o, i, j);
+        out[t] = entry_general(M, F, mode, i, j);
     }
 }
 
@@ -219,18 +234,40 @@ __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int6
 __constant__ static const double kTri7[7][3] = BEM_TRI7;
 constexpr int64_t kFieldChunk = 1024;   // boundary elements per partial sum of the field
 
+// M phi - L dphi of one element (vertices and normal from LDS column c)
+__device__ __forceinline__ bem_c64 field_term(double k, double px, double py, double pz, const double (*sv)[kTile],
+                                              const double (*sn)[kTile], int c, double area, bem_c64 phi,
+                                              bem_c64 dphi)
+{
+    bem_c64 L = c64(0, 0), Mk = c64(0, 0);
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        double x1 = kTri7[q][0], x2 = kTri7[q][1];
+        double x3 = (1.0 - x2) - x1;
+        double qx = (x1 * sv[0][c] + x2 * sv[3][c]) + x3 * sv[6][c];
+        double qy = (x1 * sv[1][c] + x2 * sv[4][c]) + x3 * sv[7][c];
+        double qz = (x1 * sv[2][c] + x2 * sv[5][c]) + x3 * sv[8][c];
+        bem_field_point_accumulate(&L, &Mk, k, px, py, pz, sn[0][c], sn[1][c], sn[2][c], qx, qy, qz, kTri7[q][2]);
+    }
+    L = c64_scale(L, area);
+    Mk = c64_scale(Mk, area);
+    return bem_field_term(L, Mk, phi, dphi);
+}
+
 // Field potential of the boundary solution at observation points
 // (calculateFieldSolutionRegular / BemOperatorField,
 // boundaryelementsolver.cpp:2262-2353), one 64-point tile x one chunk of
-// kFieldChunk elements per workgroup.  Canonical order: inside a chunk four
-// partial sums over element index mod 4 (ascending), combined
-// (s0 + s1) + (s2 + s3); k_field_reduce adds the chunks in order.
+// kFieldChunk elements per workgroup.  With an impedance plane each element's
+// term is followed by its mirror image's (:2305-2311) and the two are added
+// first.  Canonical order: inside a chunk four partial sums over element index
+// mod 4 (ascending), combined (s0 + s1) + (s2 + s3); k_field_reduce adds the
+// chunks in order.
 __global__ __launch_bounds__(kWG) void k_field(DevMesh M, double k, int64_t nobs, const double* __restrict__ obs,
                                                const bem_c64* __restrict__ phi, const bem_c64* __restrict__ dphi,
                                                int nchunk, bem_c64* __restrict__ part)
 {
-    __shared__ double sv[9][kTile];
-    __shared__ double sn[3][kTile], sa[kTile];
+    __shared__ double sv[9][kTile], rv[9][kTile];
+    __shared__ double sn[3][kTile], rn[3][kTile], sa[kTile];
     __shared__ bem_c64 sphi[kTile], sdphi[kTile];
     __shared__ bem_c64 red[4][kTile];
     const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
@@ -246,10 +283,14 @@ __global__ __launch_bounds__(kWG) void k_field(DevMesh M, double k, int64_t nobs
         __syncthreads();
         for (int idx = tid; idx < 9 * kTile; idx += kWG) {
             int d = idx / kTile, c = idx % kTile;
-            if (c < cnt) sv[d][c] = M.vtx[(int64_t)d * n + t0 + c];
+            if (c < cnt) {
+                sv[d][c] = M.vtx[(int64_t)d * n + t0 + c];
+                if (M.rvtx) rv[d][c] = M.rvtx[(int64_t)d * n + t0 + c];
+            }
         }
         for (int c = tid; c < cnt; c += kWG) {
             sn[0][c] = M.nx[t0 + c]; sn[1][c] = M.ny[t0 + c]; sn[2][c] = M.nz[t0 + c];
+            if (M.rvtx) { rn[0][c] = M.rnx[t0 + c]; rn[1][c] = M.rny[t0 + c]; rn[2][c] = M.rnz[t0 + c]; }
             sa[c] = M.area[t0 + c];
             sphi[c] = phi[t0 + c];
             sdphi[c] = dphi[t0 + c];
@@ -257,20 +298,9 @@ __global__ __launch_bounds__(kWG) void k_field(DevMesh M, double k, int64_t nobs
         __syncthreads();
         if (!live) continue;
         for (int c = cg; c < cnt; c += 4) {
-            bem_c64 L = c64(0, 0), Mk = c64(0, 0);
-#pragma unroll
-            for (int q = 0; q < 7; q++) {
-                double x1 = kTri7[q][0], x2 = kTri7[q][1];
-                double x3 = (1.0 - x2) - x1;
-                double qx = (x1 * sv[0][c] + x2 * sv[3][c]) + x3 * sv[6][c];
-                double qy = (x1 * sv[1][c] + x2 * sv[4][c]) + x3 * sv[7][c];
-                double qz = (x1 * sv[2][c] + x2 * sv[5][c]) + x3 * sv[8][c];
-                bem_field_point_accumulate(&L, &Mk, k, px, py, pz, sn[0][c], sn[1][c], sn[2][c], qx, qy, qz,
-                                           kTri7[q][2]);
-            }
-            L = c64_scale(L, sa[c]);
-            Mk = c64_scale(Mk, sa[c]);
-            acc = c64_add(acc, bem_field_term(L, Mk, sphi[c], sdphi[c]));
+            bem_c64 v = field_term(k, px, py, pz, sv, sn, c, sa[c], sphi[c], sdphi[c]);
+            if (M.rvtx) v = c64_add(v, field_term(k, px, py, pz, rv, rn, c, sa[c], sphi[c], sdphi[c]));
+            acc = c64_add(acc, v);
         }
     }
     red[cg][r] = acc;
@@ -497,7 +527,10 @@ __device__ __forceinline__ bool near_or_self(const DevMesh& M, int64_t i, int64_
     return sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim;
 }
 
-// regular entry of (i, j), or false for a self / near-singular pair
+// regular entry of (i, j), or false for a self / near-singular pair; PLANE:
+// plus the mirror image's entry (a template argument so that the plain
+// kernels keep their register budget)
+template <bool PLANE>
 __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c64* e)
 {
     const DevMesh& M = a.M;
@@ -505,14 +538,20 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
     bem_ops4 o = regular_ops_vtx<1>(a.F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j],
                                     M.nz[j], M.area[j], load_vtx(M, j));
     *e = bc_entry(a.F, a.kind, o, i, j);
+    if (PLANE) *e = c64_add(*e, refl_entry(M, a.F, a.kind, i, j));
     return true;
 }
+template <bool PLANE>
 __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c32* e)
 {
     const DevMesh& M = a.M;
     if (near_or_self(M, i, j)) return false;
     bem_c32f4 o = regular_ops_f32_edge<1>(a.F32, M, i, j);
     *e = bc_entry_f32(a.F32, a.F.subst, a.kind, o, j);
+    if (PLANE) {
+        bem_c32 r = bc_entry_f32(a.F32, a.F.subst, a.kind, refl_ops_f32(a.F32, M, i, j), j);
+        *e = c32(e->re + r.re, e->im + r.im);
+    }
     return true;
 }
 
@@ -525,11 +564,11 @@ __device__ __forceinline__ C singular_entry(const AcaArgs& a, int64_t i, int64_t
 }
 
 // any entry (the pivot probe of the extra variant)
-template <class C>
+template <bool PLANE, class C>
 __device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
 {
     C e;
-    if (regular_entry(a, i, j, &e)) return e;
+    if (regular_entry<PLANE>(a, i, j, &e)) return e;
     return singular_entry<C>(a, i, i == j ? -1 : near_find(a.M, i, (int32_t)j));
 }
 
@@ -537,7 +576,7 @@ __device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
 // Regular entries first; the few self / near-singular entries of the row
 // (CSR row r0 + i) in a second pass, so that the main loop has no divergent
 // branch.
-template <int NT, class C>
+template <int NT, class C, bool PLANE>
 __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
                         const C* U, const C* V, const C* A, C* dst, C* sw)
 {
@@ -546,7 +585,7 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     const int64_t gi = r0 + i;
     for (int64_t j = threadIdx.x; j < n; j += NT) {
         C e;
-        if (!regular_entry(a, gi, c0 + j, &e)) continue;
+        if (!regular_entry<PLANE>(a, gi, c0 + j, &e)) continue;
         dst[j] = cx_sub(e, cross_sum<true>(sw, V, n, j, r));
     }
     const int64_t p0 = a.M.nr_ptr[gi], p1 = a.M.nr_ptr[gi + 1];
@@ -563,7 +602,7 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // column j of the block minus the current cross approximation, into dst;
 // the near-singular rows of column c0 + j are the near list of row c0 + j
 // (the distance criterion is symmetric)
-template <int NT, class C>
+template <int NT, class C, bool PLANE>
 __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
                         const C* U, const C* V, const C* A, C* dst, C* sw)
 {
@@ -572,7 +611,7 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     const int64_t gj = c0 + j;
     for (int64_t i = threadIdx.x; i < m; i += NT) {
         C e;
-        if (!regular_entry(a, r0 + i, gj, &e)) continue;
+        if (!regular_entry<PLANE>(a, r0 + i, gj, &e)) continue;
         dst[i] = cx_sub(e, cross_sum<false>(sw, U, m, i, r));
     }
     const int64_t p0 = a.M.nr_ptr[gj], p1 = a.M.nr_ptr[gj + 1];
@@ -591,7 +630,7 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // variant = BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra
 // variant = BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).
 // Dynamic LDS: sw[cap] (type C) then sdots[2 cap + 2] (FP64).
-template <bool EXTRA, int NT, class C>
+template <bool EXTRA, int NT, class C, bool PLANE>
 __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred)
 {
     const int64_t m = a.rows[b], n = a.cols[b], r0 = a.row0[b], c0 = a.col0[b];
@@ -609,7 +648,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
     bem_c64 S = c64(0, 0);
     if (!EXTRA) {
         int64_t j = bem_rand_bounded(a.seed, 0, (uint64_t)b, 0, n);
-        aca_col<NT>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
+        aca_col<NT, C, PLANE>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
         ev += m;
         int64_t i = -1;
         while (r < maxRank) {
@@ -619,12 +658,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             i = am.i;
             if (am.v <= tiny2 && r >= 1) break;
             C* row = V + (int64_t)r * n;
-            aca_row<NT>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+            aca_row<NT, C, PLANE>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
             ev += n;
             ArgMax ar = block_argmax<NT>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
             C* col = U + (int64_t)r * m;
-            aca_col<NT>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+            aca_col<NT, C, PLANE>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
             ev += m;
             C alpha = cx_inv(row[ar.i]);
             if (threadIdx.x == 0) A[r] = alpha;
@@ -641,7 +680,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
         bool done = false;
         for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
             int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
-            C pe = entry_any<C>(a, r0 + i, c0 + j);
+            C pe = entry_any<PLANE, C>(a, r0 + i, c0 + j);
             ev += 1;
             if (cx_abs2(pe) < tiny2) continue;
             C approx = cx_zero<C>();
@@ -649,7 +688,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 approx = cx_add(approx, cx_mul(cx_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
             if (cx_abs2(cx_sub(pe, approx)) < tol2 * cx_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
-            aca_col<NT>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
+            aca_col<NT, C, PLANE>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
             ev += m;
             bool first = true;
             while (r < maxRank) {
@@ -660,12 +699,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 first = false;
                 if (cx_abs2(wcol[i]) <= tiny2 && r >= 1) break;
                 C* row = V + (int64_t)r * n;
-                aca_row<NT>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+                aca_row<NT, C, PLANE>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
                 ev += n;
                 ArgMax ar = block_argmax<NT>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
                 C* col = U + (int64_t)r * m;
-                aca_col<NT>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+                aca_col<NT, C, PLANE>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
                 ev += m;
                 C alpha = cx_inv(row[ar.i]);
                 if (threadIdx.x == 0) A[r] = alpha;
@@ -696,12 +735,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
 template <int NT, class C> constexpr int kAcaWaves = 1;
 template <> constexpr int kAcaWaves<64, bem_c32> = 4;
 
-template <bool EXTRA, int NT, class C>
+template <bool EXTRA, int NT, class C, bool PLANE>
 __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) char dyn[];
     __shared__ ArgMax sred[NT / 64];
-    aca_block<EXTRA, NT, C>(a, a.blist[blockIdx.x], dyn, sred);
+    aca_block<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
 }
 
 // copy the first rank columns/rows of the ACA scratch into the packed
@@ -1143,6 +1182,10 @@ struct bem_ctx {
     std::vector<double> h_nodes, h_normal, h_area, h_colloc;
     std::vector<void*> allocs_mesh;
     DevMesh mesh{};
+    // impedance plane (point, normal) and the mirror images of the mesh
+    bool have_plane = false;
+    double plane_pt[3] = {0, 0, 0}, plane_nrm[3] = {0, 0, 0};
+    std::vector<void*> allocs_plane;
     bemsetup::NearList near;
     // bc
     std::vector<bem_c64> h_alpha, h_beta, h_f;
@@ -1453,6 +1496,48 @@ DevFreq32 dev_freq32(const bem_ctx* c)
     return F;
 }
 
+// vertices a, b, c as 9 SoA rows, and the FP32 path's edge form c, a - c, b - c
+void vertex_soa(int64_t n, const double* nodes9, std::vector<double>& vtx, std::vector<double>& edg)
+{
+    vtx.resize(9 * n);
+    edg.resize(9 * n);
+    for (int64_t e = 0; e < n; e++) {
+        for (int d = 0; d < 9; d++) vtx[d * n + e] = nodes9[9 * e + d];
+        for (int d = 0; d < 3; d++) {
+            const double cv = nodes9[9 * e + 6 + d];
+            edg[d * n + e] = cv;
+            edg[(3 + d) * n + e] = nodes9[9 * e + d] - cv;
+            edg[(6 + d) * n + e] = nodes9[9 * e + 3 + d] - cv;
+        }
+    }
+}
+
+// mirror images of the current mesh in the impedance plane (or none)
+int build_plane(bem_ctx* c)
+{
+    free_list(c, c->allocs_plane);
+    DevMesh& M = c->mesh;
+    M.rvtx = M.redg = M.rnx = M.rny = M.rnz = nullptr;
+    if (!c->have_plane || c->n <= 0) return BEM_OK;
+    const int64_t n = c->n;
+    std::vector<double> rnodes, vtx, edg, nrm(3 * n);
+    std::vector<bemsetup::Vec3> rnormal;
+    bemsetup::reflect_elements(n, c->h_nodes.data(), c->plane_pt, c->plane_nrm, rnodes, rnormal);
+    vertex_soa(n, rnodes.data(), vtx, edg);
+    for (int64_t e = 0; e < n; e++) {
+        nrm[e] = rnormal[e].x; nrm[n + e] = rnormal[e].y; nrm[2 * n + e] = rnormal[e].z;
+    }
+    double *d_vtx, *d_edg, *d_nrm;
+    int rc;
+    if ((rc = dupload(c, &d_vtx, vtx.data(), vtx.size(), &c->allocs_plane))) return rc;
+    if ((rc = dupload(c, &d_edg, edg.data(), edg.size(), &c->allocs_plane))) return rc;
+    if ((rc = dupload(c, &d_nrm, nrm.data(), nrm.size(), &c->allocs_plane))) return rc;
+    M.rvtx = d_vtx;
+    M.redg = d_edg;
+    M.rnx = d_nrm; M.rny = d_nrm + n; M.rnz = d_nrm + 2 * n;
+    return BEM_OK;
+}
+
 int launch_check(bem_ctx* c, const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -1500,6 +1585,7 @@ int bem_destroy(bem_ctx_t c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_list(c, c->allocs_mesh);
+    free_list(c, c->allocs_plane);
     free_list(c, c->allocs_part);
     free_list(c, c->allocs_piv);
     for (void* p : {(void*)c->d_alpha, (void*)c->d_beta, (void*)c->d_f, (void*)c->d_subst, (void*)c->d_cphi,
@@ -1561,16 +1647,8 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
         soa[3 * n + e] = normal3[3 * e]; soa[4 * n + e] = normal3[3 * e + 1]; soa[5 * n + e] = normal3[3 * e + 2];
         soa[6 * n + e] = area[e];
     }
-    std::vector<double> vtx(9 * n), edg(9 * n);
-    for (int64_t e = 0; e < n; e++) {
-        for (int d = 0; d < 9; d++) vtx[d * n + e] = nodes9[9 * e + d];
-        for (int d = 0; d < 3; d++) {
-            const double cv = nodes9[9 * e + 6 + d];
-            edg[d * n + e] = cv;
-            edg[(3 + d) * n + e] = nodes9[9 * e + d] - cv;
-            edg[(6 + d) * n + e] = nodes9[9 * e + 3 + d] - cv;
-        }
-    }
+    std::vector<double> vtx, edg;
+    vertex_soa(n, nodes9, vtx, edg);
     double *d_soa, *d_nodes, *d_q, *d_vtx, *d_edg;
     int rc;
     if ((rc = dupload(c, &d_soa, soa.data(), soa.size(), &c->allocs_mesh))) return rc;
@@ -1616,6 +1694,28 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     M.nr_ptr = d_ptr; M.nr_row = d_prow; M.nr_col = d_col; M.nr_nsub = d_nsub; M.nr_off = d_off; M.nr_geo = d_geo;
     M.self_geo = d_self;
     M.near_lim = 4.0 * hbar;
+    if ((rc = build_plane(c))) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+int bem_set_plane(bem_ctx_t c, const double* point3, const double* normal3)
+{
+    if (!c) return BEM_ERR_ARG;
+    const bool on = normal3 && (normal3[0] != 0 || normal3[1] != 0 || normal3[2] != 0);
+    if (on && !point3) return set_err(c, BEM_ERR_ARG, "bem_set_plane: no plane point");
+    if (on && !(std::isfinite(normal3[0]) && std::isfinite(normal3[1]) && std::isfinite(normal3[2]) &&
+                std::isfinite(point3[0]) && std::isfinite(point3[1]) && std::isfinite(point3[2])))
+        return set_err(c, BEM_ERR_ARG, "bem_set_plane: non-finite plane");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->have_plane = on;
+    for (int d = 0; d < 3; d++) {
+        c->plane_pt[d] = on ? point3[d] : 0.0;
+        c->plane_nrm[d] = on ? normal3[d] : 0.0;
+    }
+    int rc = build_plane(c);
+    if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     return BEM_OK;
 }
@@ -1838,22 +1938,27 @@ struct AcaScratch {
 // workgroup of the largest class (FP64 never forms it: kHugeBlock[0])
 template <class C> constexpr int kHugeThreads = 512;
 
+template <int NT, class C>
+void launch_aca_nt(bool extra, bool plane, dim3 g, size_t shm, hipStream_t st, const AcaArgs& aq)
+{
+    if (extra) {
+        if (plane) hipLaunchKernelGGL((k_aca<true, NT, C, true>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<true, NT, C, false>), g, dim3(NT), shm, st, aq);
+    } else {
+        if (plane) hipLaunchKernelGGL((k_aca<false, NT, C, true>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C, false>), g, dim3(NT), shm, st, aq);
+    }
+}
+
 // one size class of blocks on stream st, storage type C
 template <class C>
 void launch_aca(int q, bool extra, unsigned nblocks, size_t shm, hipStream_t st, const AcaArgs& aq)
 {
-    dim3 g(nblocks);
-    if (q == 0) {
-        constexpr int NT = kHugeThreads<C>;
-        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C>), g, dim3(NT), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<false, NT, C>), g, dim3(NT), shm, st, aq);
-    } else if (q == 1) {
-        if (extra) hipLaunchKernelGGL((k_aca<true, 256, C>), g, dim3(256), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<false, 256, C>), g, dim3(256), shm, st, aq);
-    } else {
-        if (extra) hipLaunchKernelGGL((k_aca<true, 64, C>), g, dim3(64), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<false, 64, C>), g, dim3(64), shm, st, aq);
-    }
+    const dim3 g(nblocks);
+    const bool plane = aq.M.rvtx != nullptr;
+    if (q == 0) launch_aca_nt<kHugeThreads<C>, C>(extra, plane, g, shm, st, aq);
+    else if (q == 1) launch_aca_nt<256, C>(extra, plane, g, shm, st, aq);
+    else launch_aca_nt<64, C>(extra, plane, g, shm, st, aq);
 }
 
 int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
@@ -2005,11 +2110,13 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         int rc2;
         if (!c->tiles.empty()) {
             Timer t(c, 0);
+            const dim3 g((unsigned)c->tiles.size());
+            const bool plane = c->mesh.rvtx != nullptr;
             if (f32)
-                hipLaunchKernelGGL(k_dense32, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F32,
-                                   c->d_subst, kind, c->d_tiles, (bem_c32*)h->dense);
+                hipLaunchKernelGGL(plane ? k_dense32<true> : k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh,
+                                   F32, c->d_subst, kind, c->d_tiles, (bem_c32*)h->dense);
             else
-                hipLaunchKernelGGL(k_dense, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream, c->mesh, F,
+                hipLaunchKernelGGL(plane ? k_dense<true> : k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh, F,
                                    c->d_tiles, outPhi, outDPhi);
             if ((rc2 = launch_check(c, "k_dense"))) return rc2;
         }

@@ -57,6 +57,11 @@ struct DevMesh {
     const double* nr_geo;
     const double* self_geo;       // kSelf per element
     double near_lim;              // 4 * hbar
+    // mirror images in the impedance plane (rvtx == nullptr: no plane), laid
+    // out as vtx / edg / the normals
+    const double* rvtx;
+    const double* redg;
+    const double *rnx, *rny, *rnz;
 };
 
 // frequency-dependent boundary-condition coefficients (bem_detmath.h)
@@ -230,10 +235,42 @@ __device__ __forceinline__ bem_c64 bc_entry(const DevFreq& F, int kind, const be
     return kind == 0 ? bem_phi_entry(X, Y, F.subst[j], F.cphi[j]) : bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
 }
 
+// BemOperatorsReflected (boundaryelementsolver.cpp:1516-1553): the regular
+// rule over the mirror image of element j (its vertices and normal, the area
+// of element j), for any pair including i == j
+__device__ __forceinline__ bem_ops4 refl_ops(const DevMesh& M, double k, int64_t i, int64_t j)
+{
+    const double* v = M.rvtx + j;
+    const int64_t n = M.n;
+    const ElemVtx e{v[0], v[n], v[2 * n], v[3 * n], v[4 * n], v[5 * n], v[6 * n], v[7 * n], v[8 * n]};
+    return regular_ops_vtx<1>(k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.rnx[j], M.rny[j], M.rnz[j],
+                              M.area[j], e);
+}
+
+// implicitPhiMatrixReflected / implicitDPhiMatrixReflected
+// (boundaryelementsolver.cpp:1556-1643): the boundary-condition combination
+// of the column, without the jump term of the diagonal
+__device__ __forceinline__ void refl_xy(const DevMesh& M, const DevFreq& F, int64_t i, int64_t j, bem_c64* X,
+                                        bem_c64* Y)
+{
+    bem_ops4 o = refl_ops(M, F.k, i, j);
+    bem_xy(&o, F.cc, 0, X, Y);
+}
+
+__device__ __forceinline__ bem_c64 refl_entry(const DevMesh& M, const DevFreq& F, int kind, int64_t i, int64_t j)
+{
+    bem_c64 X, Y;
+    refl_xy(M, F, i, j, &X, &Y);
+    return kind == 0 ? bem_phi_entry(X, Y, F.subst[j], F.cphi[j]) : bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
+}
+
+// implicitPhiMatrixPlusReflected / implicitDPhiMatrixPlusReflected
+// (boundaryelementsolver.cpp:2042-2050): free-space entry + mirror entry
 __device__ __forceinline__ bem_c64 entry_general(const DevMesh& M, const DevFreq& F, int kind, int64_t i, int64_t j)
 {
     bem_ops4 o = ops_general(M, F.k, i, j);
-    return bc_entry(F, kind, o, i, j);
+    bem_c64 e = bc_entry(F, kind, o, i, j);
+    return M.rvtx ? c64_add(e, refl_entry(M, F, kind, i, j)) : e;
 }
 
 // ---------------------------------------------------------------- FP32 path
@@ -376,14 +413,16 @@ __device__ __forceinline__ bem_c32f4 regular_ops_f32_vtx(const DevFreq32& F, dou
 // point q = c + xi1 (a - c) + xi2 (b - c) (the rule's point, rounded
 // differently from the FP64 path's form) at two FP64 fma per coordinate
 template <int UNROLL = 1>
-__device__ __forceinline__ bem_c32f4 regular_ops_f32_edge(const DevFreq32& F, const DevMesh& M, int64_t i, int64_t j)
+__device__ __forceinline__ bem_c32f4 regular_ops_f32_edge_of(const DevFreq32& F, const DevMesh& M, int64_t i,
+                                                             int64_t j, const double* edg, const double* qnx,
+                                                             const double* qny, const double* qnz)
 {
     const int64_t n = M.n;
-    const double* g = M.edg + j;
+    const double* g = edg + j;
     const double dx = M.px[i] - g[0], dy = M.py[i] - g[n], dz = M.pz[i] - g[2 * n];
     const double ax = g[3 * n], ay = g[4 * n], az = g[5 * n], bx = g[6 * n], by = g[7 * n], bz = g[8 * n];
     const float npx = (float)M.nx[i], npy = (float)M.ny[i], npz = (float)M.nz[i];
-    const float nqx = (float)M.nx[j], nqy = (float)M.ny[j], nqz = (float)M.nz[j];
+    const float nqx = (float)qnx[j], nqy = (float)qny[j], nqz = (float)qnz[j];
     const float upnq = fmaf(npz, nqz, fmaf(npy, nqy, npx * nqx));
     bem_c32f4 o;
     o.L = o.M = o.Mt = o.N = c32(0.0f, 0.0f);
@@ -399,6 +438,18 @@ __device__ __forceinline__ bem_c32f4 regular_ops_f32_edge(const DevFreq32& F, co
     o.Mt = c32(o.Mt.re * area, o.Mt.im * area);
     o.N = c32(o.N.re * area, o.N.im * area);
     return o;
+}
+
+template <int UNROLL = 1>
+__device__ __forceinline__ bem_c32f4 regular_ops_f32_edge(const DevFreq32& F, const DevMesh& M, int64_t i, int64_t j)
+{
+    return regular_ops_f32_edge_of<UNROLL>(F, M, i, j, M.edg, M.nx, M.ny, M.nz);
+}
+
+// the FP32 form of refl_ops (mirror image of element j, edge form)
+__device__ __forceinline__ bem_c32f4 refl_ops_f32(const DevFreq32& F, const DevMesh& M, int64_t i, int64_t j)
+{
+    return regular_ops_f32_edge_of<1>(F, M, i, j, M.redg, M.rnx, M.rny, M.rnz);
 }
 
 // off-diagonal Phi / DPhi entry from the FP32 operators (bem_phi_entry,

@@ -152,6 +152,30 @@ void prepare_elements(Elements& E)
     E.hbar = E.n > 0 ? (sum / (double)E.n) / 2.0 : 0.0;
 }
 
+void reflect_elements(int64_t n, const double* nodes9, const double point[3], const double normal[3],
+                      std::vector<double>& rnodes9, std::vector<Vec3>& rnormal)
+{
+    const Vec3 u = unit(v3(normal[0], normal[1], normal[2]));
+    const double d = -dot(u, v3(point[0], point[1], point[2]));
+    const double R[3][4] = {{1.0 - 2.0 * (u.x * u.x), -2.0 * u.x * u.y, -2.0 * u.x * u.z, -2.0 * u.x * d},
+                            {-2.0 * u.x * u.y, 1.0 - 2.0 * (u.y * u.y), -2.0 * u.y * u.z, -2.0 * u.y * d},
+                            {-2.0 * u.x * u.z, -2.0 * u.y * u.z, 1.0 - 2.0 * (u.z * u.z), -2.0 * u.z * d}};
+    auto mirror = [&](const double* x, double* out) {
+        for (int r = 0; r < 3; r++) out[r] = ((R[r][0] * x[0] + R[r][1] * x[1]) + R[r][2] * x[2]) + R[r][3];
+    };
+    rnodes9.resize(9 * n);
+    rnormal.resize(n);
+    for (int64_t e = 0; e < n; e++) {
+        const double* v = nodes9 + 9 * e;
+        double* o = &rnodes9[9 * e];
+        mirror(v + 6, o);
+        mirror(v + 3, o + 3);
+        mirror(v, o + 6);
+        Tri t = tri_of(o);
+        rnormal[e] = unit(cross(t.b - t.a, t.c - t.a));
+    }
+}
+
 ClusterTree build_cluster_tree(int64_t n, const double* nodes9, int64_t leaf)
 {
     ClusterTree T;

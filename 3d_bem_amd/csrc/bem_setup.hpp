@@ -25,6 +25,18 @@ struct Elements {
 // averageTriangleDim (boundaryelements.cpp:35-46).
 void prepare_elements(Elements& E);
 
+// Mirror image of every element in the impedance plane through `point` with
+// normal `normal` (normalised here): MeshFunctions::reflectTriangles
+// (meshfunctions.cpp:349-378) as called by BoundaryElements::reflectGeometry
+// (boundaryelements.cpp:151-155).  x' = R [x; 1] with the rows of
+// R = [I - 2 n n^T | -2 n d], d = -n.point, each row summed left to right;
+// the node order is reversed (a' = R c, b' = R b, c' = R a), so the
+// recomputed normal (rnormal, as prepare_elements) is the mirror image of the
+// element's normal.  The element keeps its area (the reference copies the
+// areas before reflecting).
+void reflect_elements(int64_t n, const double* nodes9, const double point[3], const double normal[3],
+                      std::vector<double>& rnodes9, std::vector<Vec3>& rnormal);
+
 struct Cluster {
     int64_t start = 0, size = 0;
     int depth = 0;

@@ -155,6 +155,7 @@ BoundaryElementSolver::BoundaryElementSolver(int device, BoundaryElements elemen
         check(ctx_, bem_upload_mesh(ctx_, E.n, E.nodes.data(), E.normal.data(), E.area.data(), E.colloc.data(),
                                     E.averageTriangleDim),
               "bem_upload_mesh");
+        applyImpedancePlanes();
         std::vector<cplx> a(E.n), b(E.n), g(E.n);
         for (int64_t e = 0; e < E.n; e++) { a[e] = E.bc[e].a; b[e] = E.bc[e].b; g[e] = E.bc[e].g; }
         check(ctx_, bem_set_bc(ctx_, (const double*)a.data(), (const double*)b.data(), (const double*)g.data()),
@@ -171,6 +172,16 @@ BoundaryElementSolver::BoundaryElementSolver(int device, BoundaryElements elemen
         ctx_ = nullptr;
         throw;
     }
+}
+
+void BoundaryElementSolver::applyImpedancePlanes()
+{
+    const auto& P = boundaryElements.impedancePlanes;
+    if (P.size() > 1)
+        throw Error(BEM_ERR_ARG, "more than one impedance plane: the solve handles none or one "
+                                 "(hMatrixSolve, boundaryelementsolver.cpp:549-557)");
+    if (P.empty()) check(ctx_, bem_set_plane(ctx_, nullptr, nullptr), "bem_set_plane");
+    else check(ctx_, bem_set_plane(ctx_, P[0].halfSpacePlanePoint, P[0].halfSpacePlaneNormal), "bem_set_plane");
 }
 
 BoundaryElementSolver::~BoundaryElementSolver()
@@ -539,6 +550,38 @@ int bem_solver_set_preconditioner(bem_solver_t s, int kind)
 {
     if (!s || !s->s || kind < 0 || kind > 1) return BEM_ERR_ARG;
     s->preconditioner = kind;
+    return BEM_OK;
+}
+
+int bem_solver_set_impedance_plane(bem_solver_t s, const double* point3, const double* normal3)
+{
+    if (!s || !s->s) return BEM_ERR_ARG;
+    const bool on = normal3 && (normal3[0] != 0 || normal3[1] != 0 || normal3[2] != 0);
+    if (on && !point3) return BEM_ERR_ARG;
+    std::vector<bem::ImpedancePlane> planes;
+    if (on) {
+        bem::ImpedancePlane q;
+        for (int d = 0; d < 3; d++) {
+            q.halfSpacePlanePoint[d] = point3[d];
+            q.halfSpacePlaneNormal[d] = normal3[d];
+        }
+        planes.push_back(q);
+    }
+    try {
+        s->input.impedancePlanes = planes;
+        s->s->boundaryElements.impedancePlanes = planes;
+        s->s->applyImpedancePlanes();
+        for (auto& L : s->lanes) {
+            L->boundaryElements.impedancePlanes = planes;
+            L->applyImpedancePlanes();
+        }
+    } catch (const bem::Error& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return BEM_ERR_HIP;
+    }
     return BEM_OK;
 }
 

@@ -22,6 +22,12 @@ struct RobinBoundaryCondition {
     cplx a{0.0, 0.0}, b{1.0, 0.0}, g{0.0, 0.0};
 };
 
+// impedanceplane.h:10-28: a plane (point, normal) the solve mirrors the mesh in
+struct ImpedancePlane {
+    double halfSpacePlanePoint[3] = {0, 0, 0};
+    double halfSpacePlaneNormal[3] = {0, 0, 0};
+};
+
 // boundaryelements.h:35-170 (triangles + derived geometry), SoA here
 struct BoundaryElements {
     int64_t n = 0;
@@ -30,6 +36,7 @@ struct BoundaryElements {
     std::vector<double> normal, colloc, area;        // n x 3, n x 3, n
     double averageTriangleDim = 0;
     std::vector<int64_t> elementIndex;               // input index of each element
+    std::vector<ImpedancePlane> impedancePlanes;     // boundaryelements.h:97 (at most one on this path)
     void prepare();                                  // prepareBoundaryElements (:3463-3468)
 };
 
@@ -122,6 +129,9 @@ public:
     BoundaryElementSolver(int device, BoundaryElements elements, int64_t leaf, double eta);
     ~BoundaryElementSolver();
     void setFrequency(double f) { frequency = f; }
+    // the context's image plane from boundaryElements.impedancePlanes (none or
+    // one: hMatrixSolve's branches, boundaryelementsolver.cpp:549-557)
+    void applyImpedancePlanes();
     void setWaveSpeed(double c) { waveSpeed = c; }
     void setAirDensity(double rho) { airDensity = rho; }
     void calculateBoundarySolution();      // boundaryelementsolver.cpp:12-138 (H-matrix branch)

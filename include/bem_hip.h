@@ -61,6 +61,20 @@ const char* bem_last_error(bem_ctx_t ctx);
    BemOperatorsConst (boundaryelementsolver.cpp:844-894). */
 int bem_upload_mesh(bem_ctx_t ctx, int64_t n, const double* nodes9, const double* normal3, const double* area,
                     const double* colloc3, double hbar);
+/* One impedance plane, a rigid image: the plane through point3 with normal
+   normal3 (normalised here); normal3 NULL or zero removes it.  Later
+   assemblies add to every Phi / DPhi entry the regular-rule entry of the
+   column element's mirror image (no jump term, the column's boundary
+   condition), and bem_field adds the mirror images' terms.  Replaces the
+   single-plane branch of hMatrixSolve (boundaryelementsolver.cpp:549-557,
+   594-604): BoundaryElements::reflectGeometry (boundaryelements.cpp:151-155,
+   MeshFunctions::reflectTriangles meshfunctions.cpp:349-378) and
+   implicitPhiMatrixPlusReflected / implicitDPhiMatrixPlusReflected
+   (boundaryelementsolver.cpp:2042-2050 over :1516-1643), and the plane terms
+   of calculateFieldSolutionRegular (:2270-2312).  The plane is kept across
+   bem_upload_mesh (rebuilt for the new mesh); already assembled H-matrices
+   keep the operator they were assembled with. */
+int bem_set_plane(bem_ctx_t ctx, const double* point3, const double* normal3);
 /* Robin coefficients a phi + b dphi = g per element (complex, n each):
    the alpha/beta/f vectors of calculateBoundaryConditionAndSourceTermVector
    (boundaryelementsolver.cpp:3386-3429). */
@@ -179,6 +193,13 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
    default), 1 the leaf-block LU of bem_precond_create */
 int bem_solver_set_preconditioner(bem_solver_t s, int kind);
 
+/* The scene's impedance plane (BoundaryElements::impedancePlanes,
+   boundaryelements.h:97, filled from an infinite-baffle section,
+   solvingscriptreader.cpp:1070-1077): later solves and field evaluations use
+   the mirror image as bem_set_plane describes.  normal3 NULL or zero removes
+   it.  The sources are not mirrored (calculateBoundaryConditionAndSourceTermVector
+   and sourcePhiTerm use the free-field incident field, as the reference). */
+int bem_solver_set_impedance_plane(bem_solver_t s, const double* point3, const double* normal3);
 /* Sound pressure at observation points (nobs x 3, host) from the last
    bem_solver_solve: the boundary term of bem_field plus the incident point
    sources and plane wave, times i rho 2 pi f (calculateFieldSolution with

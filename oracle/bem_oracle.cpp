@@ -100,6 +100,10 @@ struct orc_problem {
     std::vector<cplx> s;             /* source term vector */
     std::vector<int> subst;
     std::vector<cplx> cphi, cd;
+    /* impedance plane: mirror images of the elements (cluster order) */
+    bool plane = false;
+    std::vector<Tri> rtri;
+    std::vector<V3> rnormal;
 };
 
 struct orc_hmat {
@@ -359,6 +363,28 @@ bem_ops4 ops_self(const orc_problem& P, int64_t i)
     return o;
 }
 
+/* BemOperatorsReflected (boundaryelementsolver.cpp:1516-1553): the regular
+   rule over the mirror image of element j, with the image's normal and the
+   area of element j */
+bem_ops4 ops_reflected(const orc_problem& P, int64_t i, int64_t j)
+{
+    bem_ops4 o = bem_ops4_zero();
+    V3 p = P.colloc[i], np = P.normal[i], nq = P.rnormal[j];
+    double upnq = dot(np, nq);
+    const Tri& t = P.rtri[j];
+    for (int q = 0; q < 16; q++) {
+        double x1 = TRI16[q][0], x2 = TRI16[q][1], w = TRI16[q][2];
+        double x3 = (1.0 - x2) - x1;
+        double qx = (x1 * t.a.x + x2 * t.b.x) + x3 * t.c.x;
+        double qy = (x1 * t.a.y + x2 * t.b.y) + x3 * t.c.y;
+        double qz = (x1 * t.a.z + x2 * t.b.z) + x3 * t.c.z;
+        bem_point_accumulate(&o, P.k, p.x, p.y, p.z, np.x, np.y, np.z, nq.x, nq.y, nq.z, upnq, qx, qy, qz, w);
+    }
+    double A = P.area[j];
+    o.L = c64_scale(o.L, A); o.M = c64_scale(o.M, A); o.Mt = c64_scale(o.Mt, A); o.N = c64_scale(o.N, A);
+    return o;
+}
+
 /* BemOperatorsConst (boundaryelementsolver.cpp:844-894) */
 bem_ops4 ops4(const orc_problem& P, int64_t i, int64_t j)
 {
@@ -373,7 +399,15 @@ cplx entry(const orc_problem& P, int kind, int64_t i, int64_t j)
     bem_ops4 o = ops4(P, i, j);
     cplx X, Y;
     bem_xy(&o, P.cc, i == j, &X, &Y);
-    return kind == 0 ? bem_phi_entry(X, Y, P.subst[j], P.cphi[j]) : bem_dphi_entry(X, Y, P.subst[j], P.cd[j]);
+    cplx e = kind == 0 ? bem_phi_entry(X, Y, P.subst[j], P.cphi[j]) : bem_dphi_entry(X, Y, P.subst[j], P.cd[j]);
+    if (!P.plane) return e;
+    /* implicitPhiMatrixPlusReflected / implicitDPhiMatrixPlusReflected
+       (:2042-2050): the free-space entry plus the mirror image's entry
+       (:1556-1643, same boundary-condition coefficients, no jump term) */
+    o = ops_reflected(P, i, j);
+    bem_xy(&o, P.cc, 0, &X, &Y);
+    return c64_add(e, kind == 0 ? bem_phi_entry(X, Y, P.subst[j], P.cphi[j])
+                                : bem_dphi_entry(X, Y, P.subst[j], P.cd[j]));
 }
 
 /* ------------------------------------------------------ cluster tree */
@@ -1020,26 +1054,66 @@ void orc_field(const orc_problem* p, int64_t nobs, const double* obs3, const dou
             const int64_t j1 = std::min(n, j0 + chunk);
             cplx part[4] = {c64(0, 0), c64(0, 0), c64(0, 0), c64(0, 0)};
             for (int64_t j = j0; j < j1; j++) {
-                const Tri& t = p->tri[j];
-                const V3& nq = p->normal[j];
-                cplx L = c64(0, 0), M = c64(0, 0);
-                for (int q = 0; q < 7; q++) {
-                    double x1 = rule[q][0], x2 = rule[q][1];
-                    double x3 = (1.0 - x2) - x1;
-                    double qx = (x1 * t.a.x + x2 * t.b.x) + x3 * t.c.x;
-                    double qy = (x1 * t.a.y + x2 * t.b.y) + x3 * t.c.y;
-                    double qz = (x1 * t.a.z + x2 * t.b.z) + x3 * t.c.z;
-                    bem_field_point_accumulate(&L, &M, p->k, px, py, pz, nq.x, nq.y, nq.z, qx, qy, qz, rule[q][2]);
-                }
-                L = c64_scale(L, p->area[j]);
-                M = c64_scale(M, p->area[j]);
-                part[j & 3] = c64_add(part[j & 3], bem_field_term(L, M, phi[j], dphi[j]));
+                auto term = [&](const Tri& t, const V3& nq) {
+                    cplx L = c64(0, 0), M = c64(0, 0);
+                    for (int q = 0; q < 7; q++) {
+                        double x1 = rule[q][0], x2 = rule[q][1];
+                        double x3 = (1.0 - x2) - x1;
+                        double qx = (x1 * t.a.x + x2 * t.b.x) + x3 * t.c.x;
+                        double qy = (x1 * t.a.y + x2 * t.b.y) + x3 * t.c.y;
+                        double qz = (x1 * t.a.z + x2 * t.b.z) + x3 * t.c.z;
+                        bem_field_point_accumulate(&L, &M, p->k, px, py, pz, nq.x, nq.y, nq.z, qx, qy, qz,
+                                                   rule[q][2]);
+                    }
+                    L = c64_scale(L, p->area[j]);
+                    M = c64_scale(M, p->area[j]);
+                    return bem_field_term(L, M, phi[j], dphi[j]);
+                };
+                cplx v = term(p->tri[j], p->normal[j]);
+                /* the mirror image's term (:2305-2311) */
+                if (p->plane) v = c64_add(v, term(p->rtri[j], p->rnormal[j]));
+                part[j & 3] = c64_add(part[j & 3], v);
             }
             cplx cs = c64_add(c64_add(part[0], part[1]), c64_add(part[2], part[3]));
             total = j0 == 0 ? cs : c64_add(total, cs);
         }
         out2[2 * i] = total.re;
         out2[2 * i + 1] = total.im;
+    }
+}
+
+/* MeshFunctions::reflectTriangles (meshfunctions.cpp:349-378) through
+   BoundaryElements::reflectGeometry (boundaryelements.cpp:151-155): the
+   plane normal normalised, x' = R [x; 1] with R = [I - 2 n n^T | -2 n d],
+   d = -n.point, each row summed left to right; nodes reversed (a' = R c,
+   b' = R b, c' = R a) and the normal recomputed from them; areas unchanged
+   (copied before the reflection) */
+void orc_set_plane(orc_problem* p, const double* point3, const double* normal3)
+{
+    p->plane = normal3 && (normal3[0] != 0 || normal3[1] != 0 || normal3[2] != 0);
+    p->rtri.clear();
+    p->rnormal.clear();
+    if (!p->plane) return;
+    V3 u = normalized(v3(normal3[0], normal3[1], normal3[2]));
+    double d = -dot(u, v3(point3[0], point3[1], point3[2]));
+    double R[3][4] = {{1.0 - 2.0 * (u.x * u.x), -2.0 * u.x * u.y, -2.0 * u.x * u.z, -2.0 * u.x * d},
+                      {-2.0 * u.x * u.y, 1.0 - 2.0 * (u.y * u.y), -2.0 * u.y * u.z, -2.0 * u.y * d},
+                      {-2.0 * u.x * u.z, -2.0 * u.y * u.z, 1.0 - 2.0 * (u.z * u.z), -2.0 * u.z * d}};
+    auto mirror = [&](V3 x) {
+        double o[3];
+        for (int r = 0; r < 3; r++) o[r] = ((R[r][0] * x.x + R[r][1] * x.y) + R[r][2] * x.z) + R[r][3];
+        return v3(o[0], o[1], o[2]);
+    };
+    p->rtri.resize(p->n);
+    p->rnormal.resize(p->n);
+    for (int64_t e = 0; e < p->n; e++) {
+        const Tri& t = p->tri[e];
+        Tri r;
+        r.a = mirror(t.c);
+        r.b = mirror(t.b);
+        r.c = mirror(t.a);
+        p->rtri[e] = r;
+        p->rnormal[e] = tri_normal(r);
     }
 }
 

@@ -100,6 +100,16 @@ int orc_solve(orc_problem* p, double acatol, int64_t maxrank, double gmres_tol, 
    pivoting LU; phi/dphi/pressure out */
 int orc_dense_solve(orc_problem* p, double rho, double freq, double* phi, double* dphi, double* pressure);
 
+/* One impedance plane through point3 with normal normal3 (a rigid image,
+   the single-plane branch of hMatrixSolve, boundaryelementsolver.cpp:549-557,
+   594-604): every Phi / DPhi entry gains the regular-rule term of the mirror
+   image of its column element (implicitPhiMatrixReflected /
+   implicitDPhiMatrixReflected, :1556-1643, added as in
+   implicit*MatrixPlusReflected, :2042-2050), and the field gains the mirror
+   images' terms (:2270-2312).  normal3 NULL or zero removes the plane.
+   orc_entries4 keeps returning the free-space operators. */
+void orc_set_plane(orc_problem* p, const double* point3, const double* normal3);
+
 /* Field potential of the boundary solution phi, dphi (n complex each,
    cluster order) at nobs observation points (nobs x 3): out[i] =
    sum_j M_j(x_i) phi_j - L_j(x_i) dphi_j with the order-5 7-point rule, as

@@ -56,6 +56,7 @@ def lib():
                                          _p, _p, _p, _p]),
             "orc_dense_solve": (ctypes.c_int, [_p, _dbl, _dbl, _p, _p, _p]),
             "orc_field": (None, [_p, ctypes.c_int64, _p, _p, _p, _p]),
+            "orc_set_plane": (None, [_p, _p, _p]),
             "orc_tables": (None, [_p, _p, _p]),
             "orc_sincos": (None, [_i64, _p, _p, _p]),
             "orc_project": (ctypes.c_int, [_p, _p, _p]),
@@ -126,6 +127,14 @@ class Problem:
         pts = _c(points if points is not None else np.zeros((0, 6)), np.float64)
         pl = _c(plane if plane is not None else np.zeros(5), np.float64)
         lib().orc_set_sources(self.h, len(pts), _ptr(pts), 1 if plane is not None else 0, _ptr(pl))
+
+    def set_plane(self, point=None, normal=None):
+        """one impedance plane (rigid image) through point with normal; None removes it"""
+        if normal is None:
+            lib().orc_set_plane(self.h, None, None)
+            return
+        pt, nr = _c(point, np.float64), _c(normal, np.float64)
+        lib().orc_set_plane(self.h, _ptr(pt), _ptr(nr))
 
     def rhs_parts(self):
         n = self.n

@@ -80,6 +80,7 @@ __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile*
 {
     __shared__ double sq[3][16][kTile];
     __shared__ double sn[3][kTile], sp[3][kTile], sa[kTile];
+    __shared__ double rv[PLANE ? 9 : 1][kTile], rn[PLANE ? 3 : 1][kTile];   // mirror images
     const Tile T = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int64_t n = M.n;
@@ -97,6 +98,10 @@ __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile*
         sn[0][c] = M.nx[j]; sn[1][c] = M.ny[j]; sn[2][c] = M.nz[j];
         sp[0][c] = M.px[j]; sp[1][c] = M.py[j]; sp[2][c] = M.pz[j];
         sa[c] = M.area[j];
+        if (PLANE) {
+            for (int d = 0; d < 9; d++) rv[d][c] = M.rvtx[d * n + j];
+            rn[0][c] = M.rnx[j]; rn[1][c] = M.rny[j]; rn[2][c] = M.rnz[j];
+        }
     }
     __syncthreads();
     const int r = tid & (kTile - 1), cg = tid >> 6;
@@ -116,8 +121,10 @@ __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile*
         int64_t d = T.dest + r + (int64_t)c * T.ld;
         bem_c64 ep = bem_phi_entry(X, Y, F.subst[j], F.cphi[j]);
         bem_c64 ed = bem_dphi_entry(X, Y, F.subst[j], F.cd[j]);
-        if (PLANE) {   // impedance plane: + the mirror image's entry
-            refl_xy(M, F, i, j, &X, &Y);
+        if (PLANE) {   // impedance plane: + the mirror image's entry (refl_xy on the staged image)
+            const ElemVtx e{rv[0][c], rv[1][c], rv[2][c], rv[3][c], rv[4][c], rv[5][c], rv[6][c], rv[7][c], rv[8][c]};
+            bem_ops4 ro = regular_ops_vtx<1>(F.k, px, py, pz, npx, npy, npz, rn[0][c], rn[1][c], rn[2][c], sa[c], e);
+            bem_xy(&ro, F.cc, 0, &X, &Y);
             ep = c64_add(ep, bem_phi_entry(X, Y, F.subst[j], F.cphi[j]));
             ed = c64_add(ed, bem_dphi_entry(X, Y, F.subst[j], F.cd[j]));
         }

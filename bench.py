@@ -18,6 +18,11 @@ after the other) is timed beside it on the same points (fp64_value,
 speedup_vs_fp64), and the 1000 Hz surface pressure of both is compared with
 the FP64 CPU restatement run on the host (cpu_baseline leg).
 
+--plane D puts a rigid impedance plane D metres below the sphere's centre
+(the plane z = -D, the single-plane branch of hMatrixSolve,
+boundaryelementsolver.cpp:549-557, 594-604): every entry then carries the
+mirror image's regular quadrature as well.
+
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -48,6 +53,13 @@ def workload(level):
     return bem, tri, bc
 
 
+def plane_of(args):
+    """(point, normal) of the rigid plane z = -D, or None"""
+    if args.plane is None:
+        return None
+    return np.array([0.0, 0.0, -args.plane]), np.array([0.0, 0.0, 1.0])
+
+
 def run_cpu_baseline(tri, bc, args):
     """The CPU restatement (oracle/) on the same workload, all host threads:
     one full frequency point."""
@@ -57,6 +69,8 @@ def run_cpu_baseline(tri, bc, args):
     t0 = time.perf_counter()
     P = oracle.Problem(tri, bc, leaf=args.leaf, eta=args.eta)
     P.set_frequency(args.freq, 343.0, 1)
+    if plane_of(args) is not None:
+        P.set_plane(*plane_of(args))
     _, _, pr, st = P.solve(args.aca_tol, gmres_tol=args.gmres_tol, seed=args.seed, rho=1.21, freq=args.freq,
                            nthreads=threads)
     t = time.perf_counter() - t0
@@ -91,9 +105,10 @@ def timed_run(bem, S, ctx_h, args, prec, freqs, lanes):
     return pr, its, stats, elapsed, prof[:6].copy(), prof[6:].copy()
 
 
-def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload):
+def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload, quads=1):
     """Dominant kernel family (largest total device time) against its bound.
-    Assembly families: algorithmic flops (F_REG per regular entry) over the
+    Assembly families: algorithmic flops (F_REG per regular quadrature,
+    quads of them per regular entry: 2 with an impedance plane) over the
     family's mean launch duration, vector-ALU peak of the precision.  MVM
     families: algorithmic bytes of the stored operator over the duration."""
     tot = avg_ms * cnt
@@ -107,12 +122,13 @@ def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload):
             entries = 2 * (n_dense - n)  # regular entries of the Phi and DPhi near field
         else:
             entries = st["evaluated_phi"] + st["evaluated_dphi"] - 2 * n_dense
-        flops = F_REG * entries / launches
+        flops = quads * F_REG * entries / launches
         ach = flops / (avg_ms[dom] * 1e-3) / 1e12
         peak = PEAK_TFLOPS[prec_name]
         rl = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
               "frac": round(ach / peak, 4), "traffic": None,
-              "algorithmic_per_launch": f"{flops:.4g} flop ({entries / launches:.4g} regular entries x {F_REG})"}
+              "algorithmic_per_launch": f"{flops:.4g} flop ({entries / launches:.4g} regular entries x "
+                                        f"{quads} x {F_REG})"}
     elif dom == 1:
         rl = {"bound": "mfma", "achieved": None, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": None,
               "traffic": None}
@@ -185,6 +201,8 @@ def main():
     ap.add_argument("--aca-tol", type=float, default=1e-5)
     ap.add_argument("--gmres-tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--plane", type=float, default=None, metavar="D",
+                    help="rigid impedance plane D m below the sphere's centre (default: none)")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -203,6 +221,8 @@ def main():
     n = len(tri)
     t0 = time.perf_counter()
     S = bem.Solver(tri, bc, leaf=args.leaf, eta=args.eta)
+    if plane_of(args) is not None:
+        S.set_impedance_plane(*plane_of(args))
     setup_s = time.perf_counter() - t0
     _, blocks = S.layout()
     ctx_h = bem.lib().bem_solver_context(S.h)
@@ -232,7 +252,8 @@ def main():
         "config": {"workload": f"pulsating sphere, icosphere level {args.level} ({n} triangles), r=0.5 m, "
                                f"{len(freqs)} frequency points {freqs[0]:.0f}-{freqs[-1]:.0f} Hz per step "
                                f"(incl. {args.freq:g} Hz), Burton-Miller, leaf {args.leaf}, eta {args.eta:g}, "
-                               f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}, {lanes} lanes",
+                               f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}, {lanes} lanes"
+                               + ("" if args.plane is None else f", rigid plane {args.plane:g} m below the centre"),
                    "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "setup_s": round(setup_s, 3),
                    "ms_per_point": round(1e3 * elapsed / points, 3),
                    "gmres_iters": int(its[ic, 0]), "residual_estimate": st["residual"],
@@ -241,7 +262,8 @@ def main():
                    "phase_ms": {k: round(st[k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
                    "kernel_ms_lane0": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
         "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0,
-                             (args.level, args.leaf, args.eta, args.aca_tol) == (5, 64, 2.0, 1e-5)),
+                             (args.level, args.leaf, args.eta, args.aca_tol, args.plane) == (5, 64, 2.0, 1e-5, None),
+                             quads=1 if args.plane is None else 2),
     }
     p_head = pr[ic, 0]
     p64 = None

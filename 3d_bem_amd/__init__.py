@@ -29,6 +29,12 @@ _SIGS = {
     "bem_upload_mesh": (_int, [_p, _i64, _p, _p, _p, _p, _dbl]),
     "bem_set_bc": (_int, [_p, _p, _p, _p]),
     "bem_set_plane": (_int, [_p, _p, _p]),
+    "bem_hlu_create": (_int, [_i64, _i64, _p, _i64, _p, _p, _p, _i64, _dbl, ctypes.POINTER(_p)]),
+    "bem_hlu_solve": (_int, [_p, _p, _p]),
+    "bem_hlu_dense": (_int, [_p, _p]),
+    "bem_hlu_stats": (_int, [_p, _p]),
+    "bem_hlu_destroy": (_int, [_p]),
+    "bem_hlu_error": (ctypes.c_char_p, []),
     "bem_solver_set_impedance_plane": (_int, [_p, _p, _p]),
     "bem_upload_partition": (_int, [_p, _i64, _p, _p, _p, _p, _p]),
     "bem_set_frequency": (_int, [_p, _dbl, _dbl, _dbl]),
@@ -46,6 +52,8 @@ _SIGS = {
     "bem_read_msh_error": (ctypes.c_char_p, []),
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),
     "bem_solver_set_preconditioner": (_int, [_p, _int]),
+    "bem_solver_set_hlu": (_int, [_p, _i64, _dbl]),
+    "bem_precond_create_hlu": (_int, [_p, _p, _i64, _p, _i64, _dbl, ctypes.POINTER(_p)]),
     "bem_precond_create": (_int, [_p, _p, ctypes.POINTER(_p)]),
     "bem_precond_destroy": (_int, [_p]),
     "bem_precond_apply": (_int, [_p, _p, _p, _p]),
@@ -303,6 +311,15 @@ class HMatrix:
         self.ctx._check(lib().bem_precond_create(self.ctx.h, self.h, ctypes.byref(p)))
         return Preconditioner(self.ctx, p)
 
+    def hlu(self, clusters2, rank=2, rel_err=0.0):
+        """host H-LU preconditioner of this operator (bem_precond_create_hlu);
+        clusters2 (nc, 2) = (start, size) of every cluster"""
+        cl = _c(np.asarray(clusters2)[:, :2], np.int64)
+        p = _p()
+        self.ctx._check(lib().bem_precond_create_hlu(self.ctx.h, self.h, len(cl), _ptr(cl), int(rank), float(rel_err),
+                                                     ctypes.byref(p)))
+        return Preconditioner(self.ctx, p)
+
     def gmres(self, b, tol, restart=300, maxit=100000, x0=None, precond=None):
         bd = self.ctx.array(b)
         xd = self.ctx.array(np.zeros(self.ctx.n, np.complex128) if x0 is None else x0)
@@ -328,6 +345,69 @@ class Preconditioner:
     def close(self):
         if getattr(self, "h", None):
             lib().bem_precond_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HLU:
+    """Host H-LU of an H-matrix given block by block (bem_hlu_create):
+    clusters2 (nc, 2) = (start, size) of every cluster, blocks5 the
+    partition, blocks[b] = dense (m, n) array or (U, alpha, V) with the block
+    U diag(alpha) V; max_rank / rel_err the truncation (0 / 0: exact)."""
+
+    def __init__(self, n, clusters2, blocks5, blocks, max_rank=2, rel_err=0.0):
+        cl = _c(np.asarray(clusters2)[:, :2], np.int64)
+        B = _c(blocks5, np.int64)
+        rank = np.empty(len(B), np.int64)
+        self._keep = []
+        for b, blk in enumerate(blocks):
+            if isinstance(blk, tuple):
+                U, al, V = blk
+                rank[b] = len(al)
+                buf = np.concatenate([np.asarray(U, np.complex128).ravel(order="F"), np.asarray(al, np.complex128),
+                                      np.asarray(V, np.complex128).ravel(order="F")])
+            else:
+                rank[b] = -1
+                buf = np.asarray(blk, np.complex128).ravel(order="F")
+            self._keep.append(np.ascontiguousarray(buf))
+        ptrs = (ctypes.c_void_p * len(B))(*[a.ctypes.data for a in self._keep])
+        h = _p()
+        rc = lib().bem_hlu_create(int(n), len(cl), _ptr(cl), len(B), _ptr(B), _ptr(rank), ptrs, int(max_rank),
+                                  float(rel_err), ctypes.byref(h))
+        self._keep = None
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_hlu_error().decode())
+        self.h, self.n = h, int(n)
+
+    def solve(self, b):
+        b = _c(b, np.complex128)
+        x = np.empty_like(b)
+        rc = lib().bem_hlu_solve(self.h, _ptr(b), _ptr(x))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_hlu_error().decode())
+        return x
+
+    def dense(self):
+        """the product L U (n, n)"""
+        out = np.empty((self.n, self.n), np.complex128, order="F")
+        rc = lib().bem_hlu_dense(self.h, _ptr(out))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_hlu_error().decode())
+        return out
+
+    def stats(self):
+        o = np.zeros(3)
+        lib().bem_hlu_stats(self.h, _ptr(o))
+        return {"entries": o[0], "max_rank": int(o[1]), "factor_s": o[2]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bem_hlu_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -423,9 +503,13 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
 
-    def set_preconditioner(self, kind):
-        """0: none, 1: leaf-block LU (bem_solver_set_preconditioner)"""
-        rc = lib().bem_solver_set_preconditioner(self.h, int(kind))
+    def set_preconditioner(self, kind, rank=2, rel_err=0.0):
+        """0: none, 1: leaf-block LU, 2: host H-LU with the truncation rank /
+        rel_err (bem_solver_set_preconditioner, bem_solver_set_hlu)"""
+        if kind == 2:
+            rc = lib().bem_solver_set_hlu(self.h, int(rank), float(rel_err))
+        else:
+            rc = lib().bem_solver_set_preconditioner(self.h, int(kind))
         if rc != BEM_OK:
             raise BemError(rc, "bem_solver_set_preconditioner")
 

@@ -1285,6 +1285,8 @@ struct bem_prec {
     PrecBlock* d_blk = nullptr;
     bem_c64* d_lu = nullptr;
     int32_t* d_piv = nullptr;
+    bem_hlu_t hlu = nullptr;        // host H-LU (bem_precond_create_hlu)
+    std::vector<bem_c64> host;      // its staging vector
 };
 
 namespace {
@@ -2487,9 +2489,51 @@ int bem_precond_create(bem_ctx_t c, bem_hmat_t h, bem_prec_t* out)
     return BEM_OK;
 }
 
+int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const int64_t* clusters2, int64_t max_rank,
+                           double rel_err, bem_prec_t* out)
+{
+    if (!c || !h || !out || h->ctx != c || nclusters <= 0 || !clusters2 || rel_err < 0)
+        return set_err(c, BEM_ERR_ARG, "bem_precond_create_hlu: bad arguments");
+    *out = nullptr;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const int64_t nb = (int64_t)c->blocks.size();
+    std::vector<int64_t> blocks5(5 * nb), rank(nb);
+    std::vector<std::vector<bem_c64>> data(nb);
+    std::vector<const double*> ptr(nb);
+    for (int64_t b = 0; b < nb; b++) {
+        const auto& B = c->blocks[b];
+        blocks5[5 * b] = B.row0; blocks5[5 * b + 1] = B.rows;
+        blocks5[5 * b + 2] = B.col0; blocks5[5 * b + 3] = B.cols;
+        blocks5[5 * b + 4] = B.adm ? 1 : 0;
+        int64_t k = -1;
+        int rc = bem_block_info(h, b, &k);
+        if (rc) return rc;
+        rank[b] = k;
+        const int64_t m = B.rows, n = B.cols;
+        data[b].resize(k < 0 ? (size_t)(m * n) : (size_t)(m * k + k + k * n));
+        bem_c64* d = data[b].data();
+        rc = k < 0 ? bem_block_download(h, b, (double*)d, nullptr, nullptr)
+                   : bem_block_download(h, b, (double*)d, (double*)(d + m * k), (double*)(d + m * k + k));
+        if (rc) return rc;
+        ptr[b] = (const double*)d;
+    }
+    bem_hlu_t L = nullptr;
+    int rc = bem_hlu_create(c->n, nclusters, clusters2, nb, blocks5.data(), rank.data(), ptr.data(), max_rank, rel_err,
+                            &L);
+    if (rc) return set_err(c, rc, std::string("bem_precond_create_hlu: ") + bem_hlu_error());
+    auto p = new bem_prec();
+    p->ctx = c;
+    p->hlu = L;
+    p->host.resize((size_t)c->n);
+    *out = p;
+    return BEM_OK;
+}
+
 int bem_precond_destroy(bem_prec_t p)
 {
     if (!p) return BEM_ERR_ARG;
+    if (p->hlu) bem_hlu_destroy(p->hlu);
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
     pfree(p->ctx, p->d_blk);
@@ -2503,6 +2547,16 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
 {
     if (!c || !p || !v || !z || p->ctx != c || v == z) return set_err(c, BEM_ERR_ARG, "bem_precond_apply: bad arguments");
     HIPC(c, hipSetDevice(c->device));
+    if (p->hlu) {   // host H-LU: the vector goes to the host and back
+        const size_t bytes = sizeof(bem_c64) * (size_t)c->n;
+        HIPC(c, hipMemcpyAsync(p->host.data(), v, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        int rc = bem_hlu_solve(p->hlu, (const double*)p->host.data(), (double*)p->host.data());
+        if (rc) return set_err(c, rc, std::string("bem_precond_apply: ") + bem_hlu_error());
+        HIPC(c, hipMemcpyAsync(z, p->host.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return BEM_OK;
+    }
     hipLaunchKernelGGL(k_blu_apply, dim3((unsigned)p->nblk), dim3(64), 0, c->stream, p->d_blk, p->d_lu, p->d_piv,
                        (const bem_c64*)v, (bem_c64*)z);
     return launch_check(c, "k_blu_apply");

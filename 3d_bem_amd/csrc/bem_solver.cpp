@@ -184,6 +184,26 @@ void BoundaryElementSolver::applyImpedancePlanes()
     else check(ctx_, bem_set_plane(ctx_, P[0].halfSpacePlanePoint, P[0].halfSpacePlaneNormal), "bem_set_plane");
 }
 
+bem_prec_t BoundaryElementSolver::makePreconditioner()
+{
+    const auto& P = bemSolverParameters;
+    bem_prec_t pc = nullptr;
+    if (P.preconditioner == 1) {
+        check(ctx_, bem_precond_create(ctx_, dPhiHMat->handle(), &pc), "bem_precond_create");
+    } else if (P.preconditioner == 2) {
+        // dPhiHMatCopy + HArithm::LUDecomposition (boundaryelementsolver.cpp:622-643)
+        std::vector<int64_t> cl;
+        for (const auto& C : clusterTree->clusters()) {
+            cl.push_back(C.start);
+            cl.push_back(C.size);
+        }
+        check(ctx_, bem_precond_create_hlu(ctx_, dPhiHMat->handle(), (int64_t)cl.size() / 2, cl.data(),
+                                           (int64_t)P.preconditionerRank, P.preconditionerRelError, &pc),
+              "bem_precond_create_hlu");
+    }
+    return pc;
+}
+
 BoundaryElementSolver::~BoundaryElementSolver()
 {
     phiHMat.reset();
@@ -338,7 +358,7 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     DeviceVector x(ctx_, E.n), phi(ctx_, E.n), dphi(ctx_, E.n), p(ctx_, E.n);
     double est = 0;
     bem_prec_t pc = nullptr;
-    if (P.preconditioner) check(ctx_, bem_precond_create(ctx_, dPhiHMat->handle(), &pc), "bem_precond_create");
+    pc = makePreconditioner();
     for (size_t k = 0; k < nr; k++) {
         x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
         iterations[k] = GMRES::gmresSolve(*dPhiHMat, x, *b[k], tol, P.gmresRestart, &est, pc);
@@ -389,7 +409,7 @@ void BoundaryElementSolver::hMatrixSolve()
     double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
     double est = 0;
     bem_prec_t pc = nullptr;
-    if (P.preconditioner) check(ctx_, bem_precond_create(ctx_, dPhiHMat->handle(), &pc), "bem_precond_create");
+    pc = makePreconditioner();
     int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est, pc);
     if (pc) bem_precond_destroy(pc);
     check(ctx_, bem_synchronize(ctx_), "sync");
@@ -432,6 +452,8 @@ struct bem_solver {
     bem::BoundaryElements input;   // input order
     std::vector<std::unique_ptr<bem::BoundaryElementSolver>> lanes;
     int preconditioner = 0;
+    long hluRank = 2;
+    double hluRelError = 0;
 };
 
 extern "C" {
@@ -530,6 +552,8 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         S.bemSolverParameters.pivotSeed = seed;
         S.bemSolverParameters.precision = precision;
         S.bemSolverParameters.preconditioner = s->preconditioner;
+        S.bemSolverParameters.preconditionerRank = s->hluRank;
+        S.bemSolverParameters.preconditionerRelError = s->hluRelError;
         S.calculateBoundarySolution();
         int64_t n = S.boundaryElements.n;
         if (phi) std::memcpy(phi, S.phiSolution.data(), n * sizeof(bem::cplx));
@@ -548,8 +572,17 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
 
 int bem_solver_set_preconditioner(bem_solver_t s, int kind)
 {
-    if (!s || !s->s || kind < 0 || kind > 1) return BEM_ERR_ARG;
+    if (!s || !s->s || kind < 0 || kind > 2) return BEM_ERR_ARG;
     s->preconditioner = kind;
+    return BEM_OK;
+}
+
+int bem_solver_set_hlu(bem_solver_t s, int64_t rank, double rel_err)
+{
+    if (!s || !s->s || rank < 0 || !(rel_err >= 0)) return BEM_ERR_ARG;
+    s->preconditioner = 2;
+    s->hluRank = (long)rank;
+    s->hluRelError = rel_err;
     return BEM_OK;
 }
 
@@ -647,6 +680,8 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 P.pivotSeed = seed;
                 P.precision = precision;
                 P.preconditioner = s->preconditioner;
+                P.preconditionerRank = s->hluRank;
+                P.preconditionerRelError = s->hluRelError;
                 std::vector<std::vector<bem::cplx>> pr;
                 std::vector<int64_t> its;
                 for (int64_t f = q; f < nf; f += L) {

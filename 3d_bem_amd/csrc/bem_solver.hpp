@@ -117,7 +117,9 @@ struct BemSolverParameters {
     long gmresRestart = 300;
     uint64_t pivotSeed = 1;
     int precision = BEM_PREC_FP64;
-    int preconditioner = 0;       // 0 none, 1 leaf-block LU (bem_precond_create)
+    int preconditioner = 0;       // 0 none, 1 leaf-block LU (bem_precond_create), 2 H-LU (bem_precond_create_hlu)
+    long preconditionerRank = 2;  // bemparameters.h:27
+    double preconditionerRelError = 0;   // bemparameters.h:28
 };
 
 struct PointSource { double x, y, z; cplx weight; };
@@ -132,6 +134,9 @@ public:
     // the context's image plane from boundaryElements.impedancePlanes (none or
     // one: hMatrixSolve's branches, boundaryelementsolver.cpp:549-557)
     void applyImpedancePlanes();
+    // the GMRES preconditioner of bemSolverParameters.preconditioner for the
+    // assembled DPhi (nullptr for none)
+    bem_prec_t makePreconditioner();
     void setWaveSpeed(double c) { waveSpeed = c; }
     void setAirDensity(double rho) { airDensity = rho; }
     void calculateBoundarySolution();      // boundaryelementsolver.cpp:12-138 (H-matrix branch)

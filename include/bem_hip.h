@@ -61,6 +61,31 @@ const char* bem_last_error(bem_ctx_t ctx);
    BemOperatorsConst (boundaryelementsolver.cpp:844-894). */
 int bem_upload_mesh(bem_ctx_t ctx, int64_t n, const double* nodes9, const double* normal3, const double* area,
                     const double* colloc3, double hbar);
+/* Host H-LU (the reference keeps the H-LU preconditioner on the host:
+   HArithm::LUDecomposition / recursiveLUDecomposition,
+   HMatrix/harithm.cpp:1192-1296, of the truncated copy
+   HMatrix(const HMatrix&, rank, relError), hmatrix.cpp:770-776, as
+   hMatrixSolve builds it, boundaryelementsolver.cpp:622-643).  clusters2:
+   (start, size) of every cluster of the cluster tree; blocks5: the partition
+   (bem_upload_partition's blocks); rank[b] = -1: data[b] is the dense block
+   (m x n column-major complex), else data[b] holds U (m x k column-major),
+   alpha (k), V (k x n column-major) one after the other (bem_block_download's
+   three arrays).  max_rank / rel_err: preconditionerRank /
+   preconditionerRelError (bemparameters.h:27-28; 0 / 0 = no truncation, an
+   exact LU).  Factors on return.  bem_hlu_solve: x = U^-1 L^-1 b (host
+   vectors, n complex; x may equal b): forwardSubstitution +
+   backwardSubstitution (harithm.cpp:1314-1412).  bem_hlu_dense: L U as a
+   dense n x n matrix (column-major complex).  bem_hlu_stats: stored complex
+   entries, largest rank, factor seconds.  Errors: bem_hlu_error(). */
+typedef struct bem_hlu* bem_hlu_t;
+int bem_hlu_create(int64_t n, int64_t nclusters, const int64_t* clusters2, int64_t nb, const int64_t* blocks5,
+                   const int64_t* rank, const double* const* data, int64_t max_rank, double rel_err, bem_hlu_t* out);
+int bem_hlu_solve(bem_hlu_t h, const double* b2, double* x2);
+int bem_hlu_dense(bem_hlu_t h, double* out2);
+int bem_hlu_stats(bem_hlu_t h, double* out3);
+int bem_hlu_destroy(bem_hlu_t h);
+const char* bem_hlu_error(void);
+
 /* One impedance plane, a rigid image: the plane through point3 with normal
    normal3 (normalised here); normal3 NULL or zero removes it.  Later
    assemblies add to every Phi / DPhi entry the regular-rule entry of the
@@ -124,6 +149,15 @@ typedef struct bem_prec* bem_prec_t;
 int bem_precond_create(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t* out);
 int bem_precond_destroy(bem_prec_t p);
 int bem_precond_apply(bem_ctx_t ctx, bem_prec_t p, const void* v_dev, void* z_dev);
+/* The H-LU preconditioner of hMatrixSolve (boundaryelementsolver.cpp:622-643,
+   usePreconditioner): the host H-LU of bem_hlu_create (truncation max_rank /
+   rel_err = preconditionerRank / preconditionerRelError) fed with the blocks
+   of the assembled H-matrix h (downloaded, FP32 blocks widened to FP64);
+   clusters2 = (start, size) of every cluster of the cluster tree.  Used by
+   bem_precond_apply / bem_gmres_prec like the leaf-block LU; each
+   application moves the vector to the host and back. */
+int bem_precond_create_hlu(bem_ctx_t ctx, bem_hmat_t h, int64_t nclusters, const int64_t* clusters2, int64_t max_rank,
+                           double rel_err, bem_prec_t* out);
 int bem_gmres_prec(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t p, const void* b_dev, void* x_dev, double tol,
                    int64_t restart, int64_t maxit, int64_t* iters, double* rel_resid);
 
@@ -190,8 +224,12 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
                      double* phi, double* dphi, double* pressure, double* stats16);
 
 /* GMRES preconditioner of bem_solver_solve / bem_solver_sweep: 0 none (the
-   default), 1 the leaf-block LU of bem_precond_create */
+   default), 1 the leaf-block LU of bem_precond_create, 2 the host H-LU of
+   bem_precond_create_hlu (usePreconditioner, bemparameters.h:26) */
 int bem_solver_set_preconditioner(bem_solver_t s, int kind);
+/* the H-LU preconditioner with preconditionerRank / preconditionerRelError
+   (bemparameters.h:27-28; defaults 2 / 0) */
+int bem_solver_set_hlu(bem_solver_t s, int64_t rank, double rel_err);
 
 /* The scene's impedance plane (BoundaryElements::impedancePlanes,
    boundaryelements.h:97, filled from an infinite-baffle section,

@@ -54,6 +54,7 @@ _SIGS = {
     "bem_solver_set_preconditioner": (_int, [_p, _int]),
     "bem_solver_set_hlu": (_int, [_p, _i64, _dbl]),
     "bem_precond_create_hlu": (_int, [_p, _p, _i64, _p, _i64, _dbl, ctypes.POINTER(_p)]),
+    "bem_precond_stats": (_int, [_p, _p]),
     "bem_precond_create": (_int, [_p, _p, ctypes.POINTER(_p)]),
     "bem_precond_destroy": (_int, [_p]),
     "bem_precond_apply": (_int, [_p, _p, _p, _p]),
@@ -342,6 +343,11 @@ class Preconditioner:
         self.ctx._check(lib().bem_precond_apply(self.ctx.h, self.h, vd.ptr, zd.ptr))
         return zd.download()
 
+    def stats(self):
+        o = np.zeros(4)
+        lib().bem_precond_stats(self.h, _ptr(o))
+        return {"setup_ms": o[0], "applies": int(o[1]), "apply_ms": o[2], "entries": o[3]}
+
     def close(self):
         if getattr(self, "h", None):
             lib().bem_precond_destroy(self.h)
@@ -419,7 +425,7 @@ class HLU:
 
 STAT_KEYS = ["t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres", "t_total", "gmres_iters", "residual",
              "max_rank_phi", "max_rank_dphi", "evaluated_phi", "evaluated_dphi", "near_blocks", "far_blocks",
-             "refine_steps", "fp32_gmres_iters", "dphi_bytes"]
+             "precond_setup_ms", "precond_apply_ms", "dphi_bytes"]
 
 
 def read_msh(path, include=None, exclude=None):

@@ -1287,6 +1287,8 @@ struct bem_prec {
     int32_t* d_piv = nullptr;
     bem_hlu_t hlu = nullptr;        // host H-LU (bem_precond_create_hlu)
     std::vector<bem_c64> host;      // its staging vector
+    double setup_ms = 0, apply_ms = 0, entries = 0;
+    int64_t applies = 0;
 };
 
 namespace {
@@ -2435,6 +2437,7 @@ int bem_precond_create(bem_ctx_t c, bem_hmat_t h, bem_prec_t* out)
 {
     if (!c || !h || !out || h->ctx != c) return set_err(c, BEM_ERR_ARG, "bem_precond_create: bad arguments");
     *out = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
     HIPC(c, hipSetDevice(c->device));
     // the diagonal leaf blocks must tile [0, n)
     std::vector<PrecBlock> blk;
@@ -2485,6 +2488,8 @@ int bem_precond_create(bem_ctx_t c, bem_hmat_t h, bem_prec_t* out)
         own.clear();
         return fail(set_err(c, BEM_ERR_NAN, "bem_precond_create: singular diagonal block"));
     }
+    p->entries = (double)lu;
+    p->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = p;
     return BEM_OK;
 }
@@ -2497,6 +2502,7 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
     *out = nullptr;
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
     const int64_t nb = (int64_t)c->blocks.size();
     std::vector<int64_t> blocks5(5 * nb), rank(nb);
     std::vector<std::vector<bem_c64>> data(nb);
@@ -2526,7 +2532,21 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
     p->ctx = c;
     p->hlu = L;
     p->host.resize((size_t)c->n);
+    double st[3];
+    bem_hlu_stats(L, st);
+    p->entries = st[0];
+    p->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = p;
+    return BEM_OK;
+}
+
+int bem_precond_stats(bem_prec_t p, double* out4)
+{
+    if (!p || !out4) return BEM_ERR_ARG;
+    out4[0] = p->setup_ms;
+    out4[1] = (double)p->applies;
+    out4[2] = p->apply_ms;
+    out4[3] = p->entries;
     return BEM_OK;
 }
 
@@ -2548,6 +2568,7 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
     if (!c || !p || !v || !z || p->ctx != c || v == z) return set_err(c, BEM_ERR_ARG, "bem_precond_apply: bad arguments");
     HIPC(c, hipSetDevice(c->device));
     if (p->hlu) {   // host H-LU: the vector goes to the host and back
+        const auto t0 = std::chrono::steady_clock::now();
         const size_t bytes = sizeof(bem_c64) * (size_t)c->n;
         HIPC(c, hipMemcpyAsync(p->host.data(), v, bytes, hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
@@ -2555,8 +2576,11 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
         if (rc) return set_err(c, rc, std::string("bem_precond_apply: ") + bem_hlu_error());
         HIPC(c, hipMemcpyAsync(z, p->host.data(), bytes, hipMemcpyHostToDevice, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
+        p->applies++;
+        p->apply_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return BEM_OK;
     }
+    p->applies++;   // asynchronous: its time is part of the GMRES iterations
     hipLaunchKernelGGL(k_blu_apply, dim3((unsigned)p->nblk), dim3(64), 0, c->stream, p->d_blk, p->d_lu, p->d_piv,
                        (const bem_c64*)v, (bem_c64*)z);
     return launch_check(c, "k_blu_apply");

@@ -367,7 +367,13 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
               "bem_reconstruct_g");
         pressure[k] = p.download();
     }
-    if (pc) bem_precond_destroy(pc);
+    if (pc) {
+        double ps[4];
+        bem_precond_stats(pc, ps);
+        stats[13] = ps[0];
+        stats[14] = ps[2];
+        bem_precond_destroy(pc);
+    }
     double t4 = now_ms();
     dPhiHMat->clear();
     stats[0] = t1 - t0; stats[1] = t2 - t1; stats[2] = t3 - t2; stats[3] = t4 - t3; stats[4] = t4 - t0;
@@ -411,7 +417,13 @@ void BoundaryElementSolver::hMatrixSolve()
     bem_prec_t pc = nullptr;
     pc = makePreconditioner();
     int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est, pc);
-    if (pc) bem_precond_destroy(pc);
+    if (pc) {
+        double ps[4];
+        bem_precond_stats(pc, ps);
+        stats[13] = ps[0];
+        stats[14] = ps[2];
+        bem_precond_destroy(pc);
+    }
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t4 = now_ms();
     // true residual (:682-684)

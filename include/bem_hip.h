@@ -158,6 +158,9 @@ int bem_precond_apply(bem_ctx_t ctx, bem_prec_t p, const void* v_dev, void* z_de
    application moves the vector to the host and back. */
 int bem_precond_create_hlu(bem_ctx_t ctx, bem_hmat_t h, int64_t nclusters, const int64_t* clusters2, int64_t max_rank,
                            double rel_err, bem_prec_t* out);
+/* [0] set-up ms (host wall clock, incl. block download and factorisation),
+   [1] applications, [2] application ms in all, [3] stored complex entries */
+int bem_precond_stats(bem_prec_t p, double* out4);
 int bem_gmres_prec(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t p, const void* b_dev, void* x_dev, double tol,
                    int64_t restart, int64_t maxit, int64_t* iters, double* rel_resid);
 
@@ -217,8 +220,9 @@ int bem_solver_set_sources(bem_solver_t s, int64_t npoint, const double* points6
    Outputs (cluster order, complex n each): phi, dphi, pressure.
    stats[0..15]: t_phi_asm, t_rhs, t_dphi_asm, t_gmres, t_total (ms),
    gmres iterations, true relative residual, max rank phi, max rank dphi,
-   evaluated phi, evaluated dphi, near blocks, far blocks, refinement steps,
-   fp32 gmres iterations, bytes of the DPhi block storage. */
+   evaluated phi, evaluated dphi, near blocks, far blocks, preconditioner
+   set-up ms, preconditioner application ms (all applications of the solve),
+   bytes of the DPhi block storage. */
 int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coupling, double aca_tol,
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);

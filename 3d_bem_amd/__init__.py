@@ -53,6 +53,7 @@ _SIGS = {
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),
     "bem_solver_set_preconditioner": (_int, [_p, _int]),
     "bem_solver_set_hlu": (_int, [_p, _i64, _dbl]),
+    "bem_solver_precond_stats": (_int, [_p, _p]),
     "bem_precond_create_hlu": (_int, [_p, _p, _i64, _p, _i64, _dbl, ctypes.POINTER(_p)]),
     "bem_precond_stats": (_int, [_p, _p]),
     "bem_precond_create": (_int, [_p, _p, ctypes.POINTER(_p)]),
@@ -344,9 +345,9 @@ class Preconditioner:
         return zd.download()
 
     def stats(self):
-        o = np.zeros(4)
+        o = np.zeros(5)
         lib().bem_precond_stats(self.h, _ptr(o))
-        return {"setup_ms": o[0], "applies": int(o[1]), "apply_ms": o[2], "entries": o[3]}
+        return {"setup_ms": o[0], "applies": int(o[1]), "apply_ms": o[2], "entries": o[3], "factor_ms": o[4]}
 
     def close(self):
         if getattr(self, "h", None):
@@ -497,6 +498,13 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def precond_stats(self):
+        """the last solve's preconditioner: set-up ms, applications, their ms,
+        stored entries, factorisation ms (bem_solver_precond_stats)"""
+        o = np.zeros(5)
+        lib().bem_solver_precond_stats(self.h, _ptr(o))
+        return {"setup_ms": o[0], "applies": int(o[1]), "apply_ms": o[2], "entries": o[3], "factor_ms": o[4]}
 
     def set_impedance_plane(self, point=None, normal=None):
         """the scene's impedance plane (a rigid image) through point with

@@ -1287,7 +1287,7 @@ struct bem_prec {
     int32_t* d_piv = nullptr;
     bem_hlu_t hlu = nullptr;        // host H-LU (bem_precond_create_hlu)
     std::vector<bem_c64> host;      // its staging vector
-    double setup_ms = 0, apply_ms = 0, entries = 0;
+    double setup_ms = 0, apply_ms = 0, entries = 0, factor_ms = 0;
     int64_t applies = 0;
 };
 
@@ -2505,24 +2505,53 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t nb = (int64_t)c->blocks.size();
     std::vector<int64_t> blocks5(5 * nb), rank(nb);
-    std::vector<std::vector<bem_c64>> data(nb);
-    std::vector<const double*> ptr(nb);
+    // the four storage arrays in one copy each, then unpacked per block
+    int64_t nd = 0, nu = 0, nv = 0, na = 0;
     for (int64_t b = 0; b < nb; b++) {
         const auto& B = c->blocks[b];
         blocks5[5 * b] = B.row0; blocks5[5 * b + 1] = B.rows;
         blocks5[5 * b + 2] = B.col0; blocks5[5 * b + 3] = B.cols;
         blocks5[5 * b + 4] = B.adm ? 1 : 0;
-        int64_t k = -1;
-        int rc = bem_block_info(h, b, &k);
-        if (rc) return rc;
-        rank[b] = k;
-        const int64_t m = B.rows, n = B.cols;
-        data[b].resize(k < 0 ? (size_t)(m * n) : (size_t)(m * k + k + k * n));
-        bem_c64* d = data[b].data();
-        rc = k < 0 ? bem_block_download(h, b, (double*)d, nullptr, nullptr)
-                   : bem_block_download(h, b, (double*)d, (double*)(d + m * k), (double*)(d + m * k + k));
-        if (rc) return rc;
-        ptr[b] = (const double*)d;
+        rank[b] = B.adm ? (int64_t)h->rank[b] : -1;
+        if (!B.adm) {
+            nd = std::max(nd, c->dense_off[b] + B.rows * B.cols);
+        } else {
+            nu = std::max(nu, h->uoff[b] + B.rows * rank[b]);
+            nv = std::max(nv, h->voff[b] + rank[b] * B.cols);
+            na = std::max(na, h->aoff[b] + rank[b]);
+        }
+    }
+    std::vector<char> hd((size_t)nd * h->esz), hu((size_t)nu * h->esz), hv((size_t)nv * h->esz), ha((size_t)na * h->esz);
+    if (nd) HIPC(c, hipMemcpyAsync(hd.data(), h->dense, hd.size(), hipMemcpyDeviceToHost, c->stream));
+    if (nu) HIPC(c, hipMemcpyAsync(hu.data(), h->U, hu.size(), hipMemcpyDeviceToHost, c->stream));
+    if (nv) HIPC(c, hipMemcpyAsync(hv.data(), h->V, hv.size(), hipMemcpyDeviceToHost, c->stream));
+    if (na) HIPC(c, hipMemcpyAsync(ha.data(), h->A, ha.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const bool f32 = h->prec == BEM_PREC_FP32;
+    auto get = [&](const std::vector<char>& src, int64_t t) -> bem_c64 {
+        if (f32) {
+            const bem_c32 v = ((const bem_c32*)src.data())[t];
+            return c64(v.re, v.im);
+        }
+        return ((const bem_c64*)src.data())[t];
+    };
+    std::vector<std::vector<bem_c64>> data(nb);
+    std::vector<const double*> ptr(nb);
+    for (int64_t b = 0; b < nb; b++) {
+        const auto& B = c->blocks[b];
+        const int64_t m = B.rows, n = B.cols, k = rank[b];
+        if (k < 0) {
+            data[b].resize((size_t)(m * n));
+            for (int64_t t = 0; t < m * n; t++) data[b][t] = get(hd, c->dense_off[b] + t);
+        } else {   // U (m x k col-major), alpha, V (k x n col-major; stored row-major)
+            data[b].resize((size_t)(m * k + k + k * n));
+            bem_c64* d = data[b].data();
+            for (int64_t t = 0; t < m * k; t++) d[t] = get(hu, h->uoff[b] + t);
+            for (int64_t l = 0; l < k; l++) d[m * k + l] = get(ha, h->aoff[b] + l);
+            for (int64_t l = 0; l < k; l++)
+                for (int64_t j = 0; j < n; j++) d[m * k + k + j * k + l] = get(hv, h->voff[b] + l * n + j);
+        }
+        ptr[b] = (const double*)data[b].data();
     }
     bem_hlu_t L = nullptr;
     int rc = bem_hlu_create(c->n, nclusters, clusters2, nb, blocks5.data(), rank.data(), ptr.data(), max_rank, rel_err,
@@ -2535,18 +2564,20 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
     double st[3];
     bem_hlu_stats(L, st);
     p->entries = st[0];
+    p->factor_ms = 1e3 * st[2];
     p->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = p;
     return BEM_OK;
 }
 
-int bem_precond_stats(bem_prec_t p, double* out4)
+int bem_precond_stats(bem_prec_t p, double* out5)
 {
-    if (!p || !out4) return BEM_ERR_ARG;
-    out4[0] = p->setup_ms;
-    out4[1] = (double)p->applies;
-    out4[2] = p->apply_ms;
-    out4[3] = p->entries;
+    if (!p || !out5) return BEM_ERR_ARG;
+    out5[0] = p->setup_ms;
+    out5[1] = (double)p->applies;
+    out5[2] = p->apply_ms;
+    out5[3] = p->entries;
+    out5[4] = p->factor_ms;
     return BEM_OK;
 }
 

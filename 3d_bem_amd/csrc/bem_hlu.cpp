@@ -364,12 +364,18 @@ void hmul(const Block& B, cd alpha, const cd* X, int64_t ldx, int64_t ncol, cd* 
         gemm_nn(B.m, ncol, k, alpha, B.U.a.data(), B.U.m, T.a.data(), T.m, Y, ldy);
         break;
     }
-    case Block::Sub:
-        for (int q = 0; q < 4; q++) {
-            const Block& S = *B.s[q];
-            hmul(S, alpha, X + (S.c0 - B.c0), ldx, ncol, Y + (S.r0 - B.r0), ldy);
-        }
+    case Block::Sub: {
+        // sons 0, 1 write the first row half, 2, 3 the second: one task each
+        const bool par = big(B);
+        for (int h = 0; h < 2; h++)
+#pragma omp task if (par) default(shared) firstprivate(h)
+            for (int q = 2 * h; q < 2 * h + 2; q++) {
+                const Block& S = *B.s[q];
+                hmul(S, alpha, X + (S.c0 - B.c0), ldx, ncol, Y + (S.r0 - B.r0), ldy);
+            }
+#pragma omp taskwait
         break;
+    }
     }
 }
 
@@ -388,12 +394,18 @@ void hmulT(const Block& B, cd alpha, const cd* X, int64_t ldx, int64_t ncol, cd*
         gemm_nn(B.n, ncol, k, alpha, B.W.a.data(), B.W.m, T.a.data(), T.m, Y, ldy);
         break;
     }
-    case Block::Sub:
-        for (int q = 0; q < 4; q++) {
-            const Block& S = *B.s[q];
-            hmulT(S, alpha, X + (S.r0 - B.r0), ldx, ncol, Y + (S.c0 - B.c0), ldy);
-        }
+    case Block::Sub: {
+        // sons 0, 2 write the first column half, 1, 3 the second
+        const bool par = big(B);
+        for (int h = 0; h < 2; h++)
+#pragma omp task if (par) default(shared) firstprivate(h)
+            for (int q = h; q < 4; q += 2) {
+                const Block& S = *B.s[q];
+                hmulT(S, alpha, X + (S.r0 - B.r0), ldx, ncol, Y + (S.c0 - B.c0), ldy);
+            }
+#pragma omp taskwait
         break;
+    }
     }
 }
 
@@ -896,8 +908,12 @@ void HLU::solve(const cd* b, cd* x) const
 {
     if (!factored_) throw Error{"H-LU: not factored"};
     if (x != b) std::memcpy(x, b, sizeof(cd) * (size_t)n_);
-    solveL_dense(*root_, x, n_, 1);   // forwardSubstitution (harithm.cpp:1314-1340)
-    solveU_dense(*root_, x, n_, 1);   // backwardSubstitution (harithm.cpp:1386-1412)
+#pragma omp parallel
+#pragma omp single
+    {
+        solveL_dense(*root_, x, n_, 1);   // forwardSubstitution (harithm.cpp:1314-1340)
+        solveU_dense(*root_, x, n_, 1);   // backwardSubstitution (harithm.cpp:1386-1412)
+    }
 }
 
 Mat HLU::product() const

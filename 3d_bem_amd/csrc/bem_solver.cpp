@@ -312,6 +312,7 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     const auto& P = bemSolverParameters;
     const size_t nr = rhs.size();
     std::memset(stats, 0, sizeof stats);
+    std::memset(precondStats, 0, sizeof precondStats);
     double t0 = now_ms();
     // source term of every right-hand side
     std::vector<std::vector<cplx>> s(nr);
@@ -368,7 +369,7 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
         pressure[k] = p.download();
     }
     if (pc) {
-        double ps[4];
+        double* ps = precondStats;
         bem_precond_stats(pc, ps);
         stats[13] = ps[0];
         stats[14] = ps[2];
@@ -387,6 +388,7 @@ void BoundaryElementSolver::hMatrixSolve()
     auto& E = boundaryElements;
     const auto& P = bemSolverParameters;
     std::memset(stats, 0, sizeof stats);
+    std::memset(precondStats, 0, sizeof precondStats);
     double t0 = now_ms();
     calculateBoundaryConditionAndSourceTermVector();
     phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
@@ -418,7 +420,7 @@ void BoundaryElementSolver::hMatrixSolve()
     pc = makePreconditioner();
     int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est, pc);
     if (pc) {
-        double ps[4];
+        double* ps = precondStats;
         bem_precond_stats(pc, ps);
         stats[13] = ps[0];
         stats[14] = ps[2];
@@ -586,6 +588,13 @@ int bem_solver_set_preconditioner(bem_solver_t s, int kind)
 {
     if (!s || !s->s || kind < 0 || kind > 2) return BEM_ERR_ARG;
     s->preconditioner = kind;
+    return BEM_OK;
+}
+
+int bem_solver_precond_stats(bem_solver_t s, double* out5)
+{
+    if (!s || !s->s || !out5) return BEM_ERR_ARG;
+    std::memcpy(out5, s->s->precondStats, sizeof s->s->precondStats);
     return BEM_OK;
 }
 

@@ -164,6 +164,7 @@ public:
     std::vector<cplx> phiSolution, dPhiSolution, soundPressure;
     double frequency = 0, waveSpeed = 343.0, airDensity = 1.2041;
     double stats[16] = {0};
+    double precondStats[5] = {0};   // bem_precond_stats of the last solve's preconditioner
     bem_ctx_t context() const { return ctx_; }
     const std::vector<bemsetup::Block>& partition() const { return phiHMat->getMinPartition(); }
     std::shared_ptr<ClusterTree> clusterTree;

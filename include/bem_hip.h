@@ -159,8 +159,9 @@ int bem_precond_apply(bem_ctx_t ctx, bem_prec_t p, const void* v_dev, void* z_de
 int bem_precond_create_hlu(bem_ctx_t ctx, bem_hmat_t h, int64_t nclusters, const int64_t* clusters2, int64_t max_rank,
                            double rel_err, bem_prec_t* out);
 /* [0] set-up ms (host wall clock, incl. block download and factorisation),
-   [1] applications, [2] application ms in all, [3] stored complex entries */
-int bem_precond_stats(bem_prec_t p, double* out4);
+   [1] applications, [2] application ms in all, [3] stored complex entries,
+   [4] factorisation ms (host H-LU; 0 for the leaf-block LU) */
+int bem_precond_stats(bem_prec_t p, double* out5);
 int bem_gmres_prec(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t p, const void* b_dev, void* x_dev, double tol,
                    int64_t restart, int64_t maxit, int64_t* iters, double* rel_resid);
 
@@ -234,6 +235,9 @@ int bem_solver_set_preconditioner(bem_solver_t s, int kind);
 /* the H-LU preconditioner with preconditionerRank / preconditionerRelError
    (bemparameters.h:27-28; defaults 2 / 0) */
 int bem_solver_set_hlu(bem_solver_t s, int64_t rank, double rel_err);
+/* bem_precond_stats of the last bem_solver_solve's preconditioner (zeros
+   when it ran without one) */
+int bem_solver_precond_stats(bem_solver_t s, double* out5);
 
 /* The scene's impedance plane (BoundaryElements::impedancePlanes,
    boundaryelements.h:97, filled from an infinite-baffle section,

@@ -1237,8 +1237,8 @@ struct bem_ctx {
     int64_t gm_cap = 0;
     // profiling
     bool prof = false;
-    double prof_ms[6] = {0};
-    int64_t prof_cnt[6] = {0};
+    double prof_ms[7] = {0};
+    int64_t prof_cnt[7] = {0};
     struct Pending { int slot; hipEvent_t e0, e1; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
@@ -2809,8 +2809,11 @@ int bem_field(bem_ctx_t c, int64_t nobs, const double* obs3, const void* phi, co
     if ((rc = dupload(c, &d_obs, soa.data(), soa.size(), &own))) { free_list(c, own); return rc; }
     if ((rc = dalloc(c, &d_part, (size_t)nobs * nchunk, &own))) { free_list(c, own); return rc; }
     if ((rc = dalloc(c, &d_out, (size_t)nobs, &own))) { free_list(c, own); return rc; }
-    hipLaunchKernelGGL(k_field, dim3(nblk(nobs, kTile), nchunk), dim3(kWG), 0, c->stream, c->mesh, c->k, nobs, d_obs,
-                       (const bem_c64*)phi, (const bem_c64*)dphi, nchunk, d_part);
+    {
+        Timer t(c, 6);
+        hipLaunchKernelGGL(k_field, dim3(nblk(nobs, kTile), nchunk), dim3(kWG), 0, c->stream, c->mesh, c->k, nobs,
+                           d_obs, (const bem_c64*)phi, (const bem_c64*)dphi, nchunk, d_part);
+    }
     hipLaunchKernelGGL(k_field_reduce, dim3(nblk(nobs, 256)), dim3(256), 0, c->stream, nobs, nchunk, d_part, d_out);
     if ((rc = launch_check(c, "k_field"))) { free_list(c, own); return rc; }
     hipError_t e = hipMemcpyAsync(out2, d_out, (size_t)nobs * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream);
@@ -2898,13 +2901,13 @@ int bem_profile(bem_ctx_t c, int enable, double* out)
     if (!c) return BEM_ERR_ARG;
     resolve_pending(c);
     if (out)
-        for (int q = 0; q < 6; q++) {
+        for (int q = 0; q < 7; q++) {
             out[q] = c->prof_cnt[q] ? c->prof_ms[q] / (double)c->prof_cnt[q] : 0.0;
-            out[6 + q] = (double)c->prof_cnt[q];
+            out[7 + q] = (double)c->prof_cnt[q];
         }
     if (enable >= 0) {
         c->prof = enable != 0;
-        for (int q = 0; q < 6; q++) { c->prof_ms[q] = 0; c->prof_cnt[q] = 0; }
+        for (int q = 0; q < 7; q++) { c->prof_ms[q] = 0; c->prof_cnt[q] = 0; }
     }
     return BEM_OK;
 }

@@ -43,7 +43,11 @@ PEAK_HBM_GBS = 8000.0
 # algorithmic flop count of one regular entry (16 points x 95 + 40; sqrt,
 # sin, cos and div count as one flop each)
 F_REG = 16 * 95 + 40
-FAMILIES = ["dense_near_field", "singular_table", "aca", "mvm_dense", "mvm_lowrank", "mvm_gather"]
+FAMILIES = ["dense_near_field", "singular_table", "aca", "mvm_dense", "mvm_lowrank", "mvm_gather", "field"]
+# algorithmic flop count of one (observation point, element) pair of the
+# field evaluation: 7 points x (15 to form the point + 38 kernel) + 4 (area)
+# + 14 (M phi - L dphi)
+F_FIELD = 7 * (15 + 38) + 18
 
 
 def workload(level):
@@ -77,6 +81,74 @@ def run_cpu_baseline(tri, bc, args):
     return pr, t, threads, st
 
 
+def run_field(args):
+    """--field NOBS: the field evaluation (bem_solver_field: the direct sum
+    of calculateFieldSolutionRegular on the device) of the solved workload at
+    NOBS seeded exterior points, --steps timed; k_field against the FP64
+    vector peak; the CPU restatement's field (oracle/, all host threads) on a
+    bounded sample of the same points with the same boundary solution."""
+    bem, tri, bc = workload(args.level)
+    n = len(tri)
+    S = bem.Solver(tri, bc, leaf=args.leaf, eta=args.eta)
+    if plane_of(args) is not None:
+        S.set_impedance_plane(*plane_of(args))
+    prec = bem.FP32 if args.precision == "fp32" else bem.FP64
+    phi, dphi, _, _ = S.solve(args.freq, c=343.0, rho=1.21, coupling=1, aca_tol=args.aca_tol,
+                              gmres_tol=args.gmres_tol, seed=args.seed, precision=prec)
+    rng = np.random.default_rng(args.seed)
+    d = rng.standard_normal((args.field, 3))
+    obs = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0.6, 3.0, (args.field, 1))
+    ctx_h = bem.lib().bem_solver_context(S.h)
+    for _ in range(args.warmup):
+        S.field(obs)
+    prof = np.zeros(14)
+    bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(ctypes.c_void_p))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f = S.field(obs)
+    elapsed = time.perf_counter() - t0
+    bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(ctypes.c_void_p))
+    avg_ms, cnt = float(prof[6]), int(prof[13])
+    flops = float(F_FIELD) * args.field * n
+    ach = flops / (avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": "field points evaluated/s at 20k tris (direct sum, FP64)",
+        "value": round(args.field * args.steps / elapsed, 1),
+        "unit": "field-points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"field of the pulsating sphere, icosphere level {args.level} ({n} triangles), "
+                               f"{args.freq:g} Hz boundary solution ({args.precision}), {args.field} points at "
+                               f"0.6-3 m (seeded)" + ("" if args.plane is None else f", rigid plane {args.plane:g} m "
+                                                      "below the centre"),
+                   "triangles": n, "points": args.field},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                     "frac": round(ach / PEAK_TFLOPS["fp64"], 4), "traffic": None, "kernel": "field",
+                     "avg_ms": round(avg_ms, 4), "launches": cnt,
+                     "algorithmic_per_launch": f"{flops:.4g} flop ({args.field} x {n} pairs x {F_FIELD})"},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        P = oracle.Problem(tri, bc, leaf=args.leaf, eta=args.eta)
+        P.set_frequency(args.freq, 343.0, 1)
+        if plane_of(args) is not None:
+            P.set_plane(*plane_of(args))
+        m = min(args.field, 2000)
+        t0 = time.perf_counter()
+        fc = P.field(obs[:m], phi, dphi)
+        t_cpu = time.perf_counter() - t0
+        fc = fc * (1j * 1.21 * 2 * np.pi * args.freq)
+        out["cpu_baseline"] = {"value": round(m / t_cpu, 1), "unit": "field-points/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the first {m} of the points with the same boundary solution, CPU "
+                                         f"restatement: {t_cpu:.2f} s"}
+        out["rel_l2_vs_cpu"] = float(np.linalg.norm(f[:m] - fc) / np.linalg.norm(fc))
+    print(json.dumps(out))
+
+
 def batch_freqs(center, batch):
     """batch points over [0.96, 1.04] x center with center itself included"""
     if batch == 1:
@@ -95,14 +167,14 @@ def timed_run(bem, S, ctx_h, args, prec, freqs, lanes):
 
     for _ in range(args.warmup):
         step()
-    prof = np.zeros(12)
+    prof = np.zeros(14)
     bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(ctypes.c_void_p))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pr, its, stats = step()
     elapsed = time.perf_counter() - t0
     bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(ctypes.c_void_p))
-    return pr, its, stats, elapsed, prof[:6].copy(), prof[6:].copy()
+    return pr, its, stats, elapsed, prof[:7].copy(), prof[7:].copy()
 
 
 def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload, quads=1):
@@ -203,6 +275,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--plane", type=float, default=None, metavar="D",
                     help="rigid impedance plane D m below the sphere's centre (default: none)")
+    ap.add_argument("--field", type=int, default=0, metavar="NOBS",
+                    help="time the field evaluation at NOBS points instead of the sweep")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -216,6 +290,9 @@ def main():
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
         sys.exit(2)
+    if args.field:
+        run_field(args)
+        return
 
     bem, tri, bc = workload(args.level)
     n = len(tri)

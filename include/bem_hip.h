@@ -200,8 +200,9 @@ int bem_memcpy_dtoh(bem_ctx_t ctx, void* dst_host, const void* src_dev, size_t b
 int bem_synchronize(bem_ctx_t ctx);
 /* average device time (ms) of the last `bem_profile` window per kernel
    family: [0] dense assembly [1] singular/near-singular fix-up [2] ACA
-   [3] MVM dense [4] MVM low-rank [5] MVM gather; counts in [6..11] */
-int bem_profile(bem_ctx_t ctx, int enable, double* out12);
+   [3] MVM dense [4] MVM low-rank [5] MVM gather [6] field evaluation;
+   counts in [7..13] */
+int bem_profile(bem_ctx_t ctx, int enable, double* out14);
 
 /* BoundaryElementSolver::hMatrixSolve (boundaryelementsolver.cpp:526-744) as
    one call: cluster tree + partition on the host, everything else on the

@@ -8,7 +8,7 @@ import numpy as np
 G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz")
 
 
-def test_golden_config1(O, meshes):
+def test_golden_sphere1280(O, meshes):
     g = np.load(G, allow_pickle=False)
     tri = meshes.icosphere(3, 0.5)
     assert np.array_equal(tri, g["tri"])

@@ -12,7 +12,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def test_config4_capsule_impedance(bem, meshes):
+def test_capsule76k_impedance(bem, meshes):
     """Capsule L = 2 m, r = 0.25 m, edge 1.07 cm (76440 triangles), 1500 Hz,
     Z = rho c (1.5 + 0.5i) on a seeded 30 % of the elements, rigid elsewhere,
     leaf 64, eta 1.5, ACA 1e-4; FP32 with the FP64-residual restarts to 1e-8.
@@ -32,7 +32,7 @@ def test_config4_capsule_impedance(bem, meshes):
     perm, _ = S.layout()
     phi, dphi, p32, st32 = S.solve(f, c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP32)
     _, _, p64, st64 = S.solve(f, c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP64)
-    print("config-4 n", n, "FP32 vs FP64", rel(p32, p64), "iters", st32["gmres_iters"], st64["gmres_iters"],
+    print("capsule n", n, "FP32 vs FP64", rel(p32, p64), "iters", st32["gmres_iters"], st64["gmres_iters"],
           "residual", st32["residual"], st64["residual"])
     assert st32["residual"] < 1e-7 and st64["residual"] < 1e-7
     assert rel(p32, p64) < 1e-4
@@ -44,7 +44,7 @@ def test_config4_capsule_impedance(bem, meshes):
     assert np.all(np.abs(dphi[~imp]) < 1e-12 * np.abs(dphi).max())
 
 
-def test_config5_ellipsoid_batch(bem, O, meshes):
+def test_sixteen_ellipsoids(bem, O, meshes):
     """16 ellipsoids, semi-axes uniform in [0.2, 1.0] m (seeded), 5120 or
     20480 triangles (250880 in all), each at a frequency uniform in
     [200, 2000] Hz with k max(axes) <= 20, normal velocity N(0, 1) per

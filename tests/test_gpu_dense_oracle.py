@@ -34,7 +34,7 @@ def dense_fp64_solution(bem, P, rho, freq, gmres_tol=1e-12):
     return pr
 
 
-def test_config2_fp64_and_fp32_against_dense_fp64(bem, O, meshes):
+def test_sphere20480_fp64_and_fp32_against_dense_fp64(bem, O, meshes):
     freq, rho = 1000.0, 1.21
     P = sphere_problem(O, meshes, level=5, leaf=64, eta=2.0, freq=freq)
     print("dense FP64 reference, n =", P.n, flush=True)

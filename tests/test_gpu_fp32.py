@@ -70,8 +70,8 @@ def test_fp32_mvm_is_exact_product_of_stored_operator(bem, L3, ctx3):
     assert np.linalg.norm(y - full @ x) / np.linalg.norm(full @ x) < 1e-13
 
 
-def test_fp32_solve_config1(bem, O, meshes):
-    """Pulsating sphere L3 (config of the small reference case) in FP32:
+def test_fp32_solve_sphere1280(bem, O, meshes):
+    """Pulsating sphere L3 (1280 triangles, 500 Hz) in FP32:
     surface pressure within 1e-6 relative L2 of the FP64 CPU solve, GMRES
     residual of the FP32 operator < 1e-9."""
     tri = meshes.icosphere(3, 0.5)
@@ -84,7 +84,7 @@ def test_fp32_solve_config1(bem, O, meshes):
     P.set_frequency(500.0, 343.0, 1)
     _, _, pro, _ = P.solve(1e-6, gmres_tol=1e-10, seed=1, rho=1.21, freq=500.0)
     rel = np.linalg.norm(pr - pro) / np.linalg.norm(pro)
-    print("FP32 config-1 pressure rel-L2 vs FP64 CPU", rel, "iters", st["gmres_iters"])
+    print("FP32 sphere-1280 pressure rel-L2 vs FP64 CPU", rel, "iters", st["gmres_iters"])
     assert rel < 1e-6
     assert st["residual"] < 1e-9
 

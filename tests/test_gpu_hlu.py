@@ -1,6 +1,6 @@
 """The H-LU preconditioner fed with GPU-assembled blocks
-(bem_precond_create_hlu, solver preconditioner 2): config 1 as specified
-(H-LU preconditioned GMRES to 1e-10) against the CPU restatement and the
+(bem_precond_create_hlu, solver preconditioner 2): the 1280-triangle
+pulsating sphere with H-LU preconditioned GMRES to 1e-10 against the CPU restatement and the
 monopole, its application against the host H-LU built from the
 restatement's own blocks, and the FP32 operator preconditioned the same
 way."""
@@ -12,7 +12,7 @@ from helpers import sphere_problem, upload_problem
 pytestmark = pytest.mark.gpu
 
 
-def test_config1_hlu_preconditioned(bem, O, meshes):
+def test_sphere1280_hlu_preconditioned(bem, O, meshes):
     """Pulsating sphere L3 (1280), 500 Hz, Burton-Miller, FP64, leaf 32,
     eta 2, ACA 1e-6, H-LU (rank 2, the reference's default) preconditioned
     GMRES to 1e-10"""
@@ -27,7 +27,7 @@ def test_config1_hlu_preconditioned(bem, O, meshes):
     P.set_frequency(500.0, 343.0, 1)
     _, _, pro, _ = P.solve(1e-6, gmres_tol=1e-10, seed=1, rho=1.21, freq=500.0)
     rel = np.linalg.norm(p1 - pro) / np.linalg.norm(pro)
-    print("config-1 H-LU: iterations", st1["gmres_iters"], "(plain", st0["gmres_iters"], ") vs restatement", rel,
+    print("sphere-1280 H-LU: iterations", st1["gmres_iters"], "(plain", st0["gmres_iters"], ") vs restatement", rel,
           "residual", st1["residual"])
     assert rel < 1e-9 and st1["residual"] < 1e-9
     assert st1["gmres_iters"] < st0["gmres_iters"]

@@ -79,7 +79,7 @@ def test_gmres_matches_oracle(L3, ctx3):
     assert np.linalg.norm(xg - xr) / np.linalg.norm(xr) < 1e-10
 
 
-def test_solver_config1_parity(bem, O, meshes):
+def test_solver_sphere1280_parity(bem, O, meshes):
     """Pulsating sphere L3, 500 Hz, leaf 32, eta 2, ACA 1e-6, GMRES 1e-10."""
     tri = meshes.icosphere(3, 0.5)
     n = len(tri)
@@ -92,7 +92,7 @@ def test_solver_config1_parity(bem, O, meshes):
     perm, blocks = S.layout()
     assert np.array_equal(perm, P.perm) and np.array_equal(blocks, P.blocks)
     rel = np.linalg.norm(pr - pro) / np.linalg.norm(pro)
-    print("config-1 pressure rel-L2 GPU vs CPU restatement", rel, "iters", st["gmres_iters"], sto["gmres_iters"])
+    print("sphere-1280 pressure rel-L2 GPU vs CPU restatement", rel, "iters", st["gmres_iters"], sto["gmres_iters"])
     assert rel < 1e-10
     k = 2 * np.pi * 500 / 343
     pex = 1j * 2 * np.pi * 500 * 1.21 * 0.5 / (1j * k * 0.5 - 1)

@@ -44,7 +44,7 @@ def test_sweep_equals_single_solves(bem, O, meshes):
             assert rel < 1e-10, (f, k, rel)
 
 
-def test_config3_sweep_fp32(bem, meshes):
+def test_sphere5120_sweep64_fp32(bem, meshes):
     """Icosphere L4 (5120 triangles), 64 log-spaced frequencies 100-2000 Hz,
     pulsating and plane-wave scattering right-hand sides (seeded direction),
     FP32 assembly, leaf 64, eta 2, ACA 1e-5, GMRES 1e-8: pulsating pressure
@@ -72,7 +72,7 @@ def test_config3_sweep_fp32(bem, meshes):
         phi = pr[fi, 1] / (1j * 2 * np.pi * f * rho)
         mie = mie_rigid_surface_phi(k, a, cos_t)
         worst_s = max(worst_s, np.linalg.norm(phi - mie) / np.linalg.norm(mie))
-    print("config-3 worst pulsating error", worst_p, "worst scattering error", worst_s)
+    print("64-frequency sweep: worst pulsating error", worst_p, "worst scattering error", worst_s)
     assert worst_p < 0.02 and worst_s < 0.05
     sel = [0, 21, 42, 63]
     pr64, _, _ = S.sweep(freqs[sel], g=g, planes=planes, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=1,

@@ -241,7 +241,8 @@ class Context:
         self._check(lib().bem_synchronize(self.h))
 
     def profile(self, enable=-1):
-        out = np.zeros(12)
+        """bem_profile: average ms per kernel family [0..6], launches [7..13]"""
+        out = np.zeros(14)
         self._check(lib().bem_profile(self.h, int(enable), _ptr(out)))
         return out
 

@@ -29,6 +29,7 @@ _SIGS = {
     "bem_upload_mesh": (_int, [_p, _i64, _p, _p, _p, _p, _dbl]),
     "bem_set_bc": (_int, [_p, _p, _p, _p]),
     "bem_set_plane": (_int, [_p, _p, _p]),
+    "bem_set_near_pairing": (_int, [_p, _int]),
     "bem_hlu_create": (_int, [_i64, _i64, _p, _i64, _p, _p, _p, _i64, _dbl, ctypes.POINTER(_p)]),
     "bem_hlu_solve": (_int, [_p, _p, _p]),
     "bem_hlu_dense": (_int, [_p, _p]),
@@ -182,6 +183,11 @@ class Context:
     def set_bc(self, alpha, beta, f):
         a, b, g = (_c(v, np.complex128) for v in (alpha, beta, f))
         self._check(lib().bem_set_bc(self.h, _ptr(a), _ptr(b), _ptr(g)))
+
+    def set_near_pairing(self, on=True):
+        """a Phi assembly also forms the DPhi near field for the next DPhi
+        assembly (bem_set_near_pairing)"""
+        self._check(lib().bem_set_near_pairing(self.h, int(bool(on))))
 
     def set_plane(self, point=None, normal=None):
         """one impedance plane (rigid image) through point with normal; None removes it"""

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile*
 // operator `kind`.  Dense blocks scatter from the table (k_scatter), the ACA
 // reads it for the rare singular entries of admissible blocks.
 __global__ __launch_bounds__(kWG) void k_singular(DevMesh M, DevFreq F, int kind, const int32_t* __restrict__ order,
-                                                  int64_t nitems, int64_t nnz, bem_c64* sing)
+                                                  int64_t nitems, int64_t nnz, bem_c64* sing, bem_c64* sing_dphi)
 {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nitems) return;
@@ -155,15 +155,25 @@ __global__ __launch_bounds__(kWG) void k_singular(DevMesh M, DevFreq F, int kind
         o = near_ops(F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j], M.nz[j],
                      M.nr_geo + M.nr_off[t], M.nr_nsub[t]);
     }
+    // sing_dphi (optional): the DPhi entry of the same pair from the same
+    // operators (the Phi assembly computing both near fields)
     bem_c64 e = bc_entry(F, kind, o, i, j);
-    sing[t] = M.rvtx ? c64_add(e, refl_entry(M, F, kind, i, j)) : e;
+    bem_c64 e2 = sing_dphi ? bc_entry(F, BEM_KIND_DPHI, o, i, j) : e;
+    if (M.rvtx) {
+        bem_c64 X, Y;
+        refl_xy(M, F, i, j, &X, &Y);
+        e = c64_add(e, kind == 0 ? bem_phi_entry(X, Y, F.subst[j], F.cphi[j]) : bem_dphi_entry(X, Y, F.subst[j], F.cd[j]));
+        if (sing_dphi) e2 = c64_add(e2, bem_dphi_entry(X, Y, F.subst[j], F.cd[j]));
+    }
+    sing[t] = e;
+    if (sing_dphi) sing_dphi[t] = e2;
 }
 
 // FP32 dense near-field tiles: as k_dense with the regular quadrature of
 // regular_ops_f32, written as complex FP32
 template <bool PLANE>
 __global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const int32_t* __restrict__ subst, int kind,
-                                                 const Tile* __restrict__ tiles, bem_c32* out)
+                                                 const Tile* __restrict__ tiles, bem_c32* out, bem_c32* out_dphi)
 {
     __shared__ double sq[3][16][kTile];
     __shared__ double sp[3][kTile];
@@ -200,11 +210,18 @@ __global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const i
         bem_c32f4 o = regular_ops_f32(F, px, py, pz, npx, npy, npz, sn[0][c], sn[1][c], sn[2][c], sa[c],
                                       &sq[0][0][c], &sq[1][0][c], &sq[2][0][c], kTile);
         bem_c32 e = bc_entry_f32(F, subst, kind, o, j);
+        bem_c32 e2 = out_dphi ? bc_entry_f32(F, subst, BEM_KIND_DPHI, o, j) : e;   // same operators
         if (PLANE) {   // impedance plane: + the mirror image's entry
-            bem_c32 m = bc_entry_f32(F, subst, kind, refl_ops_f32(F, M, i, j), j);
+            const bem_c32f4 ro = refl_ops_f32(F, M, i, j);
+            bem_c32 m = bc_entry_f32(F, subst, kind, ro, j);
             e = c32(e.re + m.re, e.im + m.im);
+            if (out_dphi) {
+                m = bc_entry_f32(F, subst, BEM_KIND_DPHI, ro, j);
+                e2 = c32(e2.re + m.re, e2.im + m.im);
+            }
         }
         out[T.dest + r + (int64_t)c * T.ld] = e;
+        if (out_dphi) out_dphi[T.dest + r + (int64_t)c * T.ld] = e2;
     }
 }
 
@@ -1223,6 +1240,16 @@ struct bem_ctx {
     std::vector<void*> allocs_piv;
     int32_t* d_sing_order = nullptr;     // evaluation order of the singular items
     bem_c64* d_sing = nullptr;           // per near pair, then per element (self)
+    // near-field pairing (bem_set_near_pairing): the Phi assembly also forms
+    // the DPhi near field (dense blocks and singular table) from the same
+    // operators and keeps it for the next DPhi assembly of the same state
+    bool pair_near = false;
+    uint64_t gen = 0;                    // bumped by every change of mesh, partition, BC, frequency, plane
+    bem_c64* d_sing2 = nullptr;          // the kept DPhi singular table
+    void* st_dense = nullptr;            // the kept DPhi dense blocks
+    int st_prec = -1;
+    uint64_t st_gen = 0;
+    const bem_c64* sing_cur = nullptr;   // table the running assembly's ACA reads
     int64_t nnz = 0;
     std::vector<int32_t> adm_list;           // admissible block ids, large first
     int32_t* d_adm_list = nullptr;
@@ -1507,6 +1534,16 @@ DevFreq32 dev_freq32(const bem_ctx* c)
     return F;
 }
 
+// a change of the operators: the kept DPhi near field no longer applies
+void bump(bem_ctx* c)
+{
+    c->gen++;
+    if (c->st_dense) {
+        pfree(c, c->st_dense);
+        c->st_dense = nullptr;
+    }
+}
+
 // vertices a, b, c as 9 SoA rows, and the FP32 path's edge form c, a - c, b - c
 void vertex_soa(int64_t n, const double* nodes9, std::vector<double>& vtx, std::vector<double>& edg)
 {
@@ -1595,6 +1632,7 @@ int bem_destroy(bem_ctx_t c)
     if (!c) return BEM_ERR_ARG;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    bump(c);
     free_list(c, c->allocs_mesh);
     free_list(c, c->allocs_plane);
     free_list(c, c->allocs_part);
@@ -1629,6 +1667,7 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
     free_list(c, c->allocs_mesh);
+    bump(c);
     c->have_part = false;   // partition, boundary conditions and frequency belong to the old mesh
     c->have_bc = false;
     c->have_freq = false;
@@ -1701,12 +1740,21 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
     if ((rc = dupload(c, &d_prow, prow.data(), prow.size(), &c->allocs_mesh))) return rc;
     if ((rc = dupload(c, &c->d_sing_order, order.data(), order.size(), &c->allocs_mesh))) return rc;
     if ((rc = dalloc(c, &c->d_sing, (size_t)(nnz + n), &c->allocs_mesh))) return rc;
+    if ((rc = dalloc(c, &c->d_sing2, (size_t)(nnz + n), &c->allocs_mesh))) return rc;
     c->nnz = nnz;
     M.nr_ptr = d_ptr; M.nr_row = d_prow; M.nr_col = d_col; M.nr_nsub = d_nsub; M.nr_off = d_off; M.nr_geo = d_geo;
     M.self_geo = d_self;
     M.near_lim = 4.0 * hbar;
     if ((rc = build_plane(c))) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
+    return BEM_OK;
+}
+
+int bem_set_near_pairing(bem_ctx_t c, int on)
+{
+    if (!c) return BEM_ERR_ARG;
+    c->pair_near = on != 0;
+    if (!c->pair_near) bump(c);
     return BEM_OK;
 }
 
@@ -1720,6 +1768,7 @@ int bem_set_plane(bem_ctx_t c, const double* point3, const double* normal3)
         return set_err(c, BEM_ERR_ARG, "bem_set_plane: non-finite plane");
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
+    bump(c);
     c->have_plane = on;
     for (int d = 0; d < 3; d++) {
         c->plane_pt[d] = on ? point3[d] : 0.0;
@@ -1748,6 +1797,7 @@ int bem_set_bc(bem_ctx_t c, const double* a, const double* b, const double* f)
     if ((rc = dupload(c, &c->d_beta, c->h_beta.data(), n, nullptr))) return rc;
     if ((rc = dupload(c, &c->d_f, c->h_f.data(), n, nullptr))) return rc;
     c->have_bc = true;
+    bump(c);
     c->have_freq = false;
     HIPC(c, hipStreamSynchronize(c->stream));
     return BEM_OK;
@@ -1803,6 +1853,7 @@ int bem_set_frequency(bem_ctx_t c, double k, double cc_re, double cc_im)
     HIPC(c, hipMemcpyAsync(c->d_cd, cd.data(), n * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     c->have_freq = true;
+    bump(c);
     return BEM_OK;
 }
 
@@ -1907,6 +1958,7 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
     if ((rc = dupload(c, &c->d_gidx, gidx.data(), gidx.size(), ap))) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     c->have_part = true;
+    bump(c);
     return BEM_OK;
 }
 
@@ -2035,7 +2087,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     a.rank_out = S.d_rank; a.flag_out = S.d_flag; a.evals_out = S.d_ev;
     a.tol = tol; a.maxrank = maxrank; a.seed = seed;
     a.piv_off = d_pivoff; a.piv = d_piv;
-    a.sing = c->d_sing; a.nnz = c->nnz;
+    a.sing = c->sing_cur ? c->sing_cur : c->d_sing; a.nnz = c->nnz;
     // dynamic LDS of aca_block: sw[cap] of the storage type, then the FP64
     // dot products sdots[2 cap + 2]
     size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
@@ -2098,34 +2150,56 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->esz = f32 ? sizeof(bem_c32) : sizeof(bem_c64);
     int64_t nb = (int64_t)c->blocks.size();
     int rc;
-    auto fail = [&](int code) { bem_hmat_destroy(h); return code; };
+    // near-field pairing: a DPhi assembly takes the near field the Phi
+    // assembly of the same state formed; a Phi assembly forms both
+    const bool adopt = kind == BEM_KIND_DPHI && c->st_dense && c->st_gen == c->gen && c->st_prec == precision;
+    const bool pairing = c->pair_near && kind == BEM_KIND_PHI;
+    void* dense2 = nullptr;
+    auto fail = [&](int code) {
+        c->sing_cur = nullptr;
+        if (dense2) pfree(c, dense2);
+        bem_hmat_destroy(h);
+        return code;
+    };
     // dense near field
-    if ((rc = dalloc_bytes(c, &h->dense, (size_t)c->dense_total * h->esz))) return fail(rc);
+    if (adopt) {
+        h->dense = c->st_dense;
+        c->st_dense = nullptr;
+    } else {
+        if (c->st_dense) {   // kept for another state or precision
+            pfree(c, c->st_dense);
+            c->st_dense = nullptr;
+        }
+        if ((rc = dalloc_bytes(c, &h->dense, (size_t)c->dense_total * h->esz))) return fail(rc);
+    }
+    if (pairing && (rc = dalloc_bytes(c, &dense2, (size_t)c->dense_total * h->esz))) return fail(rc);
+    c->sing_cur = adopt ? c->d_sing2 : c->d_sing;
     bem_c64* outPhi = kind == BEM_KIND_PHI ? (bem_c64*)h->dense : nullptr;
-    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? (bem_c64*)h->dense : nullptr;
+    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? (bem_c64*)h->dense : (bem_c64*)dense2;
     DevFreq F = dev_freq(c);
     DevFreq32 F32 = dev_freq32(c);
     hipEvent_t ev0, ev1, ev2;
     HIPC(c, hipEventCreate(&ev0)); HIPC(c, hipEventCreate(&ev1)); HIPC(c, hipEventCreate(&ev2));
     HIPC(c, hipEventRecord(ev0, c->stream));
-    {
+    if (!adopt) {
         Timer t(c, 1);
         int64_t nitems = c->nnz + c->n;
         hipLaunchKernelGGL(k_singular, dim3(nblk(nitems, kWG)), dim3(kWG), 0, c->stream, c->mesh, F, kind,
-                           c->d_sing_order, nitems, c->nnz, c->d_sing);
+                           c->d_sing_order, nitems, c->nnz, c->d_sing, pairing ? c->d_sing2 : nullptr);
         if ((rc = launch_check(c, "k_singular"))) return fail(rc);
     }
     // dense near-field tiles + scatter of their singular entries: enqueued on
     // the main stream while the large ACA classes run on the side streams
     auto dense_work = [&]() -> int {
         int rc2;
+        if (adopt) return BEM_OK;
         if (!c->tiles.empty()) {
             Timer t(c, 0);
             const dim3 g((unsigned)c->tiles.size());
             const bool plane = c->mesh.rvtx != nullptr;
             if (f32)
                 hipLaunchKernelGGL(plane ? k_dense32<true> : k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh,
-                                   F32, c->d_subst, kind, c->d_tiles, (bem_c32*)h->dense);
+                                   F32, c->d_subst, kind, c->d_tiles, (bem_c32*)h->dense, (bem_c32*)dense2);
             else
                 hipLaunchKernelGGL(plane ? k_dense<true> : k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh, F,
                                    c->d_tiles, outPhi, outDPhi);
@@ -2133,13 +2207,17 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         }
         if (!c->fixes.empty()) {
             dim3 g(nblk((int64_t)c->fixes.size(), kWG));
-            if (f32)
-                hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
-                                   (int64_t)c->fixes.size(), c->nnz, c->d_sing, (bem_c32*)h->dense);
-            else
-                hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
-                                   (int64_t)c->fixes.size(), c->nnz, c->d_sing, (bem_c64*)h->dense);
-            if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
+            for (int q = 0; q < (dense2 ? 2 : 1); q++) {
+                const bem_c64* tab = q ? c->d_sing2 : c->d_sing;
+                void* dst = q ? dense2 : h->dense;
+                if (f32)
+                    hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                       (int64_t)c->fixes.size(), c->nnz, tab, (bem_c32*)dst);
+                else
+                    hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                       (int64_t)c->fixes.size(), c->nnz, tab, (bem_c64*)dst);
+                if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
+            }
         }
         return BEM_OK;
     };
@@ -2187,8 +2265,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     for (int32_t b : c->adm_list) {
         if (flag[b] == 2) {
             free_list(c, tmp_owned);
-            bem_hmat_destroy(h);
-            return set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite value in ACA block " + std::to_string(b));
+            return fail(set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite value in ACA block " + std::to_string(b)));
         }
         (flag[b] == 1 ? over : keep).push_back(b);
     }
@@ -2203,10 +2280,10 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
             evs[b] = evs2[b];
             if (flag2[b]) {
                 free_list(c, tmp_owned);
-                bem_hmat_destroy(h);
-                return set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
-                               "bem_assemble: ACA block " + std::to_string(b) +
-                                   (flag2[b] == 2 ? " non-finite" : " needs rank > 1024; lower the tolerance or set max_rank"));
+                return fail(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
+                                    "bem_assemble: ACA block " + std::to_string(b) +
+                                        (flag2[b] == 2 ? " non-finite"
+                                                       : " needs rank > 1024; lower the tolerance or set max_rank")));
             }
         }
         c->vp.give(cap2);
@@ -2310,10 +2387,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     (void)hipEventElapsedTime(&ms_dense, ev0, ev1);
     (void)hipEventElapsedTime(&ms_aca, ev1, ev2);
     (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1); (void)hipEventDestroy(ev2);
-    if (hflag) {
-        bem_hmat_destroy(h);
-        return set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite entry in the near field");
-    }
+    if (hflag) return fail(set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite entry in the near field"));
     size_t nadm = c->adm_list.size();
     h->stats[0] = (double)evaluated;
     h->stats[1] = maxr;
@@ -2325,6 +2399,13 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->stats[7] = ms_aca;
     for (auto* v : {&cap, &rank, &flag, &rank2, &flag2, &tt, &ut}) c->vp.give(*v);
     for (auto* v : {&evs, &evs2, &toff}) c->vp.give(*v);
+    if (pairing) {   // the DPhi near field waits for the next DPhi assembly of this state
+        c->st_dense = dense2;
+        c->st_gen = c->gen;
+        c->st_prec = precision;
+        dense2 = nullptr;
+    }
+    c->sing_cur = nullptr;
     *out = h;
     return BEM_OK;
 }

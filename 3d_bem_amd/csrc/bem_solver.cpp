@@ -156,6 +156,7 @@ BoundaryElementSolver::BoundaryElementSolver(int device, BoundaryElements elemen
                                     E.averageTriangleDim),
               "bem_upload_mesh");
         applyImpedancePlanes();
+        check(ctx_, bem_set_near_pairing(ctx_, 1), "bem_set_near_pairing");
         std::vector<cplx> a(E.n), b(E.n), g(E.n);
         for (int64_t e = 0; e < E.n; e++) { a[e] = E.bc[e].a; b[e] = E.bc[e].b; g[e] = E.bc[e].g; }
         check(ctx_, bem_set_bc(ctx_, (const double*)a.data(), (const double*)b.data(), (const double*)g.data()),

@@ -100,6 +100,15 @@ const char* bem_hlu_error(void);
    bem_upload_mesh (rebuilt for the new mesh); already assembled H-matrices
    keep the operator they were assembled with. */
 int bem_set_plane(bem_ctx_t ctx, const double* point3, const double* normal3);
+
+/* Near-field pairing (off by default): a Phi assembly also forms the DPhi
+   near field (dense blocks and self / near-singular entries) from the same
+   quadrature and keeps it on the device; the next DPhi assembly with the
+   same mesh, partition, boundary conditions, frequency, plane and precision
+   takes it instead of recomputing.  The order of hMatrixSolve (Phi, then
+   DPhi, boundaryelementsolver.cpp:546-604) is the one it serves; entries
+   are unchanged. */
+int bem_set_near_pairing(bem_ctx_t ctx, int on);
 /* Robin coefficients a phi + b dphi = g per element (complex, n each):
    the alpha/beta/f vectors of calculateBoundaryConditionAndSourceTermVector
    (boundaryelementsolver.cpp:3386-3429). */

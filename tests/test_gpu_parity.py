@@ -122,3 +122,53 @@ def test_golden_entries_on_device(bem, O, meshes):
     P.set_frequency(500.0, 343.0, 1)
     ctx = upload_problem(bem, P)
     assert same_bits(ctx.entries4(g["ij"]), g["entries4"])
+
+
+def test_near_pairing_bit_exact(bem, O, meshes):
+    """near-field pairing (the DPhi near field formed by the Phi assembly and
+    taken by the next DPhi assembly): every block still equals the
+    restatement's, also after a frequency change between the two assemblies
+    (the kept near field must not be used), and FP32 blocks equal the
+    unpaired FP32 assembly"""
+    from helpers import sphere_problem, upload_problem
+    P = sphere_problem(O, meshes, level=3, leaf=32, eta=2.0, freq=600.0)
+    ctx = upload_problem(bem, P)
+    ctx.set_near_pairing(True)
+
+    def check(kind, G, H):
+        for b in range(len(P.blocks)):
+            A, B = H.block(b), G.block(b)
+            if isinstance(A, tuple):
+                for x, y in zip(A, B):
+                    assert same_bits(x, y), (kind, b)
+            else:
+                assert same_bits(A, B), (kind, b)
+
+    G0 = ctx.assemble(0, 1e-6, 0, seed=3)
+    G1 = ctx.assemble(1, 1e-6, 0, seed=3)       # takes the kept near field
+    check(0, G0, P.assemble(0, 1e-6, 0, 3))
+    check(1, G1, P.assemble(1, 1e-6, 0, 3))
+    G0.close(); G1.close()
+    # Phi at 600 Hz, then a new frequency: DPhi must be formed afresh
+    G0 = ctx.assemble(0, 1e-6, 0, seed=3)
+    P.set_frequency(750.0, 343.0, 1)
+    prm = P.params()
+    ctx.set_frequency(prm["k"], prm["cc"])
+    G1 = ctx.assemble(1, 1e-6, 0, seed=3)
+    check(1, G1, P.assemble(1, 1e-6, 0, 3))
+    G0.close(); G1.close()
+    # FP32: paired and unpaired give the same blocks
+    F0 = ctx.assemble(0, 1e-6, 0, precision=bem.FP32, seed=3)
+    F1 = ctx.assemble(1, 1e-6, 0, precision=bem.FP32, seed=3)
+    ctx.set_near_pairing(False)
+    U1 = ctx.assemble(1, 1e-6, 0, precision=bem.FP32, seed=3)
+    for b in range(len(P.blocks)):
+        A, B = F1.block(b), U1.block(b)
+        if isinstance(A, tuple):
+            for x, y in zip(A, B):
+                assert np.array_equal(x, y), b
+        else:
+            assert np.array_equal(A, B), b
+    for G in (F0, F1, U1):
+        G.close()
+    ctx.close()

@@ -32,6 +32,7 @@ _SIGS = {
     "bem_set_near_pairing": (_int, [_p, _int]),
     "bem_hlu_create": (_int, [_i64, _i64, _p, _i64, _p, _p, _p, _i64, _dbl, ctypes.POINTER(_p)]),
     "bem_hlu_solve": (_int, [_p, _p, _p]),
+    "bem_hlu_solve_flat": (_int, [_p, _p, _p]),
     "bem_hlu_dense": (_int, [_p, _p]),
     "bem_hlu_stats": (_int, [_p, _p]),
     "bem_hlu_destroy": (_int, [_p]),
@@ -398,10 +399,13 @@ class HLU:
             raise BemError(rc, lib().bem_hlu_error().decode())
         self.h, self.n = h, int(n)
 
-    def solve(self, b):
+    def solve(self, b, flat=False):
+        """x = U^-1 L^-1 b; flat: in the leaf-by-leaf order of the device
+        preconditioner (bem_hlu_solve_flat)"""
         b = _c(b, np.complex128)
         x = np.empty_like(b)
-        rc = lib().bem_hlu_solve(self.h, _ptr(b), _ptr(x))
+        f = lib().bem_hlu_solve_flat if flat else lib().bem_hlu_solve
+        rc = f(self.h, _ptr(b), _ptr(x))
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_hlu_error().decode())
         return x

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/bem_hip.h"
+#include "bem_hlu.hpp"
 #include "bem_kernels.hpp"
 #include "bem_setup.hpp"
 
@@ -1116,6 +1117,255 @@ __global__ __launch_bounds__(64) void k_blu_apply(const PrecBlock* __restrict__ 
     if (i < m) z[B.row0 + i] = y[i];
 }
 
+// H-LU substitution on the device (the flattened factors of bemhlu::Flat;
+// forwardSubstitution / backwardSubstitution, harithm.cpp:1314-1412): one
+// workgroup of kHluThreads walks the diagonal leaves of one direction in
+// order (dir 0: L y = z, increasing; dir 1: U x = y, decreasing), z in place.
+// Per leaf, with the item headers staged in LDS (kHluHdr per round): the
+// low-rank blocks that start there form t_B = W_B^T z_J from tasks of
+// kHluTChunk columns (every wavefront takes tasks; the chunk sums are added
+// in chunk order); the leaf's row slices of the blocks left of it (L) or
+// right of it (U) are applied to the solved part of z in tasks of kHluCols
+// columns, task tau going to wavefront tau mod 16 with all its loads issued
+// together, lane i holding rows i and i + 64; the partial sums are
+// subtracted from z_k in wavefront order; z_k := M z_k with M = Q^H or R^-1
+// of the leaf (wavefront w taking columns w, w + 16, ...).  Every offset was
+// checked against the buffers on the host.
+constexpr int kHluThreads = 1024;
+constexpr int kHluWaves = kHluThreads / 64;
+constexpr int kHluMaxLeaf = 128;
+constexpr int kHluCols = 6;
+struct HluLeafD { int64_t s, m, qh, rinv; };
+struct HluItemD { int64_t a, ld, c0, t; int32_t n, k; };
+static_assert(sizeof(HluLeafD) == sizeof(bemhlu::FlatLeaf), "leaf layout");
+static_assert(sizeof(HluItemD) == sizeof(bemhlu::FlatItem), "item layout");
+
+// a0 (+ a1) += columns j0 .. j0 + nj - 1 (nj <= NC) of a row slice times x,
+// all loads of the group issued together; TWO: rows lane + 64 as well
+template <int NC, bool TWO>
+__device__ __forceinline__ void hlu_cols(const bem_c64* col, int64_t ld, const bem_c64* x, int j0, int nj, int lane,
+                                         int m, bem_c64& a0, bem_c64& a1)
+{
+    bem_c64 xv[NC], f0[NC], f1[NC];
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+        const bool in = u < nj;
+        xv[u] = in ? x[j0 + u] : c64(0, 0);
+        f0[u] = in && lane < m ? col[lane + (int64_t)(j0 + u) * ld] : c64(0, 0);
+        if (TWO) f1[u] = in && lane + 64 < m ? col[lane + 64 + (int64_t)(j0 + u) * ld] : c64(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+        a0 = c64_fma(f0[u], xv[u], a0);
+        if (TWO) a1 = c64_fma(f1[u], xv[u], a1);
+    }
+}
+
+// item headers staged in LDS per round, and the columns of one t task
+constexpr int kHluHdr = 256;
+constexpr int kHluTChunk = 512;
+
+// tb[0 .. nq]: first task of each staged item (nt(item) tasks each)
+template <class NT>
+__device__ __forceinline__ void hlu_task_bases(const HluItemD* hdr, int nq, int32_t* tb, NT nt)
+{
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int i = 0; i < nq; i++) {
+            tb[i] = s;
+            s += nt(hdr[i]);
+        }
+        tb[nq] = s;
+    }
+    __syncthreads();
+}
+
+// the staged item holding task: tb[i] <= task < tb[i + 1]
+__device__ __forceinline__ int hlu_item_of(const int32_t* tb, int nq, int task)
+{
+    int lo = 0, hi = nq;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tb[mid] <= task) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kHluThreads) void k_hlu_sweep(int dir, int64_t nleaf, const HluLeafD* __restrict__ leaf,
+                                                           const int64_t* __restrict__ tptr,
+                                                           const HluItemD* __restrict__ titem,
+                                                           const int64_t* __restrict__ cptr,
+                                                           const HluItemD* __restrict__ citem,
+                                                           const bem_c64* __restrict__ data, bem_c64* tbuf,
+                                                           bem_c64* gpart, bem_c64* z)
+{
+    __shared__ bem_c64 part[kHluWaves][kHluMaxLeaf];
+    __shared__ bem_c64 acc[kHluMaxLeaf];
+    __shared__ HluItemD hdr[kHluHdr];
+    __shared__ int32_t tb[kHluHdr + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    auto t_tasks = [](const HluItemD& T) { return T.k * ((T.n + kHluTChunk - 1) / kHluTChunk); };
+    auto c_tasks = [](const HluItemD& C) { return ((C.k < 0 ? C.n : C.k) + kHluCols - 1) / kHluCols; };
+    for (int64_t step = 0; step < nleaf; step++) {
+        const int64_t k = dir == 0 ? step : nleaf - 1 - step;
+        const HluLeafD L = leaf[k];
+        const int m = (int)L.m;
+        // t vectors: (item, rank index, chunk of kHluTChunk columns) tasks over
+        // all wavefronts into gpart, then each t_B[l] sums its chunks in order
+        for (int64_t q0 = tptr[k]; q0 < tptr[k + 1]; q0 += kHluHdr) {
+            const int nq = (int)min((int64_t)kHluHdr, tptr[k + 1] - q0);
+            if (threadIdx.x < nq) hdr[threadIdx.x] = titem[q0 + threadIdx.x];
+            __syncthreads();
+            hlu_task_bases(hdr, nq, tb, t_tasks);
+            const int ntask = tb[nq];
+            for (int task = w; task < ntask; task += kHluWaves) {
+                const int i = hlu_item_of(tb, nq, task);
+                const HluItemD T = hdr[i];
+                const int nch = (T.n + kHluTChunk - 1) / kHluTChunk;
+                const int rel = task - tb[i], l = rel / nch, j0 = (rel - l * nch) * kHluTChunk;
+                const int len = min(kHluTChunk, T.n - j0);
+                const bem_c64* col = data + T.a + (int64_t)l * T.ld + j0;
+                const bem_c64* zc = z + T.c0 + j0;
+                bem_c64 wv[kHluTChunk / 64], xv[kHluTChunk / 64];
+#pragma unroll
+                for (int u = 0; u < kHluTChunk / 64; u++) {
+                    const int j = lane + 64 * u;
+                    wv[u] = j < len ? col[j] : c64(0, 0);
+                    xv[u] = j < len ? zc[j] : c64(0, 0);
+                }
+                bem_c64 v = c64(0, 0);
+#pragma unroll
+                for (int u = 0; u < kHluTChunk / 64; u++) v = c64_fma(wv[u], xv[u], v);
+                v = wave_sum(v);
+                if (lane == 0) gpart[task] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < nq) {
+                const HluItemD T = hdr[threadIdx.x];
+                const int nch = (T.n + kHluTChunk - 1) / kHluTChunk;
+                for (int l = 0; l < T.k; l++) {
+                    const bem_c64* g = gpart + tb[threadIdx.x] + l * nch;
+                    bem_c64 v = g[0];
+                    for (int ch = 1; ch < nch; ch++) v = c64_add(v, g[ch]);
+                    tbuf[T.t + l] = v;
+                }
+            }
+            __syncthreads();
+        }
+        // contributions: (item, kHluCols columns) tasks over all wavefronts
+        bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
+        for (int64_t q0 = cptr[k]; q0 < cptr[k + 1]; q0 += kHluHdr) {
+            const int nq = (int)min((int64_t)kHluHdr, cptr[k + 1] - q0);
+            if (threadIdx.x < nq) hdr[threadIdx.x] = citem[q0 + threadIdx.x];
+            __syncthreads();
+            hlu_task_bases(hdr, nq, tb, c_tasks);
+            const int ntask = tb[nq];
+            for (int task = w; task < ntask; task += kHluWaves) {
+                const int i = hlu_item_of(tb, nq, task);
+                const HluItemD C = hdr[i];
+                const int nc = C.k < 0 ? C.n : C.k;
+                const int j0 = (task - tb[i]) * kHluCols, nj = min(kHluCols, nc - j0);
+                const bem_c64* x = C.k < 0 ? z + C.c0 : tbuf + C.t;
+                const bem_c64* col = data + C.a;
+                if (m <= 64)
+                    hlu_cols<kHluCols, false>(col, C.ld, x, j0, nj, lane, m, a0, a1);
+                else
+                    for (int h = 0; h < kHluCols; h += kHluCols / 2)
+                        hlu_cols<kHluCols / 2, true>(col, C.ld, x, j0 + h, min(kHluCols / 2, nj - h), lane, m, a0,
+                                                     a1);
+            }
+            __syncthreads();
+        }
+        part[w][lane] = a0;
+        part[w][lane + 64] = a1;
+        __syncthreads();
+        if (threadIdx.x < m) {
+            bem_c64 v = z[L.s + threadIdx.x];
+            for (int g = 0; g < kHluWaves; g++) v = c64_sub(v, part[g][threadIdx.x]);
+            acc[threadIdx.x] = v;
+        }
+        __syncthreads();
+        const bem_c64* M = data + (dir == 0 ? L.qh : L.rinv);
+        bem_c64 y0 = c64(0, 0), y1 = c64(0, 0);
+        for (int j = w; j < m; j += kHluWaves) {
+            const bem_c64 av = acc[j];
+            if (lane < m) y0 = c64_fma(M[lane + j * m], av, y0);
+            if (lane + 64 < m) y1 = c64_fma(M[lane + 64 + j * m], av, y1);
+        }
+        part[w][lane] = y0;
+        part[w][lane + 64] = y1;
+        __syncthreads();
+        if (threadIdx.x < m) {
+            bem_c64 v = part[0][threadIdx.x];
+            for (int g = 1; g < kHluWaves; g++) v = c64_add(v, part[g][threadIdx.x]);
+            z[L.s + threadIdx.x] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// largest number of t tasks of one round of k_hlu_sweep (the size of gpart)
+int64_t hlu_max_t_tasks(const bemhlu::Flat& F)
+{
+    int64_t best = 1;
+    for (int d = 0; d < 2; d++)
+        for (size_t k = 0; k + 1 < F.tptr[d].size(); k++)
+            for (int64_t q0 = F.tptr[d][k]; q0 < F.tptr[d][k + 1]; q0 += kHluHdr) {
+                int64_t s = 0;
+                for (int64_t q = q0; q < std::min(q0 + kHluHdr, F.tptr[d][k + 1]); q++) {
+                    const auto& T = F.titem[d][q];
+                    s += (int64_t)T.k * ((T.n + kHluTChunk - 1) / kHluTChunk);
+                }
+                best = std::max(best, s);
+            }
+    return best;
+}
+
+// every access k_hlu_sweep makes inside its buffers (checked before upload)
+bool hlu_flat_ok(const bemhlu::Flat& F, std::string& why)
+{
+    const int64_t N = F.n, D = (int64_t)F.data.size(), T = F.tsize, K = (int64_t)F.leaf.size();
+    auto bad = [&](const std::string& w) {
+        why = w;
+        return false;
+    };
+    if (K < 1) return bad("no diagonal leaves");
+    for (const auto& L : F.leaf)
+        if (L.m < 1 || L.m > kHluMaxLeaf || L.s < 0 || L.s + L.m > N || L.qh < 0 || L.qh + L.m * L.m > D ||
+            L.rinv < 0 || L.rinv + L.m * L.m > D)
+            return bad("diagonal leaf of " + std::to_string(L.m) + " rows (at most " + std::to_string(kHluMaxLeaf) +
+                       ") or outside the factors");
+    // the last element a matrix slice touches: rows 0 .. rows - 1 of columns 0 .. cols - 1
+    auto inside = [&](int64_t a, int64_t ld, int64_t rows, int64_t cols) {
+        if (rows <= 0 || cols <= 0) return a >= 0;
+        return a >= 0 && ld >= rows && a + (rows - 1) + (cols - 1) * ld < D;
+    };
+    for (int d = 0; d < 2; d++) {
+        const auto &tp = F.tptr[d], &cp = F.cptr[d];
+        if ((int64_t)tp.size() != K + 1 || (int64_t)cp.size() != K + 1 || tp[0] != 0 || cp[0] != 0 ||
+            tp[K] != (int64_t)F.titem[d].size() || cp[K] != (int64_t)F.citem[d].size())
+            return bad("item lists");
+        for (int64_t k = 0; k < K; k++) {
+            if (tp[k + 1] < tp[k] || cp[k + 1] < cp[k]) return bad("item lists");
+            const int64_t m = F.leaf[k].m;
+            for (int64_t q = tp[k]; q < tp[k + 1]; q++) {
+                const auto& I = F.titem[d][q];
+                if (I.k < 0 || I.n < 0 || I.c0 < 0 || I.c0 + I.n > N || I.t < 0 || I.t + I.k > T ||
+                    !inside(I.a, I.ld, I.n, I.k))
+                    return bad("t item outside the factors");
+            }
+            for (int64_t q = cp[k]; q < cp[k + 1]; q++) {
+                const auto& I = F.citem[d][q];
+                const bool ok = I.k < 0 ? I.n >= 0 && I.c0 >= 0 && I.c0 + I.n <= N && inside(I.a, I.ld, m, I.n)
+                                        : I.t >= 0 && I.t + I.k <= T && inside(I.a, I.ld, m, I.k);
+                if (!ok) return bad("contribution outside the factors");
+            }
+        }
+    }
+    return true;
+}
+
 // x += z
 __global__ void k_axpy1(int64_t n, const bem_c64* z, bem_c64* x)
 {
@@ -1312,8 +1562,18 @@ struct bem_prec {
     PrecBlock* d_blk = nullptr;
     bem_c64* d_lu = nullptr;
     int32_t* d_piv = nullptr;
-    bem_hlu_t hlu = nullptr;        // host H-LU (bem_precond_create_hlu)
-    std::vector<bem_c64> host;      // its staging vector
+    // H-LU (bem_precond_create_hlu): the flattened factors on the device
+    int64_t hlu_nleaf = 0;
+    HluLeafD* hlu_leaf = nullptr;
+    int64_t* hlu_tptr[2] = {nullptr, nullptr};
+    int64_t* hlu_cptr[2] = {nullptr, nullptr};
+    HluItemD* hlu_titem[2] = {nullptr, nullptr};
+    HluItemD* hlu_citem[2] = {nullptr, nullptr};
+    bem_c64* hlu_data = nullptr;
+    bem_c64* hlu_t = nullptr;
+    bem_c64* hlu_gpart = nullptr;   // chunk sums of the t vectors of one round
+    std::vector<void*> hlu_owned;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> hlu_ev;   // per application, resolved by bem_precond_stats
     double setup_ms = 0, apply_ms = 0, entries = 0, factor_ms = 0;
     int64_t applies = 0;
 };
@@ -2638,12 +2898,47 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
     int rc = bem_hlu_create(c->n, nclusters, clusters2, nb, blocks5.data(), rank.data(), ptr.data(), max_rank, rel_err,
                             &L);
     if (rc) return set_err(c, rc, std::string("bem_precond_create_hlu: ") + bem_hlu_error());
-    auto p = new bem_prec();
-    p->ctx = c;
-    p->hlu = L;
-    p->host.resize((size_t)c->n);
+    for (auto& v : data) std::vector<bem_c64>().swap(v);
     double st[3];
     bem_hlu_stats(L, st);
+    bemhlu::Flat F;
+    try {
+        bemhlu::flatten(L, F);
+    } catch (const bemhlu::Error& e) {
+        bem_hlu_destroy(L);
+        return set_err(c, BEM_ERR_ARG, "bem_precond_create_hlu: " + e.msg);
+    }
+    bem_hlu_destroy(L);
+    std::string why;
+    if (!hlu_flat_ok(F, why)) return set_err(c, BEM_ERR_ARG, "bem_precond_create_hlu: " + why);
+    auto p = new bem_prec();
+    p->ctx = c;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+        int r = dalloc_bytes(c, dst, bytes > 0 ? bytes : 16);
+        if (r) return r;
+        p->hlu_owned.push_back(*dst);
+        if (bytes && src) HIPC(c, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        return BEM_OK;
+    };
+    p->hlu_nleaf = (int64_t)F.leaf.size();
+    rc = up((void**)&p->hlu_leaf, F.leaf.data(), F.leaf.size() * sizeof(HluLeafD));
+    for (int d = 0; d < 2 && !rc; d++) {
+        if (!rc) rc = up((void**)&p->hlu_tptr[d], F.tptr[d].data(), F.tptr[d].size() * sizeof(int64_t));
+        if (!rc) rc = up((void**)&p->hlu_cptr[d], F.cptr[d].data(), F.cptr[d].size() * sizeof(int64_t));
+        if (!rc) rc = up((void**)&p->hlu_titem[d], F.titem[d].data(), F.titem[d].size() * sizeof(HluItemD));
+        if (!rc) rc = up((void**)&p->hlu_citem[d], F.citem[d].data(), F.citem[d].size() * sizeof(HluItemD));
+    }
+    if (!rc) rc = up((void**)&p->hlu_data, F.data.data(), F.data.size() * sizeof(bem_c64));
+    if (!rc) rc = up((void**)&p->hlu_t, nullptr, (size_t)F.tsize * sizeof(bem_c64));   // written before read
+    if (!rc) rc = up((void**)&p->hlu_gpart, nullptr, (size_t)hlu_max_t_tasks(F) * sizeof(bem_c64));   // written before read
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);   // the host buffers go out of scope
+        if (e != hipSuccess) rc = set_err(c, BEM_ERR_HIP, std::string("bem_precond_create_hlu: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        bem_precond_destroy(p);
+        return rc;
+    }
     p->entries = st[0];
     p->factor_ms = 1e3 * st[2];
     p->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2654,6 +2949,14 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
 int bem_precond_stats(bem_prec_t p, double* out5)
 {
     if (!p || !out5) return BEM_ERR_ARG;
+    for (auto& e : p->hlu_ev) {   // device time of the H-LU applications so far
+        float ms = 0;
+        if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
+            p->apply_ms += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    p->hlu_ev.clear();
     out5[0] = p->setup_ms;
     out5[1] = (double)p->applies;
     out5[2] = p->apply_ms;
@@ -2665,9 +2968,13 @@ int bem_precond_stats(bem_prec_t p, double* out5)
 int bem_precond_destroy(bem_prec_t p)
 {
     if (!p) return BEM_ERR_ARG;
-    if (p->hlu) bem_hlu_destroy(p->hlu);
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
+    for (auto& e : p->hlu_ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    free_list(p->ctx, p->hlu_owned);
     pfree(p->ctx, p->d_blk);
     pfree(p->ctx, p->d_lu);
     pfree(p->ctx, p->d_piv);
@@ -2679,18 +2986,23 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
 {
     if (!c || !p || !v || !z || p->ctx != c || v == z) return set_err(c, BEM_ERR_ARG, "bem_precond_apply: bad arguments");
     HIPC(c, hipSetDevice(c->device));
-    if (p->hlu) {   // host H-LU: the vector goes to the host and back
-        const auto t0 = std::chrono::steady_clock::now();
-        const size_t bytes = sizeof(bem_c64) * (size_t)c->n;
-        HIPC(c, hipMemcpyAsync(p->host.data(), v, bytes, hipMemcpyDeviceToHost, c->stream));
-        HIPC(c, hipStreamSynchronize(c->stream));
-        int rc = bem_hlu_solve(p->hlu, (const double*)p->host.data(), (double*)p->host.data());
-        if (rc) return set_err(c, rc, std::string("bem_precond_apply: ") + bem_hlu_error());
-        HIPC(c, hipMemcpyAsync(z, p->host.data(), bytes, hipMemcpyHostToDevice, c->stream));
-        HIPC(c, hipStreamSynchronize(c->stream));
+    if (p->hlu_nleaf > 0) {   // H-LU: z = v, then the two sweeps in place
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPC(c, hipEventCreate(&e0));
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return set_err(c, BEM_ERR_HIP, "bem_precond_apply: hipEventCreate");
+        }
+        p->hlu_ev.push_back({e0, e1});
+        HIPC(c, hipEventRecord(e0, c->stream));
+        HIPC(c, hipMemcpyAsync(z, v, sizeof(bem_c64) * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+        for (int d = 0; d < 2; d++)
+            hipLaunchKernelGGL(k_hlu_sweep, dim3(1), dim3(kHluThreads), 0, c->stream, d, p->hlu_nleaf, p->hlu_leaf,
+                               p->hlu_tptr[d], p->hlu_titem[d], p->hlu_cptr[d], p->hlu_citem[d], p->hlu_data,
+                               p->hlu_t, p->hlu_gpart, (bem_c64*)z);
+        HIPC(c, hipEventRecord(e1, c->stream));
         p->applies++;
-        p->apply_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        return BEM_OK;
+        return launch_check(c, "k_hlu_sweep");
     }
     p->applies++;   // asynchronous: its time is part of the GMRES iterations
     hipLaunchKernelGGL(k_blu_apply, dim3((unsigned)p->nblk), dim3(64), 0, c->stream, p->d_blk, p->d_lu, p->d_piv,

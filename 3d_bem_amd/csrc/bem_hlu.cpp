@@ -935,13 +935,147 @@ void HLU::stats(double* out3) const
     out3[2] = t_factor_;
 }
 
+namespace {
+// factored diagonal leaves and off-diagonal leaf blocks of the tree
+void collect(const Block& B, std::vector<const Block*>& diag, std::vector<const Block*>& off)
+{
+    if (B.type == Block::Sub) {
+        for (int q = 0; q < 4; q++) collect(*B.s[q], diag, off);
+        return;
+    }
+    if (B.r0 == B.c0) {
+        if (B.type != Block::Full || B.m != B.n || B.Q.m != B.m) throw Error{"H-LU: unfactored diagonal leaf"};
+        diag.push_back(&B);
+    } else {
+        off.push_back(&B);
+    }
+}
+}  // namespace
+
+void HLU::flatten(Flat& F) const
+{
+    if (!factored_) throw Error{"H-LU: not factored"};
+    std::vector<const Block*> diag, off;
+    collect(*root_, diag, off);
+    std::sort(diag.begin(), diag.end(), [](const Block* a, const Block* b) { return a->r0 < b->r0; });
+    F = Flat();
+    F.n = n_;
+    const int64_t K = (int64_t)diag.size();
+    std::vector<int64_t> leaf_of((size_t)n_, -1);
+    int64_t s = 0;
+    for (int64_t k = 0; k < K; k++) {
+        const Block& D = *diag[k];
+        if (D.r0 != s) throw Error{"H-LU: diagonal leaves do not tile the matrix"};
+        const int64_t m = D.m;
+        for (int64_t i = 0; i < m; i++) leaf_of[(size_t)(s + i)] = k;
+        FlatLeaf L{s, m, (int64_t)F.data.size(), 0};
+        for (int64_t j = 0; j < m; j++)   // Q^H
+            for (int64_t i = 0; i < m; i++) F.data.push_back(std::conj(D.Q(j, i)));
+        L.rinv = (int64_t)F.data.size();
+        F.data.resize(F.data.size() + (size_t)(m * m), cd(0, 0));
+        cd* X = F.data.data() + L.rinv;   // R X = I by back substitution, column by column
+        for (int64_t j = 0; j < m; j++)
+            for (int64_t i = j; i >= 0; i--) {
+                cd t = i == j ? cd(1, 0) : cd(0, 0);
+                for (int64_t c = i + 1; c <= j; c++) t -= D.F(i, c) * X[c + j * m];
+                X[i + j * m] = t / D.F(i, i);
+            }
+        F.leaf.push_back(L);
+        F.maxm = std::max(F.maxm, m);
+        s += m;
+    }
+    if (s != n_) throw Error{"H-LU: diagonal leaves do not tile the matrix"};
+    std::vector<std::vector<FlatItem>> tat[2], cat[2];
+    for (int d = 0; d < 2; d++) {
+        tat[d].resize((size_t)K);
+        cat[d].resize((size_t)K);
+    }
+    for (const Block* pb : off) {
+        const Block& B = *pb;
+        const bool lower = B.r0 > B.c0;
+        const int d = lower ? 0 : 1;
+        if (lower ? B.c0 + B.n > B.r0 : B.r0 + B.m > B.c0) throw Error{"H-LU: off-diagonal block crosses the diagonal"};
+        const int64_t k0 = leaf_of[(size_t)B.r0], k1 = leaf_of[(size_t)(B.r0 + B.m - 1)];
+        if (F.leaf[k0].s != B.r0 || F.leaf[k1].s + F.leaf[k1].m != B.r0 + B.m)
+            throw Error{"H-LU: block rows are not whole diagonal leaves"};
+        if (B.type == Block::Full) {
+            const int64_t a = (int64_t)F.data.size();
+            F.data.insert(F.data.end(), B.F.a.begin(), B.F.a.end());
+            for (int64_t k = k0; k <= k1; k++)
+                cat[d][k].push_back(FlatItem{a + F.leaf[k].s - B.r0, B.m, B.c0, 0, (int32_t)B.n, -1});
+            continue;
+        }
+        const int64_t r = B.U.n;
+        if (r == 0) continue;   // a zero block
+        const int64_t au = (int64_t)F.data.size();
+        F.data.insert(F.data.end(), B.U.a.begin(), B.U.a.end());
+        const int64_t aw = (int64_t)F.data.size();
+        F.data.insert(F.data.end(), B.W.a.begin(), B.W.a.end());
+        const int64_t t = F.tsize;
+        F.tsize += r;
+        tat[d][lower ? k0 : k1].push_back(FlatItem{aw, B.n, B.c0, t, (int32_t)B.n, (int32_t)r});
+        for (int64_t k = k0; k <= k1; k++)
+            cat[d][k].push_back(FlatItem{au + F.leaf[k].s - B.r0, B.m, 0, t, 0, (int32_t)r});
+    }
+    for (int d = 0; d < 2; d++) {
+        F.tptr[d].assign(1, 0);
+        F.cptr[d].assign(1, 0);
+        for (int64_t k = 0; k < K; k++) {
+            F.titem[d].insert(F.titem[d].end(), tat[d][k].begin(), tat[d][k].end());
+            F.citem[d].insert(F.citem[d].end(), cat[d][k].begin(), cat[d][k].end());
+            F.tptr[d].push_back((int64_t)F.titem[d].size());
+            F.cptr[d].push_back((int64_t)F.citem[d].size());
+        }
+    }
+}
+
+void Flat::solve(const cd* b, cd* x) const
+{
+    if (x != b) std::memcpy(x, b, sizeof(cd) * (size_t)n);
+    std::vector<cd> t((size_t)tsize), acc((size_t)maxm);
+    const int64_t K = (int64_t)leaf.size();
+    for (int d = 0; d < 2; d++)
+        for (int64_t s = 0; s < K; s++) {
+            const int64_t k = d == 0 ? s : K - 1 - s;
+            const FlatLeaf& L = leaf[k];
+            for (int64_t q = tptr[d][k]; q < tptr[d][k + 1]; q++) {
+                const FlatItem& T = titem[d][q];
+                for (int l = 0; l < T.k; l++) {
+                    cd v(0, 0);
+                    for (int64_t j = 0; j < T.n; j++) v += data[T.a + j + l * T.ld] * x[T.c0 + j];
+                    t[T.t + l] = v;
+                }
+            }
+            for (int64_t i = 0; i < L.m; i++) acc[i] = x[L.s + i];
+            for (int64_t q = cptr[d][k]; q < cptr[d][k + 1]; q++) {
+                const FlatItem& C = citem[d][q];
+                if (C.k < 0) {
+                    for (int64_t j = 0; j < C.n; j++)
+                        for (int64_t i = 0; i < L.m; i++) acc[i] -= data[C.a + i + j * C.ld] * x[C.c0 + j];
+                } else {
+                    for (int l = 0; l < C.k; l++)
+                        for (int64_t i = 0; i < L.m; i++) acc[i] -= data[C.a + i + l * C.ld] * t[C.t + l];
+                }
+            }
+            const cd* M = data.data() + (d == 0 ? L.qh : L.rinv);
+            for (int64_t i = 0; i < L.m; i++) {
+                cd v(0, 0);
+                for (int64_t j = 0; j < L.m; j++) v += M[i + j * L.m] * acc[j];
+                x[L.s + i] = v;
+            }
+        }
+}
+
 }  // namespace bemhlu
 
 // ------------------------------------------------------------------ C ABI
 
 struct bem_hlu {
     std::unique_ptr<bemhlu::HLU> h;
+    std::unique_ptr<bemhlu::Flat> flat;   // formed on first use
 };
+
+void bemhlu::flatten(bem_hlu* h, Flat& F) { h->h->flatten(F); }
 
 namespace {
 thread_local std::string g_hlu_err;
@@ -987,6 +1121,20 @@ int bem_hlu_solve(bem_hlu_t h, const double* b2, double* x2)
     if (!h || !b2 || !x2) return BEM_ERR_ARG;
     return guarded([&]() {
         h->h->solve(reinterpret_cast<const bemhlu::cd*>(b2), reinterpret_cast<bemhlu::cd*>(x2));
+        return BEM_OK;
+    });
+}
+
+int bem_hlu_solve_flat(bem_hlu_t h, const double* b2, double* x2)
+{
+    if (!h || !b2 || !x2) return BEM_ERR_ARG;
+    return guarded([&]() {
+        if (!h->flat) {
+            auto F = std::make_unique<bemhlu::Flat>();
+            h->h->flatten(*F);
+            h->flat = std::move(F);
+        }
+        h->flat->solve(reinterpret_cast<const bemhlu::cd*>(b2), reinterpret_cast<bemhlu::cd*>(x2));
         return BEM_OK;
     });
 }

@@ -18,6 +18,8 @@
 #include <string>
 #include <vector>
 
+struct bem_hlu;
+
 namespace bemhlu {
 
 using cd = std::complex<double>;
@@ -53,6 +55,40 @@ struct Error {
     std::string msg;
 };
 
+// The factors flattened for a leaf-by-leaf substitution (the form the device
+// preconditioner runs; forwardSubstitution / backwardSubstitution,
+// harithm.cpp:1314-1412, in the same arithmetic up to the order of the
+// additions).  Direction 0 solves L y = b over the diagonal leaves in
+// increasing order, direction 1 solves U x = y in decreasing order; leaf k
+// of direction d is
+//   z_k := M_k (z_k - sum of the contributions of leaf k),
+// M_k = Q_k^H (direction 0) or R_k^-1 (direction 1), after the low-rank
+// blocks whose rows start (in that direction) at leaf k have formed their
+// vectors t_B = W_B^T z_{cols B}.  A contribution is the leaf's row slice of
+// an off-diagonal block times the solved part of z: full F (rows, columns
+// c0 ..) or low-rank U (rows) t_B.  Every matrix is column-major in data.
+struct FlatLeaf {
+    int64_t s, m;        // rows s .. s + m - 1
+    int64_t qh, rinv;    // offsets in data of Q^H and R^-1 (m x m)
+};
+struct FlatItem {
+    int64_t a;     // offset in data: F or U at the leaf's first row, or W
+    int64_t ld;    // leading dimension of that matrix
+    int64_t c0;    // first column of the block (full contribution, t item)
+    int64_t t;     // offset of t_B in the t buffer (low-rank contribution, t item)
+    int32_t n;     // columns of F (full contribution) or of the block (t item); 0 for a low-rank contribution
+    int32_t k;     // rank (low-rank contribution, t item); -1 for a full contribution
+};
+struct Flat {
+    int64_t n = 0, tsize = 0, maxm = 0;
+    std::vector<FlatLeaf> leaf;
+    std::vector<int64_t> tptr[2], cptr[2];   // per direction: items of leaf k in [ptr[k], ptr[k + 1])
+    std::vector<FlatItem> titem[2], citem[2];
+    std::vector<cd> data;
+    // x = U^-1 L^-1 b in the schedule's order (x may equal b)
+    void solve(const cd* b, cd* x) const;
+};
+
 class HLU {
 public:
     // clusters2: (start, size) of every cluster of the (row = column) cluster
@@ -70,6 +106,8 @@ public:
     int64_t n() const { return n_; }
     // [0] stored complex entries, [1] largest rank, [2] factor seconds
     void stats(double* out3) const;
+    // the factors in the leaf-by-leaf form (after factor())
+    void flatten(Flat& F) const;
 
 private:
     int64_t n_;
@@ -78,5 +116,8 @@ private:
     double t_factor_ = 0;
     bool factored_ = false;
 };
+
+// the leaf-by-leaf form of a factorisation made by bem_hlu_create
+void flatten(bem_hlu* h, Flat& F);
 
 }  // namespace bemhlu

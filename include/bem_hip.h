@@ -74,13 +74,19 @@ int bem_upload_mesh(bem_ctx_t ctx, int64_t n, const double* nodes9, const double
    preconditionerRelError (bemparameters.h:27-28; 0 / 0 = no truncation, an
    exact LU).  Factors on return.  bem_hlu_solve: x = U^-1 L^-1 b (host
    vectors, n complex; x may equal b): forwardSubstitution +
-   backwardSubstitution (harithm.cpp:1314-1412).  bem_hlu_dense: L U as a
+   backwardSubstitution (harithm.cpp:1314-1412).  bem_hlu_solve_flat: the
+   same solve in the leaf-by-leaf order the device preconditioner of
+   bem_precond_create_hlu runs (diagonal leaves in order, each after the row
+   slices of the blocks left of it in L / right of it in U, Q^H and R^-1 of
+   the leaf applied as products), on the host; equal to bem_hlu_solve up to
+   the order of the additions.  bem_hlu_dense: L U as a
    dense n x n matrix (column-major complex).  bem_hlu_stats: stored complex
    entries, largest rank, factor seconds.  Errors: bem_hlu_error(). */
 typedef struct bem_hlu* bem_hlu_t;
 int bem_hlu_create(int64_t n, int64_t nclusters, const int64_t* clusters2, int64_t nb, const int64_t* blocks5,
                    const int64_t* rank, const double* const* data, int64_t max_rank, double rel_err, bem_hlu_t* out);
 int bem_hlu_solve(bem_hlu_t h, const double* b2, double* x2);
+int bem_hlu_solve_flat(bem_hlu_t h, const double* b2, double* x2);
 int bem_hlu_dense(bem_hlu_t h, double* out2);
 int bem_hlu_stats(bem_hlu_t h, double* out3);
 int bem_hlu_destroy(bem_hlu_t h);
@@ -162,14 +168,21 @@ int bem_precond_apply(bem_ctx_t ctx, bem_prec_t p, const void* v_dev, void* z_de
    usePreconditioner): the host H-LU of bem_hlu_create (truncation max_rank /
    rel_err = preconditionerRank / preconditionerRelError) fed with the blocks
    of the assembled H-matrix h (downloaded, FP32 blocks widened to FP64);
-   clusters2 = (start, size) of every cluster of the cluster tree.  Used by
-   bem_precond_apply / bem_gmres_prec like the leaf-block LU; each
-   application moves the vector to the host and back. */
+   clusters2 = (start, size) of every cluster of the cluster tree.  The
+   factors are then flattened (bem_hlu_solve_flat's form: Q^H and R^-1 of
+   every diagonal leaf, the off-diagonal blocks' row slices per leaf) and
+   kept on the device; bem_precond_apply / bem_gmres_prec run the forward
+   and backward substitution there (forwardSubstitution /
+   backwardSubstitution, harithm.cpp:1314-1412; one workgroup walks the
+   leaves), asynchronously on the context stream like the leaf-block LU.
+   Diagonal leaves of more than 128 rows are refused (BEM_ERR_ARG). */
 int bem_precond_create_hlu(bem_ctx_t ctx, bem_hmat_t h, int64_t nclusters, const int64_t* clusters2, int64_t max_rank,
                            double rel_err, bem_prec_t* out);
-/* [0] set-up ms (host wall clock, incl. block download and factorisation),
-   [1] applications, [2] application ms in all, [3] stored complex entries,
-   [4] factorisation ms (host H-LU; 0 for the leaf-block LU) */
+/* [0] set-up ms (host wall clock, incl. block download, factorisation and
+   upload), [1] applications, [2] application ms in all (H-LU: device time
+   of the substitutions; 0 for the leaf-block LU, whose applications are
+   part of the GMRES iterations), [3] stored complex entries, [4]
+   factorisation ms (host H-LU; 0 for the leaf-block LU) */
 int bem_precond_stats(bem_prec_t p, double* out5);
 int bem_gmres_prec(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t p, const void* b_dev, void* x_dev, double tol,
                    int64_t restart, int64_t maxit, int64_t* iters, double* rel_resid);
@@ -239,7 +252,7 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
                      double* phi, double* dphi, double* pressure, double* stats16);
 
 /* GMRES preconditioner of bem_solver_solve / bem_solver_sweep: 0 none (the
-   default), 1 the leaf-block LU of bem_precond_create, 2 the host H-LU of
+   default), 1 the leaf-block LU of bem_precond_create, 2 the H-LU of
    bem_precond_create_hlu (usePreconditioner, bemparameters.h:26) */
 int bem_solver_set_preconditioner(bem_solver_t s, int kind);
 /* the H-LU preconditioner with preconditionerRank / preconditionerRelError

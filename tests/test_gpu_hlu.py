@@ -61,6 +61,40 @@ def test_apply_equals_host_hlu(bem, O, meshes):
     ctx.close()
 
 
+def test_device_substitution_wide_leaves(bem, O, meshes):
+    """diagonal leaves of 65-127 rows (leaf 128: each lane holds two rows of
+    the leaf) apply the same H-LU as the host; leaves above 128 rows are
+    refused before anything runs on the device"""
+    P = sphere_problem(O, meshes, level=3, leaf=128, eta=2.0, freq=700.0)
+    assert max(b[1] for b in P.blocks if b[0] == b[2]) > 64
+    ctx = upload_problem(bem, P)
+    G = ctx.assemble(1, 1e-6, 0, seed=5)
+    H = P.assemble(1, 1e-6, 0, 5)
+    M = G.hlu(P.clusters, 2, 0.0)
+    F = bem.HLU(P.n, P.clusters, P.blocks, [H.block(b) for b in range(len(P.blocks))], 2, 0.0)
+    try:
+        rng = np.random.default_rng(3)
+        v = rng.standard_normal(P.n) + 1j * rng.standard_normal(P.n)
+        a, b = M.apply(v), F.solve(v)
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    finally:
+        M.close()
+        F.close()
+    print("leaf 128: device vs host H-LU", err)
+    assert err < 1e-14
+    G.close()
+    ctx.close()
+    P2 = sphere_problem(O, meshes, level=3, leaf=256, eta=2.0, freq=700.0)
+    ctx2 = upload_problem(bem, P2)
+    G2 = ctx2.assemble(1, 1e-6, 0, seed=5)
+    try:
+        with pytest.raises(bem.BemError, match="at most 128"):
+            G2.hlu(P2.clusters, 2, 0.0)
+    finally:
+        G2.close()
+        ctx2.close()
+
+
 def test_fp32_operator_with_hlu(bem, meshes):
     """L4 (5120), 800 Hz, FP32 blocks with the H-LU built from them: same
     pressure as the FP64 solve, fewer iterations"""

@@ -87,6 +87,23 @@ def test_rank_two_preconditioner(bem, L3):
     assert counts[1] * 2 < counts[0]
 
 
+@pytest.mark.parametrize("rank,tol", [(0, 0.0), (0, 1e-5), (2, 0.0)])
+def test_flat_schedule_equals_recursive_solve(bem, L2, L3, rank, tol):
+    """the leaf-by-leaf substitution (the device preconditioner's order,
+    Q^H and R^-1 of each leaf as products) gives the recursive
+    forward / backward substitution's result to round-off, on the 320- and
+    1280-element operators, exact and truncated"""
+    for P, H, blocks in (L2, L3):
+        F = bem.HLU(P.n, P.clusters, P.blocks, blocks, max_rank=rank, rel_err=tol)
+        rng = np.random.default_rng(7)
+        b = rng.standard_normal(P.n) + 1j * rng.standard_normal(P.n)
+        x0, x1 = F.solve(b), F.solve(b, flat=True)
+        err = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
+        print(P.n, rank, tol, "flat vs recursive", err)
+        assert err < 1e-12
+        F.close()
+
+
 def test_bad_partition_is_an_error(bem, L2):
     P, H, blocks = L2
     keep = np.ones(len(P.blocks), bool)

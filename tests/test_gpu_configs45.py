@@ -80,3 +80,33 @@ def test_sixteen_ellipsoids(bem, O, meshes):
                 checked += 1
         S.close()
     assert total == 11 * 20480 + 5 * 5120
+
+
+def test_capsule76k_hlu_preconditioned(bem, meshes):
+    """The same capsule (FP32, ACA 1e-4, GMRES 1e-8) preconditioned by the
+    H-LU with the reference's defaults (rank 2, no tolerance): factorised on
+    the host from the GPU-assembled blocks, substituted on the device
+    (k_hlu_sweep).  Same pressure as with the leaf-block LU, a fraction of
+    its iterations."""
+    f, c, rho = 1500.0, 343.0, 1.21
+    k = 2 * np.pi * f / c
+    tri = meshes.capsule(2.0, 0.25, 0.0107)
+    n = len(tri)
+    bc = meshes.impedance_bc(n, k, 0.3, 4)
+    d = meshes.unit_direction(4)
+    S = bem.Solver(tri, bc, leaf=64, eta=1.5)
+    S.set_sources(plane=np.array([d[0], d[1], d[2], 1.0, 0.0]))
+    kw = dict(c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP32)
+    S.set_preconditioner(1)
+    _, _, p1, st1 = S.solve(f, **kw)
+    S.set_preconditioner(2)
+    _, _, p2, st2 = S.solve(f, **kw)
+    ps = S.precond_stats()
+    print("capsule H-LU: iterations", st2["gmres_iters"], "(leaf-block LU", st1["gmres_iters"], ") vs leaf-block LU",
+          rel(p2, p1), "set-up ms", ps["setup_ms"], "factorisation ms", ps["factor_ms"], "applications",
+          ps["applies"], "device ms", ps["apply_ms"])
+    assert st1["residual"] < 1e-7 and st2["residual"] < 1e-7
+    assert rel(p2, p1) < 1e-6
+    assert st2["gmres_iters"] * 3 < st1["gmres_iters"]
+    assert ps["applies"] >= st2["gmres_iters"] and ps["apply_ms"] > 0
+    S.close()

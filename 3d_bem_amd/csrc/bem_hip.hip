@@ -2835,6 +2835,26 @@ int bem_precond_create(bem_ctx_t c, bem_hmat_t h, bem_prec_t* out)
     return BEM_OK;
 }
 
+// adds the device time of the H-LU applications recorded so far to apply_ms
+// and releases their events: all of them (wait), or the finished ones
+static void hlu_resolve_events(bem_prec* p, bool wait)
+{
+    size_t keep = 0;
+    for (size_t i = 0; i < p->hlu_ev.size(); i++) {
+        auto& e = p->hlu_ev[i];
+        if (!wait && hipEventQuery(e.second) != hipSuccess) {
+            p->hlu_ev[keep++] = e;
+            continue;
+        }
+        float ms = 0;
+        if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
+            p->apply_ms += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    p->hlu_ev.resize(keep);
+}
+
 int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const int64_t* clusters2, int64_t max_rank,
                            double rel_err, bem_prec_t* out)
 {
@@ -2949,14 +2969,7 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
 int bem_precond_stats(bem_prec_t p, double* out5)
 {
     if (!p || !out5) return BEM_ERR_ARG;
-    for (auto& e : p->hlu_ev) {   // device time of the H-LU applications so far
-        float ms = 0;
-        if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
-            p->apply_ms += ms;
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    p->hlu_ev.clear();
+    hlu_resolve_events(p, true);
     out5[0] = p->setup_ms;
     out5[1] = (double)p->applies;
     out5[2] = p->apply_ms;
@@ -2987,6 +3000,7 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
     if (!c || !p || !v || !z || p->ctx != c || v == z) return set_err(c, BEM_ERR_ARG, "bem_precond_apply: bad arguments");
     HIPC(c, hipSetDevice(c->device));
     if (p->hlu_nleaf > 0) {   // H-LU: z = v, then the two sweeps in place
+        if (p->hlu_ev.size() >= 256) hlu_resolve_events(p, false);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIPC(c, hipEventCreate(&e0));
         if (hipEventCreate(&e1) != hipSuccess) {

@@ -1192,138 +1192,231 @@ __device__ __forceinline__ int hlu_item_of(const int32_t* tb, int nq, int task)
     return lo;
 }
 
-__global__ __launch_bounds__(kHluThreads) void k_hlu_sweep(int dir, int64_t nleaf, const HluLeafD* __restrict__ leaf,
-                                                           const int64_t* __restrict__ tptr,
-                                                           const HluItemD* __restrict__ titem,
-                                                           const int64_t* __restrict__ cptr,
-                                                           const HluItemD* __restrict__ citem,
-                                                           const bem_c64* __restrict__ data, bem_c64* tbuf,
-                                                           bem_c64* gpart, bem_c64* z)
+// t vectors of the items [qb, qe): (item, rank index, chunk of kHluTChunk
+// columns) tasks over all wavefronts into gpart, then each t_B[l] sums its
+// chunks in chunk order
+__device__ void hlu_t_phase(const HluItemD* __restrict__ titem, int64_t qb, int64_t qe,
+                            const bem_c64* __restrict__ data, const bem_c64* z, bem_c64* tbuf, bem_c64* gpart,
+                            HluItemD* hdr, int32_t* tb)
 {
-    __shared__ bem_c64 part[kHluWaves][kHluMaxLeaf];
-    __shared__ bem_c64 acc[kHluMaxLeaf];
-    __shared__ HluItemD hdr[kHluHdr];
-    __shared__ int32_t tb[kHluHdr + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     auto t_tasks = [](const HluItemD& T) { return T.k * ((T.n + kHluTChunk - 1) / kHluTChunk); };
-    auto c_tasks = [](const HluItemD& C) { return ((C.k < 0 ? C.n : C.k) + kHluCols - 1) / kHluCols; };
-    for (int64_t step = 0; step < nleaf; step++) {
-        const int64_t k = dir == 0 ? step : nleaf - 1 - step;
-        const HluLeafD L = leaf[k];
-        const int m = (int)L.m;
-        // t vectors: (item, rank index, chunk of kHluTChunk columns) tasks over
-        // all wavefronts into gpart, then each t_B[l] sums its chunks in order
-        for (int64_t q0 = tptr[k]; q0 < tptr[k + 1]; q0 += kHluHdr) {
-            const int nq = (int)min((int64_t)kHluHdr, tptr[k + 1] - q0);
-            if (threadIdx.x < nq) hdr[threadIdx.x] = titem[q0 + threadIdx.x];
-            __syncthreads();
-            hlu_task_bases(hdr, nq, tb, t_tasks);
-            const int ntask = tb[nq];
-            for (int task = w; task < ntask; task += kHluWaves) {
-                const int i = hlu_item_of(tb, nq, task);
-                const HluItemD T = hdr[i];
-                const int nch = (T.n + kHluTChunk - 1) / kHluTChunk;
-                const int rel = task - tb[i], l = rel / nch, j0 = (rel - l * nch) * kHluTChunk;
-                const int len = min(kHluTChunk, T.n - j0);
-                const bem_c64* col = data + T.a + (int64_t)l * T.ld + j0;
-                const bem_c64* zc = z + T.c0 + j0;
-                bem_c64 wv[kHluTChunk / 64], xv[kHluTChunk / 64];
+    for (int64_t q0 = qb; q0 < qe; q0 += kHluHdr) {
+        const int nq = (int)min((int64_t)kHluHdr, qe - q0);
+        if (threadIdx.x < nq) hdr[threadIdx.x] = titem[q0 + threadIdx.x];
+        __syncthreads();
+        hlu_task_bases(hdr, nq, tb, t_tasks);
+        const int ntask = tb[nq];
+        for (int task = w; task < ntask; task += kHluWaves) {
+            const int i = hlu_item_of(tb, nq, task);
+            const HluItemD T = hdr[i];
+            const int nch = (T.n + kHluTChunk - 1) / kHluTChunk;
+            const int rel = task - tb[i], l = rel / nch, j0 = (rel - l * nch) * kHluTChunk;
+            const int len = min(kHluTChunk, T.n - j0);
+            const bem_c64* col = data + T.a + (int64_t)l * T.ld + j0;
+            const bem_c64* zc = z + T.c0 + j0;
+            bem_c64 wv[kHluTChunk / 64], xv[kHluTChunk / 64];
 #pragma unroll
-                for (int u = 0; u < kHluTChunk / 64; u++) {
-                    const int j = lane + 64 * u;
-                    wv[u] = j < len ? col[j] : c64(0, 0);
-                    xv[u] = j < len ? zc[j] : c64(0, 0);
-                }
-                bem_c64 v = c64(0, 0);
+            for (int u = 0; u < kHluTChunk / 64; u++) {
+                const int j = lane + 64 * u;
+                wv[u] = j < len ? col[j] : c64(0, 0);
+                xv[u] = j < len ? zc[j] : c64(0, 0);
+            }
+            bem_c64 v = c64(0, 0);
 #pragma unroll
-                for (int u = 0; u < kHluTChunk / 64; u++) v = c64_fma(wv[u], xv[u], v);
-                v = wave_sum(v);
-                if (lane == 0) gpart[task] = v;
-            }
-            __syncthreads();
-            if (threadIdx.x < nq) {
-                const HluItemD T = hdr[threadIdx.x];
-                const int nch = (T.n + kHluTChunk - 1) / kHluTChunk;
-                for (int l = 0; l < T.k; l++) {
-                    const bem_c64* g = gpart + tb[threadIdx.x] + l * nch;
-                    bem_c64 v = g[0];
-                    for (int ch = 1; ch < nch; ch++) v = c64_add(v, g[ch]);
-                    tbuf[T.t + l] = v;
-                }
-            }
-            __syncthreads();
-        }
-        // contributions: (item, kHluCols columns) tasks over all wavefronts
-        bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
-        for (int64_t q0 = cptr[k]; q0 < cptr[k + 1]; q0 += kHluHdr) {
-            const int nq = (int)min((int64_t)kHluHdr, cptr[k + 1] - q0);
-            if (threadIdx.x < nq) hdr[threadIdx.x] = citem[q0 + threadIdx.x];
-            __syncthreads();
-            hlu_task_bases(hdr, nq, tb, c_tasks);
-            const int ntask = tb[nq];
-            for (int task = w; task < ntask; task += kHluWaves) {
-                const int i = hlu_item_of(tb, nq, task);
-                const HluItemD C = hdr[i];
-                const int nc = C.k < 0 ? C.n : C.k;
-                const int j0 = (task - tb[i]) * kHluCols, nj = min(kHluCols, nc - j0);
-                const bem_c64* x = C.k < 0 ? z + C.c0 : tbuf + C.t;
-                const bem_c64* col = data + C.a;
-                if (m <= 64)
-                    hlu_cols<kHluCols, false>(col, C.ld, x, j0, nj, lane, m, a0, a1);
-                else
-                    for (int h = 0; h < kHluCols; h += kHluCols / 2)
-                        hlu_cols<kHluCols / 2, true>(col, C.ld, x, j0 + h, min(kHluCols / 2, nj - h), lane, m, a0,
-                                                     a1);
-            }
-            __syncthreads();
-        }
-        part[w][lane] = a0;
-        part[w][lane + 64] = a1;
-        __syncthreads();
-        if (threadIdx.x < m) {
-            bem_c64 v = z[L.s + threadIdx.x];
-            for (int g = 0; g < kHluWaves; g++) v = c64_sub(v, part[g][threadIdx.x]);
-            acc[threadIdx.x] = v;
+            for (int u = 0; u < kHluTChunk / 64; u++) v = c64_fma(wv[u], xv[u], v);
+            v = wave_sum(v);
+            if (lane == 0) gpart[task] = v;
         }
         __syncthreads();
-        const bem_c64* M = data + (dir == 0 ? L.qh : L.rinv);
-        bem_c64 y0 = c64(0, 0), y1 = c64(0, 0);
-        for (int j = w; j < m; j += kHluWaves) {
-            const bem_c64 av = acc[j];
-            if (lane < m) y0 = c64_fma(M[lane + j * m], av, y0);
-            if (lane + 64 < m) y1 = c64_fma(M[lane + 64 + j * m], av, y1);
-        }
-        part[w][lane] = y0;
-        part[w][lane + 64] = y1;
-        __syncthreads();
-        if (threadIdx.x < m) {
-            bem_c64 v = part[0][threadIdx.x];
-            for (int g = 1; g < kHluWaves; g++) v = c64_add(v, part[g][threadIdx.x]);
-            z[L.s + threadIdx.x] = v;
+        if (threadIdx.x < nq) {
+            const HluItemD T = hdr[threadIdx.x];
+            const int nch = (T.n + kHluTChunk - 1) / kHluTChunk;
+            for (int l = 0; l < T.k; l++) {
+                const bem_c64* g = gpart + tb[threadIdx.x] + l * nch;
+                bem_c64 v = g[0];
+                for (int ch = 1; ch < nch; ch++) v = c64_add(v, g[ch]);
+                tbuf[T.t + l] = v;
+            }
         }
         __syncthreads();
     }
 }
 
-// largest number of t tasks of one round of k_hlu_sweep (the size of gpart)
-int64_t hlu_max_t_tasks(const bemhlu::Flat& F)
+// a0 (rows lane) and a1 (rows lane + 64) of this wavefront += the
+// contributions of the items [qb, qe) to a leaf of m rows: (item, kHluCols
+// columns) tasks, task tau on wavefront tau mod 16
+__device__ void hlu_c_phase(const HluItemD* __restrict__ citem, int64_t qb, int64_t qe, int m,
+                            const bem_c64* __restrict__ data, const bem_c64* z, const bem_c64* tbuf, HluItemD* hdr,
+                            int32_t* tb, bem_c64& a0, bem_c64& a1)
 {
-    int64_t best = 1;
-    for (int d = 0; d < 2; d++)
-        for (size_t k = 0; k + 1 < F.tptr[d].size(); k++)
-            for (int64_t q0 = F.tptr[d][k]; q0 < F.tptr[d][k + 1]; q0 += kHluHdr) {
-                int64_t s = 0;
-                for (int64_t q = q0; q < std::min(q0 + kHluHdr, F.tptr[d][k + 1]); q++) {
-                    const auto& T = F.titem[d][q];
-                    s += (int64_t)T.k * ((T.n + kHluTChunk - 1) / kHluTChunk);
-                }
-                best = std::max(best, s);
-            }
-    return best;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    auto c_tasks = [](const HluItemD& C) { return ((C.k < 0 ? C.n : C.k) + kHluCols - 1) / kHluCols; };
+    for (int64_t q0 = qb; q0 < qe; q0 += kHluHdr) {
+        const int nq = (int)min((int64_t)kHluHdr, qe - q0);
+        if (threadIdx.x < nq) hdr[threadIdx.x] = citem[q0 + threadIdx.x];
+        __syncthreads();
+        hlu_task_bases(hdr, nq, tb, c_tasks);
+        const int ntask = tb[nq];
+        for (int task = w; task < ntask; task += kHluWaves) {
+            const int i = hlu_item_of(tb, nq, task);
+            const HluItemD C = hdr[i];
+            const int nc = C.k < 0 ? C.n : C.k;
+            const int j0 = (task - tb[i]) * kHluCols, nj = min(kHluCols, nc - j0);
+            const bem_c64* x = C.k < 0 ? z + C.c0 : tbuf + C.t;
+            const bem_c64* col = data + C.a;
+            if (m <= 64)
+                hlu_cols<kHluCols, false>(col, C.ld, x, j0, nj, lane, m, a0, a1);
+            else
+                for (int h = 0; h < kHluCols; h += kHluCols / 2)
+                    hlu_cols<kHluCols / 2, true>(col, C.ld, x, j0 + h, min(kHluCols / 2, nj - h), lane, m, a0, a1);
+        }
+        __syncthreads();
+    }
 }
 
-// every access k_hlu_sweep makes inside its buffers (checked before upload)
-bool hlu_flat_ok(const bemhlu::Flat& F, std::string& why)
+// z_k := z_k - (sum of the wavefronts' a0 / a1); MUL: then z_k := M z_k
+template <bool MUL>
+__device__ void hlu_finish(const HluLeafD& L, const bem_c64* __restrict__ M, bem_c64 a0, bem_c64 a1, bem_c64* z,
+                           bem_c64 (*part)[kHluMaxLeaf], bem_c64* acc)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, m = (int)L.m;
+    part[w][lane] = a0;
+    part[w][lane + 64] = a1;
+    __syncthreads();
+    if (threadIdx.x < m) {
+        bem_c64 v = z[L.s + threadIdx.x];
+        for (int g = 0; g < kHluWaves; g++) v = c64_sub(v, part[g][threadIdx.x]);
+        if (MUL) acc[threadIdx.x] = v;
+        else z[L.s + threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (!MUL) return;
+    bem_c64 y0 = c64(0, 0), y1 = c64(0, 0);
+    for (int j = w; j < m; j += kHluWaves) {
+        const bem_c64 av = acc[j];
+        if (lane < m) y0 = c64_fma(M[lane + j * m], av, y0);
+        if (lane + 64 < m) y1 = c64_fma(M[lane + 64 + j * m], av, y1);
+    }
+    part[w][lane] = y0;
+    part[w][lane + 64] = y1;
+    __syncthreads();
+    if (threadIdx.x < m) {
+        bem_c64 v = part[0][threadIdx.x];
+        for (int g = 1; g < kHluWaves; g++) v = c64_add(v, part[g][threadIdx.x]);
+        z[L.s + threadIdx.x] = v;
+    }
+    __syncthreads();
+}
+
+// MODE 0: one workgroup walks the leaves kb .. ke - 1 of the panel in the
+// direction's order (t vectors, contributions, z_k := M (z_k - sum)) with
+// the panel's inner item lists; MODE 1: workgroup b forms the t vectors of
+// the outer items of leaf kb + b (gpart + b gstride); MODE 2: workgroup b
+// subtracts the outer contributions of leaf kb + b.  Outer items only read
+// z outside the panel, solved before the panel starts.
+template <int MODE>
+__global__ __launch_bounds__(kHluThreads) void k_hlu(int dir, int64_t kb, int64_t ke,
+                                                     const HluLeafD* __restrict__ leaf,
+                                                     const int64_t* __restrict__ tptr,
+                                                     const HluItemD* __restrict__ titem,
+                                                     const int64_t* __restrict__ cptr,
+                                                     const HluItemD* __restrict__ citem,
+                                                     const bem_c64* __restrict__ data, bem_c64* tbuf, bem_c64* gpart,
+                                                     int64_t gstride, bem_c64* z)
+{
+    __shared__ bem_c64 part[kHluWaves][kHluMaxLeaf];
+    __shared__ bem_c64 acc[kHluMaxLeaf];
+    __shared__ HluItemD hdr[kHluHdr];
+    __shared__ int32_t tb[kHluHdr + 1];
+    if (MODE == 1) {
+        const int64_t k = kb + blockIdx.x;
+        hlu_t_phase(titem, tptr[k], tptr[k + 1], data, z, tbuf, gpart + (int64_t)blockIdx.x * gstride, hdr, tb);
+        return;
+    }
+    if (MODE == 2) {
+        const int64_t k = kb + blockIdx.x;
+        const HluLeafD L = leaf[k];
+        bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
+        hlu_c_phase(citem, cptr[k], cptr[k + 1], (int)L.m, data, z, tbuf, hdr, tb, a0, a1);
+        hlu_finish<false>(L, nullptr, a0, a1, z, part, acc);
+        return;
+    }
+    for (int64_t step = kb; step < ke; step++) {
+        const int64_t k = dir == 0 ? step : kb + ke - 1 - step;
+        const HluLeafD L = leaf[k];
+        hlu_t_phase(titem, tptr[k], tptr[k + 1], data, z, tbuf, gpart, hdr, tb);
+        bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
+        hlu_c_phase(citem, cptr[k], cptr[k + 1], (int)L.m, data, z, tbuf, hdr, tb, a0, a1);
+        hlu_finish<true>(L, data + (dir == 0 ? L.qh : L.rinv), a0, a1, z, part, acc);
+    }
+}
+
+// The flattened factors split by panels of consecutive diagonal leaves
+// (at least kHluPanelRows rows each): per direction, the items of every
+// leaf whose block columns lie wholly outside its panel on the solved side
+// (outer: before the panel for L, after it for U) and the rest (inner).
+constexpr int64_t kHluPanelRows = 512;
+struct HluSplit {
+    std::vector<std::pair<int64_t, int64_t>> panel;   // leaves [k0, k1)
+    std::vector<int64_t> tptr[2][2], cptr[2][2];      // [direction][outer]
+    std::vector<bemhlu::FlatItem> titem[2][2], citem[2][2];
+    std::vector<char> has_t[2], has_c[2];             // [direction][panel]: outer work present
+    int64_t maxleaves = 1, gstride = 1;
+};
+
+void hlu_split(const bemhlu::Flat& F, HluSplit& S)
+{
+    const int64_t K = (int64_t)F.leaf.size();
+    std::vector<int64_t> pan((size_t)K);
+    for (int64_t k0 = 0; k0 < K;) {
+        int64_t k1 = k0, rows = 0;
+        while (k1 < K && rows < kHluPanelRows) rows += F.leaf[k1++].m;
+        for (int64_t k = k0; k < k1; k++) pan[k] = (int64_t)S.panel.size();
+        S.panel.push_back({k0, k1});
+        S.maxleaves = std::max(S.maxleaves, k1 - k0);
+        k0 = k1;
+    }
+    auto rounds = [&](const std::vector<bemhlu::FlatItem>& it, int64_t q0, int64_t q1) {
+        for (int64_t r = q0; r < q1; r += kHluHdr) {
+            int64_t s = 0;
+            for (int64_t q = r; q < std::min(r + kHluHdr, q1); q++)
+                s += (int64_t)it[q].k * ((it[q].n + kHluTChunk - 1) / kHluTChunk);
+            S.gstride = std::max(S.gstride, s);
+        }
+    };
+    for (int d = 0; d < 2; d++) {
+        S.has_t[d].assign(S.panel.size(), 0);
+        S.has_c[d].assign(S.panel.size(), 0);
+        for (int o = 0; o < 2; o++) {
+            S.tptr[d][o].assign(1, 0);
+            S.cptr[d][o].assign(1, 0);
+        }
+        for (int64_t k = 0; k < K; k++) {
+            const auto& P = S.panel[pan[k]];
+            const int64_t lo = F.leaf[P.first].s, hi = F.leaf[P.second - 1].s + F.leaf[P.second - 1].m;
+            auto outer = [&](const bemhlu::FlatItem& I) { return d == 0 ? I.c0 + I.n <= lo : I.c0 >= hi; };
+            for (int64_t q = F.tptr[d][k]; q < F.tptr[d][k + 1]; q++) {
+                const int o = outer(F.titem[d][q]) ? 1 : 0;
+                S.titem[d][o].push_back(F.titem[d][q]);
+                if (o) S.has_t[d][pan[k]] = 1;
+            }
+            for (int64_t q = F.cptr[d][k]; q < F.cptr[d][k + 1]; q++) {
+                const int o = outer(F.citem[d][q]) ? 1 : 0;
+                S.citem[d][o].push_back(F.citem[d][q]);
+                if (o) S.has_c[d][pan[k]] = 1;
+            }
+            for (int o = 0; o < 2; o++) {
+                S.tptr[d][o].push_back((int64_t)S.titem[d][o].size());
+                S.cptr[d][o].push_back((int64_t)S.citem[d][o].size());
+                rounds(S.titem[d][o], S.tptr[d][o][k], S.tptr[d][o][k + 1]);
+            }
+        }
+    }
+}
+
+// every access k_hlu makes inside its buffers (checked before upload)
+bool hlu_split_ok(const bemhlu::Flat& F, const HluSplit& S, std::string& why)
 {
     const int64_t N = F.n, D = (int64_t)F.data.size(), T = F.tsize, K = (int64_t)F.leaf.size();
     auto bad = [&](const std::string& w) {
@@ -1341,28 +1434,35 @@ bool hlu_flat_ok(const bemhlu::Flat& F, std::string& why)
         if (rows <= 0 || cols <= 0) return a >= 0;
         return a >= 0 && ld >= rows && a + (rows - 1) + (cols - 1) * ld < D;
     };
-    for (int d = 0; d < 2; d++) {
-        const auto &tp = F.tptr[d], &cp = F.cptr[d];
-        if ((int64_t)tp.size() != K + 1 || (int64_t)cp.size() != K + 1 || tp[0] != 0 || cp[0] != 0 ||
-            tp[K] != (int64_t)F.titem[d].size() || cp[K] != (int64_t)F.citem[d].size())
-            return bad("item lists");
-        for (int64_t k = 0; k < K; k++) {
-            if (tp[k + 1] < tp[k] || cp[k + 1] < cp[k]) return bad("item lists");
-            const int64_t m = F.leaf[k].m;
-            for (int64_t q = tp[k]; q < tp[k + 1]; q++) {
-                const auto& I = F.titem[d][q];
-                if (I.k < 0 || I.n < 0 || I.c0 < 0 || I.c0 + I.n > N || I.t < 0 || I.t + I.k > T ||
-                    !inside(I.a, I.ld, I.n, I.k))
-                    return bad("t item outside the factors");
-            }
-            for (int64_t q = cp[k]; q < cp[k + 1]; q++) {
-                const auto& I = F.citem[d][q];
-                const bool ok = I.k < 0 ? I.n >= 0 && I.c0 >= 0 && I.c0 + I.n <= N && inside(I.a, I.ld, m, I.n)
-                                        : I.t >= 0 && I.t + I.k <= T && inside(I.a, I.ld, m, I.k);
-                if (!ok) return bad("contribution outside the factors");
+    for (const auto& P : S.panel)
+        if (P.first < 0 || P.second > K || P.first >= P.second) return bad("panels");
+    for (int d = 0; d < 2; d++)
+        for (int o = 0; o < 2; o++) {
+            const auto &tp = S.tptr[d][o], &cp = S.cptr[d][o];
+            if ((int64_t)tp.size() != K + 1 || (int64_t)cp.size() != K + 1 || tp[0] != 0 || cp[0] != 0 ||
+                tp[K] != (int64_t)S.titem[d][o].size() || cp[K] != (int64_t)S.citem[d][o].size())
+                return bad("item lists");
+            for (int64_t k = 0; k < K; k++) {
+                if (tp[k + 1] < tp[k] || cp[k + 1] < cp[k]) return bad("item lists");
+                const int64_t m = F.leaf[k].m;
+                int64_t round = 0;
+                for (int64_t q = tp[k]; q < tp[k + 1]; q++) {
+                    const auto& I = S.titem[d][o][q];
+                    if (I.k < 0 || I.n < 0 || I.c0 < 0 || I.c0 + I.n > N || I.t < 0 || I.t + I.k > T ||
+                        !inside(I.a, I.ld, I.n, I.k))
+                        return bad("t item outside the factors");
+                    if ((q - tp[k]) % kHluHdr == 0) round = 0;
+                    round += (int64_t)I.k * ((I.n + kHluTChunk - 1) / kHluTChunk);
+                    if (round > S.gstride) return bad("t tasks of a round");
+                }
+                for (int64_t q = cp[k]; q < cp[k + 1]; q++) {
+                    const auto& I = S.citem[d][o][q];
+                    const bool ok = I.k < 0 ? I.n >= 0 && I.c0 >= 0 && I.c0 + I.n <= N && inside(I.a, I.ld, m, I.n)
+                                            : I.t >= 0 && I.t + I.k <= T && inside(I.a, I.ld, m, I.k);
+                    if (!ok) return bad("contribution outside the factors");
+                }
             }
         }
-    }
     return true;
 }
 
@@ -1565,10 +1665,13 @@ struct bem_prec {
     // H-LU (bem_precond_create_hlu): the flattened factors on the device
     int64_t hlu_nleaf = 0;
     HluLeafD* hlu_leaf = nullptr;
-    int64_t* hlu_tptr[2] = {nullptr, nullptr};
-    int64_t* hlu_cptr[2] = {nullptr, nullptr};
-    HluItemD* hlu_titem[2] = {nullptr, nullptr};
-    HluItemD* hlu_citem[2] = {nullptr, nullptr};
+    int64_t* hlu_tptr[2][2] = {};      // [direction][outer]
+    int64_t* hlu_cptr[2][2] = {};
+    HluItemD* hlu_titem[2][2] = {};
+    HluItemD* hlu_citem[2][2] = {};
+    std::vector<std::pair<int64_t, int64_t>> hlu_panel;
+    std::vector<char> hlu_has_t[2], hlu_has_c[2];
+    int64_t hlu_gstride = 1;
     bem_c64* hlu_data = nullptr;
     bem_c64* hlu_t = nullptr;
     bem_c64* hlu_gpart = nullptr;   // chunk sums of the t vectors of one round
@@ -2929,8 +3032,10 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
         return set_err(c, BEM_ERR_ARG, "bem_precond_create_hlu: " + e.msg);
     }
     bem_hlu_destroy(L);
+    HluSplit S;
+    hlu_split(F, S);
     std::string why;
-    if (!hlu_flat_ok(F, why)) return set_err(c, BEM_ERR_ARG, "bem_precond_create_hlu: " + why);
+    if (!hlu_split_ok(F, S, why)) return set_err(c, BEM_ERR_ARG, "bem_precond_create_hlu: " + why);
     auto p = new bem_prec();
     p->ctx = c;
     auto up = [&](void** dst, const void* src, size_t bytes) -> int {
@@ -2942,15 +3047,25 @@ int bem_precond_create_hlu(bem_ctx_t c, bem_hmat_t h, int64_t nclusters, const i
     };
     p->hlu_nleaf = (int64_t)F.leaf.size();
     rc = up((void**)&p->hlu_leaf, F.leaf.data(), F.leaf.size() * sizeof(HluLeafD));
-    for (int d = 0; d < 2 && !rc; d++) {
-        if (!rc) rc = up((void**)&p->hlu_tptr[d], F.tptr[d].data(), F.tptr[d].size() * sizeof(int64_t));
-        if (!rc) rc = up((void**)&p->hlu_cptr[d], F.cptr[d].data(), F.cptr[d].size() * sizeof(int64_t));
-        if (!rc) rc = up((void**)&p->hlu_titem[d], F.titem[d].data(), F.titem[d].size() * sizeof(HluItemD));
-        if (!rc) rc = up((void**)&p->hlu_citem[d], F.citem[d].data(), F.citem[d].size() * sizeof(HluItemD));
+    for (int d = 0; d < 2 && !rc; d++)
+        for (int o = 0; o < 2 && !rc; o++) {
+            if (!rc) rc = up((void**)&p->hlu_tptr[d][o], S.tptr[d][o].data(), S.tptr[d][o].size() * sizeof(int64_t));
+            if (!rc) rc = up((void**)&p->hlu_cptr[d][o], S.cptr[d][o].data(), S.cptr[d][o].size() * sizeof(int64_t));
+            if (!rc)
+                rc = up((void**)&p->hlu_titem[d][o], S.titem[d][o].data(), S.titem[d][o].size() * sizeof(HluItemD));
+            if (!rc)
+                rc = up((void**)&p->hlu_citem[d][o], S.citem[d][o].data(), S.citem[d][o].size() * sizeof(HluItemD));
+        }
+    p->hlu_panel = S.panel;
+    for (int d = 0; d < 2; d++) {
+        p->hlu_has_t[d] = S.has_t[d];
+        p->hlu_has_c[d] = S.has_c[d];
     }
+    p->hlu_gstride = S.gstride;
     if (!rc) rc = up((void**)&p->hlu_data, F.data.data(), F.data.size() * sizeof(bem_c64));
     if (!rc) rc = up((void**)&p->hlu_t, nullptr, (size_t)F.tsize * sizeof(bem_c64));   // written before read
-    if (!rc) rc = up((void**)&p->hlu_gpart, nullptr, (size_t)hlu_max_t_tasks(F) * sizeof(bem_c64));   // written before read
+    if (!rc)   // written before read
+        rc = up((void**)&p->hlu_gpart, nullptr, (size_t)(S.maxleaves * S.gstride) * sizeof(bem_c64));
     if (!rc) {
         hipError_t e = hipStreamSynchronize(c->stream);   // the host buffers go out of scope
         if (e != hipSuccess) rc = set_err(c, BEM_ERR_HIP, std::string("bem_precond_create_hlu: ") + hipGetErrorString(e));
@@ -3010,13 +3125,27 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
         p->hlu_ev.push_back({e0, e1});
         HIPC(c, hipEventRecord(e0, c->stream));
         HIPC(c, hipMemcpyAsync(z, v, sizeof(bem_c64) * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+        const int64_t np = (int64_t)p->hlu_panel.size();
         for (int d = 0; d < 2; d++)
-            hipLaunchKernelGGL(k_hlu_sweep, dim3(1), dim3(kHluThreads), 0, c->stream, d, p->hlu_nleaf, p->hlu_leaf,
-                               p->hlu_tptr[d], p->hlu_titem[d], p->hlu_cptr[d], p->hlu_citem[d], p->hlu_data,
-                               p->hlu_t, p->hlu_gpart, (bem_c64*)z);
+            for (int64_t i = 0; i < np; i++) {   // panels in the direction's order
+                const int64_t q = d == 0 ? i : np - 1 - i;
+                const int64_t k0 = p->hlu_panel[q].first, k1 = p->hlu_panel[q].second;
+                const dim3 g((unsigned)(k1 - k0)), b(kHluThreads);
+                if (p->hlu_has_t[d][q])
+                    hipLaunchKernelGGL(k_hlu<1>, g, b, 0, c->stream, d, k0, k1, p->hlu_leaf, p->hlu_tptr[d][1],
+                                       p->hlu_titem[d][1], p->hlu_cptr[d][1], p->hlu_citem[d][1], p->hlu_data,
+                                       p->hlu_t, p->hlu_gpart, p->hlu_gstride, (bem_c64*)z);
+                if (p->hlu_has_c[d][q])
+                    hipLaunchKernelGGL(k_hlu<2>, g, b, 0, c->stream, d, k0, k1, p->hlu_leaf, p->hlu_tptr[d][1],
+                                       p->hlu_titem[d][1], p->hlu_cptr[d][1], p->hlu_citem[d][1], p->hlu_data,
+                                       p->hlu_t, p->hlu_gpart, p->hlu_gstride, (bem_c64*)z);
+                hipLaunchKernelGGL(k_hlu<0>, dim3(1), b, 0, c->stream, d, k0, k1, p->hlu_leaf, p->hlu_tptr[d][0],
+                                   p->hlu_titem[d][0], p->hlu_cptr[d][0], p->hlu_citem[d][0], p->hlu_data, p->hlu_t,
+                                   p->hlu_gpart, p->hlu_gstride, (bem_c64*)z);
+            }
         HIPC(c, hipEventRecord(e1, c->stream));
         p->applies++;
-        return launch_check(c, "k_hlu_sweep");
+        return launch_check(c, "k_hlu");
     }
     p->applies++;   // asynchronous: its time is part of the GMRES iterations
     hipLaunchKernelGGL(k_blu_apply, dim3((unsigned)p->nblk), dim3(64), 0, c->stream, p->d_blk, p->d_lu, p->d_piv,

@@ -1015,7 +1015,7 @@ void HLU::flatten(Flat& F) const
         F.tsize += r;
         tat[d][lower ? k0 : k1].push_back(FlatItem{aw, B.n, B.c0, t, (int32_t)B.n, (int32_t)r});
         for (int64_t k = k0; k <= k1; k++)
-            cat[d][k].push_back(FlatItem{au + F.leaf[k].s - B.r0, B.m, 0, t, 0, (int32_t)r});
+            cat[d][k].push_back(FlatItem{au + F.leaf[k].s - B.r0, B.m, B.c0, t, (int32_t)B.n, (int32_t)r});
     }
     for (int d = 0; d < 2; d++) {
         F.tptr[d].assign(1, 0);

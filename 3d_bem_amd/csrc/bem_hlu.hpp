@@ -74,9 +74,9 @@ struct FlatLeaf {
 struct FlatItem {
     int64_t a;     // offset in data: F or U at the leaf's first row, or W
     int64_t ld;    // leading dimension of that matrix
-    int64_t c0;    // first column of the block (full contribution, t item)
+    int64_t c0;    // first column of the block
     int64_t t;     // offset of t_B in the t buffer (low-rank contribution, t item)
-    int32_t n;     // columns of F (full contribution) or of the block (t item); 0 for a low-rank contribution
+    int32_t n;     // columns of the block (of F for a full contribution)
     int32_t k;     // rank (low-rank contribution, t item); -1 for a full contribution
 };
 struct Flat {

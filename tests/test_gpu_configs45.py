@@ -86,7 +86,7 @@ def test_capsule76k_hlu_preconditioned(bem, meshes):
     """The same capsule (FP32, ACA 1e-4, GMRES 1e-8) preconditioned by the
     H-LU with the reference's defaults (rank 2, no tolerance): factorised on
     the host from the GPU-assembled blocks, substituted on the device
-    (k_hlu_sweep).  Same pressure as with the leaf-block LU, a fraction of
+    (k_hlu).  Same pressure as with the leaf-block LU, a fraction of
     its iterations."""
     f, c, rho = 1500.0, 343.0, 1.21
     k = 2 * np.pi * f / c

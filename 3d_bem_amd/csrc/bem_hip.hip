@@ -1342,13 +1342,59 @@ __global__ __launch_bounds__(kHluThreads) void k_hlu(int dir, int64_t kb, int64_
         hlu_finish<false>(L, nullptr, a0, a1, z, part, acc);
         return;
     }
+    // the next leaf's header and item ranges are loaded one step ahead
+    int64_t k = dir == 0 ? kb : ke - 1;
+    HluLeafD L = leaf[k];
+    int64_t t0 = tptr[k], t1 = tptr[k + 1], c0 = cptr[k], c1 = cptr[k + 1];
     for (int64_t step = kb; step < ke; step++) {
-        const int64_t k = dir == 0 ? step : kb + ke - 1 - step;
-        const HluLeafD L = leaf[k];
-        hlu_t_phase(titem, tptr[k], tptr[k + 1], data, z, tbuf, gpart, hdr, tb);
-        bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
-        hlu_c_phase(citem, cptr[k], cptr[k + 1], (int)L.m, data, z, tbuf, hdr, tb, a0, a1);
-        hlu_finish<true>(L, data + (dir == 0 ? L.qh : L.rinv), a0, a1, z, part, acc);
+        const int64_t kn = step + 1 < ke ? (dir == 0 ? step + 1 : kb + ke - 2 - step) : k;
+        const HluLeafD Ln = leaf[kn];
+        const int64_t t0n = tptr[kn], t1n = tptr[kn + 1], c0n = cptr[kn], c1n = cptr[kn + 1];
+        const int m = (int)L.m;
+        const bem_c64* M = data + (dir == 0 ? L.qh : L.rinv);
+        if (m <= 64) {   // Q^H / R^-1 columns w, w + 16, .. and z_k ahead of the contributions
+            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            bem_c64 mv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = w + kHluWaves * u;
+                mv[u] = lane < m && j < m ? M[lane + j * m] : c64(0, 0);
+            }
+            const bem_c64 zk = threadIdx.x < m ? z[L.s + threadIdx.x] : c64(0, 0);
+            hlu_t_phase(titem, t0, t1, data, z, tbuf, gpart, hdr, tb);
+            bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
+            hlu_c_phase(citem, c0, c1, m, data, z, tbuf, hdr, tb, a0, a1);
+            part[w][lane] = a0;
+            __syncthreads();
+            if (threadIdx.x < m) {
+                bem_c64 v = zk;
+                for (int g = 0; g < kHluWaves; g++) v = c64_sub(v, part[g][threadIdx.x]);
+                acc[threadIdx.x] = v;
+            }
+            __syncthreads();
+            bem_c64 y0 = c64(0, 0);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = w + kHluWaves * u;
+                if (j < m) y0 = c64_fma(mv[u], acc[j], y0);
+            }
+            part[w][lane] = y0;
+            __syncthreads();
+            if (threadIdx.x < m) {
+                bem_c64 v = part[0][threadIdx.x];
+                for (int g = 1; g < kHluWaves; g++) v = c64_add(v, part[g][threadIdx.x]);
+                z[L.s + threadIdx.x] = v;
+            }
+            __syncthreads();
+        } else {
+            hlu_t_phase(titem, t0, t1, data, z, tbuf, gpart, hdr, tb);
+            bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
+            hlu_c_phase(citem, c0, c1, m, data, z, tbuf, hdr, tb, a0, a1);
+            hlu_finish<true>(L, M, a0, a1, z, part, acc);
+        }
+        k = kn;
+        L = Ln;
+        t0 = t0n; t1 = t1n; c0 = c0n; c1 = c1n;
     }
 }
 

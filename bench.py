@@ -24,7 +24,7 @@ the FP64 CPU restatement run on the host (cpu_baseline leg).
 boundaryelementsolver.cpp:549-557, 594-604): every entry then carries the
 mirror image's regular quadrature as well.
 
-Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 is refused: the path runs on one GPU.
 """
 import argparse
 import ctypes
@@ -305,7 +305,7 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
+        print(json.dumps({"error": "the path runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
         sys.exit(2)
     if args.field:
         run_field(args)

@@ -37,7 +37,7 @@ constexpr int kTile = 64;       // dense tile edge
 constexpr int kCapFirst = 48;   // first-pass ACA rank capacity per block
 constexpr int kCapMax = 1024;   // second-pass capacity limit (LDS: (3 cap + 2) x 16 B <= 48 KiB)
 // ACA workgroup size classes by m + n, per storage precision [FP64, FP32]:
-// up to kSmallBlock one wavefront, up to kHugeBlock four, above eight
+// up to kSmallBlock one wavefront, up to kHugeBlock kMidThreads, above kHugeThreads
 constexpr int64_t kSmallBlock[2] = {384, 384};
 constexpr int64_t kHugeBlock[2] = {(int64_t)1 << 40, 2048};
 constexpr double kTiny = 10.0 * 2.220446049250313e-16;  // global.h:53
@@ -759,6 +759,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
 // FP32 ACA limited to 128 VGPRs: four wavefronts per SIMD at every size
 template <int NT, class C> constexpr int kAcaWaves = 1;
 template <> constexpr int kAcaWaves<64, bem_c32> = 4;
+template <> constexpr int kAcaWaves<128, bem_c32> = 4;
 
 template <bool EXTRA, int NT, class C, bool PLANE>
 __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
@@ -2408,7 +2409,12 @@ struct AcaScratch {
 };
 
 // workgroup of the largest class (FP64 never forms it: kHugeBlock[0])
-template <class C> constexpr int kHugeThreads = 512;
+template <class C> constexpr int kHugeThreads = 256;
+// workgroup of the middle class: FP32 two wavefronts (several lanes share the
+// chip), FP64 four (its points run one after the other, so one block's pivot
+// chain matters more)
+template <class C> constexpr int kMidThreads = 128;
+template <> constexpr int kMidThreads<bem_c64> = 256;
 
 template <int NT, class C>
 void launch_aca_nt(bool extra, bool plane, dim3 g, size_t shm, hipStream_t st, const AcaArgs& aq)
@@ -2429,7 +2435,7 @@ void launch_aca(int q, bool extra, unsigned nblocks, size_t shm, hipStream_t st,
     const dim3 g(nblocks);
     const bool plane = aq.M.rvtx != nullptr;
     if (q == 0) launch_aca_nt<kHugeThreads<C>, C>(extra, plane, g, shm, st, aq);
-    else if (q == 1) launch_aca_nt<256, C>(extra, plane, g, shm, st, aq);
+    else if (q == 1) launch_aca_nt<kMidThreads<C>, C>(extra, plane, g, shm, st, aq);
     else launch_aca_nt<64, C>(extra, plane, g, shm, st, aq);
 }
 
@@ -2457,8 +2463,11 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     S.owned.push_back(S.V);
     if ((rc = dalloc_bytes(c, &S.A, ta * esz))) return rc;
     S.owned.push_back(S.A);
-    // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock four,
-    // larger eight.  The huge and large classes run on their own streams so
+    // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock
+    // kMidThreads, larger kHugeThreads (measured: with several lanes sharing
+    // the chip, smaller workgroups than one block's pivot chain alone would
+    // want give the higher rate).  The huge and large classes run on their
+    // own streams so
     // that their long tails overlap the small blocks (and the caller's dense
     // tiles); within each class the largest blocks start first.
     std::vector<int32_t> cls[3];

@@ -22,7 +22,9 @@ the FP64 CPU restatement run on the host (cpu_baseline leg).
 --plane D puts a rigid impedance plane D metres below the sphere's centre
 (the plane z = -D, the single-plane branch of hMatrixSolve,
 boundaryelementsolver.cpp:549-557, 594-604): every entry then carries the
-mirror image's regular quadrature as well.
+mirror image's regular quadrature as well.  --field NOBS times the field
+evaluation at NOBS exterior points instead, --mvm the H-MVM of the assembled
+DPhi operator (FP32 and FP64 storage, against the HBM peak).
 
 Prints one JSON line.  --gpus N > 1 is refused: the path runs on one GPU.
 """
@@ -159,6 +161,60 @@ def run_field(args):
     print(json.dumps(out))
 
 
+def run_mvm(args):
+    """--mvm: H-MVM throughput of the DPhi operator at 1000 Hz, FP32 and FP64
+    storage: one single-lane solve per precision (--steps times, after
+    --warmup) with the kernel families timed by HIP events on the context
+    stream; an MVM = near-field tiles + low-rank phases + gather (the GMRES
+    MVMs dominate the counts; the one right-hand-side MVM with Phi is of the
+    same size).  Bytes per MVM: the operator's storage plus three n-vectors
+    of complex FP64 (x, y and the partial sums are re-read from cache)."""
+    bem, tri, bc = workload(args.level)
+    n = len(tri)
+    S = bem.Solver(tri, bc, leaf=args.leaf, eta=args.eta)
+    ctx_h = bem.lib().bem_solver_context(S.h)
+    res = {}
+    for name, prec in (("fp32", bem.FP32), ("fp64", bem.FP64)):
+        def solve():
+            return S.solve(args.freq, c=343.0, rho=1.21, coupling=1, aca_tol=args.aca_tol,
+                           gmres_tol=args.gmres_tol, seed=args.seed, precision=prec)
+        for _ in range(args.warmup):
+            solve()
+        prof = np.zeros(14)
+        bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(ctypes.c_void_p))
+        for _ in range(args.steps):
+            _, _, _, st = solve()
+        bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(ctypes.c_void_p))
+        avg, cnt = prof[:7], prof[7:]
+        ms = float(avg[3] + avg[4] + avg[5])   # dense tiles, low-rank phases, gather
+        nbytes = float(st["dphi_bytes"]) + 3.0 * 16.0 * n
+        res[name] = {"ms_per_mvm": ms, "mvms": int(cnt[5]), "bytes": nbytes,
+                     "gbs": nbytes / (ms * 1e-3) / 1e9, "family_ms": [round(float(avg[q]), 4) for q in (3, 4, 5)]}
+    r = res["fp32"]
+    out = {
+        "metric": "H-MVMs/s of the DPhi operator at 20k tris (FP32 storage)",
+        "value": round(1e3 / r["ms_per_mvm"], 1), "unit": "MVM/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_mvm"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64 storage, f64 vectors",
+        "data": "synthetic",
+        "config": {"workload": f"DPhi H-matrix of the pulsating sphere, icosphere level {args.level} ({n} triangles), "
+                               f"{args.freq:g} Hz, leaf {args.leaf}, eta {args.eta:g}, ACA {args.aca_tol:g}, "
+                               "one solve per step, one lane",
+                   "triangles": n, "mvms_timed": r["mvms"],
+                   "family_ms_dense_lowrank_gather": r["family_ms"],
+                   "fp64": {"mvm_per_s": round(1e3 / res["fp64"]["ms_per_mvm"], 1),
+                            "ms_per_mvm": round(res["fp64"]["ms_per_mvm"], 4),
+                            "bytes_per_mvm": res["fp64"]["bytes"], "gbs": round(res["fp64"]["gbs"], 1),
+                            "family_ms_dense_lowrank_gather": res["fp64"]["family_ms"]}},
+        "roofline": {"bound": "hbm", "achieved": round(r["gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(r["gbs"] / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "mvm",
+                     "avg_ms": round(r["ms_per_mvm"], 4),
+                     "algorithmic_per_launch": f"{r['bytes']:.4g} B (DPhi storage + 3 n-vectors)"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def batch_freqs(center, batch):
     """batch points over [0.96, 1.04] x center with center itself included"""
     if batch == 1:
@@ -291,6 +347,8 @@ def main():
                     help="rigid impedance plane D m below the sphere's centre (default: none)")
     ap.add_argument("--field", type=int, default=0, metavar="NOBS",
                     help="time the field evaluation at NOBS points instead of the sweep")
+    ap.add_argument("--mvm", action="store_true",
+                    help="time the H-MVM of the assembled DPhi operator (FP32 and FP64 storage) instead of the sweep")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -309,6 +367,9 @@ def main():
         sys.exit(2)
     if args.field:
         run_field(args)
+        return
+    if args.mvm:
+        run_mvm(args)
         return
 
     bem, tri, bc = workload(args.level)

@@ -812,10 +812,13 @@ __global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tile
 }
 
 // low-rank blocks, phase 1 (K5a): t_b[l] = alpha_l (V_b[l, :] . x_J), one
-// wavefront per (block, l0) task covering rows l0 .. l0 + kLrRows - 1 of V_b:
-// each x element is loaded once for the group and the rows' loads are in
-// flight together
+// wavefront per (block, l0) task covering rows l0 .. l0 + kLrRows - 1 of V_b.
+// FP64 storage: every row over all 64 lanes (lane-strided sums and the
+// wavefront tree, the order the restatement follows), each x element loaded
+// once for the group.  FP32 storage: 16 lanes per row with four loads in
+// flight per lane and a 16-lane tree (a sixth of the shuffles).
 constexpr int kLrRows = 4;
+static_assert(kLrRows * 16 == 64, "FP32 k_lr_t: 16 lanes per row");
 template <class S>
 __global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks, int64_t ntask,
                                               const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
@@ -832,6 +835,32 @@ __global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks,
     const int nl = min(kLrRows, rank[b] - l0);
     const S* v = V + voff[b] + (int64_t)l0 * n;
     const bem_c64* xs = x + col0[b];
+    if constexpr (sizeof(S) == sizeof(bem_c32)) {
+        const int u = lane >> 4, q = lane & 15;   // row l0 + u, columns q, q + 16, ...
+        bem_c64 p = c64(0, 0);
+        if (u < nl) {
+            const S* vr = v + (int64_t)u * n;
+            int64_t j = q;
+            for (; j + 48 < n; j += 64) {
+                S vv[4];
+                bem_c64 xv[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    vv[e] = vr[j + 16 * e];
+                    xv[e] = xs[j + 16 * e];
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) p = c64_fma(ld64(&vv[e]), xv[e], p);
+            }
+            for (; j < n; j += 16) p = c64_fma(ld64(vr + j), xs[j], p);
+        }
+        for (int s = 8; s > 0; s >>= 1) {
+            p.re = p.re + __shfl_xor(p.re, s, 16);
+            p.im = p.im + __shfl_xor(p.im, s, 16);
+        }
+        if (q == 0 && u < nl) t[toff[b] + l0 + u] = c64_mul(ld64(A + aoff[b] + l0 + u), p);
+        return;
+    }
     bem_c64 p[kLrRows];
 #pragma unroll
     for (int u = 0; u < kLrRows; u++) p[u] = c64(0, 0);

@@ -2753,9 +2753,29 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         sumr += r;
         evaluated += evs[b];
     }
-    if ((rc = dalloc_bytes(c, &h->U, (size_t)tu * h->esz))) return fail(rc);
-    if ((rc = dalloc_bytes(c, &h->V, (size_t)tv * h->esz))) return fail(rc);
-    if ((rc = dalloc_bytes(c, &h->A, (size_t)ta * h->esz))) return fail(rc);
+    // When no block overflowed its first capacity and every block has rank >= 1,
+    // the first pass's scratch is the storage: its layout per block is the
+    // packed one (rank rows of U, then of V, then alpha), with the unused
+    // capacity left between blocks.  Otherwise the rank rows are copied into
+    // packed storage (k_compact; a rank-0 block becomes a zero rank-1 block).
+    bool direct = over.empty();
+    for (int32_t b : keep)
+        if (rank[b] < 1) direct = false;
+    if (direct) {
+        for (int32_t b : c->adm_list) {
+            h->uoff[b] = S1.uoff[b];
+            h->voff[b] = S1.voff[b];
+            h->aoff[b] = S1.aoff[b];
+        }
+        h->U = S1.U; h->V = S1.V; h->A = S1.A;
+        auto& o = S1.owned;
+        o.erase(std::remove_if(o.begin(), o.end(), [&](void* q) { return q == h->U || q == h->V || q == h->A; }),
+                o.end());
+    } else {
+        if ((rc = dalloc_bytes(c, &h->U, (size_t)tu * h->esz))) return fail(rc);
+        if ((rc = dalloc_bytes(c, &h->V, (size_t)tv * h->esz))) return fail(rc);
+        if ((rc = dalloc_bytes(c, &h->A, (size_t)ta * h->esz))) return fail(rc);
+    }
     // offsets, ranks, MVM task lists and the packing lists: one upload
     std::vector<int32_t>& rk = rank;   // first-pass ranks with the re-run ranks merged in
     std::vector<void*> own2;
@@ -2810,8 +2830,8 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
                                (bem_c64*)h->V, (bem_c64*)h->A);
         return launch_check(c, "k_compact");
     };
-    if ((rc = compact(keep, pk.ptr<int32_t>(k_keep), S1))) return fail(rc);
-    if ((rc = compact(over, pk.ptr<int32_t>(k_over), S2))) return fail(rc);
+    if (!direct && (rc = compact(keep, pk.ptr<int32_t>(k_keep), S1))) return fail(rc);
+    if (!direct && (rc = compact(over, pk.ptr<int32_t>(k_over), S2))) return fail(rc);
     HIPC(c, hipEventRecord(ev2, c->stream));
     // NaN check of the dense part (the reference reports NaN blocks,
     // hmatrix.cpp:812-820)

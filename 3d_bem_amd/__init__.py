@@ -272,9 +272,10 @@ class HMatrix:
             pass
 
     def stats(self):
-        s = np.zeros(8)
+        s = np.zeros(9)
         lib().bem_hmat_stats(self.h, _ptr(s))
-        keys = ["evaluated", "max_rank", "mean_rank", "bytes", "assembly_ms", "overflow_blocks", "dense_ms", "aca_ms"]
+        keys = ["evaluated", "max_rank", "mean_rank", "bytes", "assembly_ms", "overflow_blocks", "dense_ms", "aca_ms",
+                "resident_bytes"]
         return dict(zip(keys, s.tolist()))
 
     def rank(self, b):

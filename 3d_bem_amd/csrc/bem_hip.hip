@@ -1729,7 +1729,7 @@ struct bem_hmat {
     int maxrank = 1;
     std::vector<int64_t> uoff, voff, aoff;
     std::vector<int32_t> rank;
-    double stats[8] = {0};
+    double stats[9] = {0};
 };
 
 struct bem_prec {
@@ -2427,6 +2427,7 @@ struct AcaScratch {
     int32_t *d_cap = nullptr, *d_rank = nullptr, *d_flag = nullptr, *d_list = nullptr;
     int64_t* d_ev = nullptr;
     std::vector<int64_t> uoff, voff, aoff;
+    int64_t tu = 0, tv = 0, ta = 0;   // allocated entries of U, V, alpha
     std::vector<void*> owned;
     bem_ctx* ctx = nullptr;
     ~AcaScratch()
@@ -2492,6 +2493,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     S.owned.push_back(S.V);
     if ((rc = dalloc_bytes(c, &S.A, ta * esz))) return rc;
     S.owned.push_back(S.A);
+    S.tu = tu; S.tv = tv; S.ta = ta;
     // size classes: m + n <= kSmallBlock one wavefront, <= kHugeBlock
     // kMidThreads, larger kHugeThreads (measured: with several lanes sharing
     // the chip, smaller workgroups than one block's pivot chain alone would
@@ -2864,6 +2866,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->stats[5] = (double)over.size();
     h->stats[6] = ms_dense;
     h->stats[7] = ms_aca;
+    h->stats[8] = (double)((c->dense_total + (direct ? S1.tu + S1.tv + S1.ta : tu + tv + ta)) * (int64_t)h->esz);
     for (auto* v : {&cap, &rank, &flag, &rank2, &flag2, &tt, &ut}) c->vp.give(*v);
     for (auto* v : {&evs, &evs2, &toff}) c->vp.give(*v);
     if (pairing) {   // the DPhi near field waits for the next DPhi assembly of this state

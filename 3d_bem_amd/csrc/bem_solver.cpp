@@ -120,7 +120,7 @@ void HMatrix::clear()
 }
 double HMatrix::stat(int q) const
 {
-    double s[8] = {0};
+    double s[9] = {0};
     if (h_) bem_hmat_stats(h_, s);
     return s[q];
 }
@@ -675,9 +675,14 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 const int64_t o = perm[e];
                 rhs[k].g[e] = g ? bem::cplx(g[2 * (k * n + o)], g[2 * (k * n + o) + 1]) : s->input.bc[o].g;
             }
+            // the solver's point sources belong to every right-hand side; its
+            // plane wave too unless planes5 gives each its own
+            rhs[k].pointSources = s->s->pointSources;
             if (planes5) {
                 const double* q = planes5 + 5 * k;
                 if (q[3] != 0 || q[4] != 0) rhs[k].planeWaves.push_back(bem::PlaneWave{q[0], q[1], q[2], bem::cplx(q[3], q[4])});
+            } else {
+                rhs[k].planeWaves = s->s->planeWaves;
             }
         }
         const int L = (int)std::min<int64_t>(lanes, std::max<int64_t>(nf, 1));

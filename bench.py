@@ -135,7 +135,7 @@ def run_field(args):
                                f"0.6-3 m (seeded)" + ("" if args.plane is None else f", rigid plane {args.plane:g} m "
                                                       "below the centre"),
                    "triangles": n, "points": args.field},
-        "roofline": {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
                      "frac": round(ach / PEAK_TFLOPS["fp64"], 4), "traffic": None, "kernel": "field",
                      "avg_ms": round(avg_ms, 4), "launches": cnt,
                      "algorithmic_per_launch": f"{flops:.4g} flop ({args.field} x {n} pairs x {F_FIELD})"},
@@ -263,7 +263,7 @@ def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload, 
         flops = quads * F_REG * entries / launches
         ach = flops / (avg_ms[dom] * 1e-3) / 1e12
         peak = PEAK_TFLOPS[prec_name]
-        rl = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+        rl = {"bound": "valu", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
               "frac": round(ach / peak, 4), "traffic": None,
               "algorithmic_per_launch": f"{flops:.4g} flop ({entries / launches:.4g} regular entries x "
                                         f"{quads} x {F_REG})"}
@@ -272,7 +272,7 @@ def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload, 
             # the points per second is the chip-wide rate
             rl["achieved_chip_wide"] = round(flops * launches * points_per_s / 1e12, 3)
     elif dom == 1:
-        rl = {"bound": "mfma", "achieved": None, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": None,
+        rl = {"bound": "valu", "achieved": None, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": None,
               "traffic": None}
     else:
         if dom == 3:

@@ -142,10 +142,13 @@ int bem_hmvm(bem_ctx_t ctx, bem_hmat_t h, const void* x_dev, void* y_dev, int ac
 int bem_block_info(bem_hmat_t h, int64_t b, int64_t* rank);
 int bem_block_download(bem_hmat_t h, int64_t b, double* full_or_U, double* alpha, double* V);
 /* stats[0] evaluated entries, [1] max rank, [2] mean rank, [3] bytes of block
-   storage, [4] assembly ms (host wall clock), [5] blocks re-run after
-   rank-capacity overflow, [6] device ms of the singular, dense and first ACA
-   pass, [7] device ms of the re-run and the packing */
-int bem_hmat_stats(bem_hmat_t h, double* stats8);
+   storage at the final ranks (what one H-MVM reads), [4] assembly ms (host
+   wall clock), [5] blocks re-run after rank-capacity overflow, [6] device ms
+   of the singular, dense and first ACA pass, [7] device ms of the re-run and
+   the packing, [8] device bytes the H-matrix holds (when no block
+   overflowed, the first ACA pass's storage at its capacity of 48 per block
+   is kept as is, so this exceeds [3]) */
+int bem_hmat_stats(bem_hmat_t h, double* stats9);
 
 /* GMRES::gmresSolve replacement: restarted GMRES(restart) with CGS2 Arnoldi
    on the device; x_dev holds the start vector on entry and the solution on
@@ -296,8 +299,10 @@ const char* bem_read_msh_error(void);
    every frequency Phi and DPhi are assembled once and GMRES runs once per
    right-hand side; right-hand side k has the boundary data g_k (nrhs x n
    complex, input element order; NULL: the g given at creation for every k)
-   and the plane wave planes5[k] (dir x, y, z, amplitude re, im; NULL or zero
-   amplitude: none) on the a, b given at creation.  `lanes` solver copies on
+   and the plane wave planes5[k] (dir x, y, z, amplitude re, im; zero
+   amplitude: none; planes5 NULL: the plane wave of bem_solver_set_sources,
+   if any) on the a, b given at creation; the point sources of
+   bem_solver_set_sources belong to every right-hand side.  `lanes` solver copies on
    their own streams and host threads process the frequencies concurrently
    (frequency f on lane f % lanes); every frequency point is computed exactly
    as by bem_solver_solve.  Outputs: pressure nf x nrhs x n complex (cluster

@@ -1210,6 +1210,14 @@ void orc_hmat_block(const orc_hmat* h, int64_t b, double* U, double* alpha, doub
 }
 void orc_hmvm(const orc_hmat* h, const double* x, double* y) { hmvm(*h, (const cplx*)x, (cplx*)y); }
 
+int64_t orc_normal_groups(const orc_problem* p, int64_t start, int64_t count, int32_t* gid)
+{
+    auto G = normal_groups(*p, start, count);
+    for (size_t g = 0; g < G.size(); g++)
+        for (int64_t e : G[g]) gid[e - start] = (int32_t)g;
+    return (int64_t)G.size();
+}
+
 int64_t orc_pivot_list(const orc_problem* p, int64_t b, uint64_t seed, int64_t* out, int64_t cap)
 {
     auto piv = pivot_list(*p, p->blocks[b], b, seed);

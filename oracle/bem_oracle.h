@@ -80,6 +80,10 @@ void orc_hmat_block(const orc_hmat* h, int64_t b, double* U, double* alpha, doub
 void orc_hmvm(const orc_hmat* h, const double* x, double* y);
 /* pivot list of partialPivotACAextra for one block: returns count, pairs relative */
 int64_t orc_pivot_list(const orc_problem* p, int64_t b, uint64_t seed, int64_t* out_pairs, int64_t cap);
+/* group of every element start .. start + count - 1 (cluster order) under the
+   greedy normal-orientation grouping of getNormalFilteredPivotIndices
+   (boundaryelementsolver.cpp:2052-2172); returns the number of groups */
+int64_t orc_normal_groups(const orc_problem* p, int64_t start, int64_t count, int32_t* gid);
 /* count of evaluated entries (dense + ACA rows/columns + pivot probes) of the last assemble */
 int64_t orc_hmat_evaluated(const orc_hmat* h);
 

@@ -50,6 +50,7 @@ def lib():
             "orc_hmat_block": (None, [_p, _i64, _p, _p, _p]),
             "orc_hmvm": (None, [_p, _p, _p]),
             "orc_pivot_list": (_i64, [_p, _i64, ctypes.c_uint64, _p, _i64]),
+            "orc_normal_groups": (_i64, [_p, _i64, _i64, _p]),
             "orc_hmat_evaluated": (_i64, [_p]),
             "orc_gmres": (_i64, [_p, _p, _p, _dbl, _i64, _i64, _p]),
             "orc_solve": (ctypes.c_int, [_p, _dbl, _i64, _dbl, _i64, ctypes.c_uint64, _dbl, _dbl, ctypes.c_int,
@@ -180,6 +181,12 @@ class Problem:
         out = np.empty((cap, 2), np.int64)
         c = lib().orc_pivot_list(self.h, int(b), int(seed), _ptr(out), cap)
         return out[:c].copy()
+
+    def normal_groups(self, start, count):
+        """group index of each element start .. start + count - 1 (cluster order)"""
+        gid = np.empty(int(count), np.int32)
+        ng = lib().orc_normal_groups(self.h, int(start), int(count), _ptr(gid))
+        return gid, int(ng)
 
     def solve(self, acatol, maxrank=0, gmres_tol=None, restart=300, seed=1, rho=1.21, freq=None, nthreads=0):
         n = self.n

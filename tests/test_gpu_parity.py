@@ -54,6 +54,27 @@ def test_assembly_bit_exact(L3, ctx3, kind):
     assert st["evaluated"] == H.evaluated
 
 
+def test_assembly_bit_exact_with_rank_overflow(L3, ctx3):
+    """ACA 1e-10 on the L3 blocks: ranks up to 55 / 60 exceed the first-pass
+    capacity of 48, so the overflowing blocks are re-run with their full
+    capacity and every block is packed (k_compact); the result is still
+    bit-identical to the restatement."""
+    for kind in (0, 1):
+        H = L3.assemble(kind, 1e-10, 0, 5)
+        G = ctx3.assemble(kind, 1e-10, 0, seed=5)
+        st = G.stats()
+        assert st["overflow_blocks"] > 0 and st["max_rank"] > 48
+        for b in range(len(L3.blocks)):
+            assert H.rank(b) == G.rank(b), (kind, b)
+            A, B = H.block(b), G.block(b)
+            if H.rank(b) < 0:
+                assert same_bits(A, B), (kind, b)
+            else:
+                for x, y in zip(A, B):
+                    assert same_bits(x, y), (kind, b)
+        G.close()
+
+
 def test_mvm_matches_oracle(L3, ctx3):
     H = L3.assemble(1, 1e-6, 0, 3)
     G = ctx3.assemble(1, 1e-6, 0, seed=3)

@@ -50,6 +50,7 @@ _SIGS = {
     "bem_reconstruct": (_int, [_p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_reconstruct_g": (_int, [_p, _p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_field": (_int, [_p, _i64, _p, _p, _p, _p]),
+    "bem_field_noalloc": (_int, [_p, _i64, _p, _p, _p, _p]),
     "bem_read_msh": (_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i64, _p, ctypes.POINTER(_i64)]),
     "bem_read_msh_error": (ctypes.c_char_p, []),
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),

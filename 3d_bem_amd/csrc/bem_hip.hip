@@ -19,6 +19,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -342,6 +343,13 @@ __global__ void k_field_reduce(int64_t nobs, int nchunk, const bem_c64* __restri
     for (int q = 1; q < nchunk; q++) acc = c64_add(acc, part[i * nchunk + q]);
     out[i] = acc;
 }
+
+// Fixed device scratch of bem_field_noalloc (the field's per-chunk partial
+// sums), one copy per GPU for the life of the process.  Host code never
+// passes `scratch` itself to a kernel: its device address on each GPU comes
+// from hipGetSymbolAddress (scratch_address below).
+constexpr size_t kFieldScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[kFieldScratchBytes];
 
 // -------------------------------------------------- workgroup reductions
 
@@ -1707,6 +1715,8 @@ struct bem_ctx {
     hipEvent_t stage_done = nullptr;
     bool stage_busy = false;
     VecPool vp;
+    // device address of `scratch` on this context's GPU (bem_create)
+    void* scratch_dev = nullptr;
     // GMRES: residual estimates of the last two Arnoldi columns
     double* gm_res = nullptr;
     hipEvent_t gm_ev[2] = {nullptr, nullptr};
@@ -2036,6 +2046,26 @@ unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
+// The address of `scratch` on `device` (the current device), looked up once
+// per device and kept: every GPU holds its own copy at its own address.
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;
+hipError_t scratch_address(int device, void** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = p;
+    *out = p;
+    return hipSuccess;
+}
+
 // =================================================================== C ABI
 
 extern "C" {
@@ -2058,7 +2088,8 @@ int bem_create(bem_ctx_t* out, int device)
         hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->gm_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->gm_ev[1], hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void**)&c->gm_res, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->gm_res, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        scratch_address(device, &c->scratch_dev) != hipSuccess) {
         delete c;
         return BEM_ERR_HIP;
     }
@@ -3464,6 +3495,29 @@ int bem_field(bem_ctx_t c, int64_t nobs, const double* obs3, const void* phi, co
     free_list(c, own);
     if (e != hipSuccess) return set_err(c, BEM_ERR_HIP, std::string("bem_field: ") + hipGetErrorString(e));
     return BEM_OK;
+}
+
+int bem_field_noalloc(bem_ctx_t c, int64_t nobs, const double* obs_dev, const void* phi, const void* dphi,
+                      void* out_dev)
+{
+    if (!c || nobs < 0 || (nobs && (!obs_dev || !out_dev)) || !phi || !dphi)
+        return set_err(c, BEM_ERR_ARG, "bem_field_noalloc: bad arguments");
+    if (!c->have_freq || c->n <= 0) return set_err(c, BEM_ERR_STATE, "bem_field_noalloc: mesh or frequency missing");
+    if (!c->scratch_dev) return set_err(c, BEM_ERR_STATE, "bem_field_noalloc: no scratch address for this device");
+    if (nobs == 0) return BEM_OK;
+    const int64_t nchunk = (c->n + kFieldChunk - 1) / kFieldChunk;
+    const int64_t cap = (int64_t)(kFieldScratchBytes / sizeof(bem_c64));
+    if (nobs > cap / nchunk)
+        return set_err(c, BEM_ERR_ARG, "bem_field_noalloc: nobs x ceil(n / 1024) = " + std::to_string(nobs) + " x " +
+                                           std::to_string(nchunk) + " partial sums exceed the fixed scratch of " +
+                                           std::to_string(cap) + " (32 MiB)");
+    HIPC(c, hipSetDevice(c->device));
+    bem_c64* part = (bem_c64*)c->scratch_dev;
+    hipLaunchKernelGGL(k_field, dim3(nblk(nobs, kTile), (unsigned)nchunk), dim3(kWG), 0, c->stream, c->mesh, c->k,
+                       nobs, obs_dev, (const bem_c64*)phi, (const bem_c64*)dphi, (int)nchunk, part);
+    hipLaunchKernelGGL(k_field_reduce, dim3(nblk(nobs, 256)), dim3(256), 0, c->stream, nobs, (int)nchunk, part,
+                       (bem_c64*)out_dev);
+    return launch_check(c, "k_field (noalloc)");
 }
 
 int bem_reconstruct(bem_ctx_t c, const void* x, double rho, double freq, void* phi, void* dphi, void* pr)

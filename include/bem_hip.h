@@ -211,6 +211,21 @@ int bem_reconstruct_g(bem_ctx_t ctx, const void* x_dev, const void* g_dev, doubl
 int bem_field(bem_ctx_t ctx, int64_t nobs, const double* obs3, const void* phi_dev, const void* dphi_dev,
               double* out2);
 
+/* No counterpart in the reference: bem_field for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  It allocates nothing: the
+   per-chunk partial sums go to a fixed 32 MiB device array that the library
+   holds on every GPU it runs on until the process ends, so nobs x
+   ceil(n / 1024) x 16 bytes must fit in it (else BEM_ERR_ARG).  obs_dev:
+   the points as three device rows (x[nobs], y[nobs], z[nobs]); phi_dev,
+   dphi_dev: n complex; out_dev: nobs complex, all device memory.
+   Asynchronous on the context stream (bem_synchronize waits for it).
+   Every call on one GPU shares that array, so calls that overlap on one GPU
+   (two contexts, two host threads) race; the array keeps the last call's
+   contents. */
+int bem_field_noalloc(bem_ctx_t ctx, int64_t nobs, const double* obs_dev, const void* phi_dev, const void* dphi_dev,
+                      void* out_dev);
+
 /* L, M, Mt, N (out npairs x 8) or Phi/DPhi entries (out npairs x 2) of
    (row, col) pairs given in host memory, evaluated on the device
    (BemOperatorsConst / implicitPhiMatrix2 / implicitDPhiMatrix2). */

@@ -8,6 +8,7 @@ digit, so import it with ``importlib.import_module("3d_bem_amd")``.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -51,8 +52,21 @@ _SIGS = {
     "bem_reconstruct_g": (_int, [_p, _p, _p, _dbl, _dbl, _p, _p, _p]),
     "bem_field": (_int, [_p, _i64, _p, _p, _p, _p]),
     "bem_field_noalloc": (_int, [_p, _i64, _p, _p, _p, _p]),
+    "bem_set_near_fp64": (_int, [_p, _int]),
+    "bem_hmvm_refine": (_int, [_p, _p, _p, _p, _int]),
+    "bem_gmres_ir": (_int, [_p, _p, _p, _p, _p, _p, _int, _dbl, _i64, _i64, _i64, ctypes.POINTER(_i64),
+                            ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
+    "bem_solver_set_refinement": (_int, [_p, _int, _i64]),
+    "bem_set_mfma": (_int, [_p, _int]),
+    "bem_set_graphs": (_int, [_p, _int]),
+    "bem_hmvm_multi": (_int, [_p, _p, _i64, _p, _p, _int, _int]),
+    "bem_gmres_multi": (_int, [_p, _p, _p, _i64, _p, _p, _int, _dbl, _i64, _i64, _i64, _p, _p,
+                               ctypes.POINTER(_i64)]),
+    "bem_solver_solve_dense": (_int, [_p, _dbl, _dbl, _dbl, _int, _dbl, _i64, _p, _p, _p, _p]),
+    "bem_solver_refine_stats": (_int, [_p, _p]),
     "bem_read_msh": (_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i64, _p, ctypes.POINTER(_i64)]),
     "bem_read_msh_error": (ctypes.c_char_p, []),
+    "bem_refine_triangles": (_int, [_i64, _p, _dbl, _i64, _p, ctypes.POINTER(_i64)]),
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),
     "bem_solver_set_preconditioner": (_int, [_p, _int]),
     "bem_solver_set_hlu": (_int, [_p, _i64, _dbl]),
@@ -158,6 +172,7 @@ class Context:
         self.h = h
         self.n = 0
         self.blocks = None
+        self._children = weakref.WeakSet()   # H-matrices and preconditioners of this context
 
     def _check(self, rc):
         if rc != BEM_OK:
@@ -165,7 +180,10 @@ class Context:
         return rc
 
     def close(self):
+        """destroys the context; its H-matrices and preconditioners are closed first"""
         if getattr(self, "h", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             lib().bem_destroy(self.h)
             self.h = None
 
@@ -190,6 +208,18 @@ class Context:
         """a Phi assembly also forms the DPhi near field for the next DPhi
         assembly (bem_set_near_pairing)"""
         self._check(lib().bem_set_near_pairing(self.h, int(bool(on))))
+
+    def set_near_fp64(self, on=True):
+        """later FP32 assemblies keep their near field in FP64 too (bem_set_near_fp64)"""
+        self._check(lib().bem_set_near_fp64(self.h, int(bool(on))))
+
+    def set_mfma(self, on=True):
+        """matrix-core products for four or more right-hand sides (bem_set_mfma)"""
+        self._check(lib().bem_set_mfma(self.h, int(bool(on))))
+
+    def set_graphs(self, on=True):
+        """graph-captured GMRES columns (bem_set_graphs)"""
+        self._check(lib().bem_set_graphs(self.h, int(bool(on))))
 
     def set_plane(self, point=None, normal=None):
         """one impedance plane (rigid image) through point with normal; None removes it"""
@@ -260,11 +290,14 @@ class HMatrix:
 
     def __init__(self, ctx, h):
         self.ctx, self.h = ctx, h
+        ctx._children.add(self)
 
     def close(self):
-        if getattr(self, "h", None):
+        # a destroyed context has released the H-matrix's device memory with
+        # its own (garbage collection may reach the two in either order)
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             lib().bem_hmat_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -317,6 +350,46 @@ class HMatrix:
     def mvm_dev(self, xd, yd, accumulate=False):
         self.ctx._check(lib().bem_hmvm(self.ctx.h, self.h, xd.ptr, yd.ptr, 1 if accumulate else 0))
 
+    def mvm_multi(self, X, refine=False):
+        """Y = H X for X (n, nrhs) (bem_hmvm_multi)"""
+        X = _c(X, np.complex128)
+        n, q = X.shape
+        xd, yd = self.ctx.array(X.ravel(), n=n * q), self.ctx.array(n=n * q)
+        self.ctx._check(lib().bem_hmvm_multi(self.ctx.h, self.h, q, xd.ptr, yd.ptr, 0, int(bool(refine))))
+        return yd.download().reshape(n, q)
+
+    def gmres_multi(self, B, tol, hres=None, restart=300, maxit=100000, max_steps=0):
+        """lockstep GMRES over the columns of B (n, nrhs) (bem_gmres_multi):
+        X, iterations, relative residuals, refinement steps, status"""
+        B = _c(B, np.complex128)
+        n, q = B.shape
+        bd, xd = self.ctx.array(B.ravel(), n=n * q), self.ctx.array(n=n * q)
+        its, rel, st = np.zeros(q, np.int64), np.zeros(q), _i64()
+        rc = lib().bem_gmres_multi(self.ctx.h, self.h, None if hres is None else hres.h, q, bd.ptr, xd.ptr, 1,
+                                   float(tol), int(restart), int(maxit), int(max_steps), _ptr(its), _ptr(rel),
+                                   ctypes.byref(st))
+        if rc not in (BEM_OK, BEM_ERR_NOCONV):
+            self.ctx._check(rc)
+        return xd.download().reshape(n, q), its, rel, st.value, rc
+
+    def mvm_refine(self, x):
+        """y = A_res x: the FP64 near field + stored low-rank factors (bem_hmvm_refine)"""
+        xd, yd = self.ctx.array(x), self.ctx.array()
+        self.ctx._check(lib().bem_hmvm_refine(self.ctx.h, self.h, xd.ptr, yd.ptr, 0))
+        return yd.download()
+
+    def gmres_ir(self, b, tol, hres=None, restart=300, maxit=100000, max_steps=10, precond=None):
+        """iterative refinement around GMRES (bem_gmres_ir): x, inner iterations,
+        |b - A_res x| / |b|, refinement steps, status"""
+        bd, xd = self.ctx.array(b), self.ctx.array(np.zeros(self.ctx.n, np.complex128))
+        its, rel, st = _i64(), _dbl(), _i64()
+        rc = lib().bem_gmres_ir(self.ctx.h, self.h, None if hres is None else hres.h,
+                                None if precond is None else precond.h, bd.ptr, xd.ptr, 1, float(tol), int(restart),
+                                int(maxit), int(max_steps), ctypes.byref(its), ctypes.byref(rel), ctypes.byref(st))
+        if rc not in (BEM_OK, BEM_ERR_NOCONV):
+            self.ctx._check(rc)
+        return xd.download(), its.value, rel.value, st.value, rc
+
     def preconditioner(self):
         """leaf-block LU preconditioner of this operator (bem_precond_create)"""
         p = _p()
@@ -348,6 +421,7 @@ class Preconditioner:
 
     def __init__(self, ctx, h):
         self.ctx, self.h = ctx, h
+        ctx._children.add(self)
 
     def apply(self, v):
         vd, zd = self.ctx.array(v), self.ctx.array()
@@ -360,9 +434,9 @@ class Preconditioner:
         return {"setup_ms": o[0], "applies": int(o[1]), "apply_ms": o[2], "entries": o[3], "factor_ms": o[4]}
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             lib().bem_precond_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -458,6 +532,21 @@ def read_msh(path, include=None, exclude=None):
     return tri
 
 
+def refine_triangles(tri, edge_length):
+    """bisect every triangle (n, 9) until its longest edge is at most
+    edge_length (bem_refine_triangles: ElementSection::refineTriangles)"""
+    tri = _c(np.asarray(tri).reshape(-1, 9), np.float64)
+    cnt = _i64()
+    rc = lib().bem_refine_triangles(len(tri), _ptr(tri), float(edge_length), 0, None, ctypes.byref(cnt))
+    if rc != BEM_OK:
+        raise BemError(rc, "bem_refine_triangles")
+    out = np.empty((cnt.value, 9), np.float64)
+    rc = lib().bem_refine_triangles(len(tri), _ptr(tri), float(edge_length), cnt.value, _ptr(out), ctypes.byref(cnt))
+    if rc != BEM_OK:
+        raise BemError(rc, "bem_refine_triangles")
+    return out
+
+
 class Solver:
     """BoundaryElementSolver::hMatrixSolve on the GPU (bem_solver_t).
 
@@ -508,6 +597,33 @@ class Solver:
         rc = lib().bem_solver_solve(self.h, float(freq), float(c), float(rho), int(coupling), float(aca_tol),
                                     int(maxrank), gt, int(restart), int(seed), int(precision), _ptr(phi),
                                     _ptr(dphi), _ptr(pr), _ptr(st))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
+        return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def set_refinement(self, mode, max_steps=10):
+        """FP32 solves: 0 no refinement, 1 residuals with the FP64 near field
+        and the low-rank factors, 2 residuals with FP64 H-matrices
+        (bem_solver_set_refinement)"""
+        rc = lib().bem_solver_set_refinement(self.h, int(mode), int(max_steps))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_refinement")
+
+    def refine_stats(self):
+        """the last solve's refinement: steps, inner iterations, final residual,
+        FP64 assembly ms (bem_solver_refine_stats)"""
+        o = np.zeros(4)
+        lib().bem_solver_refine_stats(self.h, _ptr(o))
+        return {"steps": int(o[0]), "inner_iters": int(o[1]), "residual": o[2], "fp64_assembly_ms": o[3]}
+
+    def solve_dense(self, freq, c=343.0, rho=1.21, coupling=1, gmres_tol=1e-12, restart=300):
+        """regularSolveWithLU on the device: dense FP64 Phi and DPhi, GMRES to
+        gmres_tol (bem_solver_solve_dense); phi, dphi, pressure (cluster order), stats"""
+        n = self.n
+        phi, dphi, pr = (np.empty(n, np.complex128) for _ in range(3))
+        st = np.zeros(16)
+        rc = lib().bem_solver_solve_dense(self.h, float(freq), float(c), float(rho), int(coupling), float(gmres_tol),
+                                          int(restart), _ptr(phi), _ptr(dphi), _ptr(pr), _ptr(st))
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))

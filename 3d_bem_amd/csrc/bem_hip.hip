@@ -242,6 +242,16 @@ __global__ void k_scatter(const Fix* __restrict__ fx, int64_t nfix, int64_t nnz,
     out[f.dest] = o;
 }
 
+// FP32 copy of an FP64 near field (the operator GMRES iterates with when the
+// FP64 one is kept for the refinement residual)
+__global__ void k_round32(int64_t len, const bem_c64* __restrict__ src, bem_c32* __restrict__ dst)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    const bem_c64 v = src[t];
+    dst[t] = c32((float)v.re, (float)v.im);
+}
+
 __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int64_t* ij, bem_c64* out)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1550,6 +1560,13 @@ bool hlu_split_ok(const bemhlu::Flat& F, const HluSplit& S, std::string& why)
     return true;
 }
 
+// v = 0 (a kernel rather than a memset: also safe under graph capture)
+__global__ void k_fill0(int64_t n, bem_c64* v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = c64(0.0, 0.0);
+}
+
 // x += z
 __global__ void k_axpy1(int64_t n, const bem_c64* z, bem_c64* x)
 {
@@ -1583,6 +1600,462 @@ __global__ void k_has_nonfinite(int64_t len, const C* v, int32_t* flag)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < len && !(isfinite(v[i].re) && isfinite(v[i].im))) *flag = 1;
+}
+
+// ------------------------------------------- several right-hand sides
+// Vectors of Q right-hand sides are interleaved, v[i * Q + q], so that one
+// pass over every block of the operator serves all Q products (Q = 1, 2, 4,
+// 8).  Partial vectors are interleaved the same way.  The products
+// accumulate in FP64 for either block storage.  For Q >= 4 the near-field
+// tiles and the U t phase of the low-rank blocks run on the FP64 matrix
+// cores (V_MFMA_F64_16X16X4F64, a complex product as four real ones); the
+// VALU kernels stay for Q < 4 and as their measured alternative.
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// C (16 x 16 complex) += A (16 x 4) B (4 x 16) for one K step of four: this
+// lane holds A[lane & 15][lane >> 4] in a and B[lane >> 4][lane & 15] in b;
+// C[r][c] is held by lane c + 16 (r % 4), element r / 4 (the layout measured
+// on gfx950 with a one-wavefront probe)
+__device__ __forceinline__ void mfma_cstep(bem_c64 a, bem_c64 b, v4d& cr, v4d& ci)
+{
+    cr = mfma_f64(a.re, b.re, cr);
+    cr = mfma_f64(-a.im, b.im, cr);
+    ci = mfma_f64(a.re, b.im, ci);
+    ci = mfma_f64(a.im, b.re, ci);
+}
+
+// near-field tiles, Q right-hand sides on the VALU
+template <class S, int Q>
+__global__ __launch_bounds__(kWG) void k_mvm_dense_q(const Tile* __restrict__ tiles, const S* __restrict__ F,
+                                                     const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+{
+    __shared__ bem_c64 red[4][kTile * Q];
+    const Tile T = tiles[blockIdx.x];
+    const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
+    bem_c64 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc[q] = c64(0, 0);
+    if (r < T.rows) {
+        const S* f = F + T.dest + r;
+        for (int c = cg; c < T.cols; c += 4) {
+            const bem_c64 fv = ld64(f + (int64_t)c * T.ld);
+            const bem_c64* xc = x + (int64_t)(T.col0 + c) * Q;
+#pragma unroll
+            for (int q = 0; q < Q; q++) acc[q] = c64_fma(fv, xc[q], acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) red[cg][r * Q + q] = acc[q];
+    __syncthreads();
+    for (int o = tid; o < T.rows * Q; o += kWG)
+        part[T.part * Q + o] = c64_add(c64_add(c64_add(red[0][o], red[1][o]), red[2][o]), red[3][o]);
+}
+
+// near-field tiles on the matrix cores: wavefront w takes rows 16 w ..
+// 16 w + 15 of the tile over its columns in K steps of four
+template <class S, int Q>
+__global__ __launch_bounds__(kWG) void k_mvm_dense_mfma(const Tile* __restrict__ tiles, const S* __restrict__ F,
+                                                        const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+{
+    static_assert(Q <= 16, "one 16-column tile of right-hand sides");
+    const Tile T = tiles[blockIdx.x];
+    const int lane = threadIdx.x & 63, rb = 16 * (threadIdx.x >> 6), lo = lane & 15, kq = lane >> 4;
+    if (rb >= T.rows) return;
+    const bool rowok = rb + lo < T.rows;
+    const S* f = F + T.dest + rb + lo;
+    v4d cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    for (int kk = 0; kk < T.cols; kk += 4) {
+        const int c = kk + kq;
+        const bool cok = c < T.cols;
+        const bem_c64 a = rowok && cok ? ld64(f + (int64_t)c * T.ld) : c64(0, 0);
+        const bem_c64 b = lo < Q && cok ? x[(int64_t)(T.col0 + c) * Q + lo] : c64(0, 0);
+        mfma_cstep(a, b, cr, ci);
+    }
+    if (lo < Q) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = rb + kq + 4 * v;
+            if (row < T.rows) part[(T.part + row) * Q + lo] = c64(cr[v], ci[v]);
+        }
+    }
+}
+
+// low-rank phase 1, Q right-hand sides: t_b[l][q] = alpha_l (V_b[l, :] . x_J[:, q]),
+// one wavefront per (block, l0) task of kLrRows rows of V (rows l0 .. l0 + R - 1
+// by this kernel, R = kLrRows / Q rounded up to one)
+template <class S, int Q>
+__global__ __launch_bounds__(kWG) void k_lr_t_q(const int32_t* __restrict__ tasks, int64_t ntask,
+                                                const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
+                                                const int32_t* __restrict__ rank, const int64_t* __restrict__ voff,
+                                                const int64_t* __restrict__ aoff, const int64_t* __restrict__ toff,
+                                                const S* __restrict__ V, const S* __restrict__ A,
+                                                const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
+{
+    constexpr int R = kLrRows / Q > 0 ? kLrRows / Q : 1;
+    const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+    if (task >= ntask) return;
+    const int lane = threadIdx.x & 63;
+    const int b = tasks[2 * task], lt = tasks[2 * task + 1];
+    const int64_t n = cols[b];
+    const bem_c64* xs = x + col0[b] * Q;
+    for (int l0 = lt; l0 < lt + kLrRows && l0 < rank[b]; l0 += R) {
+        const int nl = min(R, rank[b] - l0);
+        const S* v = V + voff[b] + (int64_t)l0 * n;
+        bem_c64 p[R][Q];
+#pragma unroll
+        for (int u = 0; u < R; u++)
+#pragma unroll
+            for (int q = 0; q < Q; q++) p[u][q] = c64(0, 0);
+        for (int64_t j = lane; j < n; j += 64) {
+            bem_c64 xv[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) xv[q] = xs[j * Q + q];
+#pragma unroll
+            for (int u = 0; u < R; u++)
+                if (u < nl) {
+                    const bem_c64 vv = ld64(v + (int64_t)u * n + j);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) p[u][q] = c64_fma(vv, xv[q], p[u][q]);
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < R; u++)
+#pragma unroll
+            for (int q = 0; q < Q; q++) p[u][q] = wave_sum(p[u][q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < R; u++)
+                if (u < nl) {
+                    const bem_c64 al = ld64(A + aoff[b] + l0 + u);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) t[(toff[b] + l0 + u) * Q + q] = c64_mul(al, p[u][q]);
+                }
+        }
+    }
+}
+
+// low-rank phase 2, Q right-hand sides on the VALU: partial_b[i][q] =
+// sum_l U_b[i, l] t_b[l][q] over a chunk of kWG rows
+template <class S, int Q>
+__global__ __launch_bounds__(kWG) void k_lr_u_q(const int32_t* __restrict__ tasks, const int64_t* __restrict__ rows,
+                                                const int32_t* __restrict__ rank, const int64_t* __restrict__ uoff,
+                                                const int64_t* __restrict__ toff, const int64_t* __restrict__ poff,
+                                                const S* __restrict__ U, const bem_c64* __restrict__ t,
+                                                bem_c64* __restrict__ part)
+{
+    const int b = tasks[2 * blockIdx.x], chunk = tasks[2 * blockIdx.x + 1];
+    const int64_t m = rows[b];
+    const int k = rank[b];
+    const int64_t i = (int64_t)chunk * kWG + threadIdx.x;
+    if (i >= m) return;
+    const S* u = U + uoff[b] + i;
+    const bem_c64* tb = t + toff[b] * Q;   // the same k Q values for every thread (cached)
+    bem_c64 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc[q] = c64(0, 0);
+    for (int l = 0; l < k; l++) {
+        const bem_c64 uv = ld64(u + (int64_t)l * m);
+#pragma unroll
+        for (int q = 0; q < Q; q++) acc[q] = c64_fma(uv, tb[l * Q + q], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) part[(poff[b] + i) * Q + q] = acc[q];
+}
+
+// low-rank phase 2 on the matrix cores: the chunk's kWG rows as 16 row tiles,
+// wavefront w taking tiles w, w + 4, ...; K = the block's rank in steps of four
+template <class S, int Q>
+__global__ __launch_bounds__(kWG) void k_lr_u_mfma(const int32_t* __restrict__ tasks, const int64_t* __restrict__ rows,
+                                                   const int32_t* __restrict__ rank, const int64_t* __restrict__ uoff,
+                                                   const int64_t* __restrict__ toff, const int64_t* __restrict__ poff,
+                                                   const S* __restrict__ U, const bem_c64* __restrict__ t,
+                                                   bem_c64* __restrict__ part)
+{
+    const int b = tasks[2 * blockIdx.x], chunk = tasks[2 * blockIdx.x + 1];
+    const int64_t m = rows[b];
+    const int k = rank[b];
+    const int lane = threadIdx.x & 63, lo = lane & 15, kq = lane >> 4;
+    const S* u = U + uoff[b];
+    const bem_c64* tb = t + toff[b] * Q;
+    for (int tile = threadIdx.x >> 6; tile < kWG / 16; tile += kWG / 64) {
+        const int64_t r0 = (int64_t)chunk * kWG + 16 * tile;
+        if (r0 >= m) break;
+        const bool rowok = r0 + lo < m;
+        v4d cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+        for (int kk = 0; kk < k; kk += 4) {
+            const int l = kk + kq;
+            const bool lok = l < k;
+            const bem_c64 a = rowok && lok ? ld64(u + (r0 + lo) + (int64_t)l * m) : c64(0, 0);
+            const bem_c64 bb = lo < Q && lok ? tb[l * Q + lo] : c64(0, 0);
+            mfma_cstep(a, bb, cr, ci);
+        }
+        if (lo < Q) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t row = r0 + kq + 4 * v;
+                if (row < m) part[(poff[b] + row) * Q + lo] = c64(cr[v], ci[v]);
+            }
+        }
+    }
+}
+
+// y[i][q] (+)= sum of the partial vectors covering row i, in partition order
+// per lane group as k_mvm_gather
+template <int Q>
+__global__ void k_mvm_gather_q(int64_t n, const int64_t* __restrict__ gptr, const int64_t* __restrict__ gidx,
+                               const bem_c64* __restrict__ part, bem_c64* y, int accumulate)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = g >> 4;
+    const int lq = (int)(g & 15);
+    bem_c64 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc[q] = c64(0, 0);
+    if (i < n)
+        for (int64_t p = gptr[i] + lq; p < gptr[i + 1]; p += 16) {
+            const bem_c64* src = part + gidx[p] * Q;
+#pragma unroll
+            for (int q = 0; q < Q; q++) acc[q] = c64_add(acc[q], src[q]);
+        }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        for (int s = 8; s > 0; s >>= 1) {
+            acc[q].re = acc[q].re + __shfl_down(acc[q].re, s, 16);
+            acc[q].im = acc[q].im + __shfl_down(acc[q].im, s, 16);
+        }
+    if (i < n && lq == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) y[i * Q + q] = accumulate ? c64_add(y[i * Q + q], acc[q]) : acc[q];
+    }
+}
+
+// ---- lockstep GMRES over Q right-hand sides.  The Krylov bases are
+// interleaved like the vectors (V[(l n + i) Q + q]); the current column j is
+// read from device memory (*jp), so that one iteration's launches form a
+// graph that is captured once and replayed for every j.
+
+constexpr int kArnMax = 8;   // right-hand sides of one lockstep solve
+
+// X = column *jp of the bases (the product's input)
+template <int Q>
+__global__ void k_basis_col(int64_t n, const int* __restrict__ jp, const bem_c64* __restrict__ V, bem_c64* __restrict__ X)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n * Q) X[t] = V[(int64_t)(*jp) * n * Q + t];
+}
+
+// part[(l nchunk + chunk) Q + q] = sum over the chunk's rows of conj(V_l) w
+// for l <= *jp; one wavefront per (chunk, group of vectors)
+template <int Q>
+__global__ __launch_bounds__(kWG) void k_cgs_dots_q(int64_t n, const int* __restrict__ jp,
+                                                    const bem_c64* __restrict__ V, const bem_c64* __restrict__ w,
+                                                    bem_c64* __restrict__ part)
+{
+    constexpr int G = kDotVecs / Q > 0 ? kDotVecs / Q : 1;
+    const int cnt = *jp + 1;
+    const int l0 = (blockIdx.y * (kWG / 64) + (threadIdx.x >> 6)) * G;
+    if (l0 >= cnt) return;
+    const int lane = threadIdx.x & 63;
+    const int nv = min(G, cnt - l0);
+    const int64_t i0 = (int64_t)blockIdx.x * kDotChunk, i1 = i0 + kDotChunk < n ? i0 + kDotChunk : n;
+    bem_c64 p[G][Q];
+#pragma unroll
+    for (int u = 0; u < G; u++)
+#pragma unroll
+        for (int q = 0; q < Q; q++) p[u][q] = c64(0, 0);
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
+        bem_c64 wi[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) wi[q] = w[i * Q + q];
+#pragma unroll
+        for (int u = 0; u < G; u++)
+            if (u < nv) {
+                const bem_c64* vr = V + ((int64_t)(l0 + u) * n + i) * Q;
+#pragma unroll
+                for (int q = 0; q < Q; q++) p[u][q] = c64_fma(c64_conj(vr[q]), wi[q], p[u][q]);
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++)
+#pragma unroll
+        for (int q = 0; q < Q; q++) p[u][q] = wave_sum(p[u][q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < G; u++)
+            if (u < nv)
+#pragma unroll
+                for (int q = 0; q < Q; q++) part[((int64_t)(l0 + u) * gridDim.x + blockIdx.x) * Q + q] = p[u][q];
+    }
+}
+
+// h[l Q + q] = sum over chunks; hacc = h (first pass) or += h.  One
+// wavefront per l <= *jp.
+template <int Q>
+__global__ __launch_bounds__(64) void k_cgs_reduce_q(const int* __restrict__ jp, int nchunk,
+                                                     const bem_c64* __restrict__ part, bem_c64* h, bem_c64* hacc,
+                                                     int first)
+{
+    const int l = blockIdx.x;
+    if (l > *jp) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        bem_c64 s = c64(0, 0);
+        for (int c = threadIdx.x; c < nchunk; c += 64) s = c64_add(s, part[((int64_t)l * nchunk + c) * Q + q]);
+        s = wave_sum(s);
+        if (threadIdx.x == 0) {
+            h[l * Q + q] = s;
+            hacc[l * Q + q] = first ? s : c64_add(hacc[l * Q + q], s);
+        }
+    }
+}
+
+// w -= sum_{l <= *jp} V_l h_l
+template <int Q>
+__global__ void k_orth_update_q(int64_t n, const int* __restrict__ jp, const bem_c64* __restrict__ V,
+                                const bem_c64* __restrict__ h, bem_c64* __restrict__ w)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const int q = (int)(t % Q);
+    const int cnt = *jp + 1;
+    bem_c64 acc = w[t];
+    for (int l = 0; l < cnt; l++) acc = c64_fms(h[l * Q + q], V[(int64_t)l * n * Q + t], acc);
+    w[t] = acc;
+}
+
+// partial sums of |w_q|^2 per workgroup: part[block Q + q]
+template <int Q>
+__global__ __launch_bounds__(kWG) void k_norm2_part_q(int64_t n, const bem_c64* __restrict__ w, double* part)
+{
+    __shared__ double red[kWG / 64][Q];
+    double p[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) p[q] = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG)
+#pragma unroll
+        for (int q = 0; q < Q; q++) p[q] = p[q] + c64_abs2(w[i * Q + q]);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        p[q] = wave_sum(p[q]);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = p[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < Q) {
+        const int q = threadIdx.x;
+        part[blockIdx.x * Q + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    }
+}
+
+template <int Q>
+__global__ void k_norm2_final_q(int nparts, const double* __restrict__ part, double* out)
+{
+    const int q = threadIdx.x;
+    if (blockIdx.x != 0 || q >= Q) return;
+    double s = 0;
+    for (int p = 0; p < nparts; p++) s += part[p * Q + q];
+    out[q] = sqrt(s);
+}
+
+// V column *jp + 1 = w / |w| (0 for a zero norm)
+template <int Q>
+__global__ void k_scale_next_q(int64_t n, const int* __restrict__ jp, const bem_c64* __restrict__ w,
+                               const double* __restrict__ nrm, bem_c64* __restrict__ V)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const double d = nrm[t % Q];
+    V[(int64_t)(*jp + 1) * n * Q + t] = d > 0 ? c64_scale(w[t], 1.0 / d) : c64(0, 0);
+}
+
+// Arnoldi column *jp for every right-hand side (thread q), as k_arnoldi:
+// H_q column j, the Givens rotations, g_q and resid[q] = |g_q[j + 1]| / |b_q|;
+// then *jp += 1.  H is Q x (m + 1) x m, cs_c / cs_s / g are Q x (m + 2).
+template <int Q>
+__global__ void k_arnoldi_q(int* jp, int m, const bem_c64* __restrict__ hacc, const double* __restrict__ hn,
+                            bem_c64* H, double* cs_c, bem_c64* cs_s, bem_c64* g, const double* __restrict__ bn,
+                            double* resid)
+{
+    const int q = threadIdx.x;
+    const int j = *jp;
+    if (q < Q) {
+        bem_c64* h = H + ((int64_t)q * m + j) * (m + 1);
+        double* cc_q = cs_c + (int64_t)q * (m + 2);
+        bem_c64* ss_q = cs_s + (int64_t)q * (m + 2);
+        bem_c64* g_q = g + (int64_t)q * (m + 2);
+        for (int l = 0; l <= j; l++) h[l] = hacc[l * Q + q];
+        h[j + 1] = c64(hn[q], 0.0);
+        for (int l = 0; l < j; l++) {
+            bem_c64 a0 = h[l], b0 = h[l + 1];
+            h[l] = c64_add(c64_scale(a0, cc_q[l]), c64_mul(ss_q[l], b0));
+            h[l + 1] = c64_sub(c64_scale(b0, cc_q[l]), c64_mul(c64_conj(ss_q[l]), a0));
+        }
+        double na = sqrt(c64_abs2(h[j])), nb2 = sqrt(c64_abs2(h[j + 1]));
+        double tt = sqrt(na * na + nb2 * nb2);
+        double cc;
+        bem_c64 ss;
+        if (na == 0) { cc = 0; ss = c64(1, 0); }
+        else {
+            cc = na / tt;
+            ss = c64_scale(c64_mul(c64_scale(h[j], 1.0 / na), c64_conj(h[j + 1])), 1.0 / tt);
+        }
+        cc_q[j] = cc;
+        ss_q[j] = ss;
+        h[j] = c64_add(c64_scale(h[j], cc), c64_mul(ss, h[j + 1]));
+        h[j + 1] = c64(0, 0);
+        bem_c64 gj = g_q[j];
+        g_q[j] = c64_scale(gj, cc);
+        g_q[j + 1] = c64_neg(c64_mul(c64_conj(ss), gj));
+        resid[q] = bn[q] > 0 ? sqrt(c64_abs2(g_q[j + 1])) / bn[q] : 0.0;
+    }
+    __syncthreads();
+    if (q == 0) *jp = j + 1;
+}
+
+// x_q += sum_{l < cnt} V_l y[l Q + q]
+template <int Q>
+__global__ void k_combine_q(int64_t n, int cnt, const bem_c64* __restrict__ V, const bem_c64* __restrict__ y,
+                            bem_c64* __restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const int q = (int)(t % Q);
+    bem_c64 acc = x[t];
+    for (int l = 0; l < cnt; l++) acc = c64_fma(V[(int64_t)l * n * Q + t], y[l * Q + q], acc);
+    x[t] = acc;
+}
+
+// V column 0 = r / beta (per right-hand side; 0 where beta = 0)
+template <int Q>
+__global__ void k_scale_first_q(int64_t n, const bem_c64* __restrict__ r, const double* __restrict__ beta,
+                                bem_c64* __restrict__ V)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const double d = beta[t % Q];
+    V[t] = d > 0 ? c64_scale(r[t], 1.0 / d) : c64(0, 0);
+}
+
+// r = b - r over n Q entries
+__global__ void k_residual_n(int64_t len, const bem_c64* __restrict__ b, bem_c64* __restrict__ r)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < len) r[t] = c64_sub(b[t], r[t]);
+}
+
+// dst[i Q + q] = src[i ld + q] (q < nq), 0 for nq <= q < Q
+__global__ void k_repack(int64_t n, int nq, int64_t ld, int Q, const bem_c64* __restrict__ src, bem_c64* __restrict__ dst)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const int64_t i = t / Q;
+    const int q = (int)(t % Q);
+    dst[t] = q < nq ? src[i * ld + q] : c64(0, 0);
 }
 
 // ------------------------------------------------------------ host side
@@ -1681,7 +2154,12 @@ struct bem_ctx {
     uint64_t gen = 0;                    // bumped by every change of mesh, partition, BC, frequency, plane
     bem_c64* d_sing2 = nullptr;          // the kept DPhi singular table
     void* st_dense = nullptr;            // the kept DPhi dense blocks
+    void* st_dense64 = nullptr;          // and their FP64 form (FP32 operators with near64)
     int st_prec = -1;
+    bool st_near64 = false;
+    // FP32 assemblies form the near field in FP64 and keep it beside the FP32
+    // copy (bem_set_near_fp64): the refinement residual's operator
+    bool near64 = false;
     uint64_t st_gen = 0;
     const bem_c64* sing_cur = nullptr;   // table the running assembly's ACA reads
     int64_t nnz = 0;
@@ -1719,6 +2197,15 @@ struct bem_ctx {
     void* scratch_dev = nullptr;
     // GMRES: residual estimates of the last two Arnoldi columns
     double* gm_res = nullptr;
+    // several right-hand sides: Q-interleaved partial vectors (part_total x
+    // partq_cap), the lockstep GMRES's pinned residual ring (2 x kArnMax),
+    // matrix cores for Q >= 4 products, graph-captured Arnoldi iterations
+    bem_c64* d_partq = nullptr;
+    int partq_cap = 0;
+    double* gm_resq = nullptr;
+    bool use_mfma = true;
+    bool use_graphs = true;
+    bool capturing = false;   // Timer records no events while a graph is captured
     hipEvent_t gm_ev[2] = {nullptr, nullptr};
 };
 
@@ -1727,6 +2214,7 @@ struct bem_hmat {
     int kind = 0, prec = 0;
     size_t esz = sizeof(bem_c64);   // bytes per stored entry (FP64 or FP32 complex)
     void* dense = nullptr;
+    void* dense64 = nullptr;   // FP64 near field of an FP32 operator (bem_set_near_fp64), else null
     void *U = nullptr, *V = nullptr, *A = nullptr;
     void* meta = nullptr;    // one allocation holding the offset, rank and task arrays below
     int64_t *d_uoff = nullptr, *d_voff = nullptr, *d_aoff = nullptr;
@@ -1737,6 +2225,9 @@ struct bem_hmat {
     int64_t* d_toff = nullptr;
     bem_c64* d_t = nullptr;
     int maxrank = 1;
+    int64_t t_total = 0;      // sum of the ranks (entries of d_t)
+    bem_c64* d_tq = nullptr;  // t vectors of Q right-hand sides (t_total x tq_cap)
+    int tq_cap = 0;
     std::vector<int64_t> uoff, voff, aoff;
     std::vector<int32_t> rank;
     double stats[9] = {0};
@@ -1812,7 +2303,7 @@ void resolve_pending(bem_ctx* c)
 
 Timer::Timer(bem_ctx* c_, int s, hipStream_t stream) : c(c_), slot(s), st(stream ? stream : c_->stream)
 {
-    if (c->prof) {
+    if (c->prof && !c->capturing) {
         e0 = pool_get(c);
         e1 = pool_get(c);
         (void)hipEventRecord(e0, st);
@@ -1821,7 +2312,7 @@ Timer::Timer(bem_ctx* c_, int s, hipStream_t stream) : c(c_), slot(s), st(stream
 
 Timer::~Timer()
 {
-    if (c->prof) {
+    if (e1) {
         (void)hipEventRecord(e1, st);
         c->pending.push_back(bem_ctx::Pending{slot, e0, e1});
         if (c->pending.size() > 4096) resolve_pending(c);
@@ -1984,13 +2475,23 @@ DevFreq32 dev_freq32(const bem_ctx* c)
 }
 
 // a change of the operators: the kept DPhi near field no longer applies
-void bump(bem_ctx* c)
+// release a kept DPhi near field
+void drop_kept(bem_ctx* c)
 {
-    c->gen++;
     if (c->st_dense) {
         pfree(c, c->st_dense);
         c->st_dense = nullptr;
     }
+    if (c->st_dense64) {
+        pfree(c, c->st_dense64);
+        c->st_dense64 = nullptr;
+    }
+}
+
+void bump(bem_ctx* c)
+{
+    c->gen++;
+    drop_kept(c);
 }
 
 // vertices a, b, c as 9 SoA rows, and the FP32 path's edge form c, a - c, b - c
@@ -2089,6 +2590,7 @@ int bem_create(bem_ctx_t* out, int device)
         hipEventCreateWithFlags(&c->gm_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->gm_ev[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&c->gm_res, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->gm_resq, 2 * kArnMax * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         scratch_address(device, &c->scratch_dev) != hipSuccess) {
         delete c;
         return BEM_ERR_HIP;
@@ -2108,7 +2610,7 @@ int bem_destroy(bem_ctx_t c)
     free_list(c, c->allocs_part);
     free_list(c, c->allocs_piv);
     for (void* p : {(void*)c->d_alpha, (void*)c->d_beta, (void*)c->d_f, (void*)c->d_subst, (void*)c->d_cphi,
-                    (void*)c->d_cd, (void*)c->d_cphi32, (void*)c->d_cd32, (void*)c->d_gm})
+                    (void*)c->d_cd, (void*)c->d_cphi32, (void*)c->d_cd32, (void*)c->d_gm, (void*)c->d_partq})
         pfree(c, p);
     for (auto& kv : c->live) (void)hipFree(kv.first);
     c->live.clear();
@@ -2122,6 +2624,7 @@ int bem_destroy(bem_ctx_t c)
     (void)hipEventDestroy(c->stage_done);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->gm_res) (void)hipHostFree(c->gm_res);
+    if (c->gm_resq) (void)hipHostFree(c->gm_resq);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return BEM_OK;
@@ -2225,6 +2728,14 @@ int bem_set_near_pairing(bem_ctx_t c, int on)
     if (!c) return BEM_ERR_ARG;
     c->pair_near = on != 0;
     if (!c->pair_near) bump(c);
+    return BEM_OK;
+}
+
+int bem_set_near_fp64(bem_ctx_t c, int on)
+{
+    if (!c) return BEM_ERR_ARG;
+    if (c->near64 != (on != 0)) bump(c);   // a kept near field of the other form no longer applies
+    c->near64 = on != 0;
     return BEM_OK;
 }
 
@@ -2427,6 +2938,9 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
     if ((rc = dupload(c, &c->d_gptr, gptr.data(), gptr.size(), ap))) return rc;
     if ((rc = dupload(c, &c->d_gidx, gidx.data(), gidx.size(), ap))) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
+    pfree(c, c->d_partq);   // sized for the previous partition
+    c->d_partq = nullptr;
+    c->partq_cap = 0;
     c->have_part = true;
     bump(c);
     return BEM_OK;
@@ -2438,7 +2952,8 @@ int bem_hmat_destroy(bem_hmat_t h)
     (void)hipSetDevice(h->ctx->device);
     (void)hipStreamSynchronize(h->ctx->stream);
     for (hipStream_t t : h->ctx->side) (void)hipStreamSynchronize(t);
-    for (void* p : {(void*)h->dense, (void*)h->U, (void*)h->V, (void*)h->A, h->meta, (void*)h->d_t})
+    for (void* p : {(void*)h->dense, h->dense64, (void*)h->U, (void*)h->V, (void*)h->A, h->meta, (void*)h->d_t,
+                    (void*)h->d_tq})
         pfree(h->ctx, p);
     for (auto* v : {&h->uoff, &h->voff, &h->aoff}) h->ctx->vp.give(*v);
     h->ctx->vp.give(h->rank);
@@ -2632,30 +3147,41 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     int rc;
     // near-field pairing: a DPhi assembly takes the near field the Phi
     // assembly of the same state formed; a Phi assembly forms both
-    const bool adopt = kind == BEM_KIND_DPHI && c->st_dense && c->st_gen == c->gen && c->st_prec == precision;
+    // FP32 operator with its near field formed in FP64 (kept for the
+    // refinement residual, rounded for the FP32 copy)
+    const bool n64 = f32 && c->near64;
+    const bool adopt = kind == BEM_KIND_DPHI && c->st_dense && c->st_gen == c->gen && c->st_prec == precision &&
+                       c->st_near64 == n64;
     const bool pairing = c->pair_near && kind == BEM_KIND_PHI;
-    void* dense2 = nullptr;
+    void *dense2 = nullptr, *dense2_64 = nullptr;
     auto fail = [&](int code) {
         c->sing_cur = nullptr;
         if (dense2) pfree(c, dense2);
+        if (dense2_64) pfree(c, dense2_64);
         bem_hmat_destroy(h);
         return code;
     };
     // dense near field
     if (adopt) {
         h->dense = c->st_dense;
+        h->dense64 = c->st_dense64;
         c->st_dense = nullptr;
+        c->st_dense64 = nullptr;
     } else {
-        if (c->st_dense) {   // kept for another state or precision
-            pfree(c, c->st_dense);
-            c->st_dense = nullptr;
-        }
+        drop_kept(c);   // kept for another state or precision
         if ((rc = dalloc_bytes(c, &h->dense, (size_t)c->dense_total * h->esz))) return fail(rc);
+        if (n64 && (rc = dalloc_bytes(c, &h->dense64, (size_t)c->dense_total * sizeof(bem_c64)))) return fail(rc);
     }
     if (pairing && (rc = dalloc_bytes(c, &dense2, (size_t)c->dense_total * h->esz))) return fail(rc);
+    if (pairing && n64 && (rc = dalloc_bytes(c, &dense2_64, (size_t)c->dense_total * sizeof(bem_c64))))
+        return fail(rc);
     c->sing_cur = adopt ? c->d_sing2 : c->d_sing;
-    bem_c64* outPhi = kind == BEM_KIND_PHI ? (bem_c64*)h->dense : nullptr;
-    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? (bem_c64*)h->dense : (bem_c64*)dense2;
+    // FP64 near-field targets: the operator's own storage (FP64 path) or its
+    // kept FP64 form (FP32 with near64)
+    bem_c64* own64 = (bem_c64*)(n64 ? h->dense64 : h->dense);
+    bem_c64* pair64 = (bem_c64*)(n64 ? dense2_64 : dense2);
+    bem_c64* outPhi = kind == BEM_KIND_PHI ? own64 : nullptr;
+    bem_c64* outDPhi = kind == BEM_KIND_DPHI ? own64 : pair64;
     DevFreq F = dev_freq(c);
     DevFreq32 F32 = dev_freq32(c);
     hipEvent_t ev0, ev1, ev2;
@@ -2677,7 +3203,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
             Timer t(c, 0);
             const dim3 g((unsigned)c->tiles.size());
             const bool plane = c->mesh.rvtx != nullptr;
-            if (f32)
+            if (f32 && !n64)
                 hipLaunchKernelGGL(plane ? k_dense32<true> : k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh,
                                    F32, c->d_subst, kind, c->d_tiles, (bem_c32*)h->dense, (bem_c32*)dense2);
             else
@@ -2689,15 +3215,23 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
             dim3 g(nblk((int64_t)c->fixes.size(), kWG));
             for (int q = 0; q < (dense2 ? 2 : 1); q++) {
                 const bem_c64* tab = q ? c->d_sing2 : c->d_sing;
-                void* dst = q ? dense2 : h->dense;
-                if (f32)
+                if (f32 && !n64)
                     hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
-                                       (int64_t)c->fixes.size(), c->nnz, tab, (bem_c32*)dst);
+                                       (int64_t)c->fixes.size(), c->nnz, tab, (bem_c32*)(q ? dense2 : h->dense));
                 else
                     hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
-                                       (int64_t)c->fixes.size(), c->nnz, tab, (bem_c64*)dst);
+                                       (int64_t)c->fixes.size(), c->nnz, tab, q ? pair64 : own64);
                 if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
             }
+        }
+        if (n64 && c->dense_total) {   // the FP32 copies GMRES iterates with
+            const dim3 g(nblk(c->dense_total, 256));
+            hipLaunchKernelGGL(k_round32, g, dim3(256), 0, c->stream, c->dense_total, (const bem_c64*)h->dense64,
+                               (bem_c32*)h->dense);
+            if (dense2)
+                hipLaunchKernelGGL(k_round32, g, dim3(256), 0, c->stream, c->dense_total, (const bem_c64*)dense2_64,
+                                   (bem_c32*)dense2);
+            if ((rc2 = launch_check(c, "k_round32"))) return rc2;
         }
         return BEM_OK;
     };
@@ -2835,6 +3369,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         }
     }
     h->nttask = ntt;
+    h->t_total = tot;
     h->nutask = nchunks;
     Packer pk;
     size_t k_uoff = pk.add(h->uoff), k_voff = pk.add(h->voff), k_aoff = pk.add(h->aoff);
@@ -2897,14 +3432,18 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     h->stats[5] = (double)over.size();
     h->stats[6] = ms_dense;
     h->stats[7] = ms_aca;
-    h->stats[8] = (double)((c->dense_total + (direct ? S1.tu + S1.tv + S1.ta : tu + tv + ta)) * (int64_t)h->esz);
+    h->stats[8] = (double)((c->dense_total + (direct ? S1.tu + S1.tv + S1.ta : tu + tv + ta)) * (int64_t)h->esz +
+                           (h->dense64 ? c->dense_total * (int64_t)sizeof(bem_c64) : 0));
     for (auto* v : {&cap, &rank, &flag, &rank2, &flag2, &tt, &ut}) c->vp.give(*v);
     for (auto* v : {&evs, &evs2, &toff}) c->vp.give(*v);
     if (pairing) {   // the DPhi near field waits for the next DPhi assembly of this state
         c->st_dense = dense2;
+        c->st_dense64 = dense2_64;
         c->st_gen = c->gen;
         c->st_prec = precision;
+        c->st_near64 = n64;
         dense2 = nullptr;
+        dense2_64 = nullptr;
     }
     c->sing_cur = nullptr;
     *out = h;
@@ -2934,6 +3473,31 @@ int hmvm_blocks(bem_ctx* c, bem_hmat* h, const bem_c64* xd)
         hipLaunchKernelGGL(k_lr_u<S>, dim3((unsigned)h->nutask), dim3(kWG), h->maxrank * sizeof(bem_c64), c->stream,
                            h->d_utask, c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part, (const S*)h->U,
                            h->d_t, c->d_part);
+        if ((rc = launch_check(c, "k_lr"))) return rc;
+    }
+    return BEM_OK;
+}
+
+// The refinement operator of an FP32 H-matrix with a kept FP64 near field:
+// the near-field tiles from the FP64 blocks, the low-rank blocks from the
+// FP32 factors, all accumulated in FP64
+int hmvm_blocks_near64(bem_ctx* c, bem_hmat* h, const bem_c64* xd)
+{
+    int rc;
+    if (!c->tiles.empty()) {
+        Timer t(c, 3);
+        hipLaunchKernelGGL(k_mvm_dense<bem_c64>, dim3((unsigned)c->tiles.size()), dim3(kWG), 0, c->stream,
+                           c->d_tiles, (const bem_c64*)h->dense64, xd, c->d_part);
+        if ((rc = launch_check(c, "k_mvm_dense"))) return rc;
+    }
+    if (h->nttask > 0) {
+        Timer t(c, 4);
+        hipLaunchKernelGGL(k_lr_t<bem_c32>, dim3(nblk(h->nttask, kWG / 64)), dim3(kWG), 0, c->stream, h->d_ttask,
+                           h->nttask, c->d_col0, c->d_cols, h->d_rank, h->d_voff, h->d_aoff, h->d_toff,
+                           (const bem_c32*)h->V, (const bem_c32*)h->A, xd, h->d_t);
+        hipLaunchKernelGGL(k_lr_u<bem_c32>, dim3((unsigned)h->nutask), dim3(kWG), h->maxrank * sizeof(bem_c64),
+                           c->stream, h->d_utask, c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part,
+                           (const bem_c32*)h->U, h->d_t, c->d_part);
         if ((rc = launch_check(c, "k_lr"))) return rc;
     }
     return BEM_OK;
@@ -2976,6 +3540,22 @@ int bem_hmvm(bem_ctx_t c, bem_hmat_t h, const void* x, void* y, int accumulate)
         if ((rc = launch_check(c, "k_mvm_gather"))) return rc;
     }
     return BEM_OK;
+}
+
+int bem_hmvm_refine(bem_ctx_t c, bem_hmat_t h, const void* x, void* y, int accumulate)
+{
+    if (!c || !h || !x || !y || h->ctx != c) return set_err(c, BEM_ERR_ARG, "bem_hmvm_refine: bad arguments");
+    if (h->prec == BEM_PREC_FP64) return bem_hmvm(c, h, x, y, accumulate);
+    if (!h->dense64)
+        return set_err(c, BEM_ERR_STATE, "bem_hmvm_refine: FP32 operator without an FP64 near field "
+                                         "(assemble after bem_set_near_fp64(ctx, 1))");
+    HIPC(c, hipSetDevice(c->device));
+    int rc = hmvm_blocks_near64(c, h, (const bem_c64*)x);
+    if (rc) return rc;
+    Timer t(c, 5);
+    hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(16 * c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr, c->d_gidx,
+                       c->d_part, (bem_c64*)y, accumulate);
+    return launch_check(c, "k_mvm_gather");
 }
 
 int bem_block_info(bem_hmat_t h, int64_t b, int64_t* rank)
@@ -3291,6 +3871,16 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
     return launch_check(c, "k_blu_apply");
 }
 
+}  // extern "C"
+
+namespace {
+template <int Q>
+int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
+                   int64_t maxit, int64_t* its_out, double* rel_out);
+}  // namespace
+
+extern "C" {
+
 int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int64_t restart, int64_t maxit,
               int64_t* iters, double* rel_resid)
 {
@@ -3303,6 +3893,16 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
     if (!c || !h || !b || !x || restart < 1 || maxit < 0 || h->ctx != c || (pc && pc->ctx != c))
         return set_err(c, BEM_ERR_ARG, "bem_gmres: bad arguments");
     HIPC(c, hipSetDevice(c->device));
+    if (!pc && c->use_graphs) {   // the same iteration as below, one column per replayed graph
+        int64_t its = 0;
+        double rel = 0;
+        int rc = gmres_lockstep<1>(c, h, (const bem_c64*)b, (bem_c64*)x, &tol, restart, maxit, &its, &rel);
+        if (rc) return rc;
+        if (iters) *iters = its;
+        if (rel_resid) *rel_resid = rel;
+        if (rel >= tol) return set_err(c, BEM_ERR_NOCONV, "bem_gmres: maxit reached above tolerance");
+        return BEM_OK;
+    }
     const int64_t n = c->n;
     const int64_t m = restart;
     const int nchunk = (int)((n + kDotChunk - 1) / kDotChunk);
@@ -3446,6 +4046,606 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
     if (iters) *iters = it;
     if (rel_resid) *rel_resid = rel;
     if (rel >= tol) return set_err(c, BEM_ERR_NOCONV, "bem_gmres: maxit reached above tolerance");
+    return BEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int pow2_q(int64_t nrhs) { return nrhs <= 1 ? 1 : nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : 8; }
+
+// Q-interleaved partial vectors (context) and t vectors (operator) of at
+// least Q right-hand sides
+int ensure_multi(bem_ctx* c, bem_hmat* h, int Q)
+{
+    int rc;
+    if (c->partq_cap < Q) {
+        pfree(c, c->d_partq);
+        c->d_partq = nullptr;
+        c->partq_cap = 0;
+        if ((rc = dalloc(c, &c->d_partq, (size_t)std::max<int64_t>(c->part_total, 1) * Q, nullptr))) return rc;
+        c->partq_cap = Q;
+    }
+    if (h->tq_cap < Q) {
+        pfree(c, h->d_tq);
+        h->d_tq = nullptr;
+        h->tq_cap = 0;
+        if ((rc = dalloc(c, &h->d_tq, (size_t)std::max<int64_t>(h->t_total, 1) * Q, nullptr))) return rc;
+        h->tq_cap = Q;
+    }
+    return BEM_OK;
+}
+
+// Y (+)= H X for Q interleaved right-hand sides; DS: storage of the near
+// field read (dense), LS: storage of the low-rank factors
+template <class DS, class LS, int Q>
+int hmvm_q(bem_ctx* c, bem_hmat* h, const void* dense, const bem_c64* X, bem_c64* Y, int accumulate)
+{
+    int rc;
+    bem_c64* part = Q == 1 ? c->d_part : c->d_partq;
+    bem_c64* tv = Q == 1 ? h->d_t : h->d_tq;
+    if (Q > 1 && (rc = ensure_multi(c, h, Q))) return rc;
+    if (Q > 1) {
+        part = c->d_partq;
+        tv = h->d_tq;
+    }
+    const bool mf = c->use_mfma && Q >= 4;
+    if (!c->tiles.empty()) {
+        Timer t(c, 3);
+        const dim3 g((unsigned)c->tiles.size());
+        if (Q == 1)
+            hipLaunchKernelGGL(k_mvm_dense<DS>, g, dim3(kWG), 0, c->stream, c->d_tiles, (const DS*)dense, X, part);
+        else if (mf)
+            hipLaunchKernelGGL((k_mvm_dense_mfma<DS, Q>), g, dim3(kWG), 0, c->stream, c->d_tiles, (const DS*)dense, X,
+                               part);
+        else
+            hipLaunchKernelGGL((k_mvm_dense_q<DS, Q>), g, dim3(kWG), 0, c->stream, c->d_tiles, (const DS*)dense, X,
+                               part);
+        if ((rc = launch_check(c, "k_mvm_dense"))) return rc;
+    }
+    if (h->nttask > 0) {
+        Timer t(c, 4);
+        const dim3 gt(nblk(h->nttask, kWG / 64)), gu((unsigned)h->nutask);
+        if (Q == 1) {
+            hipLaunchKernelGGL(k_lr_t<LS>, gt, dim3(kWG), 0, c->stream, h->d_ttask, h->nttask, c->d_col0, c->d_cols,
+                               h->d_rank, h->d_voff, h->d_aoff, h->d_toff, (const LS*)h->V, (const LS*)h->A, X, tv);
+            hipLaunchKernelGGL(k_lr_u<LS>, gu, dim3(kWG), h->maxrank * sizeof(bem_c64), c->stream, h->d_utask,
+                               c->d_rows, h->d_rank, h->d_uoff, h->d_toff, c->d_lr_part, (const LS*)h->U, tv, part);
+        } else {
+            hipLaunchKernelGGL((k_lr_t_q<LS, Q>), gt, dim3(kWG), 0, c->stream, h->d_ttask, h->nttask, c->d_col0,
+                               c->d_cols, h->d_rank, h->d_voff, h->d_aoff, h->d_toff, (const LS*)h->V,
+                               (const LS*)h->A, X, tv);
+            if (mf)
+                hipLaunchKernelGGL((k_lr_u_mfma<LS, Q>), gu, dim3(kWG), 0, c->stream, h->d_utask, c->d_rows, h->d_rank,
+                                   h->d_uoff, h->d_toff, c->d_lr_part, (const LS*)h->U, tv, part);
+            else
+                hipLaunchKernelGGL((k_lr_u_q<LS, Q>), gu, dim3(kWG), 0, c->stream, h->d_utask, c->d_rows, h->d_rank,
+                                   h->d_uoff, h->d_toff, c->d_lr_part, (const LS*)h->U, tv, part);
+        }
+        if ((rc = launch_check(c, "k_lr"))) return rc;
+    }
+    Timer t(c, 5);
+    if (Q == 1)
+        hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(16 * c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr,
+                           c->d_gidx, part, Y, accumulate);
+    else
+        hipLaunchKernelGGL(k_mvm_gather_q<Q>, dim3(nblk(16 * c->n, 256)), dim3(256), 0, c->stream, c->n, c->d_gptr,
+                           c->d_gidx, part, Y, accumulate);
+    return launch_check(c, "k_mvm_gather");
+}
+
+// the product with the operator's own storage (refine = false) or with its
+// refinement operator (FP32 blocks with their FP64 near field)
+template <int Q>
+int hmvm_multi(bem_ctx* c, bem_hmat* h, bool refine, const bem_c64* X, bem_c64* Y, int accumulate)
+{
+    if (h->prec == BEM_PREC_FP64) return hmvm_q<bem_c64, bem_c64, Q>(c, h, h->dense, X, Y, accumulate);
+    if (refine) {
+        if (!h->dense64)
+            return set_err(c, BEM_ERR_STATE, "refinement product: FP32 operator without an FP64 near field");
+        return hmvm_q<bem_c64, bem_c32, Q>(c, h, h->dense64, X, Y, accumulate);
+    }
+    return hmvm_q<bem_c32, bem_c32, Q>(c, h, h->dense, X, Y, accumulate);
+}
+
+// |v_q| of Q interleaved vectors into nrm[0 .. Q) (device), using part
+template <int Q>
+int norms_q(bem_ctx* c, const bem_c64* v, double* part, double* nrm)
+{
+    const int npart = (int)std::min<int64_t>(256, (c->n + kWG - 1) / kWG);
+    hipLaunchKernelGGL(k_norm2_part_q<Q>, dim3(npart), dim3(kWG), 0, c->stream, c->n, v, part);
+    hipLaunchKernelGGL(k_norm2_final_q<Q>, dim3(1), dim3(64), 0, c->stream, npart, part, nrm);
+    return launch_check(c, "norms");
+}
+
+// Restarted GMRES(m) in lockstep over Q interleaved right-hand sides B
+// (start and solution X, in place): one product of the operator per Arnoldi
+// column serves all Q; CGS2, Givens rotations and the residual estimates per
+// right-hand side on the device as bem_gmres_prec.  Right-hand side q stops
+// at the first column whose estimate is below tol[q] (its later columns are
+// computed but not used); every restart recomputes b_q - A x_q.  With
+// graphs on, one column's launches are captured once per call and replayed.
+template <int Q>
+int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
+                   int64_t maxit, int64_t* its_out, double* rel_out)
+{
+    const int64_t n = c->n;
+    const int nchunk = (int)((n + kDotChunk - 1) / kDotChunk);
+    const int npart = (int)std::min<int64_t>(256, (n + kWG - 1) / kWG);
+    std::vector<void*> own;
+    auto fail = [&](int code) {
+        free_list(c, own);
+        return code;
+    };
+    bem_c64 *Vb, *w, *r, *xb, *dh, *dhacc, *dH, *dg, *dss, *dy, *dpart;
+    double *dcc, *dnp, *dnrm, *dbn, *dres;
+    int* jp;
+    int rc;
+    // buffers of the product exist before a graph is captured (no allocation inside)
+    if (Q > 1 && (rc = ensure_multi(c, h, Q))) return rc;
+    if ((rc = dalloc(c, &Vb, (size_t)((m + 1) * n * Q), &own)) || (rc = dalloc(c, &w, (size_t)(n * Q), &own)) ||
+        (rc = dalloc(c, &r, (size_t)(n * Q), &own)) || (rc = dalloc(c, &xb, (size_t)(n * Q), &own)) ||
+        (rc = dalloc(c, &dh, (size_t)((m + 2) * Q), &own)) || (rc = dalloc(c, &dhacc, (size_t)((m + 2) * Q), &own)) ||
+        (rc = dalloc(c, &dH, (size_t)(Q * (m + 1) * m), &own)) || (rc = dalloc(c, &dg, (size_t)(Q * (m + 2)), &own)) ||
+        (rc = dalloc(c, &dss, (size_t)(Q * (m + 2)), &own)) || (rc = dalloc(c, &dy, (size_t)(m * Q + 1), &own)) ||
+        (rc = dalloc(c, &dpart, (size_t)((m + 1) * nchunk * Q), &own)) ||
+        (rc = dalloc(c, &dcc, (size_t)(Q * (m + 2)), &own)) || (rc = dalloc(c, &dnp, (size_t)(npart * Q), &own)) ||
+        (rc = dalloc(c, &dnrm, (size_t)Q, &own)) || (rc = dalloc(c, &dbn, (size_t)Q, &own)) ||
+        (rc = dalloc(c, &dres, (size_t)Q, &own)) || (rc = dalloc(c, &jp, (size_t)1, &own)))
+        return fail(rc);
+    double bnh[Q], beta[Q];
+    auto fetch = [&](double* dst, const double* src, int cnt) -> int {
+        HIPC(c, hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return BEM_OK;
+    };
+    if ((rc = norms_q<Q>(c, B, dnp, dbn)) || (rc = fetch(bnh, dbn, Q))) return fail(rc);
+    // one Arnoldi column for every right-hand side (column *jp)
+    const int ng = (int)(((m + 1) + std::max(1, kDotVecs / Q) - 1) / std::max(1, kDotVecs / Q));
+    auto iteration = [&]() -> int {
+        int rc2;
+        hipLaunchKernelGGL(k_basis_col<Q>, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n, jp, Vb, xb);
+        if ((rc2 = hmvm_multi<Q>(c, h, false, xb, w, 0))) return rc2;
+        for (int pass = 0; pass < 2; pass++) {   // classical Gram-Schmidt, twice
+            hipLaunchKernelGGL(k_cgs_dots_q<Q>, dim3(nchunk, (ng + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0,
+                               c->stream, n, jp, Vb, w, dpart);
+            hipLaunchKernelGGL(k_cgs_reduce_q<Q>, dim3((unsigned)(m + 1)), dim3(64), 0, c->stream, jp, nchunk, dpart,
+                               dh, dhacc, pass == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_orth_update_q<Q>, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n, jp, Vb, dh, w);
+        }
+        if ((rc2 = norms_q<Q>(c, w, dnp, dnrm))) return rc2;
+        hipLaunchKernelGGL(k_scale_next_q<Q>, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n, jp, w, dnrm, Vb);
+        hipLaunchKernelGGL(k_arnoldi_q<Q>, dim3(1), dim3(64), 0, c->stream, jp, (int)m, dhacc, dnrm, dH, dcc, dss, dg,
+                           dbn, dres);
+        return launch_check(c, "arnoldi");
+    };
+    hipGraphExec_t gexec = nullptr;
+    auto cleanup = [&](int code) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        return fail(code);
+    };
+    if (c->use_graphs) {
+        hipGraph_t graph = nullptr;
+        c->capturing = true;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        int rc2 = e == hipSuccess ? iteration() : BEM_OK;
+        hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(c->stream, &graph) : e;
+        c->capturing = false;
+        if (e != hipSuccess || e2 != hipSuccess || rc2) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return cleanup(rc2 ? rc2 : set_err(c, BEM_ERR_HIP, std::string("GMRES graph capture: ") +
+                                                                   hipGetErrorString(e != hipSuccess ? e : e2)));
+        }
+        e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            gexec = nullptr;
+            return cleanup(set_err(c, BEM_ERR_HIP, std::string("GMRES graph instantiate: ") + hipGetErrorString(e)));
+        }
+    }
+    auto launch_col = [&](int64_t jj) -> int {
+        if (gexec) {
+            HIPC(c, hipGraphLaunch(gexec, c->stream));
+        } else {
+            int rc2 = iteration();
+            if (rc2) return rc2;
+        }
+        HIPC(c, hipMemcpyAsync(c->gm_resq + (jj & 1) * kArnMax, dres, Q * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIPC(c, hipEventRecord(c->gm_ev[jj & 1], c->stream));
+        return BEM_OK;
+    };
+    bool done[Q];
+    int64_t its[Q];
+    double rel[Q];
+    for (int q = 0; q < Q; q++) {
+        done[q] = !(bnh[q] > 0);
+        its[q] = 0;
+        rel[q] = 0;
+    }
+    std::vector<bem_c64> H((size_t)Q * (m + 1) * m), g((size_t)Q * (m + 2)), y((size_t)m * Q + 1);
+    int64_t it = 0;
+    while (true) {
+        // r = b - A x per right-hand side
+        if ((rc = hmvm_multi<Q>(c, h, false, X, r, 0))) return cleanup(rc);
+        hipLaunchKernelGGL(k_residual_n, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, B, r);
+        if ((rc = norms_q<Q>(c, r, dnp, dnrm)) || (rc = fetch(beta, dnrm, Q))) return cleanup(rc);
+        bool all = true;
+        for (int q = 0; q < Q; q++) {
+            rel[q] = bnh[q] > 0 ? beta[q] / bnh[q] : 0.0;
+            if (rel[q] < tol[q]) done[q] = true;
+            all = all && done[q];
+        }
+        if (all || it >= maxit) break;
+        hipLaunchKernelGGL(k_scale_first_q<Q>, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n, r, dnrm, Vb);
+        std::fill(g.begin(), g.end(), c64(0, 0));
+        for (int q = 0; q < Q; q++) g[(size_t)q * (m + 2)] = c64(beta[q], 0);
+        const int zero = 0;
+        HIPC(c, hipMemcpyAsync(dg, g.data(), g.size() * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipMemcpyAsync(jp, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        // one column kept in flight ahead of the convergence check (the
+        // columns after a right-hand side's last one are not used for it)
+        const int64_t cap = std::min<int64_t>(m, maxit - it);
+        int64_t conv[Q];
+        for (int q = 0; q < Q; q++) conv[q] = -1;
+        int64_t j = 0, nl = 0;
+        if ((rc = launch_col(0))) return cleanup(rc);
+        nl = 1;
+        while (j < nl) {
+            if (nl < cap) {
+                if ((rc = launch_col(nl))) return cleanup(rc);
+                nl++;
+            }
+            HIPC(c, hipEventSynchronize(c->gm_ev[j & 1]));
+            const double* rs = c->gm_resq + (j & 1) * kArnMax;
+            bool allc = true;
+            for (int q = 0; q < Q; q++) {
+                if (!done[q] && conv[q] < 0 && rs[q] < tol[q]) conv[q] = j;
+                allc = allc && (done[q] || conv[q] >= 0);
+            }
+            j++;
+            if (allc) break;
+        }
+        HIPC(c, hipStreamSynchronize(c->stream));
+        it += j;
+        HIPC(c, hipMemcpyAsync(H.data(), dH, H.size() * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipMemcpyAsync(g.data(), dg, g.size() * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        std::fill(y.begin(), y.end(), c64(0, 0));
+        for (int q = 0; q < Q; q++) {
+            const int64_t jq = done[q] ? 0 : (conv[q] >= 0 ? conv[q] + 1 : j);
+            its[q] += jq;
+            const bem_c64* Hq = H.data() + (size_t)q * m * (m + 1);
+            const bem_c64* gq = g.data() + (size_t)q * (m + 2);
+            for (int64_t l = jq - 1; l >= 0; l--) {   // y = R^-1 g
+                bem_c64 s = gq[l];
+                for (int64_t k2 = l + 1; k2 < jq; k2++) s = c64_sub(s, c64_mul(Hq[k2 * (m + 1) + l], y[k2 * Q + q]));
+                y[l * Q + q] = c64_div(s, Hq[l * (m + 1) + l]);
+            }
+        }
+        HIPC(c, hipMemcpyAsync(dy, y.data(), (size_t)j * Q * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_combine_q<Q>, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n, (int)j, Vb, dy, X);
+        if ((rc = launch_check(c, "k_combine_q"))) return cleanup(rc);
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < Q; q++) {
+        if (its_out) its_out[q] = its[q];
+        if (rel_out) rel_out[q] = rel[q];
+    }
+    return cleanup(BEM_OK);
+}
+
+// n x nq (row stride ld) <-> n x Q interleaved copies around the Q-templated code
+int repack(bem_ctx* c, int64_t nq, int64_t ld, int Q, const bem_c64* src, bem_c64* dst)
+{
+    hipLaunchKernelGGL(k_repack, dim3(nblk(c->n * Q, 256)), dim3(256), 0, c->stream, c->n, (int)nq, ld, Q, src, dst);
+    return launch_check(c, "k_repack");
+}
+int unpack(bem_ctx* c, int64_t nq, int64_t ld, int Q, const bem_c64* src, bem_c64* dst)
+{
+    hipLaunchKernelGGL(k_repack, dim3(nblk(c->n * nq, 256)), dim3(256), 0, c->stream, c->n, (int)Q, (int64_t)Q,
+                       (int)nq, src, dst);
+    (void)ld;
+    return launch_check(c, "k_repack");
+}
+
+int gmres_lockstep_any(bem_ctx* c, bem_hmat* h, int Q, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
+                       int64_t maxit, int64_t* its, double* rel)
+{
+    switch (Q) {
+    case 1: return gmres_lockstep<1>(c, h, B, X, tol, m, maxit, its, rel);
+    case 2: return gmres_lockstep<2>(c, h, B, X, tol, m, maxit, its, rel);
+    case 4: return gmres_lockstep<4>(c, h, B, X, tol, m, maxit, its, rel);
+    default: return gmres_lockstep<8>(c, h, B, X, tol, m, maxit, its, rel);
+    }
+}
+
+int hmvm_multi_any(bem_ctx* c, bem_hmat* h, int Q, bool refine, const bem_c64* X, bem_c64* Y, int acc)
+{
+    switch (Q) {
+    case 1: return hmvm_multi<1>(c, h, refine, X, Y, acc);
+    case 2: return hmvm_multi<2>(c, h, refine, X, Y, acc);
+    case 4: return hmvm_multi<4>(c, h, refine, X, Y, acc);
+    default: return hmvm_multi<8>(c, h, refine, X, Y, acc);
+    }
+}
+
+int norms_any(bem_ctx* c, int Q, const bem_c64* v, double* part, double* nrm)
+{
+    switch (Q) {
+    case 1: return norms_q<1>(c, v, part, nrm);
+    case 2: return norms_q<2>(c, v, part, nrm);
+    case 4: return norms_q<4>(c, v, part, nrm);
+    default: return norms_q<8>(c, v, part, nrm);
+    }
+}
+
+// Iterative refinement over Q interleaved right-hand sides (the loop of
+// bem_gmres_ir): r = b - A_res x in FP64, inner lockstep GMRES on h with
+// the per-right-hand-side relative tolerances min(0.5, 0.5 tol / rel_q)
+int gmres_ir_q(bem_ctx* c, bem_hmat* h, bem_hmat* hres, int Q, const bem_c64* B, bem_c64* X, bool x_is_zero,
+               double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* its_out, double* rel_out,
+               int64_t* steps_out)
+{
+    const int64_t n = c->n;
+    std::vector<void*> own;
+    bem_c64 *r, *d;
+    double *np, *nrm;
+    int rc;
+    auto done = [&](int code) {
+        free_list(c, own);
+        return code;
+    };
+    if ((rc = dalloc(c, &r, (size_t)(n * Q), &own)) || (rc = dalloc(c, &d, (size_t)(n * Q), &own)) ||
+        (rc = dalloc(c, &np, (size_t)(256 * Q), &own)) || (rc = dalloc(c, &nrm, (size_t)Q, &own)))
+        return done(rc);
+    std::vector<double> bn(Q), rn(Q), rel(Q, 0.0), tq(Q);
+    auto fetch = [&](double* dst) -> int {
+        HIPC(c, hipMemcpyAsync(dst, nrm, Q * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return BEM_OK;
+    };
+    if ((rc = norms_any(c, Q, B, np, nrm)) || (rc = fetch(bn.data()))) return done(rc);
+    std::vector<int64_t> its(Q, 0), inner(Q);
+    int64_t st = 0, total = 0;
+    bool zero = x_is_zero;
+    if (zero) {
+        hipLaunchKernelGGL(k_fill0, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, X);
+        if ((rc = launch_check(c, "k_fill0"))) return done(rc);
+    }
+    while (true) {
+        if (zero) {
+            HIPC(c, hipMemcpyAsync(r, B, (size_t)(n * Q) * sizeof(bem_c64), hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            if ((rc = hmvm_multi_any(c, hres, Q, true, X, r, 0))) return done(rc);
+            hipLaunchKernelGGL(k_residual_n, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, B, r);
+        }
+        if ((rc = norms_any(c, Q, r, np, nrm)) || (rc = fetch(rn.data()))) return done(rc);
+        bool all = true;
+        for (int q = 0; q < Q; q++) {
+            rel[q] = bn[q] > 0 ? rn[q] / bn[q] : 0.0;
+            // converged right-hand sides get an inner tolerance of 1: no columns
+            tq[q] = rel[q] < tol ? 1.0 : std::min(0.5, 0.5 * tol / rel[q]);
+            all = all && rel[q] < tol;
+        }
+        if (all || st >= max_steps || total >= maxit) break;
+        hipLaunchKernelGGL(k_fill0, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, d);
+        rc = gmres_lockstep_any(c, h, Q, r, d, tq.data(), restart, maxit - total, inner.data(), nullptr);
+        if (rc != BEM_OK) return done(rc);
+        int64_t mx = 0;
+        for (int q = 0; q < Q; q++) {
+            its[q] += inner[q];
+            mx = std::max(mx, inner[q]);
+        }
+        total += std::max<int64_t>(mx, 1);
+        hipLaunchKernelGGL(k_axpy1, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, (const bem_c64*)d, X);
+        if ((rc = launch_check(c, "k_axpy1"))) return done(rc);
+        zero = false;
+        st++;
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < Q; q++) {
+        if (its_out) its_out[q] = its[q];
+        if (rel_out) rel_out[q] = rel[q];
+    }
+    if (steps_out) *steps_out = st;
+    return done(BEM_OK);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bem_set_mfma(bem_ctx_t c, int on)
+{
+    if (!c) return BEM_ERR_ARG;
+    c->use_mfma = on != 0;
+    return BEM_OK;
+}
+
+int bem_set_graphs(bem_ctx_t c, int on)
+{
+    if (!c) return BEM_ERR_ARG;
+    c->use_graphs = on != 0;
+    return BEM_OK;
+}
+
+int bem_hmvm_multi(bem_ctx_t c, bem_hmat_t h, int64_t nrhs, const void* x, void* y, int accumulate, int refine)
+{
+    if (!c || !h || !x || !y || h->ctx != c || nrhs < 1 || nrhs > kArnMax)
+        return set_err(c, BEM_ERR_ARG, "bem_hmvm_multi: bad arguments (1 <= nrhs <= 8)");
+    HIPC(c, hipSetDevice(c->device));
+    const int Q = pow2_q(nrhs);
+    if (Q == nrhs) return hmvm_multi_any(c, h, Q, refine != 0, (const bem_c64*)x, (bem_c64*)y, accumulate);
+    std::vector<void*> own;
+    bem_c64 *xq, *yq;
+    int rc;
+    if ((rc = dalloc(c, &xq, (size_t)(c->n * Q), &own)) || (rc = dalloc(c, &yq, (size_t)(c->n * Q), &own)) ||
+        (rc = repack(c, nrhs, nrhs, Q, (const bem_c64*)x, xq)) ||
+        (accumulate && (rc = repack(c, nrhs, nrhs, Q, (const bem_c64*)y, yq))) ||
+        (rc = hmvm_multi_any(c, h, Q, refine != 0, xq, yq, accumulate)) ||
+        (rc = unpack(c, nrhs, nrhs, Q, yq, (bem_c64*)y))) {
+        free_list(c, own);
+        return rc;
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    free_list(c, own);
+    return BEM_OK;
+}
+
+int bem_gmres_multi(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, const void* b, void* x, int x_is_zero,
+                    double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* iters, double* rel_resid,
+                    int64_t* steps)
+{
+    if (!c || !h || !b || !x || h->ctx != c || (hres && hres->ctx != c) || nrhs < 1 || nrhs > kArnMax ||
+        restart < 1 || maxit < 0 || max_steps < 0 || !(tol > 0))
+        return set_err(c, BEM_ERR_ARG, "bem_gmres_multi: bad arguments (1 <= nrhs <= 8)");
+    HIPC(c, hipSetDevice(c->device));
+    const int Q = pow2_q(nrhs);
+    const bool ir = hres != nullptr || max_steps > 0;
+    if (ir && !hres) hres = h;
+    if (ir && hres->prec == BEM_PREC_FP32 && !hres->dense64)
+        return set_err(c, BEM_ERR_STATE, "bem_gmres_multi: the residual operator is FP32 without an FP64 near field");
+    std::vector<void*> own;
+    bem_c64 *bq = (bem_c64*)b, *xq = (bem_c64*)x;
+    int rc = BEM_OK;
+    if (Q != nrhs) {
+        if ((rc = dalloc(c, &bq, (size_t)(c->n * Q), &own)) || (rc = dalloc(c, &xq, (size_t)(c->n * Q), &own)) ||
+            (rc = repack(c, nrhs, nrhs, Q, (const bem_c64*)b, bq)) ||
+            (rc = repack(c, nrhs, nrhs, Q, (const bem_c64*)x, xq))) {
+            free_list(c, own);
+            return rc;
+        }
+    }
+    std::vector<int64_t> its(Q, 0);
+    std::vector<double> rel(Q, 0.0);
+    int64_t st = 0;
+    if (ir) {
+        rc = gmres_ir_q(c, h, hres, Q, bq, xq, x_is_zero != 0, tol, restart, maxit, max_steps, its.data(), rel.data(),
+                        &st);
+    } else {
+        if (x_is_zero) {
+            hipLaunchKernelGGL(k_fill0, dim3(nblk(c->n * Q, 256)), dim3(256), 0, c->stream, c->n * Q, xq);
+            rc = launch_check(c, "k_fill0");
+        }
+        std::vector<double> tq(Q, tol);
+        if (!rc) rc = gmres_lockstep_any(c, h, Q, bq, xq, tq.data(), restart, maxit, its.data(), rel.data());
+    }
+    if (!rc && Q != nrhs) rc = unpack(c, nrhs, nrhs, Q, xq, (bem_c64*)x);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = set_err(c, BEM_ERR_HIP, std::string("bem_gmres_multi: ") + hipGetErrorString(e));
+    }
+    free_list(c, own);
+    if (rc) return rc;
+    bool conv = true;
+    for (int64_t q = 0; q < nrhs; q++) {
+        if (iters) iters[q] = its[q];
+        if (rel_resid) rel_resid[q] = rel[q];
+        conv = conv && rel[q] < tol;
+    }
+    if (steps) *steps = st;
+    if (!conv) return set_err(c, BEM_ERR_NOCONV, "bem_gmres_multi: maxit or max_steps reached above tolerance");
+    return BEM_OK;
+}
+
+int bem_gmres_ir(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, bem_prec_t pc, const void* b, void* x, int x_is_zero,
+                 double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* iters, double* rel_resid,
+                 int64_t* steps)
+{
+    if (!hres) hres = h;
+    if (!c || !h || !b || !x || restart < 1 || maxit < 0 || max_steps < 0 || h->ctx != c || hres->ctx != c ||
+        (pc && pc->ctx != c))
+        return set_err(c, BEM_ERR_ARG, "bem_gmres_ir: bad arguments");
+    if (hres->prec == BEM_PREC_FP32 && !hres->dense64)
+        return set_err(c, BEM_ERR_STATE, "bem_gmres_ir: the residual operator is FP32 without an FP64 near field");
+    HIPC(c, hipSetDevice(c->device));
+    if (!pc) {   // the lockstep solver (graph-captured columns) with one right-hand side
+        int64_t its = 0, st = 0;
+        double rel = 0;
+        int rc = gmres_ir_q(c, h, hres, 1, (const bem_c64*)b, (bem_c64*)x, x_is_zero != 0, tol, restart, maxit,
+                            max_steps, &its, &rel, &st);
+        if (rc) return rc;
+        if (iters) *iters = its;
+        if (rel_resid) *rel_resid = rel;
+        if (steps) *steps = st;
+        if (rel >= tol) return set_err(c, BEM_ERR_NOCONV, "bem_gmres_ir: residual of the refinement operator above tol");
+        return BEM_OK;
+    }
+    const int64_t n = c->n;
+    std::vector<void*> own;
+    bem_c64 *r = nullptr, *d = nullptr;
+    double* nrm = nullptr;
+    int rc;
+    auto done = [&](int code) {
+        free_list(c, own);
+        return code;
+    };
+    if ((rc = dalloc(c, &r, (size_t)n, &own)) || (rc = dalloc(c, &d, (size_t)n, &own)) ||
+        (rc = dalloc(c, &nrm, (size_t)257, &own)))
+        return done(rc);
+    const int npart = (int)std::min<int64_t>(256, (n + kWG - 1) / kWG);
+    // |v| of a device vector, to the host
+    auto norm = [&](const bem_c64* v, double* out) -> int {
+        hipLaunchKernelGGL(k_norm2_part, dim3(npart), dim3(kWG), 0, c->stream, n, v, nrm);
+        hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, c->stream, npart, nrm, nrm + 256);
+        int rc2 = launch_check(c, "norm");
+        if (rc2) return rc2;
+        HIPC(c, hipMemcpyAsync(out, nrm + 256, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return BEM_OK;
+    };
+    double bn = 0, rn = 0;
+    if ((rc = norm((const bem_c64*)b, &bn))) return done(rc);
+    int64_t it = 0, st = 0;
+    double rel = 0;
+    if (bn > 0) {
+        bool zero = x_is_zero != 0;
+        while (true) {
+            // r = b - A_res x, accumulated in FP64 (x = 0: r = b)
+            if (zero) {
+                HIPC(c, hipMemcpyAsync(r, b, (size_t)n * sizeof(bem_c64), hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                if ((rc = bem_hmvm_refine(c, hres, x, r, 0))) return done(rc);
+                hipLaunchKernelGGL(k_residual, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (const bem_c64*)b, r);
+            }
+            if ((rc = norm(r, &rn))) return done(rc);
+            rel = rn / bn;
+            if (rel < tol || st >= max_steps || it >= maxit) break;
+            // d = A^-1 r with GMRES on the inner operator to the relative
+            // tolerance that brings |b - A_res x| under tol |b|, then x += d
+            const double tau = std::min(0.5, 0.5 * tol / rel);
+            hipLaunchKernelGGL(k_fill0, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, d);
+            int64_t its = 0;
+            double rr = 0;
+            rc = bem_gmres_prec(c, h, pc, r, d, tau, restart, maxit - it, &its, &rr);
+            if (rc != BEM_OK && rc != BEM_ERR_NOCONV) return done(rc);
+            it += its;
+            if (zero) {
+                HIPC(c, hipMemcpyAsync(x, d, (size_t)n * sizeof(bem_c64), hipMemcpyDeviceToDevice, c->stream));
+                zero = false;
+            } else {
+                hipLaunchKernelGGL(k_axpy1, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (const bem_c64*)d,
+                                   (bem_c64*)x);
+            }
+            if ((rc = launch_check(c, "k_axpy1"))) return done(rc);
+            st++;
+        }
+        if (zero) {   // no step taken: x = 0
+            hipLaunchKernelGGL(k_fill0, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (bem_c64*)x);
+            if ((rc = launch_check(c, "k_fill0"))) return done(rc);
+        }
+    } else {
+        hipLaunchKernelGGL(k_fill0, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (bem_c64*)x);
+        if ((rc = launch_check(c, "k_fill0"))) return done(rc);
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (iters) *iters = it;
+    if (rel_resid) *rel_resid = rel;
+    if (steps) *steps = st;
+    done(BEM_OK);
+    if (rel >= tol) return set_err(c, BEM_ERR_NOCONV, "bem_gmres_ir: residual of the refinement operator above tol");
     return BEM_OK;
 }
 

@@ -1,12 +1,17 @@
 // GMSH .msh reader for the boundary mesh (mshReader::readMsh,
 // mshreader.cpp:10-428): triangles (element type 2) and quadrilaterals
-// (type 3, split into two triangles) filtered by the reference's
-// include / exclude lists.  The reference parses the ASCII format 2; format
+// (type 3, split into two triangles along the shorter diagonal as
+// global::quadrilateralsToTriangles, global.cpp) filtered by the reference's
+// include / exclude lists; and the edge-length refinement of the triangles
+// (ElementSection::refineTriangles / recursiveRefineTriangle,
+// SolvingScript/elementSection.cpp:688-800, called by
+// SolvingScriptReader::refineElements, solvingscriptreader.cpp:1104-1128).  The reference parses the ASCII format 2; format
 // 4.1 (entity blocks) is read as well, with an element's geometrical group =
 // its entity tag and its physical group = the entity's first physical tag.
 #include <algorithm>
 #include <array>
 #include <cctype>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -22,6 +27,13 @@
 namespace {
 
 struct Range { uint64_t lo, hi; };
+
+// ElementSection::lineLength (elementSection.cpp:1001-1005)
+double line_length(const double* p, const double* q)
+{
+    const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+    return std::sqrt((dx * dx + dy * dy) + dz * dz);
+}
 
 std::string trim(const std::string& s)
 {
@@ -245,13 +257,68 @@ int read_file(const char* path, const char* inc, const char* exc, std::vector<do
         }
         return true;
     };
+    auto dist = [&](uint64_t a, uint64_t b) {
+        const auto &p = nodes[a], &q = nodes[b];
+        return line_length(p.data(), q.data());
+    };
     for (const Elem& e : elems) {
         if (!f.included(e.phys, e.geo)) continue;
-        bool ok = put(e.node[0], e.node[1], e.node[2]);
-        if (ok && e.type == 3) ok = put(e.node[0], e.node[2], e.node[3]);
+        bool ok;
+        if (e.type == 2) {
+            ok = put(e.node[0], e.node[1], e.node[2]);
+        } else {   // the shorter diagonal: (1, 2, 4) + (2, 3, 4) or (4, 1, 3) + (1, 2, 3)
+            ok = true;
+            for (int k = 0; k < 4; k++) ok = ok && nodes.count(e.node[k]);
+            if (ok && dist(e.node[0], e.node[2]) > dist(e.node[1], e.node[3]))
+                ok = put(e.node[0], e.node[1], e.node[3]) && put(e.node[1], e.node[2], e.node[3]);
+            else if (ok)
+                ok = put(e.node[3], e.node[0], e.node[2]) && put(e.node[0], e.node[1], e.node[2]);
+        }
         if (!ok) { err = "element with an undefined node"; return BEM_ERR_ARG; }
     }
     return BEM_OK;
+}
+
+// indexOfMaxOfThree (global.cpp:593-617): 1, 2 or 3, the later index on ties
+int index_of_max3(double a, double b, double c)
+{
+    if (a > b) return a > c ? 1 : 3;
+    return b > c ? 2 : 3;
+}
+
+// recursiveRefineTriangle: bisect the longest edge (edge 1 = node1-node2,
+// 2 = node2-node3, 3 = node3-node1) at its midpoint until every longest edge
+// is at most L; the children keep the parent's orientation and come out
+// depth first, child 1 before child 2
+void refine_triangle(const double* t, int longest, double L, std::vector<double>& out)
+{
+    const double* n1 = t;
+    const double* n2 = t + 3;
+    const double* n3 = t + 6;
+    const double* e0 = longest == 1 ? n1 : (longest == 2 ? n2 : n3);
+    const double* e1 = longest == 1 ? n2 : (longest == 2 ? n3 : n1);
+    if (line_length(e0, e1) <= L) {
+        out.insert(out.end(), t, t + 9);
+        return;
+    }
+    double mid[3];
+    for (int d = 0; d < 3; d++) mid[d] = (e0[d] + e1[d]) / 2;
+    double c1[9], c2[9];
+    auto set = [](double* dst, const double* a, const double* b, const double* c) {
+        for (int d = 0; d < 3; d++) {
+            dst[d] = a[d];
+            dst[3 + d] = b[d];
+            dst[6 + d] = c[d];
+        }
+    };
+    switch (longest) {
+    case 1: set(c1, n1, mid, n3); set(c2, mid, n2, n3); break;
+    case 2: set(c1, n1, n2, mid); set(c2, mid, n3, n1); break;
+    default: set(c1, n1, n2, mid); set(c2, mid, n2, n3); break;
+    }
+    for (const double* c : {(const double*)c1, (const double*)c2})
+        refine_triangle(c, index_of_max3(line_length(c, c + 3), line_length(c + 3, c + 6), line_length(c + 6, c)), L,
+                        out);
 }
 
 thread_local std::string g_msh_err;
@@ -281,5 +348,21 @@ int bem_read_msh(const char* path, const char* include_csv, const char* exclude_
 }
 
 const char* bem_read_msh_error(void) { return g_msh_err.c_str(); }
+
+int bem_refine_triangles(int64_t n, const double* tri9, double edge_length, int64_t cap, double* out9, int64_t* nout)
+{
+    if (n < 0 || (n && !tri9) || !nout || !(edge_length > 0) || cap < 0) return BEM_ERR_ARG;
+    std::vector<double> out;
+    for (int64_t e = 0; e < n; e++) {
+        const double* t = tri9 + 9 * e;
+        const int mx = index_of_max3(line_length(t, t + 3), line_length(t + 3, t + 6), line_length(t + 6, t));
+        refine_triangle(t, mx, edge_length, out);
+    }
+    *nout = (int64_t)(out.size() / 9);
+    if (!out9) return BEM_OK;   // count only
+    if (*nout > cap) return BEM_ERR_ARG;
+    if (!out.empty()) std::memcpy(out9, out.data(), out.size() * sizeof(double));
+    return BEM_OK;
+}
 
 }  // extern "C"

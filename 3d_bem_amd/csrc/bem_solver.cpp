@@ -2,6 +2,7 @@
 // solver-level C ABI (bem_solver_*).  Compiled with -ffp-contract=off.
 #include "bem_solver.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -145,7 +146,7 @@ int64_t GMRES::gmresSolve(HMatrix& A, DeviceVector& x, const DeviceVector& b, do
 }
 
 BoundaryElementSolver::BoundaryElementSolver(int device, BoundaryElements elements, int64_t leaf, double eta)
-    : boundaryElements(std::move(elements)), eta_(eta)
+    : boundaryElements(std::move(elements)), device_(device), eta_(eta)
 {
     int rc = bem_create(&ctx_, device);
     if (rc != BEM_OK) throw Error(rc, "bem_create failed (no HIP device?)");
@@ -304,6 +305,68 @@ std::vector<cplx> BoundaryElementSolver::calculateFieldSolution(const std::vecto
     return pressure;
 }
 
+// refinement mode of the current parameters (FP32 operators only)
+static int refineMode(const BemSolverParameters& P)
+{
+    return P.precision == BEM_PREC_FP32 ? P.refine : 0;
+}
+
+// Phi for b = s + Phi g: the GMRES operators' precision, or with refinement
+// the FP64-accurate operator the residuals are measured with (mode 1: the
+// FP32 Phi with its FP64 near field; mode 2: Phi in FP64)
+void BoundaryElementSolver::assembleRhsOperator()
+{
+    const auto& P = bemSolverParameters;
+    const int R = refineMode(P);
+    double t0 = now_ms();
+    phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, R == 2 ? BEM_PREC_FP64 : P.precision);
+    if (R == 2) refineStats[3] += now_ms() - t0;
+}
+
+void BoundaryElementSolver::rhsProduct(DeviceVector& b, const DeviceVector& g)
+{
+    if (refineMode(bemSolverParameters) == 1)
+        check(ctx_, bem_hmvm_refine(ctx_, phiHMat->handle(), g.data(), b.data(), 1), "b = s + Phi g");
+    else
+        check(ctx_, bem_hmvm(ctx_, phiHMat->handle(), g.data(), b.data(), 1), "b = s + Phi g");
+}
+
+// mode 2: DPhi in FP64 first (it takes the near field the FP64 Phi
+// assembly paired with it), then the FP32 DPhi of the iterations
+void BoundaryElementSolver::assembleResidualOperator()
+{
+    const auto& P = bemSolverParameters;
+    dPhiResHMat.reset();
+    if (refineMode(P) != 2) return;
+    double t0 = now_ms();
+    dPhiResHMat = std::make_unique<HMatrix>(ctx_);
+    dPhiResHMat->setClusterTrees(clusterTree, clusterTree);
+    dPhiResHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, BEM_PREC_FP64);
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    refineStats[3] += now_ms() - t0;
+}
+
+int64_t BoundaryElementSolver::solveSystem(DeviceVector& x, const DeviceVector& b, double tol, bem_prec_t pc,
+                                           double* rel)
+{
+    const auto& P = bemSolverParameters;
+    const int R = refineMode(P);
+    if (R == 0) {
+        x.upload(std::vector<cplx>(x.size(), cplx(0, 0)));
+        return GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, rel, pc);
+    }
+    int64_t its = 0, steps = 0;
+    double r = 0;
+    int rc = bem_gmres_ir(ctx_, dPhiHMat->handle(), R == 2 ? dPhiResHMat->handle() : dPhiHMat->handle(), pc, b.data(),
+                          x.data(), 1, tol, P.gmresRestart, 100000, P.refineMaxSteps, &its, &r, &steps);
+    if (rc != BEM_OK && rc != BEM_ERR_NOCONV) check(ctx_, rc, "bem_gmres_ir");
+    refineStats[0] += (double)steps;
+    refineStats[1] += (double)its;
+    refineStats[2] = r;
+    if (rel) *rel = r;
+    return its;
+}
+
 void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHandSide>& rhs,
                                                        std::vector<std::vector<cplx>>& pressure,
                                                        std::vector<int64_t>& iterations)
@@ -314,6 +377,8 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     const size_t nr = rhs.size();
     std::memset(stats, 0, sizeof stats);
     std::memset(precondStats, 0, sizeof precondStats);
+    std::memset(refineStats, 0, sizeof refineStats);
+    check(ctx_, bem_set_near_fp64(ctx_, refineMode(P) == 1), "bem_set_near_fp64");
     double t0 = now_ms();
     // source term of every right-hand side
     std::vector<std::vector<cplx>> s(nr);
@@ -330,6 +395,10 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     }
     pointSources = keepPoints;
     planeWaves = keepPlanes;
+    if (nr > 1 && nr <= 8 && P.preconditioner == 0 && P.lockstepRhs) {
+        solveLockstep(rhs, s, anyG, t0, pressure, iterations);
+        return;
+    }
     std::vector<std::unique_ptr<DeviceVector>> b, g;
     for (size_t k = 0; k < nr; k++) {
         b.push_back(std::make_unique<DeviceVector>(ctx_, E.n));
@@ -339,15 +408,16 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     }
     double t1 = t0;
     if (anyG) {   // b_k = s_k + Phi g_k (:578-579)
-        phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+        assembleRhsOperator();
         stats[7] = phiHMat->stat(1);
         stats[9] = phiHMat->stat(0);
         t1 = now_ms();
-        for (size_t k = 0; k < nr; k++) HArithm::MVM(*b[k], *phiHMat, *g[k]);
+        for (size_t k = 0; k < nr; k++) rhsProduct(*b[k], *g[k]);
         phiHMat->clear();
     }
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t2 = now_ms();
+    assembleResidualOperator();
     dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t3 = now_ms();
@@ -362,8 +432,7 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     bem_prec_t pc = nullptr;
     pc = makePreconditioner();
     for (size_t k = 0; k < nr; k++) {
-        x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
-        iterations[k] = GMRES::gmresSolve(*dPhiHMat, x, *b[k], tol, P.gmresRestart, &est, pc);
+        iterations[k] = solveSystem(x, *b[k], tol, pc, &est);
         check(ctx_, bem_reconstruct_g(ctx_, x.data(), g[k]->data(), airDensity, frequency, phi.data(), dphi.data(),
                                       p.data()),
               "bem_reconstruct_g");
@@ -378,9 +447,147 @@ void BoundaryElementSolver::calculateBoundarySolutions(const std::vector<RightHa
     }
     double t4 = now_ms();
     dPhiHMat->clear();
+    dPhiResHMat.reset();
     stats[0] = t1 - t0; stats[1] = t2 - t1; stats[2] = t3 - t2; stats[3] = t4 - t3; stats[4] = t4 - t0;
     stats[5] = nr ? (double)iterations[nr - 1] : 0;
     stats[6] = est;
+}
+
+// calculateBoundarySolutions for 2..8 right-hand sides without a
+// preconditioner: b = s + Phi g for all of them with one pass over Phi, one
+// lockstep GMRES (with refinement when set) whose Arnoldi columns share one
+// product with DPhi, then the reconstruction per right-hand side
+void BoundaryElementSolver::solveLockstep(const std::vector<RightHandSide>& rhs,
+                                          const std::vector<std::vector<cplx>>& s, bool anyG, double t0,
+                                          std::vector<std::vector<cplx>>& pressure, std::vector<int64_t>& iterations)
+{
+    auto& E = boundaryElements;
+    const auto& P = bemSolverParameters;
+    const int R = refineMode(P);
+    const int64_t n = E.n, nr = (int64_t)rhs.size();
+    std::vector<cplx> Si((size_t)(n * nr)), Gi((size_t)(n * nr));
+    for (int64_t k = 0; k < nr; k++)
+        for (int64_t i = 0; i < n; i++) {
+            Si[i * nr + k] = s[k][i];
+            Gi[i * nr + k] = rhs[k].g[i];
+        }
+    DeviceVector B(ctx_, n * nr), G(ctx_, n * nr), X(ctx_, n * nr);
+    B.upload(Si);
+    G.upload(Gi);
+    double t1 = t0;
+    if (anyG) {   // b_k = s_k + Phi g_k (:578-579), Phi read once for all k
+        assembleRhsOperator();
+        stats[7] = phiHMat->stat(1);
+        stats[9] = phiHMat->stat(0);
+        t1 = now_ms();
+        check(ctx_, bem_hmvm_multi(ctx_, phiHMat->handle(), nr, G.data(), B.data(), 1, R == 1), "b = s + Phi g");
+        check(ctx_, bem_synchronize(ctx_), "sync");
+        phiHMat->clear();
+    }
+    double t2 = now_ms();
+    assembleResidualOperator();
+    dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    check(ctx_, bem_synchronize(ctx_), "sync");
+    double t3 = now_ms();
+    stats[8] = dPhiHMat->stat(1);
+    stats[10] = dPhiHMat->stat(0);
+    stats[15] = dPhiHMat->stat(3);
+    const double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
+    std::vector<int64_t> its((size_t)nr, 0);
+    std::vector<double> rel((size_t)nr, 0.0);
+    int64_t steps = 0;
+    bem_hmat_t hres = R == 2 ? dPhiResHMat->handle() : (R == 1 ? dPhiHMat->handle() : nullptr);
+    int rc = bem_gmres_multi(ctx_, dPhiHMat->handle(), hres, nr, B.data(), X.data(), 1, tol, P.gmresRestart, 100000,
+                             R ? P.refineMaxSteps : 0, its.data(), rel.data(), &steps);
+    if (rc != BEM_OK && rc != BEM_ERR_NOCONV) check(ctx_, rc, "bem_gmres_multi");
+    if (R) {
+        refineStats[0] = (double)steps;
+        for (int64_t k = 0; k < nr; k++) refineStats[1] += (double)its[k];
+        refineStats[2] = *std::max_element(rel.begin(), rel.end());
+    }
+    std::vector<cplx> Xi = X.download();
+    DeviceVector x(ctx_, n), g(ctx_, n), phi(ctx_, n), dphi(ctx_, n), p(ctx_, n);
+    std::vector<cplx> xk((size_t)n);
+    pressure.assign((size_t)nr, {});
+    iterations.assign((size_t)nr, 0);
+    for (int64_t k = 0; k < nr; k++) {
+        for (int64_t i = 0; i < n; i++) xk[i] = Xi[i * nr + k];
+        x.upload(xk);
+        g.upload(rhs[k].g);
+        check(ctx_, bem_reconstruct_g(ctx_, x.data(), g.data(), airDensity, frequency, phi.data(), dphi.data(),
+                                      p.data()),
+              "bem_reconstruct_g");
+        pressure[k] = p.download();
+        iterations[k] = its[k];
+    }
+    double t4 = now_ms();
+    dPhiHMat->clear();
+    dPhiResHMat.reset();
+    stats[0] = t1 - t0; stats[1] = t2 - t1; stats[2] = t3 - t2; stats[3] = t4 - t3; stats[4] = t4 - t0;
+    stats[5] = (double)its[(size_t)nr - 1];
+    stats[6] = rel[(size_t)nr - 1];
+}
+
+void BoundaryElementSolver::regularSolve(double tol, long restart)
+{
+    setWavenumberAndCoupling();
+    calculateBoundaryConditionAndSourceTermVector();
+    auto& E = boundaryElements;
+    std::memset(stats, 0, sizeof stats);
+    double t0 = now_ms();
+    bem_ctx_t d = nullptr;
+    int rc = bem_create(&d, device_);
+    if (rc != BEM_OK) throw Error(rc, "regularSolve: bem_create failed");
+    bem_hmat_t h = nullptr;
+    try {
+        check(d, bem_upload_mesh(d, E.n, E.nodes.data(), E.normal.data(), E.area.data(), E.colloc.data(),
+                                 E.averageTriangleDim),
+              "bem_upload_mesh");
+        const auto& PL = E.impedancePlanes;
+        if (!PL.empty())
+            check(d, bem_set_plane(d, PL[0].halfSpacePlanePoint, PL[0].halfSpacePlaneNormal), "bem_set_plane");
+        std::vector<cplx> a(E.n), bb(E.n), g(E.n);
+        for (int64_t e = 0; e < E.n; e++) { a[e] = E.bc[e].a; bb[e] = E.bc[e].b; g[e] = E.bc[e].g; }
+        check(d, bem_set_bc(d, (const double*)a.data(), (const double*)bb.data(), (const double*)g.data()), "bem_set_bc");
+        const int64_t z = 0, n = E.n;
+        const uint8_t dense = 0;
+        check(d, bem_upload_partition(d, 1, &z, &n, &z, &n, &dense), "bem_upload_partition");
+        check(d, bem_set_frequency(d, wavenumber, couplingParameter.real(), couplingParameter.imag()), "bem_set_frequency");
+        DeviceVector b(d, E.n), f(d, E.n), x(d, E.n);
+        f.upload(g);
+        b.upload(sourceTermVector);
+        check(d, bem_assemble(d, BEM_KIND_PHI, 0.0, 0, BEM_PREC_FP64, 1, &h), "dense Phi");
+        check(d, bem_hmvm(d, h, f.data(), b.data(), 1), "b = s + Phi f");
+        check(d, bem_synchronize(d), "sync");
+        bem_hmat_destroy(h);
+        h = nullptr;
+        check(d, bem_assemble(d, BEM_KIND_DPHI, 0.0, 0, BEM_PREC_FP64, 1, &h), "dense DPhi");
+        double t1 = now_ms();
+        int64_t its = 0;
+        double r = 0;
+        x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
+        rc = bem_gmres(d, h, b.data(), x.data(), tol, restart, 100000, &its, &r);
+        if (rc != BEM_OK && rc != BEM_ERR_NOCONV) check(d, rc, "dense GMRES");
+        bem_hmat_destroy(h);
+        h = nullptr;
+        DeviceVector phi(d, E.n), dphi(d, E.n), p(d, E.n);
+        check(d, bem_reconstruct(d, x.data(), airDensity, frequency, phi.data(), dphi.data(), p.data()),
+              "bem_reconstruct");
+        phiSolution = phi.download();
+        dPhiSolution = dphi.download();
+        soundPressure = p.download();
+        double t2 = now_ms();
+        stats[0] = t1 - t0;
+        stats[3] = t2 - t1;
+        stats[4] = t2 - t0;
+        stats[5] = (double)its;
+        stats[6] = r;
+    } catch (...) {
+        if (h) bem_hmat_destroy(h);
+        bem_destroy(d);
+        throw;
+    }
+    bem_destroy(d);
 }
 
 // BoundaryElementSolver::hMatrixSolve (boundaryelementsolver.cpp:526-744)
@@ -390,9 +597,11 @@ void BoundaryElementSolver::hMatrixSolve()
     const auto& P = bemSolverParameters;
     std::memset(stats, 0, sizeof stats);
     std::memset(precondStats, 0, sizeof precondStats);
+    std::memset(refineStats, 0, sizeof refineStats);
+    check(ctx_, bem_set_near_fp64(ctx_, refineMode(P) == 1), "bem_set_near_fp64");
     double t0 = now_ms();
     calculateBoundaryConditionAndSourceTermVector();
-    phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    assembleRhsOperator();
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t1 = now_ms();
     stats[7] = phiHMat->stat(1);
@@ -403,9 +612,11 @@ void BoundaryElementSolver::hMatrixSolve()
     for (int64_t e = 0; e < E.n; e++) fv[e] = E.bc[e].g;
     f.upload(fv);
     b.upload(sourceTermVector);
-    HArithm::MVM(b, *phiHMat, f);
+    rhsProduct(b, f);
+    check(ctx_, bem_synchronize(ctx_), "sync");
     phiHMat->clear();
     double t2 = now_ms();
+    assembleResidualOperator();
     dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t3 = now_ms();
@@ -414,12 +625,11 @@ void BoundaryElementSolver::hMatrixSolve()
     stats[15] = dPhiHMat->stat(3);
     // GMRES with tolerance 1e-3 acaRelativeError by default (:670); start
     // vector zero
-    x.upload(std::vector<cplx>(E.n, cplx(0, 0)));
     double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
     double est = 0;
     bem_prec_t pc = nullptr;
     pc = makePreconditioner();
-    int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, &est, pc);
+    int64_t its = solveSystem(x, b, tol, pc, &est);
     if (pc) {
         double* ps = precondStats;
         bem_precond_stats(pc, ps);
@@ -429,12 +639,18 @@ void BoundaryElementSolver::hMatrixSolve()
     }
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t4 = now_ms();
-    // true residual (:682-684)
-    DeviceVector r(ctx_, E.n);
-    check(ctx_, bem_hmvm(ctx_, dPhiHMat->handle(), x.data(), r.data(), 0), "residual");
-    std::vector<cplx> rv = r.download(), bv = b.download();
+    // true residual (:682-684); with refinement the residual of the
+    // refinement operator that ended it
     double num = 0, den = 0;
-    for (int64_t e = 0; e < E.n; e++) { num += std::norm(rv[e] - bv[e]); den += std::norm(bv[e]); }
+    if (refineMode(P) == 0) {
+        DeviceVector r(ctx_, E.n);
+        check(ctx_, bem_hmvm(ctx_, dPhiHMat->handle(), x.data(), r.data(), 0), "residual");
+        std::vector<cplx> rv = r.download(), bv = b.download();
+        for (int64_t e = 0; e < E.n; e++) { num += std::norm(rv[e] - bv[e]); den += std::norm(bv[e]); }
+    } else {
+        num = est * est;
+        den = 1;
+    }
     // reconstruction (:729-740) and pressure (:2251-2261)
     DeviceVector phi(ctx_, E.n), dphi(ctx_, E.n), p(ctx_, E.n);
     check(ctx_, bem_reconstruct(ctx_, x.data(), airDensity, frequency, phi.data(), dphi.data(), p.data()),
@@ -443,6 +659,7 @@ void BoundaryElementSolver::hMatrixSolve()
     dPhiSolution = dphi.download();
     soundPressure = p.download();
     dPhiHMat->clear();
+    dPhiResHMat.reset();
     double t5 = now_ms();
     int64_t nnear = 0, nfar = 0;
     for (auto& B : phiHMat->getMinPartition()) (B.adm ? nfar : nnear)++;
@@ -469,6 +686,8 @@ struct bem_solver {
     int preconditioner = 0;
     long hluRank = 2;
     double hluRelError = 0;
+    int refine = 0;
+    long refineMaxSteps = 10;
 };
 
 extern "C" {
@@ -569,7 +788,35 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         S.bemSolverParameters.preconditioner = s->preconditioner;
         S.bemSolverParameters.preconditionerRank = s->hluRank;
         S.bemSolverParameters.preconditionerRelError = s->hluRelError;
+        S.bemSolverParameters.refine = s->refine;
+        S.bemSolverParameters.refineMaxSteps = s->refineMaxSteps;
         S.calculateBoundarySolution();
+        int64_t n = S.boundaryElements.n;
+        if (phi) std::memcpy(phi, S.phiSolution.data(), n * sizeof(bem::cplx));
+        if (dphi) std::memcpy(dphi, S.dPhiSolution.data(), n * sizeof(bem::cplx));
+        if (pressure) std::memcpy(pressure, S.soundPressure.data(), n * sizeof(bem::cplx));
+        if (stats16) std::memcpy(stats16, S.stats, sizeof S.stats);
+    } catch (const bem::Error& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return BEM_ERR_HIP;
+    }
+    return BEM_OK;
+}
+
+int bem_solver_solve_dense(bem_solver_t s, double freq, double c, double rho, int coupling, double gmres_tol,
+                           int64_t restart, double* phi, double* dphi, double* pressure, double* stats16)
+{
+    if (!s || !s->s || coupling < 0 || coupling > 2 || restart < 1 || !(gmres_tol > 0)) return BEM_ERR_ARG;
+    try {
+        auto& S = *s->s;
+        S.setFrequency(freq);
+        S.setWaveSpeed(c);
+        S.setAirDensity(rho);
+        S.bemSolverParameters.coupling = (bem::BemCoupling)coupling;
+        S.regularSolve(gmres_tol, (long)restart);
         int64_t n = S.boundaryElements.n;
         if (phi) std::memcpy(phi, S.phiSolution.data(), n * sizeof(bem::cplx));
         if (dphi) std::memcpy(dphi, S.dPhiSolution.data(), n * sizeof(bem::cplx));
@@ -596,6 +843,21 @@ int bem_solver_precond_stats(bem_solver_t s, double* out5)
 {
     if (!s || !s->s || !out5) return BEM_ERR_ARG;
     std::memcpy(out5, s->s->precondStats, sizeof s->s->precondStats);
+    return BEM_OK;
+}
+
+int bem_solver_set_refinement(bem_solver_t s, int mode, int64_t max_steps)
+{
+    if (!s || !s->s || mode < 0 || mode > 2 || max_steps < 0) return BEM_ERR_ARG;
+    s->refine = mode;
+    s->refineMaxSteps = (long)max_steps;
+    return BEM_OK;
+}
+
+int bem_solver_refine_stats(bem_solver_t s, double* out4)
+{
+    if (!s || !s->s || !out4) return BEM_ERR_ARG;
+    std::memcpy(out4, s->s->refineStats, sizeof s->s->refineStats);
     return BEM_OK;
 }
 
@@ -709,6 +971,8 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 P.preconditioner = s->preconditioner;
                 P.preconditionerRank = s->hluRank;
                 P.preconditionerRelError = s->hluRelError;
+                P.refine = s->refine;
+                P.refineMaxSteps = s->refineMaxSteps;
                 std::vector<std::vector<bem::cplx>> pr;
                 std::vector<int64_t> its;
                 for (int64_t f = q; f < nf; f += L) {

@@ -120,6 +120,16 @@ struct BemSolverParameters {
     int preconditioner = 0;       // 0 none, 1 leaf-block LU (bem_precond_create), 2 H-LU (bem_precond_create_hlu)
     long preconditionerRank = 2;  // bemparameters.h:27
     double preconditionerRelError = 0;   // bemparameters.h:28
+    // FP32 operators only: iterative refinement of the GMRES solution against
+    // an FP64-accurate operator (bem_gmres_ir): 0 none, 1 the FP32
+    // operators' own FP64 near field + their low-rank factors
+    // (bem_set_near_fp64), 2 FP64 H-matrices assembled beside them (the
+    // plain path's operators)
+    int refine = 0;
+    long refineMaxSteps = 10;
+    // 2..8 right-hand sides without a preconditioner: one lockstep GMRES
+    // (bem_gmres_multi) instead of one GMRES per right-hand side
+    bool lockstepRhs = true;
 };
 
 struct PointSource { double x, y, z; cplx weight; };
@@ -140,6 +150,13 @@ public:
     void setWaveSpeed(double c) { waveSpeed = c; }
     void setAirDensity(double rho) { airDensity = rho; }
     void calculateBoundarySolution();      // boundaryelementsolver.cpp:12-138 (H-matrix branch)
+    // regularSolveWithLU (boundaryelementsolver.cpp:140-331): every entry of
+    // Phi and DPhi assembled densely in FP64 on the device (a one-block
+    // partition on a context of its own), b = s + Phi f, and the dense system
+    // solved by GMRES to gmresTol instead of the reference's partial-pivoting
+    // LU; fills phiSolution, dPhiSolution, soundPressure like
+    // calculateBoundarySolution.  Holds two n x n complex matrices.
+    void regularSolve(double gmresTol, long restart);
     // One right-hand side of a multi-right-hand-side solve on the same a, b:
     // boundary data g (cluster order) and its own sources.
     struct RightHandSide {
@@ -165,6 +182,9 @@ public:
     double frequency = 0, waveSpeed = 343.0, airDensity = 1.2041;
     double stats[16] = {0};
     double precondStats[5] = {0};   // bem_precond_stats of the last solve's preconditioner
+    // refinement of the last solve: steps, inner GMRES iterations, final
+    // |b - A_res x| / |b|, ms spent assembling FP64 residual operators (mode 2)
+    double refineStats[4] = {0};
     bem_ctx_t context() const { return ctx_; }
     const std::vector<bemsetup::Block>& partition() const { return phiHMat->getMinPartition(); }
     std::shared_ptr<ClusterTree> clusterTree;
@@ -173,11 +193,22 @@ private:
     void setWavenumberAndCoupling();
     void calculateBoundaryConditionAndSourceTermVector();
     bem_ctx_t ctx_ = nullptr;
+    int device_ = 0;
     double wavenumber = 0;
     cplx couplingParameter{0, 0};
     std::vector<cplx> sourceTermVector;
-    std::unique_ptr<HMatrix> phiHMat, dPhiHMat;
+    std::unique_ptr<HMatrix> phiHMat, dPhiHMat, dPhiResHMat;
     double eta_;
+    // b (+)= Phi g with the operator the refinement measures residuals with
+    void rhsProduct(DeviceVector& b, const DeviceVector& g);
+    // the right-hand side operator and the residual DPhi of the refinement
+    // mode (before the DPhi the GMRES iterations use)
+    void assembleRhsOperator();
+    void assembleResidualOperator();
+    void solveLockstep(const std::vector<RightHandSide>& rhs, const std::vector<std::vector<cplx>>& s, bool anyG,
+                       double t0, std::vector<std::vector<cplx>>& pressure, std::vector<int64_t>& iterations);
+    // GMRES (refine 0) or GMRES with refinement; returns inner iterations
+    int64_t solveSystem(DeviceVector& x, const DeviceVector& b, double tol, bem_prec_t pc, double* rel);
 };
 
 struct Error : std::exception {

@@ -10,14 +10,20 @@ condition reconstruction and sound pressure (boundaryelementsolver.cpp:
 geometry are set up once, before the timed region, and reported as setup.
 
 The headline is the mixed-precision batched form (--precision fp32,
-default): regular quadrature, ACA and block storage in complex FP32, self and
-near-singular entries in FP64, GMRES in FP64 over the FP32 operator with the
-restart residual recomputed with FP64 accumulation, and the batch's points
-solved concurrently on --lanes solver lanes (default 8, with 16 hardware
-queues: the benchmark runs as a child process with GPU_MAX_HW_QUEUES=16).  The plain FP64 path (one point
-after the other) is timed beside it on the same points (fp64_value,
-speedup_vs_fp64), and the 1000 Hz surface pressure of both is compared with
-the FP64 CPU restatement run on the host (cpu_baseline leg).
+default): regular far-field quadrature, ACA and block storage in complex
+FP32; self and near-singular entries and (with --refine 1, the default) the
+whole near field in FP64; GMRES in FP64 arithmetic iterating on the FP32
+operator inside an iterative refinement whose residuals b - A x are taken in
+FP64 with the FP64 near field and the stored low-rank factors (--refine 2:
+with FP64 H-matrices; 0: no refinement); the batch's points solved
+concurrently on --lanes solver lanes (default 8, with 16 hardware queues:
+the benchmark runs as a child process with GPU_MAX_HW_QUEUES=16).  The plain
+FP64 path (one point after the other) is timed beside it on the same points
+(fp64_value, speedup_vs_fp64), the unrefined FP32 form too (fp32_norefine);
+the 1000 Hz surface pressures are compared with the FP64 H-matrix path, with
+an FP64 dense-assembly solve (bem_solver_solve_dense: every entry of the
+dense operators, GMRES to 1e-12) and with the FP64 CPU restatement run on
+the host (cpu_baseline leg, all host threads and one thread).
 
 --plane D puts a rigid impedance plane D metres below the sphere's centre
 (the plane z = -D, the single-plane branch of hMatrixSolve,
@@ -76,12 +82,23 @@ def plane_of(args):
     return np.array([0.0, 0.0, -args.plane]), np.array([0.0, 0.0, 1.0])
 
 
-def run_cpu_baseline(tri, bc, args):
-    """The CPU restatement (oracle/) on the same workload, all host threads:
-    one full frequency point."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def run_cpu_baseline(tri, bc, args, threads=None):
+    """The CPU restatement (oracle/) on the same workload, all host threads
+    (or `threads`): one full frequency point."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     P = oracle.Problem(tri, bc, leaf=args.leaf, eta=args.eta)
     P.set_frequency(args.freq, 343.0, 1)
@@ -349,8 +366,14 @@ def main():
                     help="time the field evaluation at NOBS points instead of the sweep")
     ap.add_argument("--mvm", action="store_true",
                     help="time the H-MVM of the assembled DPhi operator (FP32 and FP64 storage) instead of the sweep")
+    ap.add_argument("--refine", type=int, choices=[0, 1, 2], default=1,
+                    help="FP32 form: 0 no refinement, 1 FP64 residuals with the FP64 near field and the low-rank "
+                         "factors (default), 2 FP64 residuals with FP64 H-matrices")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-norefine-leg", action="store_true", help="skip the unrefined FP32 comparison run")
+    ap.add_argument("--no-dense", action="store_true", help="skip the FP64 dense-assembly comparison solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-1thread", action="store_true", help="skip the one-thread CPU restatement leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the plain copy kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -385,8 +408,11 @@ def main():
     freqs = batch_freqs(args.freq, args.batch)
     ic = int(np.argmin(np.abs(freqs - args.freq)))
     lanes = args.lanes if prec == bem.FP32 else 1
+    refine = args.refine if prec == bem.FP32 else 0
+    S.set_refinement(refine)
 
     pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes)
+    rstats = S.refine_stats()
     points = args.steps * len(freqs)
     print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, lanes {lanes}", file=sys.stderr, flush=True)
     st = stats[ic]
@@ -410,6 +436,7 @@ def main():
                                f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}, {lanes} lanes"
                                + ("" if args.plane is None else f", rigid plane {args.plane:g} m below the centre"),
                    "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "setup_s": round(setup_s, 3),
+                   "refine": refine, "refine_stats_last_point_lane0": rstats,
                    "ms_per_point": round(1e3 * elapsed / points, 3),
                    "gmres_iters": int(its[ic, 0]), "residual_estimate": st["residual"],
                    "max_rank_phi": st["max_rank_phi"], "max_rank_dphi": st["max_rank_dphi"],
@@ -420,8 +447,18 @@ def main():
                              (args.level, args.leaf, args.eta, args.aca_tol, args.plane) == (5, 64, 2.0, 1e-5, None),
                              quads=1 if args.plane is None else 2, points_per_s=points / elapsed),
     }
+    out["roofline"]["duration_source"] = ("HIP events around each launch on lane 0's launching streams during the "
+                                          "timed steps (a launch's duration includes time it shares the chip "
+                                          "with the other lanes' kernels)")
     p_head = pr[ic, 0]
     p64 = None
+    pr0 = None
+    if refine and not args.no_norefine_leg:
+        S.set_refinement(0)
+        pr0, _, _, el0, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, lanes)
+        S.set_refinement(refine)
+        print(f"# fp32 without refinement: {1e3 * el0 / points:.2f} ms/point", file=sys.stderr, flush=True)
+        out["fp32_norefine"] = {"value": round(points / el0, 4), "ms_per_point": round(1e3 * el0 / points, 3)}
     if not args.no_fp64_leg and prec != bem.FP64:
         pr64, _, _, el64, _, _ = timed_run(bem, S, ctx_h, args, bem.FP64, freqs, 1)
         print(f"# fp64: {1e3 * el64 / points:.2f} ms/point", file=sys.stderr, flush=True)
@@ -430,15 +467,37 @@ def main():
         out["fp64_ms_per_point"] = round(1e3 * el64 / points, 3)
         out["speedup_vs_fp64"] = round(el64 / elapsed, 3)
         out["rel_l2_vs_fp64_gpu"] = float(np.linalg.norm(p_head - p64) / np.linalg.norm(p64))
+        if pr0 is not None:
+            out["fp32_norefine"]["rel_l2_vs_fp64_gpu"] = float(np.linalg.norm(pr0[ic, 0] - p64) / np.linalg.norm(p64))
+    if not args.no_dense:
+        t0 = time.perf_counter()
+        _, _, pd, std = S.solve_dense(args.freq, c=343.0, rho=1.21, coupling=1, gmres_tol=1e-12)
+        t_dense = time.perf_counter() - t0
+        print(f"# dense FP64 solve: {t_dense:.2f} s, {std['gmres_iters']:.0f} iterations", file=sys.stderr, flush=True)
+        rd = lambda p: float(np.linalg.norm(p - pd) / np.linalg.norm(pd))
+        out["rel_l2_vs_dense_fp64"] = rd(p_head)
+        out["dense_fp64"] = {"seconds": round(t_dense, 3), "gmres_iters": int(std["gmres_iters"]),
+                             "gmres_residual": std["residual"]}
+        if p64 is not None:
+            out["fp64_rel_l2_vs_dense_fp64"] = rd(p64)
+        if pr0 is not None:
+            out["fp32_norefine"]["rel_l2_vs_dense_fp64"] = rd(pr0[ic, 0])
     if not args.no_cpu_baseline:
         pr_cpu, t_cpu, threads, st_cpu = run_cpu_baseline(tri, bc, args)
         out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 5), "unit": "freq-points/s", "cores": threads,
-                               "kind": "port",
+                               "kind": "port", "cpu_model": cpu_model(),
                                "sample": f"one full frequency point ({args.freq:g} Hz) of the same workload ({n} "
-                                         f"triangles, FP64, set-up included) with the CPU restatement: {t_cpu:.1f} s"}
+                                         f"triangles, FP64, set-up included) with the CPU restatement: {t_cpu:.1f} s",
+                               "phase_ms": {k: round(st_cpu[k], 1) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm",
+                                                                            "t_gmres")}}
         out["rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(p_head - pr_cpu) / np.linalg.norm(pr_cpu))
         if p64 is not None:
             out["fp64_rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(p64 - pr_cpu) / np.linalg.norm(pr_cpu))
+        if not args.no_cpu_1thread:
+            _, t1, _, st1 = run_cpu_baseline(tri, bc, args, threads=1)
+            out["cpu_baseline"]["one_thread"] = {"value": round(1.0 / t1, 5), "cores": 1, "seconds": round(t1, 1),
+                                                 "phase_ms": {k: round(st1[k], 1) for k in
+                                                              ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")}}
     print(json.dumps(out))
 
 

@@ -48,8 +48,10 @@ extern "C" {
 typedef struct bem_ctx* bem_ctx_t;
 typedef struct bem_hmat* bem_hmat_t;
 typedef struct bem_solver* bem_solver_t;
+typedef struct bem_prec* bem_prec_t;
 
-/* context on HIP device `device` (BoundaryElementSolver owns one per solve) */
+/* context on HIP device `device` (BoundaryElementSolver owns one per solve);
+   destroy its H-matrices and preconditioners before it */
 int bem_create(bem_ctx_t* out, int device);
 int bem_destroy(bem_ctx_t ctx);
 const char* bem_last_error(bem_ctx_t ctx);
@@ -155,6 +157,55 @@ int bem_hmat_stats(bem_hmat_t h, double* stats9);
    return.  Returns BEM_ERR_NOCONV when maxit is reached above tol. */
 int bem_gmres(bem_ctx_t ctx, bem_hmat_t h, const void* b_dev, void* x_dev, double tol, int64_t restart,
               int64_t maxit, int64_t* iters, double* rel_resid);
+/* Mixed-precision refinement (no counterpart in the reference, whose solve
+   is FP64 throughout).  bem_set_near_fp64(ctx, 1): later FP32 assemblies
+   form their near field (dense blocks and self / near-singular entries) in
+   FP64, keep it beside the FP32 copy they round from it, and so carry an
+   FP64-accurate refinement operator: the FP64 near field plus the stored
+   low-rank factors, products accumulated in FP64.  bem_hmvm_refine:
+   y (+)= that operator times x (FP64 operators: bem_hmvm).
+   bem_gmres_ir: iterative refinement around GMRES,
+     r = b - A_res x (FP64), stop when |r| < tol |b|,
+     d = GMRES(A, r) to the relative tolerance min(0.5, 0.5 tol |b| / |r|),
+     x += d,
+   with A = h (the FP32 operator the Krylov iterations use, preconditioner
+   pc as bem_gmres_prec) and A_res = hres (an FP64 H-matrix, or an FP32 one
+   with its FP64 near field; NULL: h itself).  x_is_zero: x holds 0 on entry
+   (saves the first residual product).  iters: inner GMRES iterations;
+   rel_resid: |b - A_res x| / |b|; steps: refinement steps.  The residual
+   check replaces GMRES's own (gmres.h:170-180) and the true residual of
+   hMatrixSolve (boundaryelementsolver.cpp:682-684). */
+int bem_set_near_fp64(bem_ctx_t ctx, int on);
+int bem_hmvm_refine(bem_ctx_t ctx, bem_hmat_t h, const void* x_dev, void* y_dev, int accumulate);
+int bem_gmres_ir(bem_ctx_t ctx, bem_hmat_t h, bem_hmat_t hres, bem_prec_t pc, const void* b_dev, void* x_dev,
+                 int x_is_zero, double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* iters,
+                 double* rel_resid, int64_t* steps);
+
+/* Several right-hand sides (no counterpart in the reference, which solves
+   them one after another): vectors of nrhs right-hand sides (1 <= nrhs <= 8)
+   interleaved, v[i * nrhs + q], in device memory.
+   bem_hmvm_multi: Y (+)= H X for all nrhs with one pass over the blocks
+   (refine = 1: with the refinement operator of bem_hmvm_refine); for four
+   or more right-hand sides the near-field tiles and the U t phase of the
+   low-rank blocks run on the FP64 matrix cores unless bem_set_mfma(ctx, 0).
+   bem_gmres_multi: GMRES(restart) in lockstep over the nrhs right-hand
+   sides B (start / solution X; x_is_zero: X is taken as 0), one product per
+   Arnoldi column for all of them, each stopping at its own tolerance;
+   hres != NULL or max_steps > 0: inside bem_gmres_ir's refinement loop with
+   the residual operator hres (NULL: h).  iters / rel_resid: per right-hand
+   side (columns used, final |b - A x| / |b|); steps: refinement steps.
+   With bem_set_graphs(ctx, 1) (the default) the launches of one Arnoldi
+   column are captured once per solve as a HIP graph and replayed per
+   column; bem_gmres / bem_gmres_ir without a preconditioner run on the
+   same solver with one right-hand side. */
+int bem_set_mfma(bem_ctx_t ctx, int on);
+int bem_set_graphs(bem_ctx_t ctx, int on);
+int bem_hmvm_multi(bem_ctx_t ctx, bem_hmat_t h, int64_t nrhs, const void* x_dev, void* y_dev, int accumulate,
+                   int refine);
+int bem_gmres_multi(bem_ctx_t ctx, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, const void* b_dev, void* x_dev,
+                    int x_is_zero, double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* iters,
+                    double* rel_resid, int64_t* steps);
+
 /* Leaf-block preconditioner of an assembled operator: LU with partial
    pivoting (FP64) of every diagonal leaf block, i.e. the leaf level of the
    reference's H-LU preconditioner (HMatrix(const HMatrix&, rank, err) +
@@ -163,7 +214,6 @@ int bem_gmres(bem_ctx_t ctx, bem_hmat_t h, const void* b_dev, void* x_dev, doubl
    (leaf size <= 64).  bem_gmres_prec runs GMRES on A M^-1 (right
    preconditioning; x is the solution of A x = b); pc = NULL is bem_gmres.
    bem_precond_apply: z = M^-1 v (device vectors, n complex, z != v). */
-typedef struct bem_prec* bem_prec_t;
 int bem_precond_create(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t* out);
 int bem_precond_destroy(bem_prec_t p);
 int bem_precond_apply(bem_ctx_t ctx, bem_prec_t p, const void* v_dev, void* z_dev);
@@ -269,6 +319,17 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);
 
+/* regularSolveWithLU (boundaryelementsolver.cpp:140-331), the reference's
+   dense path: every entry of Phi and DPhi in FP64 (the same entry
+   generator as the H-matrix assembly) on a context of its own, b = s + Phi f,
+   the dense system solved by GMRES to gmres_tol (the reference factorises
+   it by partial-pivoting LU), reconstruction and pressure as
+   bem_solver_solve.  Needs 2 n^2 x 16 bytes of device memory (13.4 GB at
+   n = 20480).  stats16: [0] assembly ms, [3] GMRES ms, [4] total ms,
+   [5] iterations, [6] GMRES relative residual, the rest 0. */
+int bem_solver_solve_dense(bem_solver_t s, double freq, double c, double rho, int coupling, double gmres_tol,
+                           int64_t restart, double* phi, double* dphi, double* pressure, double* stats16);
+
 /* GMRES preconditioner of bem_solver_solve / bem_solver_sweep: 0 none (the
    default), 1 the leaf-block LU of bem_precond_create, 2 the H-LU of
    bem_precond_create_hlu (usePreconditioner, bemparameters.h:26) */
@@ -279,6 +340,17 @@ int bem_solver_set_hlu(bem_solver_t s, int64_t rank, double rel_err);
 /* bem_precond_stats of the last bem_solver_solve's preconditioner (zeros
    when it ran without one) */
 int bem_solver_precond_stats(bem_solver_t s, double* out5);
+/* Refinement of the FP32 form (no counterpart in the reference): mode 0
+   none (the default), 1 residuals with the FP32 operators' own FP64 near
+   field and their low-rank factors in FP64 accumulation (bem_set_near_fp64;
+   b = s + Phi g with the same operator), 2 residuals with FP64 H-matrices
+   assembled beside the FP32 ones (the plain path's Phi and DPhi); GMRES
+   iterates on the FP32 DPhi inside bem_gmres_ir, at most max_steps
+   refinement steps.  FP64 solves ignore it.  bem_solver_refine_stats of
+   the last solve: [0] refinement steps, [1] inner GMRES iterations, [2]
+   final |b - A_res x| / |b|, [3] ms assembling the FP64 operators of mode 2. */
+int bem_solver_set_refinement(bem_solver_t s, int mode, int64_t max_steps);
+int bem_solver_refine_stats(bem_solver_t s, double* out4);
 
 /* The scene's impedance plane (BoundaryElements::impedancePlanes,
    boundaryelements.h:97, filled from an infinite-baffle section,
@@ -296,7 +368,10 @@ int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* p
 
 /* Boundary mesh from a GMSH .msh file (mshReader::readMsh,
    mshreader.cpp:10-428): ASCII format 2 (as the reference parses) or 4.1.
-   Triangles, and quadrilaterals split into (1, 2, 3) and (1, 3, 4), of the
+   Triangles, and quadrilaterals split along their shorter diagonal into
+   (1, 2, 4) + (2, 3, 4) or (4, 1, 3) + (1, 2, 3) (global::
+   quadrilateralsToTriangles; the reference's aspect-ratio splitting of
+   quadrilaterals before it is not applied), of the
    included groups: include_csv / exclude_csv are comma-separated entries
    "a...b" (geometrical group range, either end may be empty), a number
    (geometrical group) or a physical group name; "all" or an empty include
@@ -307,6 +382,15 @@ int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* p
 int bem_read_msh(const char* path, const char* include_csv, const char* exclude_csv, int64_t cap, double* nodes9,
                  int64_t* ntri);
 const char* bem_read_msh_error(void);
+/* Edge-length refinement of a triangle mesh (ElementSection::refineTriangles
+   / recursiveRefineTriangle, SolvingScript/elementSection.cpp:688-800, as
+   SolvingScriptReader::refineElements applies it, solvingscriptreader.cpp:
+   1104-1128, with EdgeLength = the section's edge length, or c / (2
+   MeshFrequency)): every triangle whose longest edge exceeds edge_length is
+   bisected at that edge's midpoint, recursively, the children in the
+   parent's orientation and in depth-first order.  tri9: n x 9 input; out9:
+   cap x 9 (NULL: only *nout is set). */
+int bem_refine_triangles(int64_t n, const double* tri9, double edge_length, int64_t cap, double* out9, int64_t* nout);
 
 /* Frequency sweep with several right-hand sides per frequency
    (BemSolverController::run's loop, bemsolvercontroller.cpp:59-79, with the

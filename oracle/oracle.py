@@ -288,3 +288,47 @@ def project(tri9, p):
     t = _c(tri9, np.float64); pp = _c(p, np.float64); out = np.empty(3)
     r = lib().orc_project(_ptr(t), _ptr(pp), _ptr(out))
     return r, out
+
+
+def refine_triangles(tri, edge_length):
+    """Restatement of ElementSection::refineTriangles / recursiveRefineTriangle
+    (SolvingScript/elementSection.cpp:688-800) with lineLength (:1001-1005),
+    middleOfLine (:993-999) and global::indexOfMaxOfThree (global.cpp:593-617),
+    in plain Python floats (the same IEEE double operations): every triangle
+    whose longest edge is above edge_length is split at that edge's midpoint,
+    depth first."""
+    import math
+
+    def length(p, q):
+        dx, dy, dz = p[0] - q[0], p[1] - q[1], p[2] - q[2]
+        return math.sqrt(dx * dx + dy * dy + dz * dz)
+
+    def imax(a, b, c):
+        if a > b:
+            return 1 if a > c else 3
+        return 2 if b > c else 3
+
+    def mid(p, q):
+        return ((p[0] + q[0]) / 2, (p[1] + q[1]) / 2, (p[2] + q[2]) / 2)
+
+    out = []
+
+    def rec(n1, n2, n3, longest):
+        e = {1: (n1, n2), 2: (n2, n3), 3: (n3, n1)}[longest]
+        if length(*e) <= edge_length:
+            out.append(n1 + n2 + n3)
+            return
+        m = mid(*e)
+        if longest == 1:
+            kids = ((n1, m, n3), (m, n2, n3))
+        elif longest == 2:
+            kids = ((n1, n2, m), (m, n3, n1))
+        else:
+            kids = ((n1, n2, m), (m, n2, n3))
+        for a, b, c in kids:
+            rec(a, b, c, imax(length(a, b), length(b, c), length(c, a)))
+
+    for t in np.asarray(tri, dtype=np.float64).reshape(-1, 9).tolist():
+        n1, n2, n3 = tuple(t[0:3]), tuple(t[3:6]), tuple(t[6:9])
+        rec(n1, n2, n3, imax(length(n1, n2), length(n2, n3), length(n3, n1)))
+    return np.array(out, dtype=np.float64).reshape(-1, 9)

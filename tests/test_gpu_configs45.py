@@ -15,11 +15,12 @@ def rel(a, b):
 def test_capsule76k_impedance(bem, meshes):
     """Capsule L = 2 m, r = 0.25 m, edge 1.07 cm (76440 triangles), 1500 Hz,
     Z = rho c (1.5 + 0.5i) on a seeded 30 % of the elements, rigid elsewhere,
-    leaf 64, eta 1.5, ACA 1e-4; FP32 with the FP64-residual restarts to 1e-8.
-    The FP32 pressure agrees with the FP64 path at the ACA tolerance, both
-    converge (GMRES(300) without the reference's H-LU preconditioner needs
-    several restarts here), and the reconstructed phi, dphi satisfy the
-    impedance / rigid boundary conditions element by element."""
+    leaf 64, eta 1.5, ACA 1e-4, leaf-block LU preconditioner.  The FP32
+    operator alone agrees with the FP64 path at the ACA tolerance (different
+    pivots); with FP64-residual refinement (mode 2: residuals with the FP64
+    H-matrices, GMRES on the FP32 one) it reaches the FP64 solution to 1e-8.
+    The reconstructed phi, dphi satisfy the impedance / rigid boundary
+    conditions element by element."""
     f, c, rho = 1500.0, 343.0, 1.21
     k = 2 * np.pi * f / c
     tri = meshes.capsule(2.0, 0.25, 0.0107)
@@ -29,13 +30,21 @@ def test_capsule76k_impedance(bem, meshes):
     d = meshes.unit_direction(4)
     S = bem.Solver(tri, bc, leaf=64, eta=1.5)
     S.set_sources(plane=np.array([d[0], d[1], d[2], 1.0, 0.0]))
+    S.set_preconditioner(1)
     perm, _ = S.layout()
-    phi, dphi, p32, st32 = S.solve(f, c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP32)
-    _, _, p64, st64 = S.solve(f, c=c, rho=rho, aca_tol=1e-4, gmres_tol=1e-8, seed=1, precision=bem.FP64)
-    print("capsule n", n, "FP32 vs FP64", rel(p32, p64), "iters", st32["gmres_iters"], st64["gmres_iters"],
-          "residual", st32["residual"], st64["residual"])
-    assert st32["residual"] < 1e-7 and st64["residual"] < 1e-7
+    kw = dict(c=c, rho=rho, aca_tol=1e-4, seed=1)
+    phi, dphi, p32, st32 = S.solve(f, gmres_tol=1e-8, precision=bem.FP32, **kw)
+    _, _, p64, st64 = S.solve(f, gmres_tol=1e-10, precision=bem.FP64, **kw)
+    S.set_refinement(2)
+    _, _, pref, stref = S.solve(f, gmres_tol=1e-10, precision=bem.FP32, **kw)
+    rs = S.refine_stats()
+    S.set_refinement(0)
+    print("capsule n", n, "FP32 vs FP64", rel(p32, p64), "refined FP32 vs FP64", rel(pref, p64), "iters",
+          st32["gmres_iters"], st64["gmres_iters"], stref["gmres_iters"], "refinement", rs)
+    assert st32["residual"] < 1e-7 and st64["residual"] < 1e-9
     assert rel(p32, p64) < 1e-4
+    assert rs["residual"] < 1e-10 and rs["steps"] >= 2
+    assert rel(pref, p64) < 1e-8
     a = (bc[:, 0] + 1j * bc[:, 1])[perm]                  # cluster order
     imp = a != 0
     assert 0.25 < imp.mean() < 0.35

@@ -42,8 +42,15 @@ def test_quadrilaterals_split_and_errors(bem, tmp_path):
     path.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n4\n1 0 0 0\n2 1 0 0\n3 1 1 0\n4 0 1 0\n"
                     "$EndNodes\n$Elements\n1\n1 3 2 1 1 1 2 3 4\n$EndElements\n")
     tri = bem.read_msh(path)
+    # equal diagonals: (4, 1, 3) + (1, 2, 3) (global::quadrilateralsToTriangles)
     assert tri.shape == (2, 9)
-    assert np.array_equal(tri[0], [0, 0, 0, 1, 0, 0, 1, 1, 0]) and np.array_equal(tri[1], [0, 0, 0, 1, 1, 0, 0, 1, 0])
+    assert np.array_equal(tri[0], [0, 1, 0, 0, 0, 0, 1, 1, 0]) and np.array_equal(tri[1], [0, 0, 0, 1, 0, 0, 1, 1, 0])
+    # diagonal 1-3 longer than 2-4: (1, 2, 4) + (2, 3, 4)
+    path2 = tmp_path / "q2.msh"
+    path2.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n4\n1 0 0 0\n2 1 0 0\n3 2 1 0\n4 1 1 0\n"
+                     "$EndNodes\n$Elements\n1\n1 3 2 1 1 1 2 3 4\n$EndElements\n")
+    tri = bem.read_msh(path2)
+    assert np.array_equal(tri[0], [0, 0, 0, 1, 0, 0, 1, 1, 0]) and np.array_equal(tri[1], [1, 0, 0, 2, 1, 0, 1, 1, 0])
     with pytest.raises(bem.BemError):
         bem.read_msh(tmp_path / "missing.msh")
     bad = tmp_path / "bad.msh"
@@ -53,13 +60,13 @@ def test_quadrilaterals_split_and_errors(bem, tmp_path):
         bem.read_msh(bad)
 
 
-REF_SPHERE = "/root/reference/Example/Sphere/Sphere.msh"
+REF_SPHERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "Sphere.msh")
 
 
-@pytest.mark.skipif(not os.path.exists(REF_SPHERE), reason="reference example mesh not present")
 def test_reference_example_sphere():
-    """The reference's own example (format 4.1): a closed surface whose
-    vertices lie on the radius-5 sphere."""
+    """The reference's own example mesh (Example/Sphere/Sphere.msh, format
+    4.1, kept as a fixture): a closed surface whose vertices lie on the
+    radius-5 sphere."""
     import importlib
     bem = importlib.import_module("3d_bem_amd")
     tri = bem.read_msh(REF_SPHERE)

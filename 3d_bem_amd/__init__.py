@@ -43,6 +43,8 @@ _SIGS = {
     "bem_set_frequency": (_int, [_p, _dbl, _dbl, _dbl]),
     "bem_assemble": (_int, [_p, _int, _dbl, _i64, _int, _u64, ctypes.POINTER(_p)]),
     "bem_hmat_destroy": (_int, [_p]),
+    "bem_assemble_batch": (_int, [_p, _int, _i64, _p, _dbl, _i64, _int, _u64, _p]),
+    "bem_set_aca_schedule": (_int, [_p, _int, _i64, _i64, _int]),
     "bem_hmvm": (_int, [_p, _p, _p, _p, _int]),
     "bem_block_info": (_int, [_p, _i64, ctypes.POINTER(_i64)]),
     "bem_block_download": (_int, [_p, _i64, _p, _p, _p]),
@@ -57,6 +59,9 @@ _SIGS = {
     "bem_gmres_ir": (_int, [_p, _p, _p, _p, _p, _p, _int, _dbl, _i64, _i64, _i64, ctypes.POINTER(_i64),
                             ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
     "bem_solver_set_refinement": (_int, [_p, _int, _i64]),
+    "bem_solver_set_sweep_batch": (_int, [_p, _int]),
+    "bem_gmres_batch": (_int, [_p, _i64, _p, _p, _i64, _p, _p, _int, _dbl, _i64, _i64, _i64, _p, _p,
+                               ctypes.POINTER(_i64)]),
     "bem_set_mfma": (_int, [_p, _int]),
     "bem_set_graphs": (_int, [_p, _int]),
     "bem_hmvm_multi": (_int, [_p, _p, _i64, _p, _p, _int, _int]),
@@ -244,6 +249,40 @@ class Context:
         self._check(lib().bem_assemble(self.h, int(kind), float(tol), int(maxrank), int(precision), int(seed),
                                        ctypes.byref(h)))
         return HMatrix(self, h)
+
+    def assemble_batch(self, kind, kcc, tol, maxrank=0, precision=FP64, seed=1):
+        """bem_assemble_batch: kcc (nf, 3) = (k, cc re, cc im) per frequency;
+        one HMatrix per frequency"""
+        kcc = _c(np.asarray(kcc).reshape(-1, 3), np.float64)
+        outs = (_p * len(kcc))()
+        self._check(lib().bem_assemble_batch(self.h, int(kind), len(kcc), _ptr(kcc), float(tol), int(maxrank),
+                                             int(precision), int(seed), ctypes.cast(outs, _p)))
+        return [HMatrix(self, _p(h)) for h in outs]
+
+    def gmres_batch(self, hs, B, tol, hres=None, restart=300, maxit=100000, max_steps=0):
+        """lockstep GMRES over several operators (bem_gmres_batch): B (n,
+        nops * nrhs), column o * nrhs + r a right-hand side of hs[o]; X,
+        iterations, relative residuals per column, refinement steps, status"""
+        B = _c(B, np.complex128)
+        n, q = B.shape
+        nops = len(hs)
+        if q % nops:
+            raise ValueError("B needs nrhs columns per operator")
+        hv = (_p * nops)(*[h.h for h in hs])
+        rv = None if hres is None else (_p * nops)(*[h.h for h in hres])
+        bd, xd = self.array(B.ravel(), n=n * q), self.array(n=n * q)
+        its, rel, st = np.zeros(q, np.int64), np.zeros(q), _i64()
+        rc = lib().bem_gmres_batch(self.h, nops, ctypes.cast(hv, _p), None if rv is None else ctypes.cast(rv, _p),
+                                   q // nops, bd.ptr, xd.ptr, 1, float(tol), int(restart), int(maxit), int(max_steps),
+                                   _ptr(its), _ptr(rel), ctypes.byref(st))
+        if rc not in (BEM_OK, BEM_ERR_NOCONV):
+            self._check(rc)
+        return xd.download().reshape(n, q), its, rel, st.value, rc
+
+    def set_aca_schedule(self, precision, small_bound, huge_bound, queue=-1):
+        """ACA size-class bounds (m + n) for one precision and the batched
+        launches' work queue (bem_set_aca_schedule)"""
+        self._check(lib().bem_set_aca_schedule(self.h, int(precision), int(small_bound), int(huge_bound), int(queue)))
 
     def entries4(self, ij):
         ij = _c(ij, np.int64).reshape(-1, 2)
@@ -600,6 +639,13 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def set_sweep_batch(self, batch):
+        """sweeps in frequency chunks of batch with batched assembly and
+        lockstep GMRES on one context (bem_solver_set_sweep_batch; 0: lanes)"""
+        rc = lib().bem_solver_set_sweep_batch(self.h, int(batch))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_sweep_batch")
 
     def set_refinement(self, mode, max_steps=10):
         """FP32 solves: 0 no refinement, 1 residuals with the FP64 near field

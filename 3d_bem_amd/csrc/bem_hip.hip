@@ -462,6 +462,23 @@ struct AcaArgs {
     const int32_t* piv;
     const bem_c64* sing;  // self / near-singular entries (k_singular), FP64
     int64_t nnz;
+    // several frequencies in one launch (nf > 0): list entries are items
+    // f * nbk + b; frequency f has wavenumber kf[f], coupling ccf[f], its
+    // singular table singf[f], factor storage Uf/Vf/Af[f] (the same per-block
+    // offsets for every f) and its results at rank_out / flag_out /
+    // evals_out + f * nbk
+    int nf;
+    int64_t nbk;
+    const double* kf;
+    const bem_c64* ccf;
+    const bem_c64* const* singf;
+    void* const* Uf;
+    void* const* Vf;
+    void* const* Af;
+    // persistent launch: workgroups take list positions from this counter
+    // (zero at launch) until nitems are taken; null: one workgroup per item
+    int* queue;
+    int nitems;
 };
 
 // The dot products are processed kDotGroup at a time per wavefront: the
@@ -779,12 +796,62 @@ template <int NT, class C> constexpr int kAcaWaves = 1;
 template <> constexpr int kAcaWaves<64, bem_c32> = 4;
 template <> constexpr int kAcaWaves<128, bem_c32> = 4;
 
+// one listed item: a block of the single operator, or (nf > 0) block b of
+// frequency f with that frequency's wavenumber, coupling, table and storage
 template <bool EXTRA, int NT, class C, bool PLANE>
+__device__ __forceinline__ void aca_item(const AcaArgs& a, int item, char* dyn, ArgMax* sred)
+{
+    if (a.nf == 0) {
+        aca_block<EXTRA, NT, C, PLANE>(a, item, dyn, sred);
+        return;
+    }
+    const int f = (int)(item / a.nbk);
+    const int b = (int)(item - (int64_t)f * a.nbk);
+    AcaArgs ai = a;
+    const double k = a.kf[f];
+    const bem_c64 cc = a.ccf[f];
+    ai.F.k = k;
+    ai.F.cc = cc;
+    ai.F32.k = (float)k;
+    ai.F32.krev = k / (2.0 * BEM_PI);
+    ai.F32.cc = c32((float)cc.re, (float)cc.im);
+    ai.sing = a.singf[f];
+    ai.U = a.Uf[f];
+    ai.V = a.Vf[f];
+    ai.A = a.Af[f];
+    ai.rank_out = a.rank_out + (int64_t)f * a.nbk;
+    ai.flag_out = a.flag_out + (int64_t)f * a.nbk;
+    ai.evals_out = a.evals_out + (int64_t)f * a.nbk;
+    aca_block<EXTRA, NT, C, PLANE>(ai, b, dyn, sred);
+}
+
+// MODE 0: one workgroup per listed block of the single operator; 1: one per
+// listed (frequency, block) item; 2: persistent workgroups taking items
+// from the queue counter (separate instantiations, so that the plain kernel
+// keeps its register budget)
+template <bool EXTRA, int NT, class C, bool PLANE, int MODE>
 __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) char dyn[];
     __shared__ ArgMax sred[NT / 64];
-    aca_block<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
+    if constexpr (MODE == 0) {
+        aca_block<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
+        return;
+    }
+    if constexpr (MODE == 1) {
+        aca_item<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
+        return;
+    }
+    __shared__ int s_pos;
+    while (true) {   // every workgroup leaves once the counter passes nitems
+        if (threadIdx.x == 0) s_pos = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int pos = s_pos;
+        __syncthreads();
+        if (pos >= a.nitems) break;
+        aca_item<EXTRA, NT, C, PLANE>(a, a.blist[pos], dyn, sred);
+        __syncthreads();
+    }
 }
 
 // copy the first rank columns/rows of the ACA scratch into the packed
@@ -2048,6 +2115,18 @@ __global__ void k_residual_n(int64_t len, const bem_c64* __restrict__ b, bem_c64
     if (t < len) r[t] = c64_sub(b[t], r[t]);
 }
 
+// w columns between two interleaved layouts: dst[i ds + doff + q] =
+// src[i ss + soff + q] for q < w
+__global__ void k_cols(int64_t n, int w, const bem_c64* __restrict__ src, int ss, int soff, bem_c64* __restrict__ dst,
+                       int ds, int doff)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * w) return;
+    const int64_t i = t / w;
+    const int q = (int)(t % w);
+    dst[i * ds + doff + q] = src[i * ss + soff + q];
+}
+
 // dst[i Q + q] = src[i ld + q] (q < nq), 0 for nq <= q < Q
 __global__ void k_repack(int64_t n, int nq, int64_t ld, int Q, const bem_c64* __restrict__ src, bem_c64* __restrict__ dst)
 {
@@ -2205,6 +2284,12 @@ struct bem_ctx {
     double* gm_resq = nullptr;
     bool use_mfma = true;
     bool use_graphs = true;
+    // ACA size classes by m + n per storage precision [FP64, FP32]: up to
+    // cls_small one wavefront, up to cls_huge the middle class, above the
+    // largest; batched assemblies on persistent workgroups (queue counter)
+    int64_t cls_small[2] = {kSmallBlock[0], kSmallBlock[1]};
+    int64_t cls_huge[2] = {kHugeBlock[0], kHugeBlock[1]};
+    bool aca_queue = false;
     bool capturing = false;   // Timer records no events while a graph is captured
     hipEvent_t gm_ev[2] = {nullptr, nullptr};
 };
@@ -2995,12 +3080,23 @@ template <> constexpr int kMidThreads<bem_c64> = 256;
 template <int NT, class C>
 void launch_aca_nt(bool extra, bool plane, dim3 g, size_t shm, hipStream_t st, const AcaArgs& aq)
 {
+    const int mode = aq.nf == 0 ? 0 : (aq.queue ? 2 : 1);   // batches run without a plane (checked by the caller)
+    if (mode == 1) {
+        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
+        return;
+    }
+    if (mode == 2) {
+        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 2>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C, false, 2>), g, dim3(NT), shm, st, aq);
+        return;
+    }
     if (extra) {
-        if (plane) hipLaunchKernelGGL((k_aca<true, NT, C, true>), g, dim3(NT), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<true, NT, C, false>), g, dim3(NT), shm, st, aq);
+        if (plane) hipLaunchKernelGGL((k_aca<true, NT, C, true, 0>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<true, NT, C, false, 0>), g, dim3(NT), shm, st, aq);
     } else {
-        if (plane) hipLaunchKernelGGL((k_aca<false, NT, C, true>), g, dim3(NT), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<false, NT, C, false>), g, dim3(NT), shm, st, aq);
+        if (plane) hipLaunchKernelGGL((k_aca<false, NT, C, true, 0>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C, false, 0>), g, dim3(NT), shm, st, aq);
     }
 }
 
@@ -3051,7 +3147,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     for (int32_t b : list) {
         int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
         const int pi = prec == BEM_PREC_FP32 ? 1 : 0;
-        cls[s2 > kHugeBlock[pi] ? 0 : (s2 > kSmallBlock[pi] ? 1 : 2)].push_back(b);
+        cls[s2 > c->cls_huge[pi] ? 0 : (s2 > c->cls_small[pi] ? 1 : 2)].push_back(b);
     }
     std::vector<int32_t> ordered;
     for (auto& v : cls) ordered.insert(ordered.end(), v.begin(), v.end());
@@ -3083,6 +3179,9 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     a.tol = tol; a.maxrank = maxrank; a.seed = seed;
     a.piv_off = d_pivoff; a.piv = d_piv;
     a.sing = c->sing_cur ? c->sing_cur : c->d_sing; a.nnz = c->nnz;
+    a.nf = 0; a.nbk = nb; a.kf = nullptr; a.ccf = nullptr; a.singf = nullptr;
+    a.Uf = a.Vf = a.Af = nullptr;
+    a.queue = nullptr; a.nitems = 0;
     // dynamic LDS of aca_block: sw[cap] of the storage type, then the FP64
     // dot products sdots[2 cap + 2]
     size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
@@ -3114,6 +3213,314 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     std::memcpy(ev.data(), h, (size_t)nb * 8);
     std::memcpy(rank.data(), h + (size_t)nb * 8, (size_t)nb * 4);
     std::memcpy(flag.data(), h + (size_t)nb * 12, (size_t)nb * 4);
+    return BEM_OK;
+}
+
+// Packs the ACA results of h's assembly (first pass S1, re-run of the
+// overflowing blocks S2, final ranks `rank`) into h's storage -- or keeps the
+// first pass's scratch as the storage when nothing overflowed and no block
+// has rank 0 -- builds the MVM task lists, checks the near field for
+// non-finite entries and fills h->stats except the timings ([4], [6], [7]).
+int finish_hmat(bem_ctx* c, bem_hmat* h, bool f32, AcaScratch& S1, AcaScratch& S2, std::vector<int32_t>& rank,
+                std::vector<int64_t>& evs, std::vector<int32_t>& over, std::vector<int32_t>& keep, hipEvent_t ev_done)
+{
+    const int64_t nb = (int64_t)c->blocks.size();
+    int rc;
+    // packed storage at the final ranks
+    h->uoff = c->vp.take<int64_t>(nb, 0); h->voff = c->vp.take<int64_t>(nb, 0);
+    h->aoff = c->vp.take<int64_t>(nb, 0); h->rank = c->vp.take<int32_t>(nb, -1);
+    int64_t tu = 0, tv = 0, ta = 0;
+    double maxr = 0, sumr = 0;
+    int64_t evaluated = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        const auto& B = c->blocks[b];
+        if (!B.adm) { evaluated += B.rows * B.cols; continue; }
+        int32_t r = rank[b] > 0 ? rank[b] : 1;
+        h->rank[b] = r;
+        h->uoff[b] = tu; h->voff[b] = tv; h->aoff[b] = ta;
+        tu += B.rows * r; tv += B.cols * r; ta += r;
+        maxr = std::max(maxr, (double)r);
+        sumr += r;
+        evaluated += evs[b];
+    }
+    // When no block overflowed its first capacity and every block has rank >= 1,
+    // the first pass's scratch is the storage: its layout per block is the
+    // packed one (rank rows of U, then of V, then alpha), with the unused
+    // capacity left between blocks.  Otherwise the rank rows are copied into
+    // packed storage (k_compact; a rank-0 block becomes a zero rank-1 block).
+    bool direct = over.empty();
+    for (int32_t b : keep)
+        if (rank[b] < 1) direct = false;
+    if (direct) {
+        for (int32_t b : c->adm_list) {
+            h->uoff[b] = S1.uoff[b];
+            h->voff[b] = S1.voff[b];
+            h->aoff[b] = S1.aoff[b];
+        }
+        h->U = S1.U; h->V = S1.V; h->A = S1.A;
+        auto& o = S1.owned;
+        o.erase(std::remove_if(o.begin(), o.end(), [&](void* q) { return q == h->U || q == h->V || q == h->A; }),
+                o.end());
+    } else {
+        if ((rc = dalloc_bytes(c, &h->U, (size_t)tu * h->esz))) return rc;
+        if ((rc = dalloc_bytes(c, &h->V, (size_t)tv * h->esz))) return rc;
+        if ((rc = dalloc_bytes(c, &h->A, (size_t)ta * h->esz))) return rc;
+    }
+    // offsets, ranks, MVM task lists and the packing lists: one upload
+    std::vector<int32_t>& rk = rank;   // first-pass ranks with the re-run ranks merged in
+    std::vector<void*> own2;
+    // (block, l) tasks of the first MVM phase, (block, row chunk) tasks of
+    // the second; sized first, then filled by index
+    std::vector<int64_t> toff = c->vp.take<int64_t>(nb, 0);
+    int64_t tot = 0, ntt = 0, nchunks = 0;
+    for (int32_t b : c->adm_list) {
+        tot += h->rank[b];
+        ntt += (h->rank[b] + kLrRows - 1) / kLrRows;
+        nchunks += (c->blocks[b].rows + kWG - 1) / kWG;
+    }
+    std::vector<int32_t> tt = c->vp.take<int32_t>((size_t)2 * ntt, 0), ut = c->vp.take<int32_t>((size_t)2 * nchunks, 0);
+    {
+        int64_t pt = 0, pu = 0, acc = 0;
+        for (int32_t b : c->adm_list) {
+            const int r = h->rank[b];
+            toff[b] = acc;
+            acc += r;
+            h->maxrank = std::max(h->maxrank, r);
+            for (int l = 0; l < r; l += kLrRows, pt++) { tt[2 * pt] = b; tt[2 * pt + 1] = l; }
+            const int64_t nq = (c->blocks[b].rows + kWG - 1) / kWG;
+            for (int64_t q = 0; q < nq; q++, pu++) { ut[2 * pu] = b; ut[2 * pu + 1] = (int32_t)q; }
+        }
+    }
+    h->nttask = ntt;
+    h->t_total = tot;
+    h->nutask = nchunks;
+    Packer pk;
+    size_t k_uoff = pk.add(h->uoff), k_voff = pk.add(h->voff), k_aoff = pk.add(h->aoff);
+    size_t k_rank = pk.add(h->rank), k_srank = pk.add(rk), k_tt = pk.add(tt), k_ut = pk.add(ut);
+    size_t k_toff = pk.add(toff), k_keep = pk.add(keep), k_over = pk.add(over);
+    rc = pk.upload(c, nullptr);
+    h->meta = pk.dbase;   // owned by h from here on, also on failure
+    if (rc) return rc;
+    h->d_uoff = pk.ptr<int64_t>(k_uoff); h->d_voff = pk.ptr<int64_t>(k_voff); h->d_aoff = pk.ptr<int64_t>(k_aoff);
+    h->d_rank = pk.ptr<int32_t>(k_rank);
+    int32_t* d_srank = pk.ptr<int32_t>(k_srank);
+    h->d_ttask = pk.ptr<int32_t>(k_tt); h->d_utask = pk.ptr<int32_t>(k_ut); h->d_toff = pk.ptr<int64_t>(k_toff);
+    if ((rc = dalloc(c, &h->d_t, tot, nullptr))) return rc;
+    auto compact = [&](const std::vector<int32_t>& lst, const int32_t* d_lst, const AcaScratch& Sx) -> int {
+        if (lst.empty()) return BEM_OK;
+        dim3 g((unsigned)lst.size());
+        if (f32)
+            hipLaunchKernelGGL(k_compact<bem_c32>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
+                               Sx.d_uoff, Sx.d_voff, Sx.d_aoff, (const bem_c32*)Sx.U, (const bem_c32*)Sx.V,
+                               (const bem_c32*)Sx.A, d_srank, h->d_uoff, h->d_voff, h->d_aoff, (bem_c32*)h->U,
+                               (bem_c32*)h->V, (bem_c32*)h->A);
+        else
+            hipLaunchKernelGGL(k_compact<bem_c64>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
+                               Sx.d_uoff, Sx.d_voff, Sx.d_aoff, (const bem_c64*)Sx.U, (const bem_c64*)Sx.V,
+                               (const bem_c64*)Sx.A, d_srank, h->d_uoff, h->d_voff, h->d_aoff, (bem_c64*)h->U,
+                               (bem_c64*)h->V, (bem_c64*)h->A);
+        return launch_check(c, "k_compact");
+    };
+    if (!direct && (rc = compact(keep, pk.ptr<int32_t>(k_keep), S1))) return rc;
+    if (!direct && (rc = compact(over, pk.ptr<int32_t>(k_over), S2))) return rc;
+    if (ev_done) HIPC(c, hipEventRecord(ev_done, c->stream));
+    // NaN check of the dense part (the reference reports NaN blocks,
+    // hmatrix.cpp:812-820)
+    int32_t* d_flag;
+    if ((rc = dalloc(c, &d_flag, 1, &own2))) return rc;
+    HIPC(c, hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
+    if (c->dense_total) {
+        if (f32)
+            hipLaunchKernelGGL(k_has_nonfinite<bem_c32>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
+                               c->dense_total, (const bem_c32*)h->dense, d_flag);
+        else
+            hipLaunchKernelGGL(k_has_nonfinite<bem_c64>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
+                               c->dense_total, (const bem_c64*)h->dense, d_flag);
+    }
+    int32_t hflag = 0;
+    HIPC(c, hipMemcpyAsync(&hflag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    free_list(c, own2);
+    if (hflag) return set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite entry in the near field");
+    size_t nadm = c->adm_list.size();
+    h->stats[0] = (double)evaluated;
+    h->stats[1] = maxr;
+    h->stats[2] = nadm ? sumr / (double)nadm : 0;
+    h->stats[3] = (double)((c->dense_total + tu + tv + ta) * (int64_t)h->esz);
+    h->stats[5] = (double)over.size();
+    h->stats[8] = (double)((c->dense_total + (direct ? S1.tu + S1.tv + S1.ta : tu + tv + ta)) * (int64_t)h->esz +
+                           (h->dense64 ? c->dense_total * (int64_t)sizeof(bem_c64) : 0));
+    for (auto* v : {&tt, &ut}) c->vp.give(*v);
+    c->vp.give(toff);
+    return BEM_OK;
+}
+
+// one frequency of a batched assembly
+struct FreqJob {
+    double k;
+    bem_c64 cc;
+    const bem_c64* sing;
+};
+
+// run_aca for several frequencies at once: one launch per size class over
+// the flattened (frequency, block) items, same-size blocks of all
+// frequencies next to each other, largest first; persistent workgroups on a
+// queue counter when c->aca_queue.  Frequency f's factors go to S[f]
+// (identical per-block offsets) and its ranks / flags / evaluation counts
+// to rank[f] / flag[f] / ev[f].  The offset arrays live in S[0].
+int run_aca_batch(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed,
+                  const std::vector<FreqJob>& jobs, const std::vector<int32_t>& list, const std::vector<int32_t>& cap,
+                  const int64_t* d_pivoff, const int32_t* d_piv, const std::function<int()>& main_work,
+                  std::vector<AcaScratch>& S, std::vector<std::vector<int32_t>>& rank,
+                  std::vector<std::vector<int32_t>>& flag, std::vector<std::vector<int64_t>>& ev)
+{
+    const int64_t nb = (int64_t)c->blocks.size();
+    const int nf = (int)jobs.size();
+    const size_t esz = prec == BEM_PREC_FP32 ? sizeof(bem_c32) : sizeof(bem_c64);
+    int rc;
+    std::vector<int64_t> uoff(nb, 0), voff(nb, 0), aoff(nb, 0);
+    int64_t tu = 0, tv = 0, ta = 0;
+    int maxcap = 1;
+    for (int32_t b : list) {
+        const auto& B = c->blocks[b];
+        uoff[b] = tu; voff[b] = tv; aoff[b] = ta;
+        tu += B.rows * cap[b]; tv += B.cols * cap[b]; ta += cap[b];
+        maxcap = std::max(maxcap, cap[b]);
+    }
+    std::vector<uint64_t> Up(nf), Vp(nf), Ap(nf), Sp(nf);
+    std::vector<double> kf(nf);
+    std::vector<bem_c64> ccf(nf);
+    for (int f = 0; f < nf; f++) {
+        AcaScratch& s = S[f];
+        s.ctx = c;
+        s.uoff = c->vp.take<int64_t>(nb, 0); s.voff = c->vp.take<int64_t>(nb, 0); s.aoff = c->vp.take<int64_t>(nb, 0);
+        std::copy(uoff.begin(), uoff.end(), s.uoff.begin());
+        std::copy(voff.begin(), voff.end(), s.voff.begin());
+        std::copy(aoff.begin(), aoff.end(), s.aoff.begin());
+        if ((rc = dalloc_bytes(c, &s.U, tu * esz))) return rc;
+        s.owned.push_back(s.U);
+        if ((rc = dalloc_bytes(c, &s.V, tv * esz))) return rc;
+        s.owned.push_back(s.V);
+        if ((rc = dalloc_bytes(c, &s.A, ta * esz))) return rc;
+        s.owned.push_back(s.A);
+        s.tu = tu; s.tv = tv; s.ta = ta;
+        Up[f] = (uint64_t)s.U; Vp[f] = (uint64_t)s.V; Ap[f] = (uint64_t)s.A; Sp[f] = (uint64_t)jobs[f].sing;
+        kf[f] = jobs[f].k;
+        ccf[f] = jobs[f].cc;
+    }
+    const int pi = prec == BEM_PREC_FP32 ? 1 : 0;
+    std::vector<int32_t> cls[3];
+    for (int32_t b : list) {
+        int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
+        cls[s2 > c->cls_huge[pi] ? 0 : (s2 > c->cls_small[pi] ? 1 : 2)].push_back(b);
+    }
+    std::vector<int32_t> items;
+    size_t ncls[3];
+    for (int q = 0; q < 3; q++) {
+        for (int32_t b : cls[q])
+            for (int f = 0; f < nf; f++) items.push_back((int32_t)(f * nb + b));
+        ncls[q] = cls[q].size() * nf;
+    }
+    if ((int64_t)nf * nb >= ((int64_t)1 << 31)) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: too many items");
+    Packer pk;
+    size_t k_uoff = pk.add(uoff), k_voff = pk.add(voff), k_aoff = pk.add(aoff), k_cap = pk.add(cap);
+    size_t k_items = pk.add(items), k_kf = pk.add(kf), k_ccf = pk.add(ccf), k_sp = pk.add(Sp);
+    size_t k_up = pk.add(Up), k_vp = pk.add(Vp), k_ap = pk.add(Ap);
+    if ((rc = pk.upload(c, &S[0].owned))) return rc;
+    for (int f = 0; f < nf; f++) {   // the shared offsets (owned by S[0])
+        S[f].d_uoff = pk.ptr<int64_t>(k_uoff); S[f].d_voff = pk.ptr<int64_t>(k_voff);
+        S[f].d_aoff = pk.ptr<int64_t>(k_aoff); S[f].d_cap = pk.ptr<int32_t>(k_cap);
+    }
+    void* res = nullptr;
+    if ((rc = dalloc_bytes(c, &res, (size_t)nf * nb * 16 + 256))) return rc;
+    S[0].owned.push_back(res);
+    int64_t* d_ev = (int64_t*)res;
+    int32_t* d_rank = (int32_t*)((char*)res + (size_t)nf * nb * 8);
+    int32_t* d_flag = d_rank + (size_t)nf * nb;
+    int* d_queue = (int*)((char*)res + (size_t)nf * nb * 16);
+    if (c->aca_queue) HIPC(c, hipMemsetAsync(d_queue, 0, 3 * sizeof(int), c->stream));
+    AcaArgs a;
+    a.M = c->mesh;
+    a.F = dev_freq(c);
+    a.F32 = dev_freq32(c);
+    a.kind = kind;
+    a.row0 = c->d_row0; a.rows = c->d_rows; a.col0 = c->d_col0; a.cols = c->d_cols;
+    a.uoff = S[0].d_uoff; a.voff = S[0].d_voff; a.aoff = S[0].d_aoff; a.cap = S[0].d_cap;
+    a.U = a.V = a.A = nullptr;
+    a.rank_out = d_rank; a.flag_out = d_flag; a.evals_out = d_ev;
+    a.tol = tol; a.maxrank = maxrank; a.seed = seed;
+    a.piv_off = d_pivoff; a.piv = d_piv;
+    a.sing = nullptr; a.nnz = c->nnz;
+    a.nf = nf; a.nbk = nb;
+    a.kf = pk.ptr<double>(k_kf); a.ccf = pk.ptr<bem_c64>(k_ccf);
+    a.singf = pk.ptr<const bem_c64*>(k_sp);
+    a.Uf = pk.ptr<void*>(k_up); a.Vf = pk.ptr<void*>(k_vp); a.Af = pk.ptr<void*>(k_ap);
+    a.queue = nullptr; a.nitems = 0;
+    size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
+    HIPC(c, hipEventRecord(c->ev_fork, c->stream));
+    size_t off = 0;
+    // resident workgroups of each class on the chip (256 CUs x 4 SIMDs x
+    // 4 waves per SIMD, divided by the class's wavefronts per workgroup)
+    const int nt[3] = {pi ? kHugeThreads<bem_c32> : kHugeThreads<bem_c64>,
+                       pi ? kMidThreads<bem_c32> : kMidThreads<bem_c64>, 64};
+    for (int q = 0; q < 3; q++) {
+        AcaArgs aq = a;
+        aq.blist = pk.ptr<int32_t>(k_items) + off;
+        off += ncls[q];
+        if (q == 2 && (rc = main_work())) return rc;
+        if (!ncls[q]) continue;
+        hipStream_t st = q < 2 ? c->side[q] : c->stream;
+        if (q < 2) HIPC(c, hipStreamWaitEvent(st, c->ev_fork, 0));
+        unsigned grid = (unsigned)ncls[q];
+        if (c->aca_queue) {
+            aq.queue = d_queue + q;
+            aq.nitems = (int)ncls[q];
+            grid = (unsigned)std::min<size_t>(ncls[q], (size_t)(256 * 4 * 4 * 64 / nt[q]));
+        }
+        Timer t(c, 2, st);
+        bool x = kind == BEM_KIND_DPHI;
+        if (prec == BEM_PREC_FP32) launch_aca<bem_c32>(q, x, grid, shm, st, aq);
+        else launch_aca<bem_c64>(q, x, grid, shm, st, aq);
+        if ((rc = launch_check(c, "k_aca (batch)"))) return rc;
+    }
+    for (int q = 0; q < 2; q++) {
+        HIPC(c, hipEventRecord(c->ev_join[q], c->side[q]));
+        HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join[q], 0));
+    }
+    std::vector<char> h((size_t)nf * nb * 16);
+    HIPC(c, hipMemcpyAsync(h.data(), res, h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    rank.assign(nf, {}); flag.assign(nf, {}); ev.assign(nf, {});
+    for (int f = 0; f < nf; f++) {
+        rank[f] = c->vp.take<int32_t>(nb, 0); flag[f] = c->vp.take<int32_t>(nb, 0); ev[f] = c->vp.take<int64_t>(nb, 0);
+        std::memcpy(ev[f].data(), h.data() + (size_t)f * nb * 8, (size_t)nb * 8);
+        std::memcpy(rank[f].data(), h.data() + (size_t)nf * nb * 8 + (size_t)f * nb * 4, (size_t)nb * 4);
+        std::memcpy(flag[f].data(), h.data() + (size_t)nf * nb * 12 + (size_t)f * nb * 4, (size_t)nb * 4);
+    }
+    return BEM_OK;
+}
+
+// the DPhi variant's pivot lists of the partition (getNormalFilteredPivotIndices
+// for every admissible block), built on the host once per mesh, partition
+// and seed and kept on the device
+int ensure_pivot_lists(bem_ctx* c, uint64_t seed)
+{
+    if (c->piv_ok && c->piv_seed == seed) return BEM_OK;
+    int rc;
+    free_list(c, c->allocs_piv);
+    c->piv_ok = false;
+    bemsetup::Elements E;
+    E.n = c->n;
+    E.normal.resize(c->n);
+    for (int64_t e = 0; e < c->n; e++)
+        E.normal[e] = bemsetup::Vec3{c->h_normal[3 * e], c->h_normal[3 * e + 1], c->h_normal[3 * e + 2]};
+    std::vector<int64_t> pivoff;
+    std::vector<int32_t> piv;
+    bemsetup::build_pivot_lists(E, c->blocks, seed, pivoff, piv);
+    if ((rc = dupload(c, &c->d_pivoff, pivoff.data(), pivoff.size(), &c->allocs_piv))) return rc;
+    if ((rc = dupload(c, &c->d_piv, piv.data(), piv.size(), &c->allocs_piv))) return rc;
+    c->piv_ok = true;
+    c->piv_seed = seed;
     return BEM_OK;
 }
 
@@ -3240,22 +3647,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     int64_t* d_pivoff = nullptr;
     int32_t* d_piv = nullptr;
     if (kind == BEM_KIND_DPHI) {
-        if (!c->piv_ok || c->piv_seed != seed) {
-            free_list(c, c->allocs_piv);
-            c->piv_ok = false;
-            bemsetup::Elements E;
-            E.n = c->n;
-            E.normal.resize(c->n);
-            for (int64_t e = 0; e < c->n; e++)
-                E.normal[e] = bemsetup::Vec3{c->h_normal[3 * e], c->h_normal[3 * e + 1], c->h_normal[3 * e + 2]};
-            std::vector<int64_t> pivoff;
-            std::vector<int32_t> piv;
-            bemsetup::build_pivot_lists(E, c->blocks, seed, pivoff, piv);
-            if ((rc = dupload(c, &c->d_pivoff, pivoff.data(), pivoff.size(), &c->allocs_piv))) return fail(rc);
-            if ((rc = dupload(c, &c->d_piv, piv.data(), piv.size(), &c->allocs_piv))) return fail(rc);
-            c->piv_ok = true;
-            c->piv_seed = seed;
-        }
+        if ((rc = ensure_pivot_lists(c, seed))) return fail(rc);
         d_pivoff = c->d_pivoff;
         d_piv = c->d_piv;
     }
@@ -3303,139 +3695,16 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
         c->vp.give(cap2);
     }
     free_list(c, tmp_owned);
-    // packed storage at the final ranks
-    h->uoff = c->vp.take<int64_t>(nb, 0); h->voff = c->vp.take<int64_t>(nb, 0);
-    h->aoff = c->vp.take<int64_t>(nb, 0); h->rank = c->vp.take<int32_t>(nb, -1);
-    int64_t tu = 0, tv = 0, ta = 0;
-    double maxr = 0, sumr = 0;
-    int64_t evaluated = 0;
-    for (int64_t b = 0; b < nb; b++) {
-        const auto& B = c->blocks[b];
-        if (!B.adm) { evaluated += B.rows * B.cols; continue; }
-        int32_t r = rank[b] > 0 ? rank[b] : 1;
-        h->rank[b] = r;
-        h->uoff[b] = tu; h->voff[b] = tv; h->aoff[b] = ta;
-        tu += B.rows * r; tv += B.cols * r; ta += r;
-        maxr = std::max(maxr, (double)r);
-        sumr += r;
-        evaluated += evs[b];
-    }
-    // When no block overflowed its first capacity and every block has rank >= 1,
-    // the first pass's scratch is the storage: its layout per block is the
-    // packed one (rank rows of U, then of V, then alpha), with the unused
-    // capacity left between blocks.  Otherwise the rank rows are copied into
-    // packed storage (k_compact; a rank-0 block becomes a zero rank-1 block).
-    bool direct = over.empty();
-    for (int32_t b : keep)
-        if (rank[b] < 1) direct = false;
-    if (direct) {
-        for (int32_t b : c->adm_list) {
-            h->uoff[b] = S1.uoff[b];
-            h->voff[b] = S1.voff[b];
-            h->aoff[b] = S1.aoff[b];
-        }
-        h->U = S1.U; h->V = S1.V; h->A = S1.A;
-        auto& o = S1.owned;
-        o.erase(std::remove_if(o.begin(), o.end(), [&](void* q) { return q == h->U || q == h->V || q == h->A; }),
-                o.end());
-    } else {
-        if ((rc = dalloc_bytes(c, &h->U, (size_t)tu * h->esz))) return fail(rc);
-        if ((rc = dalloc_bytes(c, &h->V, (size_t)tv * h->esz))) return fail(rc);
-        if ((rc = dalloc_bytes(c, &h->A, (size_t)ta * h->esz))) return fail(rc);
-    }
-    // offsets, ranks, MVM task lists and the packing lists: one upload
-    std::vector<int32_t>& rk = rank;   // first-pass ranks with the re-run ranks merged in
-    std::vector<void*> own2;
-    // (block, l) tasks of the first MVM phase, (block, row chunk) tasks of
-    // the second; sized first, then filled by index
-    std::vector<int64_t> toff = c->vp.take<int64_t>(nb, 0);
-    int64_t tot = 0, ntt = 0, nchunks = 0;
-    for (int32_t b : c->adm_list) {
-        tot += h->rank[b];
-        ntt += (h->rank[b] + kLrRows - 1) / kLrRows;
-        nchunks += (c->blocks[b].rows + kWG - 1) / kWG;
-    }
-    std::vector<int32_t> tt = c->vp.take<int32_t>((size_t)2 * ntt, 0), ut = c->vp.take<int32_t>((size_t)2 * nchunks, 0);
-    {
-        int64_t pt = 0, pu = 0, acc = 0;
-        for (int32_t b : c->adm_list) {
-            const int r = h->rank[b];
-            toff[b] = acc;
-            acc += r;
-            h->maxrank = std::max(h->maxrank, r);
-            for (int l = 0; l < r; l += kLrRows, pt++) { tt[2 * pt] = b; tt[2 * pt + 1] = l; }
-            const int64_t nq = (c->blocks[b].rows + kWG - 1) / kWG;
-            for (int64_t q = 0; q < nq; q++, pu++) { ut[2 * pu] = b; ut[2 * pu + 1] = (int32_t)q; }
-        }
-    }
-    h->nttask = ntt;
-    h->t_total = tot;
-    h->nutask = nchunks;
-    Packer pk;
-    size_t k_uoff = pk.add(h->uoff), k_voff = pk.add(h->voff), k_aoff = pk.add(h->aoff);
-    size_t k_rank = pk.add(h->rank), k_srank = pk.add(rk), k_tt = pk.add(tt), k_ut = pk.add(ut);
-    size_t k_toff = pk.add(toff), k_keep = pk.add(keep), k_over = pk.add(over);
-    rc = pk.upload(c, nullptr);
-    h->meta = pk.dbase;   // owned by h from here on, also on failure
-    if (rc) return fail(rc);
-    h->d_uoff = pk.ptr<int64_t>(k_uoff); h->d_voff = pk.ptr<int64_t>(k_voff); h->d_aoff = pk.ptr<int64_t>(k_aoff);
-    h->d_rank = pk.ptr<int32_t>(k_rank);
-    int32_t* d_srank = pk.ptr<int32_t>(k_srank);
-    h->d_ttask = pk.ptr<int32_t>(k_tt); h->d_utask = pk.ptr<int32_t>(k_ut); h->d_toff = pk.ptr<int64_t>(k_toff);
-    if ((rc = dalloc(c, &h->d_t, tot, nullptr))) return fail(rc);
-    auto compact = [&](const std::vector<int32_t>& lst, const int32_t* d_lst, const AcaScratch& Sx) -> int {
-        if (lst.empty()) return BEM_OK;
-        dim3 g((unsigned)lst.size());
-        if (f32)
-            hipLaunchKernelGGL(k_compact<bem_c32>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
-                               Sx.d_uoff, Sx.d_voff, Sx.d_aoff, (const bem_c32*)Sx.U, (const bem_c32*)Sx.V,
-                               (const bem_c32*)Sx.A, d_srank, h->d_uoff, h->d_voff, h->d_aoff, (bem_c32*)h->U,
-                               (bem_c32*)h->V, (bem_c32*)h->A);
-        else
-            hipLaunchKernelGGL(k_compact<bem_c64>, g, dim3(kWG), 0, c->stream, d_lst, c->d_rows, c->d_cols,
-                               Sx.d_uoff, Sx.d_voff, Sx.d_aoff, (const bem_c64*)Sx.U, (const bem_c64*)Sx.V,
-                               (const bem_c64*)Sx.A, d_srank, h->d_uoff, h->d_voff, h->d_aoff, (bem_c64*)h->U,
-                               (bem_c64*)h->V, (bem_c64*)h->A);
-        return launch_check(c, "k_compact");
-    };
-    if (!direct && (rc = compact(keep, pk.ptr<int32_t>(k_keep), S1))) return fail(rc);
-    if (!direct && (rc = compact(over, pk.ptr<int32_t>(k_over), S2))) return fail(rc);
-    HIPC(c, hipEventRecord(ev2, c->stream));
-    // NaN check of the dense part (the reference reports NaN blocks,
-    // hmatrix.cpp:812-820)
-    int32_t* d_flag;
-    if ((rc = dalloc(c, &d_flag, 1, &own2))) return fail(rc);
-    HIPC(c, hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
-    if (c->dense_total) {
-        if (f32)
-            hipLaunchKernelGGL(k_has_nonfinite<bem_c32>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
-                               c->dense_total, (const bem_c32*)h->dense, d_flag);
-        else
-            hipLaunchKernelGGL(k_has_nonfinite<bem_c64>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
-                               c->dense_total, (const bem_c64*)h->dense, d_flag);
-    }
-    int32_t hflag = 0;
-    HIPC(c, hipMemcpyAsync(&hflag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
-    free_list(c, own2);
+    if ((rc = finish_hmat(c, h, f32, S1, S2, rank, evs, over, keep, ev2))) return fail(rc);
     float ms_dense = 0, ms_aca = 0;
     (void)hipEventElapsedTime(&ms_dense, ev0, ev1);
     (void)hipEventElapsedTime(&ms_aca, ev1, ev2);
     (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1); (void)hipEventDestroy(ev2);
-    if (hflag) return fail(set_err(c, BEM_ERR_NAN, "bem_assemble: non-finite entry in the near field"));
-    size_t nadm = c->adm_list.size();
-    h->stats[0] = (double)evaluated;
-    h->stats[1] = maxr;
-    h->stats[2] = nadm ? sumr / (double)nadm : 0;
-    h->stats[3] = (double)((c->dense_total + tu + tv + ta) * (int64_t)h->esz);
     h->stats[4] = now_ms() - t0;
-    h->stats[5] = (double)over.size();
     h->stats[6] = ms_dense;
     h->stats[7] = ms_aca;
-    h->stats[8] = (double)((c->dense_total + (direct ? S1.tu + S1.tv + S1.ta : tu + tv + ta)) * (int64_t)h->esz +
-                           (h->dense64 ? c->dense_total * (int64_t)sizeof(bem_c64) : 0));
-    for (auto* v : {&cap, &rank, &flag, &rank2, &flag2, &tt, &ut}) c->vp.give(*v);
-    for (auto* v : {&evs, &evs2, &toff}) c->vp.give(*v);
+    for (auto* v : {&cap, &rank, &flag, &rank2, &flag2}) c->vp.give(*v);
+    for (auto* v : {&evs, &evs2}) c->vp.give(*v);
     if (pairing) {   // the DPhi near field waits for the next DPhi assembly of this state
         c->st_dense = dense2;
         c->st_dense64 = dense2_64;
@@ -3447,6 +3716,188 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
     }
     c->sing_cur = nullptr;
     *out = h;
+    return BEM_OK;
+}
+
+int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, double tol, int64_t maxrank,
+                       int precision, uint64_t seed, bem_hmat_t* outs)
+{
+    if (!c || !outs || !kcc3 || nf < 1 || nf > 64) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: bad arguments");
+    if (!c->have_part || !c->have_freq)
+        return set_err(c, BEM_ERR_STATE, "bem_assemble_batch: partition or boundary conditions missing");
+    if (kind != BEM_KIND_PHI && kind != BEM_KIND_DPHI) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: bad kind");
+    if (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)
+        return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: bad precision");
+    if (!(tol >= 0)) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: negative tolerance");
+    if (c->mesh.rvtx) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: not with an impedance plane");
+    for (int64_t f = 0; f < nf; f++)
+        if (!(kcc3[3 * f] > 0) || !std::isfinite(kcc3[3 * f]))
+            return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: k must be > 0");
+    HIPC(c, hipSetDevice(c->device));
+    const double t0 = now_ms();
+    const double k0 = c->k;
+    const bem_c64 cc0 = c->cc;
+    const int64_t nb = (int64_t)c->blocks.size();
+    const bool f32 = precision == BEM_PREC_FP32;
+    const size_t esz = f32 ? sizeof(bem_c32) : sizeof(bem_c64);
+    std::vector<bem_hmat*> H((size_t)nf, nullptr);
+    std::vector<bem_c64*> sing((size_t)nf, nullptr);
+    std::vector<void*> own;
+    auto use = [&](int64_t f) {
+        c->k = kcc3[3 * f];
+        c->cc = c64(kcc3[3 * f + 1], kcc3[3 * f + 2]);
+    };
+    auto fail = [&](int code) {
+        c->k = k0;
+        c->cc = cc0;
+        c->sing_cur = nullptr;
+        for (auto* h : H)
+            if (h) bem_hmat_destroy(h);
+        free_list(c, own);
+        return code;
+    };
+    int rc;
+    drop_kept(c);   // a near field kept for a paired assembly does not belong to these
+    const int64_t nitems = c->nnz + c->n;
+    const bool n64 = f32 && c->near64;   // FP32 operators with their FP64 near field (bem_set_near_fp64)
+    for (int64_t f = 0; f < nf; f++) {
+        H[f] = new bem_hmat();
+        H[f]->ctx = c;
+        H[f]->kind = kind;
+        H[f]->prec = precision;
+        H[f]->esz = esz;
+        if ((rc = dalloc_bytes(c, &H[f]->dense, (size_t)c->dense_total * esz))) return fail(rc);
+        if (n64 && (rc = dalloc_bytes(c, &H[f]->dense64, (size_t)c->dense_total * sizeof(bem_c64)))) return fail(rc);
+        if ((rc = dalloc(c, &sing[f], (size_t)nitems, &own))) return fail(rc);
+    }
+    hipEvent_t ev0, ev1, ev2;
+    HIPC(c, hipEventCreate(&ev0)); HIPC(c, hipEventCreate(&ev1)); HIPC(c, hipEventCreate(&ev2));
+    HIPC(c, hipEventRecord(ev0, c->stream));
+    for (int64_t f = 0; f < nf; f++) {   // self and near-singular entries per frequency
+        use(f);
+        Timer t(c, 1);
+        hipLaunchKernelGGL(k_singular, dim3(nblk(nitems, kWG)), dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), kind,
+                           c->d_sing_order, nitems, c->nnz, sing[f], nullptr);
+        if ((rc = launch_check(c, "k_singular"))) return fail(rc);
+    }
+    // the dense tiles of every frequency, on the main stream while the large
+    // ACA classes run on the side streams
+    auto dense_work = [&]() -> int {
+        int rc2;
+        for (int64_t f = 0; f < nf; f++) {
+            use(f);
+            bem_c64* d64 = (bem_c64*)(n64 ? H[f]->dense64 : H[f]->dense);   // the FP64 near field, if any
+            if (!c->tiles.empty()) {
+                Timer t(c, 0);
+                const dim3 g((unsigned)c->tiles.size());
+                if (f32 && !n64)
+                    hipLaunchKernelGGL(k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq32(c), c->d_subst,
+                                       kind, c->d_tiles, (bem_c32*)H[f]->dense, (bem_c32*)nullptr);
+                else
+                    hipLaunchKernelGGL(k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), c->d_tiles,
+                                       kind == BEM_KIND_PHI ? d64 : nullptr, kind == BEM_KIND_DPHI ? d64 : nullptr);
+                if ((rc2 = launch_check(c, "k_dense"))) return rc2;
+            }
+            if (!c->fixes.empty()) {
+                const dim3 g(nblk((int64_t)c->fixes.size(), kWG));
+                if (f32 && !n64)
+                    hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                       (int64_t)c->fixes.size(), c->nnz, sing[f], (bem_c32*)H[f]->dense);
+                else
+                    hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                       (int64_t)c->fixes.size(), c->nnz, sing[f], d64);
+                if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
+            }
+            if (n64 && c->dense_total) {
+                hipLaunchKernelGGL(k_round32, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream, c->dense_total,
+                                   (const bem_c64*)d64, (bem_c32*)H[f]->dense);
+                if ((rc2 = launch_check(c, "k_round32"))) return rc2;
+            }
+        }
+        return BEM_OK;
+    };
+    int64_t* d_pivoff = nullptr;
+    int32_t* d_piv = nullptr;
+    if (kind == BEM_KIND_DPHI) {
+        if ((rc = ensure_pivot_lists(c, seed))) return fail(rc);
+        d_pivoff = c->d_pivoff;
+        d_piv = c->d_piv;
+    }
+    auto full_cap = [&](int32_t b) {
+        int64_t m = std::min(c->blocks[b].rows, c->blocks[b].cols);
+        if (maxrank > 0) m = std::min(m, maxrank);
+        return (int32_t)m;
+    };
+    std::vector<int32_t> cap(nb, 0);
+    for (int32_t b : c->adm_list) cap[b] = std::min<int32_t>(full_cap(b), kCapFirst);
+    std::vector<FreqJob> jobs((size_t)nf);
+    for (int64_t f = 0; f < nf; f++) jobs[f] = FreqJob{kcc3[3 * f], c64(kcc3[3 * f + 1], kcc3[3 * f + 2]), sing[f]};
+    std::vector<AcaScratch> S1((size_t)nf);
+    std::vector<std::vector<int32_t>> rank, flag;
+    std::vector<std::vector<int64_t>> evs;
+    if ((rc = run_aca_batch(c, kind, precision, tol, maxrank, seed, jobs, c->adm_list, cap, d_pivoff, d_piv, dense_work,
+                            S1, rank, flag, evs)))
+        return fail(rc);
+    HIPC(c, hipEventRecord(ev1, c->stream));
+    for (int64_t f = 0; f < nf; f++) {
+        std::vector<int32_t> over, keep;
+        for (int32_t b : c->adm_list) {
+            if (flag[f][b] == 2)
+                return fail(set_err(c, BEM_ERR_NAN, "bem_assemble_batch: non-finite value in ACA block " +
+                                                        std::to_string(b)));
+            (flag[f][b] == 1 ? over : keep).push_back(b);
+        }
+        AcaScratch S2;
+        if (!over.empty()) {   // the overflowing blocks of this frequency with their full capacity
+            std::vector<int32_t> cap2(nb, 0), rank2, flag2;
+            std::vector<int64_t> evs2;
+            for (int32_t b : over) cap2[b] = std::min<int32_t>(full_cap(b), kCapMax);
+            use(f);
+            c->sing_cur = sing[f];
+            rc = run_aca(c, kind, precision, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, [] { return BEM_OK; }, S2,
+                         rank2, flag2, evs2);
+            c->sing_cur = nullptr;
+            if (rc) return fail(rc);
+            for (int32_t b : over) {
+                rank[f][b] = rank2[b];
+                evs[f][b] = evs2[b];
+                if (flag2[b])
+                    return fail(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
+                                        "bem_assemble_batch: ACA block " + std::to_string(b) +
+                                            (flag2[b] == 2 ? " non-finite" : " needs rank > 1024")));
+            }
+            c->vp.give(rank2); c->vp.give(flag2); c->vp.give(evs2);
+        }
+        if ((rc = finish_hmat(c, H[f], f32, S1[f], S2, rank[f], evs[f], over, keep, nullptr))) return fail(rc);
+        c->vp.give(rank[f]); c->vp.give(flag[f]); c->vp.give(evs[f]);
+    }
+    HIPC(c, hipEventRecord(ev2, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    float ms_dense = 0, ms_aca = 0;
+    (void)hipEventElapsedTime(&ms_dense, ev0, ev1);
+    (void)hipEventElapsedTime(&ms_aca, ev1, ev2);
+    (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1); (void)hipEventDestroy(ev2);
+    const double wall = now_ms() - t0;
+    for (int64_t f = 0; f < nf; f++) {   // the batch's times, shared out evenly
+        H[f]->stats[4] = wall / nf;
+        H[f]->stats[6] = ms_dense / nf;
+        H[f]->stats[7] = ms_aca / nf;
+        outs[f] = H[f];
+        H[f] = nullptr;
+    }
+    c->k = k0;
+    c->cc = cc0;
+    free_list(c, own);
+    return BEM_OK;
+}
+
+int bem_set_aca_schedule(bem_ctx_t c, int precision, int64_t small_bound, int64_t huge_bound, int queue)
+{
+    if (!c || (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32) || small_bound < 0 || huge_bound < small_bound)
+        return set_err(c, BEM_ERR_ARG, "bem_set_aca_schedule: bad arguments");
+    c->cls_small[precision] = small_bound;
+    c->cls_huge[precision] = huge_bound;
+    if (queue >= 0) c->aca_queue = queue != 0;
     return BEM_OK;
 }
 
@@ -3874,8 +4325,16 @@ int bem_precond_apply(bem_ctx_t c, bem_prec_t p, const void* v, void* z)
 }  // extern "C"
 
 namespace {
+// The operator of a lockstep solve: one H-matrix for all Q columns, or
+// (several) operator g for the columns g R .. g R + R - 1; refine: products
+// with the operators' refinement form (bem_hmvm_refine)
+struct OpSet {
+    std::vector<bem_hmat*> ops;
+    int R = 0;
+    bool refine = false;
+};
 template <int Q>
-int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
+int gmres_lockstep(bem_ctx* c, const OpSet& S, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
                    int64_t maxit, int64_t* its_out, double* rel_out);
 }  // namespace
 
@@ -3896,7 +4355,9 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
     if (!pc && c->use_graphs) {   // the same iteration as below, one column per replayed graph
         int64_t its = 0;
         double rel = 0;
-        int rc = gmres_lockstep<1>(c, h, (const bem_c64*)b, (bem_c64*)x, &tol, restart, maxit, &its, &rel);
+        OpSet S;
+        S.ops = {h};
+        int rc = gmres_lockstep<1>(c, S, (const bem_c64*)b, (bem_c64*)x, &tol, restart, maxit, &its, &rel);
         if (rc) return rc;
         if (iters) *iters = its;
         if (rel_resid) *rel_resid = rel;
@@ -4149,6 +4610,39 @@ int hmvm_multi(bem_ctx* c, bem_hmat* h, bool refine, const bem_c64* X, bem_c64* 
     return hmvm_q<bem_c32, bem_c32, Q>(c, h, h->dense, X, Y, accumulate);
 }
 
+int hmvm_multi_any(bem_ctx* c, bem_hmat* h, int Q, bool refine, const bem_c64* X, bem_c64* Y, int acc);
+
+// buffers of every operator of S for products of its column count
+int ensure_ops(bem_ctx* c, const OpSet& S, int Q)
+{
+    const int w = S.ops.size() == 1 ? Q : S.R;
+    if (w == 1) return BEM_OK;
+    int rc;
+    for (bem_hmat* h : S.ops)
+        if ((rc = ensure_multi(c, h, w))) return rc;
+    return BEM_OK;
+}
+
+// Y = A X for Q interleaved columns: one product of the single operator, or
+// per operator g its R columns gathered into tx, multiplied into ty and
+// scattered back (tx, ty: n R each, allocated by the caller)
+template <int Q>
+int apply_ops(bem_ctx* c, const OpSet& S, const bem_c64* X, bem_c64* Y, bem_c64* tx, bem_c64* ty)
+{
+    if (S.ops.size() == 1) return hmvm_multi<Q>(c, S.ops[0], S.refine, X, Y, 0);
+    int rc;
+    const int R = S.R;
+    for (size_t g = 0; g < S.ops.size(); g++) {
+        hipLaunchKernelGGL(k_cols, dim3(nblk(c->n * R, 256)), dim3(256), 0, c->stream, c->n, R, X, Q, (int)(g * R), tx,
+                           R, 0);
+        if ((rc = hmvm_multi_any(c, S.ops[g], R, S.refine, tx, ty, 0))) return rc;
+        hipLaunchKernelGGL(k_cols, dim3(nblk(c->n * R, 256)), dim3(256), 0, c->stream, c->n, R, (const bem_c64*)ty, R,
+                           0, Y, Q, (int)(g * R));
+        if ((rc = launch_check(c, "k_cols"))) return rc;
+    }
+    return BEM_OK;
+}
+
 // |v_q| of Q interleaved vectors into nrm[0 .. Q) (device), using part
 template <int Q>
 int norms_q(bem_ctx* c, const bem_c64* v, double* part, double* nrm)
@@ -4167,7 +4661,7 @@ int norms_q(bem_ctx* c, const bem_c64* v, double* part, double* nrm)
 // computed but not used); every restart recomputes b_q - A x_q.  With
 // graphs on, one column's launches are captured once per call and replayed.
 template <int Q>
-int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
+int gmres_lockstep(bem_ctx* c, const OpSet& S, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
                    int64_t maxit, int64_t* its_out, double* rel_out)
 {
     const int64_t n = c->n;
@@ -4183,7 +4677,10 @@ int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const 
     int* jp;
     int rc;
     // buffers of the product exist before a graph is captured (no allocation inside)
-    if (Q > 1 && (rc = ensure_multi(c, h, Q))) return rc;
+    if ((rc = ensure_ops(c, S, Q))) return rc;
+    bem_c64 *tx = nullptr, *ty = nullptr;
+    if (S.ops.size() > 1 && ((rc = dalloc(c, &tx, (size_t)(n * S.R), &own)) || (rc = dalloc(c, &ty, (size_t)(n * S.R), &own))))
+        return fail(rc);
     if ((rc = dalloc(c, &Vb, (size_t)((m + 1) * n * Q), &own)) || (rc = dalloc(c, &w, (size_t)(n * Q), &own)) ||
         (rc = dalloc(c, &r, (size_t)(n * Q), &own)) || (rc = dalloc(c, &xb, (size_t)(n * Q), &own)) ||
         (rc = dalloc(c, &dh, (size_t)((m + 2) * Q), &own)) || (rc = dalloc(c, &dhacc, (size_t)((m + 2) * Q), &own)) ||
@@ -4206,7 +4703,7 @@ int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const 
     auto iteration = [&]() -> int {
         int rc2;
         hipLaunchKernelGGL(k_basis_col<Q>, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n, jp, Vb, xb);
-        if ((rc2 = hmvm_multi<Q>(c, h, false, xb, w, 0))) return rc2;
+        if ((rc2 = apply_ops<Q>(c, S, xb, w, tx, ty))) return rc2;
         for (int pass = 0; pass < 2; pass++) {   // classical Gram-Schmidt, twice
             hipLaunchKernelGGL(k_cgs_dots_q<Q>, dim3(nchunk, (ng + kWG / 64 - 1) / (kWG / 64)), dim3(kWG), 0,
                                c->stream, n, jp, Vb, w, dpart);
@@ -4268,7 +4765,7 @@ int gmres_lockstep(bem_ctx* c, bem_hmat* h, const bem_c64* B, bem_c64* X, const 
     int64_t it = 0;
     while (true) {
         // r = b - A x per right-hand side
-        if ((rc = hmvm_multi<Q>(c, h, false, X, r, 0))) return cleanup(rc);
+        if ((rc = apply_ops<Q>(c, S, X, r, tx, ty))) return cleanup(rc);
         hipLaunchKernelGGL(k_residual_n, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, B, r);
         if ((rc = norms_q<Q>(c, r, dnp, dnrm)) || (rc = fetch(beta, dnrm, Q))) return cleanup(rc);
         bool all = true;
@@ -4350,14 +4847,24 @@ int unpack(bem_ctx* c, int64_t nq, int64_t ld, int Q, const bem_c64* src, bem_c6
     return launch_check(c, "k_repack");
 }
 
-int gmres_lockstep_any(bem_ctx* c, bem_hmat* h, int Q, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
+int gmres_lockstep_any(bem_ctx* c, const OpSet& S, int Q, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
                        int64_t maxit, int64_t* its, double* rel)
 {
     switch (Q) {
-    case 1: return gmres_lockstep<1>(c, h, B, X, tol, m, maxit, its, rel);
-    case 2: return gmres_lockstep<2>(c, h, B, X, tol, m, maxit, its, rel);
-    case 4: return gmres_lockstep<4>(c, h, B, X, tol, m, maxit, its, rel);
-    default: return gmres_lockstep<8>(c, h, B, X, tol, m, maxit, its, rel);
+    case 1: return gmres_lockstep<1>(c, S, B, X, tol, m, maxit, its, rel);
+    case 2: return gmres_lockstep<2>(c, S, B, X, tol, m, maxit, its, rel);
+    case 4: return gmres_lockstep<4>(c, S, B, X, tol, m, maxit, its, rel);
+    default: return gmres_lockstep<8>(c, S, B, X, tol, m, maxit, its, rel);
+    }
+}
+
+int apply_ops_any(bem_ctx* c, const OpSet& S, int Q, const bem_c64* X, bem_c64* Y, bem_c64* tx, bem_c64* ty)
+{
+    switch (Q) {
+    case 1: return apply_ops<1>(c, S, X, Y, tx, ty);
+    case 2: return apply_ops<2>(c, S, X, Y, tx, ty);
+    case 4: return apply_ops<4>(c, S, X, Y, tx, ty);
+    default: return apply_ops<8>(c, S, X, Y, tx, ty);
     }
 }
 
@@ -4384,13 +4891,13 @@ int norms_any(bem_ctx* c, int Q, const bem_c64* v, double* part, double* nrm)
 // Iterative refinement over Q interleaved right-hand sides (the loop of
 // bem_gmres_ir): r = b - A_res x in FP64, inner lockstep GMRES on h with
 // the per-right-hand-side relative tolerances min(0.5, 0.5 tol / rel_q)
-int gmres_ir_q(bem_ctx* c, bem_hmat* h, bem_hmat* hres, int Q, const bem_c64* B, bem_c64* X, bool x_is_zero,
+int gmres_ir_q(bem_ctx* c, const OpSet& inner, const OpSet& res, int Q, const bem_c64* B, bem_c64* X, bool x_is_zero,
                double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* its_out, double* rel_out,
                int64_t* steps_out)
 {
     const int64_t n = c->n;
     std::vector<void*> own;
-    bem_c64 *r, *d;
+    bem_c64 *r, *d, *tx = nullptr, *ty = nullptr;
     double *np, *nrm;
     int rc;
     auto done = [&](int code) {
@@ -4400,6 +4907,9 @@ int gmres_ir_q(bem_ctx* c, bem_hmat* h, bem_hmat* hres, int Q, const bem_c64* B,
     if ((rc = dalloc(c, &r, (size_t)(n * Q), &own)) || (rc = dalloc(c, &d, (size_t)(n * Q), &own)) ||
         (rc = dalloc(c, &np, (size_t)(256 * Q), &own)) || (rc = dalloc(c, &nrm, (size_t)Q, &own)))
         return done(rc);
+    if (res.ops.size() > 1 &&
+        ((rc = dalloc(c, &tx, (size_t)(n * res.R), &own)) || (rc = dalloc(c, &ty, (size_t)(n * res.R), &own))))
+        return done(rc);
     std::vector<double> bn(Q), rn(Q), rel(Q, 0.0), tq(Q);
     auto fetch = [&](double* dst) -> int {
         HIPC(c, hipMemcpyAsync(dst, nrm, Q * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -4407,7 +4917,7 @@ int gmres_ir_q(bem_ctx* c, bem_hmat* h, bem_hmat* hres, int Q, const bem_c64* B,
         return BEM_OK;
     };
     if ((rc = norms_any(c, Q, B, np, nrm)) || (rc = fetch(bn.data()))) return done(rc);
-    std::vector<int64_t> its(Q, 0), inner(Q);
+    std::vector<int64_t> its(Q, 0), inner_its(Q);
     int64_t st = 0, total = 0;
     bool zero = x_is_zero;
     if (zero) {
@@ -4418,7 +4928,7 @@ int gmres_ir_q(bem_ctx* c, bem_hmat* h, bem_hmat* hres, int Q, const bem_c64* B,
         if (zero) {
             HIPC(c, hipMemcpyAsync(r, B, (size_t)(n * Q) * sizeof(bem_c64), hipMemcpyDeviceToDevice, c->stream));
         } else {
-            if ((rc = hmvm_multi_any(c, hres, Q, true, X, r, 0))) return done(rc);
+            if ((rc = apply_ops_any(c, res, Q, X, r, tx, ty))) return done(rc);
             hipLaunchKernelGGL(k_residual_n, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, B, r);
         }
         if ((rc = norms_any(c, Q, r, np, nrm)) || (rc = fetch(rn.data()))) return done(rc);
@@ -4431,12 +4941,12 @@ int gmres_ir_q(bem_ctx* c, bem_hmat* h, bem_hmat* hres, int Q, const bem_c64* B,
         }
         if (all || st >= max_steps || total >= maxit) break;
         hipLaunchKernelGGL(k_fill0, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, d);
-        rc = gmres_lockstep_any(c, h, Q, r, d, tq.data(), restart, maxit - total, inner.data(), nullptr);
+        rc = gmres_lockstep_any(c, inner, Q, r, d, tq.data(), restart, maxit - total, inner_its.data(), nullptr);
         if (rc != BEM_OK) return done(rc);
         int64_t mx = 0;
         for (int q = 0; q < Q; q++) {
-            its[q] += inner[q];
-            mx = std::max(mx, inner[q]);
+            its[q] += inner_its[q];
+            mx = std::max(mx, inner_its[q]);
         }
         total += std::max<int64_t>(mx, 1);
         hipLaunchKernelGGL(k_axpy1, dim3(nblk(n * Q, 256)), dim3(256), 0, c->stream, n * Q, (const bem_c64*)d, X);
@@ -4522,7 +5032,11 @@ int bem_gmres_multi(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, co
     std::vector<double> rel(Q, 0.0);
     int64_t st = 0;
     if (ir) {
-        rc = gmres_ir_q(c, h, hres, Q, bq, xq, x_is_zero != 0, tol, restart, maxit, max_steps, its.data(), rel.data(),
+        OpSet in, rs;
+        in.ops = {h};
+        rs.ops = {hres};
+        rs.refine = true;
+        rc = gmres_ir_q(c, in, rs, Q, bq, xq, x_is_zero != 0, tol, restart, maxit, max_steps, its.data(), rel.data(),
                         &st);
     } else {
         if (x_is_zero) {
@@ -4530,7 +5044,9 @@ int bem_gmres_multi(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, co
             rc = launch_check(c, "k_fill0");
         }
         std::vector<double> tq(Q, tol);
-        if (!rc) rc = gmres_lockstep_any(c, h, Q, bq, xq, tq.data(), restart, maxit, its.data(), rel.data());
+        OpSet in;
+        in.ops = {h};
+        if (!rc) rc = gmres_lockstep_any(c, in, Q, bq, xq, tq.data(), restart, maxit, its.data(), rel.data());
     }
     if (!rc && Q != nrhs) rc = unpack(c, nrhs, nrhs, Q, xq, (bem_c64*)x);
     if (!rc) {
@@ -4550,6 +5066,80 @@ int bem_gmres_multi(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, co
     return BEM_OK;
 }
 
+int bem_gmres_batch(bem_ctx_t c, int64_t nops, const bem_hmat_t* hs, const bem_hmat_t* hres, int64_t nrhs,
+                    const void* b, void* x, int x_is_zero, double tol, int64_t restart, int64_t maxit, int64_t max_steps,
+                    int64_t* iters, double* rel_resid, int64_t* steps)
+{
+    if (!c || nops < 1 || !hs || !b || !x || (nrhs != 1 && nrhs != 2 && nrhs != 4) || nops * nrhs > kArnMax ||
+        restart < 1 || maxit < 0 || max_steps < 0 || !(tol > 0))
+        return set_err(c, BEM_ERR_ARG, "bem_gmres_batch: bad arguments (nrhs 1, 2 or 4; nops x nrhs <= 8)");
+    for (int64_t o = 0; o < nops; o++) {
+        if (!hs[o] || hs[o]->ctx != c || (hres && (!hres[o] || hres[o]->ctx != c)))
+            return set_err(c, BEM_ERR_ARG, "bem_gmres_batch: operator of another context");
+        const bem_hmat* hr = hres ? hres[o] : hs[o];
+        if ((hres || max_steps > 0) && hr->prec == BEM_PREC_FP32 && !hr->dense64)
+            return set_err(c, BEM_ERR_STATE, "bem_gmres_batch: FP32 residual operator without an FP64 near field");
+    }
+    HIPC(c, hipSetDevice(c->device));
+    const int64_t nq = nops * nrhs;
+    const int Q = pow2_q(nq);
+    const bool ir = hres != nullptr || max_steps > 0;
+    // operator groups of R = nrhs columns; the padding columns (zero
+    // right-hand sides) reuse the first operator
+    OpSet in, rs;
+    in.R = rs.R = (int)nrhs;
+    rs.refine = true;
+    for (int g = 0; g < Q / nrhs; g++) {
+        const int64_t o = g < nops ? g : 0;
+        in.ops.push_back(hs[o]);
+        rs.ops.push_back(hres ? hres[o] : hs[o]);
+    }
+    if (Q == (int)nrhs) {   // one group: the multi-right-hand-side product
+        in.ops.resize(1);
+        rs.ops.resize(1);
+    }
+    std::vector<void*> own;
+    bem_c64 *bq = (bem_c64*)b, *xq = (bem_c64*)x;
+    int rc = BEM_OK;
+    if (Q != nq) {
+        if ((rc = dalloc(c, &bq, (size_t)(c->n * Q), &own)) || (rc = dalloc(c, &xq, (size_t)(c->n * Q), &own)) ||
+            (rc = repack(c, nq, nq, Q, (const bem_c64*)b, bq)) || (rc = repack(c, nq, nq, Q, (const bem_c64*)x, xq))) {
+            free_list(c, own);
+            return rc;
+        }
+    }
+    std::vector<int64_t> its(Q, 0);
+    std::vector<double> rel(Q, 0.0);
+    int64_t st = 0;
+    if (ir) {
+        rc = gmres_ir_q(c, in, rs, Q, bq, xq, x_is_zero != 0, tol, restart, maxit, max_steps, its.data(), rel.data(),
+                        &st);
+    } else {
+        if (x_is_zero) {
+            hipLaunchKernelGGL(k_fill0, dim3(nblk(c->n * Q, 256)), dim3(256), 0, c->stream, c->n * Q, xq);
+            rc = launch_check(c, "k_fill0");
+        }
+        std::vector<double> tq(Q, tol);
+        if (!rc) rc = gmres_lockstep_any(c, in, Q, bq, xq, tq.data(), restart, maxit, its.data(), rel.data());
+    }
+    if (!rc && Q != nq) rc = unpack(c, nq, nq, Q, xq, (bem_c64*)x);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = set_err(c, BEM_ERR_HIP, std::string("bem_gmres_batch: ") + hipGetErrorString(e));
+    }
+    free_list(c, own);
+    if (rc) return rc;
+    bool conv = true;
+    for (int64_t q = 0; q < nq; q++) {
+        if (iters) iters[q] = its[q];
+        if (rel_resid) rel_resid[q] = rel[q];
+        conv = conv && rel[q] < tol;
+    }
+    if (steps) *steps = st;
+    if (!conv) return set_err(c, BEM_ERR_NOCONV, "bem_gmres_batch: maxit or max_steps reached above tolerance");
+    return BEM_OK;
+}
+
 int bem_gmres_ir(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, bem_prec_t pc, const void* b, void* x, int x_is_zero,
                  double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* iters, double* rel_resid,
                  int64_t* steps)
@@ -4564,7 +5154,11 @@ int bem_gmres_ir(bem_ctx_t c, bem_hmat_t h, bem_hmat_t hres, bem_prec_t pc, cons
     if (!pc) {   // the lockstep solver (graph-captured columns) with one right-hand side
         int64_t its = 0, st = 0;
         double rel = 0;
-        int rc = gmres_ir_q(c, h, hres, 1, (const bem_c64*)b, (bem_c64*)x, x_is_zero != 0, tol, restart, maxit,
+        OpSet in, rs;
+        in.ops = {h};
+        rs.ops = {hres};
+        rs.refine = true;
+        int rc = gmres_ir_q(c, in, rs, 1, (const bem_c64*)b, (bem_c64*)x, x_is_zero != 0, tol, restart, maxit,
                             max_steps, &its, &rel, &st);
         if (rc) return rc;
         if (iters) *iters = its;

@@ -13,6 +13,13 @@
 namespace bem {
 namespace {
 
+double bem_hmat_stats_value(bem_hmat_t h, int q)
+{
+    double s[9] = {0};
+    if (h) bem_hmat_stats(h, s);
+    return s[q];
+}
+
 void check(bem_ctx_t ctx, int rc, const char* what)
 {
     if (rc != BEM_OK) throw Error(rc, std::string(what) + ": " + bem_last_error(ctx));
@@ -528,6 +535,142 @@ void BoundaryElementSolver::solveLockstep(const std::vector<RightHandSide>& rhs,
     stats[6] = rel[(size_t)nr - 1];
 }
 
+void BoundaryElementSolver::calculateBoundarySolutionsBatch(const std::vector<double>& freqs,
+                                                            const std::vector<RightHandSide>& rhs,
+                                                            std::vector<std::vector<std::vector<cplx>>>& pressure,
+                                                            std::vector<std::vector<int64_t>>& iterations,
+                                                            std::vector<double>& stats16)
+{
+    auto& E = boundaryElements;
+    const auto& P = bemSolverParameters;
+    const int64_t nf = (int64_t)freqs.size(), nr = (int64_t)rhs.size(), n = E.n, nq = nf * nr;
+    if (nf < 1 || (nr != 1 && nr != 2 && nr != 4) || nq > 8)
+        throw Error(BEM_ERR_ARG, "batched solve: 1, 2 or 4 right-hand sides and at most 8 (frequency, right-hand side) pairs");
+    if (P.preconditioner != 0 || !E.impedancePlanes.empty())
+        throw Error(BEM_ERR_ARG, "batched solve: no preconditioner and no impedance plane");
+    const int R = refineMode(P);
+    const double t0 = now_ms();
+    check(ctx_, bem_set_near_fp64(ctx_, R == 1), "bem_set_near_fp64");
+    // wavenumber, coupling and source terms per frequency (the context's
+    // boundary-condition arrays are set once: they do not depend on f)
+    std::vector<double> kcc(3 * nf);
+    std::vector<std::vector<cplx>> src(nq);
+    auto keepPoints = pointSources;
+    auto keepPlanes = planeWaves;
+    bool anyG = false;
+    for (int64_t f = nf - 1; f >= 0; f--) {   // ends with the first frequency set on the context
+        setFrequency(freqs[f]);
+        setWavenumberAndCoupling();
+        kcc[3 * f] = wavenumber;
+        kcc[3 * f + 1] = couplingParameter.real();
+        kcc[3 * f + 2] = couplingParameter.imag();
+        for (int64_t k = 0; k < nr; k++) {
+            if ((int64_t)rhs[k].g.size() != n) throw Error(BEM_ERR_ARG, "right-hand side of the wrong length");
+            pointSources = rhs[k].pointSources;
+            planeWaves = rhs[k].planeWaves;
+            calculateBoundaryConditionAndSourceTermVector();
+            src[f * nr + k] = sourceTermVector;
+            for (const cplx& v : rhs[k].g) anyG = anyG || v != cplx(0, 0);
+        }
+    }
+    pointSources = keepPoints;
+    planeWaves = keepPlanes;
+    auto destroy = [](std::vector<bem_hmat_t>& v) {
+        for (bem_hmat_t h : v)
+            if (h) bem_hmat_destroy(h);
+        v.clear();
+    };
+    std::vector<bem_hmat_t> phis(nf, nullptr), dphis(nf, nullptr), dres;
+    std::vector<cplx> Bi((size_t)(n * nq));
+    DeviceVector B(ctx_, n * nq), X(ctx_, n * nq), G(ctx_, n * nr), Bf(ctx_, n * nr);
+    std::vector<cplx> Gi((size_t)(n * nr));
+    for (int64_t k = 0; k < nr; k++)
+        for (int64_t i = 0; i < n; i++) Gi[i * nr + k] = rhs[k].g[i];
+    G.upload(Gi);
+    double t1 = now_ms(), t2, t3;
+    try {
+        if (anyG) {   // b = s + Phi g, Phi of each frequency read once for its right-hand sides
+            check(ctx_, bem_assemble_batch(ctx_, BEM_KIND_PHI, nf, kcc.data(), P.acaRelativeError, (int64_t)P.acaMaxRank,
+                                           R == 2 ? BEM_PREC_FP64 : P.precision, P.pivotSeed, phis.data()),
+                  "Phi batch");
+            t1 = now_ms();
+        }
+        std::vector<cplx> sf((size_t)(n * nr));
+        for (int64_t f = 0; f < nf; f++) {
+            for (int64_t k = 0; k < nr; k++)
+                for (int64_t i = 0; i < n; i++) sf[i * nr + k] = src[f * nr + k][i];
+            if (anyG) {
+                Bf.upload(sf);
+                check(ctx_, bem_hmvm_multi(ctx_, phis[f], nr, G.data(), Bf.data(), 1, R == 1), "b = s + Phi g");
+                sf = Bf.download();
+            }
+            for (int64_t k = 0; k < nr; k++)
+                for (int64_t i = 0; i < n; i++) Bi[i * nq + f * nr + k] = sf[i * nr + k];
+        }
+        destroy(phis);
+        B.upload(Bi);
+        t2 = now_ms();
+        if (R == 2) {
+            dres.assign(nf, nullptr);
+            check(ctx_, bem_assemble_batch(ctx_, BEM_KIND_DPHI, nf, kcc.data(), P.acaRelativeError,
+                                           (int64_t)P.acaMaxRank, BEM_PREC_FP64, P.pivotSeed, dres.data()),
+                  "DPhi FP64 batch");
+        }
+        check(ctx_, bem_assemble_batch(ctx_, BEM_KIND_DPHI, nf, kcc.data(), P.acaRelativeError, (int64_t)P.acaMaxRank,
+                                       P.precision, P.pivotSeed, dphis.data()),
+              "DPhi batch");
+        t3 = now_ms();
+        const double tol = P.gmresTolerance > 0 ? P.gmresTolerance : 1e-3 * P.acaRelativeError;
+        std::vector<int64_t> its(nq, 0);
+        std::vector<double> rel(nq, 0.0);
+        int64_t steps = 0;
+        int rc = bem_gmres_batch(ctx_, nf, dphis.data(), R == 2 ? dres.data() : nullptr, nr, B.data(), X.data(), 1, tol,
+                                 P.gmresRestart, 100000, R ? P.refineMaxSteps : 0, its.data(), rel.data(), &steps);
+        if (rc != BEM_OK && rc != BEM_ERR_NOCONV) check(ctx_, rc, "bem_gmres_batch");
+        std::vector<cplx> Xi = X.download();
+        DeviceVector x(ctx_, n), g(ctx_, n), phi(ctx_, n), dphi(ctx_, n), p(ctx_, n);
+        std::vector<cplx> xk((size_t)n);
+        pressure.assign(nf, std::vector<std::vector<cplx>>(nr));
+        iterations.assign(nf, std::vector<int64_t>(nr, 0));
+        for (int64_t f = 0; f < nf; f++)
+            for (int64_t k = 0; k < nr; k++) {
+                for (int64_t i = 0; i < n; i++) xk[i] = Xi[i * nq + f * nr + k];
+                x.upload(xk);
+                g.upload(rhs[k].g);
+                check(ctx_, bem_reconstruct_g(ctx_, x.data(), g.data(), airDensity, freqs[f], phi.data(), dphi.data(),
+                                              p.data()),
+                      "bem_reconstruct_g");
+                pressure[f][k] = p.download();
+                iterations[f][k] = its[f * nr + k];
+            }
+        const double t4 = now_ms();
+        stats16.assign(16 * nf, 0.0);
+        for (int64_t f = 0; f < nf; f++) {
+            double* st = stats16.data() + 16 * f;
+            st[0] = (t1 - t0) / nf; st[1] = (t2 - t1) / nf; st[2] = (t3 - t2) / nf; st[3] = (t4 - t3) / nf;
+            st[4] = (t4 - t0) / nf;
+            st[5] = (double)its[f * nr + nr - 1];
+            st[6] = rel[f * nr + nr - 1];
+            st[8] = bem_hmat_stats_value(dphis[f], 1);
+            st[10] = bem_hmat_stats_value(dphis[f], 0);
+            st[15] = bem_hmat_stats_value(dphis[f], 3);
+        }
+        if (R) {
+            refineStats[0] = (double)steps;
+            refineStats[1] = 0;
+            for (int64_t q = 0; q < nq; q++) refineStats[1] += (double)its[q];
+            refineStats[2] = *std::max_element(rel.begin(), rel.end());
+        }
+    } catch (...) {
+        destroy(phis);
+        destroy(dphis);
+        destroy(dres);
+        throw;
+    }
+    destroy(dphis);
+    destroy(dres);
+}
+
 void BoundaryElementSolver::regularSolve(double tol, long restart)
 {
     setWavenumberAndCoupling();
@@ -688,6 +831,7 @@ struct bem_solver {
     double hluRelError = 0;
     int refine = 0;
     long refineMaxSteps = 10;
+    int sweepBatch = 0;   // bem_solver_set_sweep_batch
 };
 
 extern "C" {
@@ -854,6 +998,13 @@ int bem_solver_set_refinement(bem_solver_t s, int mode, int64_t max_steps)
     return BEM_OK;
 }
 
+int bem_solver_set_sweep_batch(bem_solver_t s, int batch)
+{
+    if (!s || !s->s || batch < 0) return BEM_ERR_ARG;
+    s->sweepBatch = batch;
+    return BEM_OK;
+}
+
 int bem_solver_refine_stats(bem_solver_t s, double* out4)
 {
     if (!s || !s->s || !out4) return BEM_ERR_ARG;
@@ -946,6 +1097,39 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
             } else {
                 rhs[k].planeWaves = s->s->planeWaves;
             }
+        }
+        if (s->sweepBatch > 1) {   // batched assembly and lockstep GMRES over frequency chunks, one context
+            auto& S = *s->s;
+            S.setWaveSpeed(c);
+            S.setAirDensity(rho);
+            auto& P = S.bemSolverParameters;
+            P.acaRelativeError = aca_tol;
+            P.acaMaxRank = (unsigned long)max_rank;
+            P.coupling = (bem::BemCoupling)coupling;
+            P.gmresTolerance = gmres_tol;
+            P.gmresRestart = (long)restart;
+            P.pivotSeed = seed;
+            P.precision = precision;
+            P.preconditioner = s->preconditioner;
+            P.refine = s->refine;
+            P.refineMaxSteps = s->refineMaxSteps;
+            const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(s->sweepBatch, 8 / nrhs));
+            for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
+                const int64_t f1 = std::min(nf, f0 + chunk);
+                std::vector<double> fr(freqs + f0, freqs + f1);
+                std::vector<std::vector<std::vector<bem::cplx>>> pr;
+                std::vector<std::vector<int64_t>> its;
+                std::vector<double> st;
+                S.calculateBoundarySolutionsBatch(fr, rhs, pr, its, st);
+                for (int64_t f = f0; f < f1; f++) {
+                    for (int64_t k = 0; k < nrhs; k++) {
+                        std::memcpy(pressure + 2 * ((f * nrhs + k) * n), pr[f - f0][k].data(), n * sizeof(bem::cplx));
+                        if (iters) iters[f * nrhs + k] = its[f - f0][k];
+                    }
+                    if (stats16) std::memcpy(stats16 + 16 * f, st.data() + 16 * (f - f0), 16 * sizeof(double));
+                }
+            }
+            return BEM_OK;
         }
         const int L = (int)std::min<int64_t>(lanes, std::max<int64_t>(nf, 1));
         while ((int)s->lanes.size() < L - 1)

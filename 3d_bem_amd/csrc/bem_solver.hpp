@@ -170,6 +170,15 @@ public:
     // gives for it alone
     void calculateBoundarySolutions(const std::vector<RightHandSide>& rhs, std::vector<std::vector<cplx>>& pressure,
                                     std::vector<int64_t>& iterations);
+    // calculateBoundarySolutions for several frequencies at once: Phi and DPhi
+    // of all of them from one batched assembly each (bem_assemble_batch), the
+    // GMRES of every (frequency, right-hand side) pair in one lockstep solve
+    // (bem_gmres_batch); freqs.size() * rhs.size() <= 8, rhs.size() 1, 2 or 4,
+    // no preconditioner, no impedance plane.  pressure[f][k], iterations[f][k];
+    // stats (16 per frequency) carry the batch's times shared out evenly.
+    void calculateBoundarySolutionsBatch(const std::vector<double>& freqs, const std::vector<RightHandSide>& rhs,
+                                         std::vector<std::vector<std::vector<cplx>>>& pressure,
+                                         std::vector<std::vector<int64_t>>& iterations, std::vector<double>& stats16);
     // calculateFieldSolution with the direct sum (calculateFieldSolutionRegular,
     // boundaryelementsolver.cpp:2192-2316): sound pressure at observation
     // points from the last boundary solution, incident sources included

@@ -241,9 +241,12 @@ def batch_freqs(center, batch):
     return f
 
 
-def timed_run(bem, S, ctx_h, args, prec, freqs, lanes):
+def timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sweep_batch=0):
     """warmup + timed steps (one sweep of the batch each); kernel family
-    times from HIP events recorded on lane 0's launching streams"""
+    times from HIP events recorded on lane 0's launching streams (with
+    sweep_batch > 1: on the one context that runs every point)"""
+    S.set_sweep_batch(sweep_batch)
+
     def step():
         return S.sweep(freqs, c=343.0, rho=1.21, coupling=1, aca_tol=args.aca_tol, gmres_tol=args.gmres_tol,
                        seed=args.seed, precision=prec, lanes=lanes)
@@ -352,6 +355,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=8, help="frequency points per step")
     ap.add_argument("--lanes", type=int, default=8, help="concurrent solver lanes of the FP32 form")
+    ap.add_argument("--sweep-batch", type=int, default=0, metavar="B",
+                    help="FP32 form: frequency chunks of B points with batched assembly and lockstep GMRES on one "
+                         "context instead of the lanes (0: lanes)")
+    ap.add_argument("--no-batch-leg", action="store_true",
+                    help="skip the comparison run of the other FP32 scheduling (lanes or batched chunks)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--freq", type=float, default=1000.0)
@@ -407,16 +415,19 @@ def main():
     prec = bem.FP32 if args.precision == "fp32" else bem.FP64
     freqs = batch_freqs(args.freq, args.batch)
     ic = int(np.argmin(np.abs(freqs - args.freq)))
-    lanes = args.lanes if prec == bem.FP32 else 1
+    batched = prec == bem.FP32 and args.sweep_batch > 1 and args.plane is None
+    lanes = args.lanes if prec == bem.FP32 and not batched else 1
+    sb = args.sweep_batch if batched else 0
     refine = args.refine if prec == bem.FP32 else 0
     S.set_refinement(refine)
 
-    pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes)
+    pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sb)
     rstats = S.refine_stats()
     points = args.steps * len(freqs)
-    print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, lanes {lanes}", file=sys.stderr, flush=True)
+    sched = f"batches of {sb}" if batched else f"{lanes} lanes"
+    print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, {sched}", file=sys.stderr, flush=True)
     st = stats[ic]
-    points0 = args.steps * len(range(0, len(freqs), max(lanes, 1)))
+    points0 = args.steps * len(range(0, len(freqs), max(lanes, 1)))   # points whose launches lane 0 records
     out = {
         "metric": "freq-points solved/s at 20k tris (FP32+refine vs FP64); rel-L2 err vs FP64 oracle",
         "value": round(points / elapsed, 4),
@@ -433,9 +444,9 @@ def main():
         "config": {"workload": f"pulsating sphere, icosphere level {args.level} ({n} triangles), r=0.5 m, "
                                f"{len(freqs)} frequency points {freqs[0]:.0f}-{freqs[-1]:.0f} Hz per step "
                                f"(incl. {args.freq:g} Hz), Burton-Miller, leaf {args.leaf}, eta {args.eta:g}, "
-                               f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}, {lanes} lanes"
+                               f"ACA {args.aca_tol:g}, GMRES {args.gmres_tol:g}, {args.precision}, {sched}"
                                + ("" if args.plane is None else f", rigid plane {args.plane:g} m below the centre"),
-                   "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "setup_s": round(setup_s, 3),
+                   "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "sweep_batch": sb, "setup_s": round(setup_s, 3),
                    "refine": refine, "refine_stats_last_point_lane0": rstats,
                    "ms_per_point": round(1e3 * elapsed / points, 3),
                    "gmres_iters": int(its[ic, 0]), "residual_estimate": st["residual"],
@@ -453,9 +464,20 @@ def main():
     p_head = pr[ic, 0]
     p64 = None
     pr0 = None
+    if prec == bem.FP32 and args.plane is None and not args.no_batch_leg:
+        # the other scheduling of the same FP32 points: lanes, or batched chunks of 8
+        ob = 0 if batched else 8
+        ol = args.lanes if batched else 1
+        prb, _, _, elb, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, ol, ob)
+        key = "lanes_leg" if batched else "sweep_batch_leg"
+        print(f"# {key}: {1e3 * elb / points:.2f} ms/point", file=sys.stderr, flush=True)
+        out[key] = {"value": round(points / elb, 4), "ms_per_point": round(1e3 * elb / points, 3),
+                    "lanes": ol, "sweep_batch": ob,
+                    "max_rel_diff_vs_headline": float(max(np.linalg.norm(prb[i, 0] - pr[i, 0]) /
+                                                          np.linalg.norm(pr[i, 0]) for i in range(len(freqs))))}
     if refine and not args.no_norefine_leg:
         S.set_refinement(0)
-        pr0, _, _, el0, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, lanes)
+        pr0, _, _, el0, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sb)
         S.set_refinement(refine)
         print(f"# fp32 without refinement: {1e3 * el0 / points:.2f} ms/point", file=sys.stderr, flush=True)
         out["fp32_norefine"] = {"value": round(points / el0, 4), "ms_per_point": round(1e3 * el0 / points, 3)}

@@ -134,6 +134,24 @@ int bem_set_frequency(bem_ctx_t ctx, double k, double cc_re, double cc_im);
 int bem_assemble(bem_ctx_t ctx, int kind, double aca_tol, int64_t max_rank, int precision, uint64_t seed,
                  bem_hmat_t* out);
 int bem_hmat_destroy(bem_hmat_t h);
+/* Batched assembly (no counterpart in the reference, whose frequency loop
+   assembles one operator at a time, bemsolvercontroller.cpp:59-79): the
+   operator `kind` at nf frequencies (kcc3: nf x (k, cc re, cc im)) on the
+   context's mesh, partition and boundary conditions, returned as nf
+   handles in outs, each equal to bem_assemble at that frequency without
+   near-field pairing (the FP64 near field of bem_set_near_fp64 is kept).  The ACA runs as one
+   launch per size class over the flattened (frequency, block) list, the
+   same-size blocks of all frequencies adjacent and the largest first; the
+   self / near-singular tables and dense tiles per frequency.  Not with an
+   impedance plane; 1 <= nf <= 64.  The context's frequency is unchanged.
+   bem_set_aca_schedule: the ACA size-class bounds on m + n for one
+   precision (one wavefront up to small_bound, the middle workgroup shape
+   up to huge_bound, the largest above) and, for queue >= 0, whether batched
+   ACA launches run as persistent workgroups that take items from a device
+   counter (1) or one workgroup per item (0).  Results do not depend on it. */
+int bem_assemble_batch(bem_ctx_t ctx, int kind, int64_t nf, const double* kcc3, double aca_tol, int64_t max_rank,
+                       int precision, uint64_t seed, bem_hmat_t* outs);
+int bem_set_aca_schedule(bem_ctx_t ctx, int precision, int64_t small_bound, int64_t huge_bound, int queue);
 /* HArithm::MVM: y (+)= H x on device pointers (complex double, cluster
    order); accumulate = 1 adds to y as the reference does. */
 int bem_hmvm(bem_ctx_t ctx, bem_hmat_t h, const void* x_dev, void* y_dev, int accumulate);
@@ -205,6 +223,17 @@ int bem_hmvm_multi(bem_ctx_t ctx, bem_hmat_t h, int64_t nrhs, const void* x_dev,
 int bem_gmres_multi(bem_ctx_t ctx, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, const void* b_dev, void* x_dev,
                     int x_is_zero, double tol, int64_t restart, int64_t maxit, int64_t max_steps, int64_t* iters,
                     double* rel_resid, int64_t* steps);
+
+/* bem_gmres_multi over several operators at once (the frequencies of a
+   batch): column q = o * nrhs + r of B / X (interleaved, nops * nrhs <= 8,
+   nrhs 1, 2 or 4) is right-hand side r of operator hs[o]; every Arnoldi
+   column applies each operator to its own columns inside the same replayed
+   graph.  hres: NULL, or the residual operator of each hs[o] (refinement
+   as bem_gmres_multi; max_steps > 0 with hres NULL: each operator's own
+   refinement form).  iters / rel_resid per column. */
+int bem_gmres_batch(bem_ctx_t ctx, int64_t nops, const bem_hmat_t* hs, const bem_hmat_t* hres, int64_t nrhs,
+                    const void* b_dev, void* x_dev, int x_is_zero, double tol, int64_t restart, int64_t maxit,
+                    int64_t max_steps, int64_t* iters, double* rel_resid, int64_t* steps);
 
 /* Leaf-block preconditioner of an assembled operator: LU with partial
    pivoting (FP64) of every diagonal leaf block, i.e. the leaf level of the
@@ -407,6 +436,13 @@ int bem_refine_triangles(int64_t n, const double* tri9, double edge_length, int6
    as by bem_solver_solve.  Outputs: pressure nf x nrhs x n complex (cluster
    order), iters nf x nrhs, stats nf x 16 (bem_solver_solve's layout, for the
    last right-hand side). */
+/* batch > 1: bem_solver_sweep processes the frequencies in chunks of
+   min(batch, 8 / nrhs) on the solver's one context: Phi and DPhi of a
+   chunk from one batched assembly each (bem_assemble_batch), the GMRES of
+   all its (frequency, right-hand side) pairs in one lockstep solve
+   (bem_gmres_batch), so no host lanes and no extra hardware queues; nrhs
+   1, 2 or 4, no preconditioner, no impedance plane.  0 or 1: the lanes. */
+int bem_solver_set_sweep_batch(bem_solver_t s, int batch);
 int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, double rho, int coupling,
                      double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      int64_t nrhs, const double* g, const double* planes5, int lanes, double* pressure,

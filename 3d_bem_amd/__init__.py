@@ -60,6 +60,7 @@ _SIGS = {
                             ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
     "bem_solver_set_refinement": (_int, [_p, _int, _i64]),
     "bem_solver_set_sweep_batch": (_int, [_p, _int]),
+    "bem_set_fuse_ops": (_int, [_p, _int]),
     "bem_gmres_batch": (_int, [_p, _i64, _p, _p, _i64, _p, _p, _int, _dbl, _i64, _i64, _i64, _p, _p,
                                ctypes.POINTER(_i64)]),
     "bem_set_mfma": (_int, [_p, _int]),
@@ -258,6 +259,10 @@ class Context:
         self._check(lib().bem_assemble_batch(self.h, int(kind), len(kcc), _ptr(kcc), float(tol), int(maxrank),
                                              int(precision), int(seed), ctypes.cast(outs, _p)))
         return [HMatrix(self, _p(h)) for h in outs]
+
+    def set_fuse_ops(self, on):
+        """bem_set_fuse_ops: products of several operators in one pass (default on)"""
+        self._check(lib().bem_set_fuse_ops(self.h, int(bool(on))))
 
     def gmres_batch(self, hs, B, tol, hres=None, restart=300, maxit=100000, max_steps=0):
         """lockstep GMRES over several operators (bem_gmres_batch): B (n,

@@ -877,23 +877,32 @@ __global__ __launch_bounds__(kWG) void k_compact(const int32_t* blist, const int
 // Block storage is complex FP64 (bem_c64) or FP32 (bem_c32); products
 // accumulate in FP64 and vectors are FP64 in both cases.
 
-// near-field tiles (K4): partial[tile][row] = sum_c F[row, c] x[c]
-template <class S>
-__global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tiles, const S* __restrict__ F,
-                                                   const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+// near-field tiles (K4): partial[tile][row] = sum_c F[row, c] x[c]; the
+// body takes x and the partial vectors with element stride XS (one column
+// of XS interleaved ones)
+template <class S, int XS>
+__device__ __forceinline__ void mvm_dense_tile(const Tile& T, const S* __restrict__ F, const bem_c64* __restrict__ x,
+                                               bem_c64* __restrict__ part)
 {
     __shared__ bem_c64 red[4][kTile];
-    const Tile T = tiles[blockIdx.x];
     const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
     bem_c64 acc = c64(0, 0);
     if (r < T.rows) {
         const S* f = F + T.dest + r;
-        for (int c = cg; c < T.cols; c += 4) acc = c64_fma(ld64(f + (int64_t)c * T.ld), x[T.col0 + c], acc);
+        for (int c = cg; c < T.cols; c += 4)
+            acc = c64_fma(ld64(f + (int64_t)c * T.ld), x[(int64_t)(T.col0 + c) * XS], acc);
     }
     red[cg][r] = acc;
     __syncthreads();
     if (cg == 0 && r < T.rows)
-        part[T.part + r] = c64_add(c64_add(c64_add(red[0][r], red[1][r]), red[2][r]), red[3][r]);
+        part[(T.part + r) * XS] = c64_add(c64_add(c64_add(red[0][r], red[1][r]), red[2][r]), red[3][r]);
+}
+
+template <class S>
+__global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tiles, const S* __restrict__ F,
+                                                   const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+{
+    mvm_dense_tile<S, 1>(tiles[blockIdx.x], F, x, part);
 }
 
 // low-rank blocks, phase 1 (K5a): t_b[l] = alpha_l (V_b[l, :] . x_J), one
@@ -904,22 +913,21 @@ __global__ __launch_bounds__(kWG) void k_mvm_dense(const Tile* __restrict__ tile
 // flight per lane and a 16-lane tree (a sixth of the shuffles).
 constexpr int kLrRows = 4;
 static_assert(kLrRows * 16 == 64, "FP32 k_lr_t: 16 lanes per row");
-template <class S>
-__global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks, int64_t ntask,
-                                              const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
-                                              const int32_t* __restrict__ rank, const int64_t* __restrict__ voff,
-                                              const int64_t* __restrict__ aoff, const int64_t* __restrict__ toff,
-                                              const S* __restrict__ V, const S* __restrict__ A,
-                                              const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
+// (body with x of element stride XS)
+template <class S, int XS>
+__device__ __forceinline__ void lr_t_task(int64_t task, const int32_t* __restrict__ tasks,
+                                          const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
+                                          const int32_t* __restrict__ rank, const int64_t* __restrict__ voff,
+                                          const int64_t* __restrict__ aoff, const int64_t* __restrict__ toff,
+                                          const S* __restrict__ V, const S* __restrict__ A,
+                                          const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
 {
-    const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
-    if (task >= ntask) return;
     const int lane = threadIdx.x & 63;
     const int b = tasks[2 * task], l0 = tasks[2 * task + 1];
     const int64_t n = cols[b];
     const int nl = min(kLrRows, rank[b] - l0);
     const S* v = V + voff[b] + (int64_t)l0 * n;
-    const bem_c64* xs = x + col0[b];
+    const bem_c64* xs = x + col0[b] * XS;
     if constexpr (sizeof(S) == sizeof(bem_c32)) {
         const int u = lane >> 4, q = lane & 15;   // row l0 + u, columns q, q + 16, ...
         bem_c64 p = c64(0, 0);
@@ -932,12 +940,12 @@ __global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks,
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     vv[e] = vr[j + 16 * e];
-                    xv[e] = xs[j + 16 * e];
+                    xv[e] = xs[(j + 16 * e) * XS];
                 }
 #pragma unroll
                 for (int e = 0; e < 4; e++) p = c64_fma(ld64(&vv[e]), xv[e], p);
             }
-            for (; j < n; j += 16) p = c64_fma(ld64(vr + j), xs[j], p);
+            for (; j < n; j += 16) p = c64_fma(ld64(vr + j), xs[j * XS], p);
         }
         for (int s = 8; s > 0; s >>= 1) {
             p.re = p.re + __shfl_xor(p.re, s, 16);
@@ -950,7 +958,7 @@ __global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks,
 #pragma unroll
     for (int u = 0; u < kLrRows; u++) p[u] = c64(0, 0);
     for (int64_t j = lane; j < n; j += 64) {
-        const bem_c64 xv = xs[j];
+        const bem_c64 xv = xs[j * XS];
 #pragma unroll
         for (int u = 0; u < kLrRows; u++)
             if (u < nl) p[u] = c64_fma(ld64(v + (int64_t)u * n + j), xv, p[u]);
@@ -964,17 +972,30 @@ __global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks,
     }
 }
 
+template <class S>
+__global__ __launch_bounds__(kWG) void k_lr_t(const int32_t* __restrict__ tasks, int64_t ntask,
+                                              const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
+                                              const int32_t* __restrict__ rank, const int64_t* __restrict__ voff,
+                                              const int64_t* __restrict__ aoff, const int64_t* __restrict__ toff,
+                                              const S* __restrict__ V, const S* __restrict__ A,
+                                              const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
+{
+    const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+    if (task >= ntask) return;
+    lr_t_task<S, 1>(task, tasks, col0, cols, rank, voff, aoff, toff, V, A, x, t);
+}
+
 // low-rank blocks, phase 2 (K5b): partial_b[i] = sum_l U_b[i, l] t_b[l] over a
 // chunk of kWG rows of one block
-template <class S>
-__global__ __launch_bounds__(kWG) void k_lr_u(const int32_t* __restrict__ tasks, const int64_t* __restrict__ rows,
-                                              const int32_t* __restrict__ rank, const int64_t* __restrict__ uoff,
-                                              const int64_t* __restrict__ toff, const int64_t* __restrict__ poff,
-                                              const S* __restrict__ U, const bem_c64* __restrict__ t,
-                                              bem_c64* __restrict__ part)
+// (body: task `task`, t staged in st, partial vectors of element stride XS)
+template <class S, int XS>
+__device__ __forceinline__ void lr_u_task(int64_t task, const int32_t* __restrict__ tasks,
+                                          const int64_t* __restrict__ rows, const int32_t* __restrict__ rank,
+                                          const int64_t* __restrict__ uoff, const int64_t* __restrict__ toff,
+                                          const int64_t* __restrict__ poff, const S* __restrict__ U,
+                                          const bem_c64* __restrict__ t, bem_c64* st, bem_c64* __restrict__ part)
 {
-    extern __shared__ bem_c64 st[];
-    const int b = tasks[2 * blockIdx.x], chunk = tasks[2 * blockIdx.x + 1];
+    const int b = tasks[2 * task], chunk = tasks[2 * task + 1];
     const int64_t m = rows[b];
     const int k = rank[b];
     for (int l = threadIdx.x; l < k; l += kWG) st[l] = t[toff[b] + l];
@@ -984,7 +1005,18 @@ __global__ __launch_bounds__(kWG) void k_lr_u(const int32_t* __restrict__ tasks,
     const S* u = U + uoff[b] + i;
     bem_c64 acc = c64(0, 0);
     for (int l = 0; l < k; l++) acc = c64_fma(ld64(u + (int64_t)l * m), st[l], acc);
-    part[poff[b] + i] = acc;
+    part[(poff[b] + i) * XS] = acc;
+}
+
+template <class S>
+__global__ __launch_bounds__(kWG) void k_lr_u(const int32_t* __restrict__ tasks, const int64_t* __restrict__ rows,
+                                              const int32_t* __restrict__ rank, const int64_t* __restrict__ uoff,
+                                              const int64_t* __restrict__ toff, const int64_t* __restrict__ poff,
+                                              const S* __restrict__ U, const bem_c64* __restrict__ t,
+                                              bem_c64* __restrict__ part)
+{
+    extern __shared__ bem_c64 st[];
+    lr_u_task<S, 1>(blockIdx.x, tasks, rows, rank, uoff, toff, poff, U, t, st, part);
 }
 
 // y_i (+)= sum of the partial vectors covering row i in partition order:
@@ -1697,31 +1729,38 @@ __device__ __forceinline__ void mfma_cstep(bem_c64 a, bem_c64 b, v4d& cr, v4d& c
     ci = mfma_f64(a.im, b.re, ci);
 }
 
-// near-field tiles, Q right-hand sides on the VALU
-template <class S, int Q>
-__global__ __launch_bounds__(kWG) void k_mvm_dense_q(const Tile* __restrict__ tiles, const S* __restrict__ F,
-                                                     const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+// near-field tiles, Q right-hand sides on the VALU (body: R columns of
+// vectors with XS interleaved ones)
+template <class S, int R, int XS>
+__device__ __forceinline__ void mvm_dense_tile_q(const Tile& T, const S* __restrict__ F,
+                                                 const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
 {
-    __shared__ bem_c64 red[4][kTile * Q];
-    const Tile T = tiles[blockIdx.x];
+    __shared__ bem_c64 red[4][kTile * R];
     const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
-    bem_c64 acc[Q];
+    bem_c64 acc[R];
 #pragma unroll
-    for (int q = 0; q < Q; q++) acc[q] = c64(0, 0);
+    for (int q = 0; q < R; q++) acc[q] = c64(0, 0);
     if (r < T.rows) {
         const S* f = F + T.dest + r;
         for (int c = cg; c < T.cols; c += 4) {
             const bem_c64 fv = ld64(f + (int64_t)c * T.ld);
-            const bem_c64* xc = x + (int64_t)(T.col0 + c) * Q;
+            const bem_c64* xc = x + (int64_t)(T.col0 + c) * XS;
 #pragma unroll
-            for (int q = 0; q < Q; q++) acc[q] = c64_fma(fv, xc[q], acc[q]);
+            for (int q = 0; q < R; q++) acc[q] = c64_fma(fv, xc[q], acc[q]);
         }
     }
 #pragma unroll
-    for (int q = 0; q < Q; q++) red[cg][r * Q + q] = acc[q];
+    for (int q = 0; q < R; q++) red[cg][r * R + q] = acc[q];
     __syncthreads();
-    for (int o = tid; o < T.rows * Q; o += kWG)
-        part[T.part * Q + o] = c64_add(c64_add(c64_add(red[0][o], red[1][o]), red[2][o]), red[3][o]);
+    for (int o = tid; o < T.rows * R; o += kWG)
+        part[(T.part + o / R) * XS + o % R] = c64_add(c64_add(c64_add(red[0][o], red[1][o]), red[2][o]), red[3][o]);
+}
+
+template <class S, int Q>
+__global__ __launch_bounds__(kWG) void k_mvm_dense_q(const Tile* __restrict__ tiles, const S* __restrict__ F,
+                                                     const bem_c64* __restrict__ x, bem_c64* __restrict__ part)
+{
+    mvm_dense_tile_q<S, Q, Q>(tiles[blockIdx.x], F, x, part);
 }
 
 // near-field tiles on the matrix cores: wavefront w takes rows 16 w ..
@@ -1754,8 +1793,58 @@ __global__ __launch_bounds__(kWG) void k_mvm_dense_mfma(const Tile* __restrict__
 }
 
 // low-rank phase 1, Q right-hand sides: t_b[l][q] = alpha_l (V_b[l, :] . x_J[:, q]),
-// one wavefront per (block, l0) task of kLrRows rows of V (rows l0 .. l0 + R - 1
-// by this kernel, R = kLrRows / Q rounded up to one)
+// one wavefront per (block, l0) task of kLrRows rows of V (rows l0 .. l0 + NR - 1
+// per pass, NR = kLrRows / Q rounded up to one).  Body: R columns of x with
+// XS interleaved ones, t with R.
+template <class S, int R, int XS>
+__device__ __forceinline__ void lr_t_task_q(int64_t task, const int32_t* __restrict__ tasks,
+                                            const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
+                                            const int32_t* __restrict__ rank, const int64_t* __restrict__ voff,
+                                            const int64_t* __restrict__ aoff, const int64_t* __restrict__ toff,
+                                            const S* __restrict__ V, const S* __restrict__ A,
+                                            const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
+{
+    constexpr int NR = kLrRows / R > 0 ? kLrRows / R : 1;
+    const int lane = threadIdx.x & 63;
+    const int b = tasks[2 * task], lt = tasks[2 * task + 1];
+    const int64_t n = cols[b];
+    const bem_c64* xs = x + col0[b] * XS;
+    for (int l0 = lt; l0 < lt + kLrRows && l0 < rank[b]; l0 += NR) {
+        const int nl = min(NR, rank[b] - l0);
+        const S* v = V + voff[b] + (int64_t)l0 * n;
+        bem_c64 p[NR][R];
+#pragma unroll
+        for (int u = 0; u < NR; u++)
+#pragma unroll
+            for (int q = 0; q < R; q++) p[u][q] = c64(0, 0);
+        for (int64_t j = lane; j < n; j += 64) {
+            bem_c64 xv[R];
+#pragma unroll
+            for (int q = 0; q < R; q++) xv[q] = xs[j * XS + q];
+#pragma unroll
+            for (int u = 0; u < NR; u++)
+                if (u < nl) {
+                    const bem_c64 vv = ld64(v + (int64_t)u * n + j);
+#pragma unroll
+                    for (int q = 0; q < R; q++) p[u][q] = c64_fma(vv, xv[q], p[u][q]);
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < NR; u++)
+#pragma unroll
+            for (int q = 0; q < R; q++) p[u][q] = wave_sum(p[u][q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < NR; u++)
+                if (u < nl) {
+                    const bem_c64 al = ld64(A + aoff[b] + l0 + u);
+#pragma unroll
+                    for (int q = 0; q < R; q++) t[(toff[b] + l0 + u) * R + q] = c64_mul(al, p[u][q]);
+                }
+        }
+    }
+}
+
 template <class S, int Q>
 __global__ __launch_bounds__(kWG) void k_lr_t_q(const int32_t* __restrict__ tasks, int64_t ntask,
                                                 const int64_t* __restrict__ col0, const int64_t* __restrict__ cols,
@@ -1764,51 +1853,40 @@ __global__ __launch_bounds__(kWG) void k_lr_t_q(const int32_t* __restrict__ task
                                                 const S* __restrict__ V, const S* __restrict__ A,
                                                 const bem_c64* __restrict__ x, bem_c64* __restrict__ t)
 {
-    constexpr int R = kLrRows / Q > 0 ? kLrRows / Q : 1;
     const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
     if (task >= ntask) return;
-    const int lane = threadIdx.x & 63;
-    const int b = tasks[2 * task], lt = tasks[2 * task + 1];
-    const int64_t n = cols[b];
-    const bem_c64* xs = x + col0[b] * Q;
-    for (int l0 = lt; l0 < lt + kLrRows && l0 < rank[b]; l0 += R) {
-        const int nl = min(R, rank[b] - l0);
-        const S* v = V + voff[b] + (int64_t)l0 * n;
-        bem_c64 p[R][Q];
-#pragma unroll
-        for (int u = 0; u < R; u++)
-#pragma unroll
-            for (int q = 0; q < Q; q++) p[u][q] = c64(0, 0);
-        for (int64_t j = lane; j < n; j += 64) {
-            bem_c64 xv[Q];
-#pragma unroll
-            for (int q = 0; q < Q; q++) xv[q] = xs[j * Q + q];
-#pragma unroll
-            for (int u = 0; u < R; u++)
-                if (u < nl) {
-                    const bem_c64 vv = ld64(v + (int64_t)u * n + j);
-#pragma unroll
-                    for (int q = 0; q < Q; q++) p[u][q] = c64_fma(vv, xv[q], p[u][q]);
-                }
-        }
-#pragma unroll
-        for (int u = 0; u < R; u++)
-#pragma unroll
-            for (int q = 0; q < Q; q++) p[u][q] = wave_sum(p[u][q]);
-        if (lane == 0) {
-#pragma unroll
-            for (int u = 0; u < R; u++)
-                if (u < nl) {
-                    const bem_c64 al = ld64(A + aoff[b] + l0 + u);
-#pragma unroll
-                    for (int q = 0; q < Q; q++) t[(toff[b] + l0 + u) * Q + q] = c64_mul(al, p[u][q]);
-                }
-        }
-    }
+    lr_t_task_q<S, Q, Q>(task, tasks, col0, cols, rank, voff, aoff, toff, V, A, x, t);
 }
 
 // low-rank phase 2, Q right-hand sides on the VALU: partial_b[i][q] =
-// sum_l U_b[i, l] t_b[l][q] over a chunk of kWG rows
+// sum_l U_b[i, l] t_b[l][q] over a chunk of kWG rows (body: t of R columns,
+// partial vectors with XS interleaved ones)
+template <class S, int R, int XS>
+__device__ __forceinline__ void lr_u_task_q(int64_t task, const int32_t* __restrict__ tasks,
+                                            const int64_t* __restrict__ rows, const int32_t* __restrict__ rank,
+                                            const int64_t* __restrict__ uoff, const int64_t* __restrict__ toff,
+                                            const int64_t* __restrict__ poff, const S* __restrict__ U,
+                                            const bem_c64* __restrict__ t, bem_c64* __restrict__ part)
+{
+    const int b = tasks[2 * task], chunk = tasks[2 * task + 1];
+    const int64_t m = rows[b];
+    const int k = rank[b];
+    const int64_t i = (int64_t)chunk * kWG + threadIdx.x;
+    if (i >= m) return;
+    const S* u = U + uoff[b] + i;
+    const bem_c64* tb = t + toff[b] * R;   // the same k R values for every thread (cached)
+    bem_c64 acc[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) acc[q] = c64(0, 0);
+    for (int l = 0; l < k; l++) {
+        const bem_c64 uv = ld64(u + (int64_t)l * m);
+#pragma unroll
+        for (int q = 0; q < R; q++) acc[q] = c64_fma(uv, tb[l * R + q], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) part[(poff[b] + i) * XS + q] = acc[q];
+}
+
 template <class S, int Q>
 __global__ __launch_bounds__(kWG) void k_lr_u_q(const int32_t* __restrict__ tasks, const int64_t* __restrict__ rows,
                                                 const int32_t* __restrict__ rank, const int64_t* __restrict__ uoff,
@@ -1816,23 +1894,7 @@ __global__ __launch_bounds__(kWG) void k_lr_u_q(const int32_t* __restrict__ task
                                                 const S* __restrict__ U, const bem_c64* __restrict__ t,
                                                 bem_c64* __restrict__ part)
 {
-    const int b = tasks[2 * blockIdx.x], chunk = tasks[2 * blockIdx.x + 1];
-    const int64_t m = rows[b];
-    const int k = rank[b];
-    const int64_t i = (int64_t)chunk * kWG + threadIdx.x;
-    if (i >= m) return;
-    const S* u = U + uoff[b] + i;
-    const bem_c64* tb = t + toff[b] * Q;   // the same k Q values for every thread (cached)
-    bem_c64 acc[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) acc[q] = c64(0, 0);
-    for (int l = 0; l < k; l++) {
-        const bem_c64 uv = ld64(u + (int64_t)l * m);
-#pragma unroll
-        for (int q = 0; q < Q; q++) acc[q] = c64_fma(uv, tb[l * Q + q], acc[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; q++) part[(poff[b] + i) * Q + q] = acc[q];
+    lr_u_task_q<S, Q, Q>(blockIdx.x, tasks, rows, rank, uoff, toff, poff, U, t, part);
 }
 
 // low-rank phase 2 on the matrix cores: the chunk's kWG rows as 16 row tiles,
@@ -1899,6 +1961,129 @@ __global__ void k_mvm_gather_q(int64_t n, const int64_t* __restrict__ gptr, cons
     if (i < n && lq == 0) {
 #pragma unroll
         for (int q = 0; q < Q; q++) y[i * Q + q] = accumulate ? c64_add(y[i * Q + q], acc[q]) : acc[q];
+    }
+}
+
+// ---- products of several operators in one pass (the frequencies of a
+// batch): operator g = blockIdx.y owns columns g R .. g R + R - 1 of the Q
+// interleaved vectors.  The operators share the context's partition (tiles,
+// block extents, partial-vector offsets, the row-chunk tasks of phase 2);
+// each has its own stored values, ranks and phase-1 tasks.  The input is
+// first regrouped by operator (xg: operator g's R columns, R-interleaved, at
+// g n R) and the partial vectors are kept the same way (operator g's at
+// g P R, P partials per column), so that every operator's kernels read and
+// write contiguous vectors exactly as its single-operator kernels do; with
+// R = 1 the arithmetic is theirs.  A padding group (pad[g]: columns beyond
+// the solve's right-hand sides) writes zero partial vectors and reads
+// nothing.
+constexpr int kMaxOps = 8;
+struct OpsArg {
+    int32_t pad[kMaxOps];
+    const void* F[kMaxOps];                  // near-field storage read
+    const int32_t* tt[kMaxOps];              // phase-1 (block, l0) tasks
+    const int32_t* ut[kMaxOps];              // phase-2 (block, row chunk) tasks
+    int64_t ntt[kMaxOps], nut[kMaxOps];
+    const int32_t* rank[kMaxOps];
+    const int64_t *voff[kMaxOps], *aoff[kMaxOps], *toff[kMaxOps], *uoff[kMaxOps];
+    const void *U[kMaxOps], *V[kMaxOps], *A[kMaxOps];
+    bem_c64* t[kMaxOps];                     // R columns per operator
+    int64_t n, P;                            // rows; partial vector entries per column
+};
+
+// xg[(g n + i) R + r] = x[i Q + g R + r]
+template <int R, int Q>
+__global__ void k_group_cols(int64_t n, const bem_c64* __restrict__ x, bem_c64* __restrict__ xg)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * Q) return;
+    const int64_t i = e / Q;
+    const int q = (int)(e - i * Q), g = q / R, r = q - g * R;
+    xg[((int64_t)g * n + i) * R + r] = x[e];
+}
+
+template <class S, int R>
+__global__ __launch_bounds__(kWG) void k_mvm_dense_ops(const Tile* __restrict__ tiles, OpsArg O,
+                                                       const bem_c64* __restrict__ xg, bem_c64* __restrict__ part)
+{
+    const int g = blockIdx.y;
+    bem_c64* pg = part + (int64_t)g * O.P * R;
+    if (O.pad[g]) {
+        const Tile T = tiles[blockIdx.x];
+        for (int o = threadIdx.x; o < T.rows * R; o += kWG) pg[T.part * R + o] = c64(0, 0);
+        return;
+    }
+    const S* F = (const S*)O.F[g];
+    const bem_c64* x = xg + (int64_t)g * O.n * R;
+    if constexpr (R == 1) mvm_dense_tile<S, 1>(tiles[blockIdx.x], F, x, pg);
+    else mvm_dense_tile_q<S, R, R>(tiles[blockIdx.x], F, x, pg);
+}
+
+template <class S, int R>
+__global__ __launch_bounds__(kWG) void k_lr_t_ops(OpsArg O, const int64_t* __restrict__ col0,
+                                                  const int64_t* __restrict__ cols, const bem_c64* __restrict__ xg)
+{
+    const int g = blockIdx.y;
+    const int64_t task = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+    if (task >= O.ntt[g]) return;
+    const bem_c64* x = xg + (int64_t)g * O.n * R;
+    if constexpr (R == 1)
+        lr_t_task<S, 1>(task, O.tt[g], col0, cols, O.rank[g], O.voff[g], O.aoff[g], O.toff[g], (const S*)O.V[g],
+                        (const S*)O.A[g], x, O.t[g]);
+    else
+        lr_t_task_q<S, R, R>(task, O.tt[g], col0, cols, O.rank[g], O.voff[g], O.aoff[g], O.toff[g],
+                             (const S*)O.V[g], (const S*)O.A[g], x, O.t[g]);
+}
+
+template <class S, int R>
+__global__ __launch_bounds__(kWG) void k_lr_u_ops(OpsArg O, const int64_t* __restrict__ rows,
+                                                  const int64_t* __restrict__ poff, bem_c64* __restrict__ part)
+{
+    extern __shared__ bem_c64 st[];
+    const int g = blockIdx.y;
+    if ((int64_t)blockIdx.x >= O.nut[g]) return;   // uniform per workgroup
+    bem_c64* pg = part + (int64_t)g * O.P * R;
+    if (O.pad[g]) {
+        const int b = O.ut[g][2 * blockIdx.x], chunk = O.ut[g][2 * blockIdx.x + 1];
+        const int64_t i = (int64_t)chunk * kWG + threadIdx.x;
+        if (i < rows[b])
+            for (int q = 0; q < R; q++) pg[(poff[b] + i) * R + q] = c64(0, 0);
+        return;
+    }
+    if constexpr (R == 1)
+        lr_u_task<S, 1>(blockIdx.x, O.ut[g], rows, O.rank[g], O.uoff[g], O.toff[g], poff, (const S*)O.U[g], O.t[g],
+                        st, pg);
+    else
+        lr_u_task_q<S, R, R>(blockIdx.x, O.ut[g], rows, O.rank[g], O.uoff[g], O.toff[g], poff, (const S*)O.U[g],
+                             O.t[g], pg);
+}
+
+// y[i Q + g R + r] = sum of operator g's partial vectors covering row i (in
+// the order and with the shuffle tree of k_mvm_gather / k_mvm_gather_q)
+template <int R, int Q>
+__global__ void k_mvm_gather_ops(int64_t n, int64_t P, const int64_t* __restrict__ gptr,
+                                 const int64_t* __restrict__ gidx, const bem_c64* __restrict__ part, bem_c64* y)
+{
+    const int64_t gl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = gl >> 4;
+    const int lq = (int)(gl & 15);
+    bem_c64 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc[q] = c64(0, 0);
+    if (i < n)
+        for (int64_t p = gptr[i] + lq; p < gptr[i + 1]; p += 16) {
+            const int64_t o = gidx[p] * R;
+#pragma unroll
+            for (int q = 0; q < Q; q++) acc[q] = c64_add(acc[q], part[(int64_t)(q / R) * P * R + o + q % R]);
+        }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        for (int s = 8; s > 0; s >>= 1) {
+            acc[q].re = acc[q].re + __shfl_down(acc[q].re, s, 16);
+            acc[q].im = acc[q].im + __shfl_down(acc[q].im, s, 16);
+        }
+    if (i < n && lq == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) y[i * Q + q] = acc[q];
     }
 }
 
@@ -2236,9 +2421,22 @@ struct bem_ctx {
     void* st_dense64 = nullptr;          // and their FP64 form (FP32 operators with near64)
     int st_prec = -1;
     bool st_near64 = false;
+    // the same for a batched Phi assembly: the DPhi near field and singular
+    // table of each of its frequencies, kept for the next batched DPhi
+    // assembly of the same state and frequency list
+    struct KeptBatch {
+        std::vector<double> kcc;
+        std::vector<void*> dense, dense64, sing;
+        uint64_t gen = 0;
+        int prec = -1;
+        bool near64 = false;
+    } kb;
     // FP32 assemblies form the near field in FP64 and keep it beside the FP32
     // copy (bem_set_near_fp64): the refinement residual's operator
     bool near64 = false;
+    // products of several operators (the frequencies of a batched solve) in
+    // one pass per kernel, not one operator after another (bem_set_fuse_ops)
+    bool fuse_ops = true;
     uint64_t st_gen = 0;
     const bem_c64* sing_cur = nullptr;   // table the running assembly's ACA reads
     int64_t nnz = 0;
@@ -2281,6 +2479,7 @@ struct bem_ctx {
     // matrix cores for Q >= 4 products, graph-captured Arnoldi iterations
     bem_c64* d_partq = nullptr;
     int partq_cap = 0;
+    bem_c64* d_xq = nullptr;             // n x partq_cap: vectors regrouped by operator (fused products)
     double* gm_resq = nullptr;
     bool use_mfma = true;
     bool use_graphs = true;
@@ -2563,6 +2762,12 @@ DevFreq32 dev_freq32(const bem_ctx* c)
 // release a kept DPhi near field
 void drop_kept(bem_ctx* c)
 {
+    for (auto* v : {&c->kb.dense, &c->kb.dense64, &c->kb.sing}) {
+        for (void* q : *v)
+            if (q) pfree(c, q);
+        v->clear();
+    }
+    c->kb.kcc.clear();
     if (c->st_dense) {
         pfree(c, c->st_dense);
         c->st_dense = nullptr;
@@ -2695,7 +2900,8 @@ int bem_destroy(bem_ctx_t c)
     free_list(c, c->allocs_part);
     free_list(c, c->allocs_piv);
     for (void* p : {(void*)c->d_alpha, (void*)c->d_beta, (void*)c->d_f, (void*)c->d_subst, (void*)c->d_cphi,
-                    (void*)c->d_cd, (void*)c->d_cphi32, (void*)c->d_cd32, (void*)c->d_gm, (void*)c->d_partq})
+                    (void*)c->d_cd, (void*)c->d_cphi32, (void*)c->d_cd32, (void*)c->d_gm, (void*)c->d_partq,
+                    (void*)c->d_xq})
         pfree(c, p);
     for (auto& kv : c->live) (void)hipFree(kv.first);
     c->live.clear();
@@ -3024,7 +3230,9 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
     if ((rc = dupload(c, &c->d_gidx, gidx.data(), gidx.size(), ap))) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     pfree(c, c->d_partq);   // sized for the previous partition
+    pfree(c, c->d_xq);
     c->d_partq = nullptr;
+    c->d_xq = nullptr;
     c->partq_cap = 0;
     c->have_part = true;
     bump(c);
@@ -3757,60 +3965,100 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
         return code;
     };
     int rc;
-    drop_kept(c);   // a near field kept for a paired assembly does not belong to these
     const int64_t nitems = c->nnz + c->n;
     const bool n64 = f32 && c->near64;   // FP32 operators with their FP64 near field (bem_set_near_fp64)
+    // near-field pairing: a batched DPhi assembly takes the near fields the
+    // batched Phi assembly of the same state and frequencies formed; a
+    // batched Phi assembly forms both
+    const std::vector<double> kccv(kcc3, kcc3 + 3 * nf);
+    const bool adopt = kind == BEM_KIND_DPHI && c->kb.gen == c->gen && c->kb.prec == precision &&
+                       c->kb.near64 == n64 && c->kb.kcc == kccv;
+    const bool pairing = c->pair_near && kind == BEM_KIND_PHI;
+    std::vector<void*> pd((size_t)nf, nullptr), pd64((size_t)nf, nullptr), ps((size_t)nf, nullptr);
+    auto fail_pair = [&](int code) {
+        for (auto* v : {&pd, &pd64, &ps})
+            for (void* q : *v)
+                if (q) pfree(c, q);
+        return fail(code);
+    };
     for (int64_t f = 0; f < nf; f++) {
         H[f] = new bem_hmat();
         H[f]->ctx = c;
         H[f]->kind = kind;
         H[f]->prec = precision;
         H[f]->esz = esz;
-        if ((rc = dalloc_bytes(c, &H[f]->dense, (size_t)c->dense_total * esz))) return fail(rc);
-        if (n64 && (rc = dalloc_bytes(c, &H[f]->dense64, (size_t)c->dense_total * sizeof(bem_c64)))) return fail(rc);
-        if ((rc = dalloc(c, &sing[f], (size_t)nitems, &own))) return fail(rc);
+        if (adopt) {   // the kept blocks become the operators' own; the tables are freed with the batch
+            H[f]->dense = c->kb.dense[f];
+            H[f]->dense64 = c->kb.dense64[f];
+            sing[f] = (bem_c64*)c->kb.sing[f];
+            own.push_back(sing[f]);
+            c->kb.dense[f] = c->kb.dense64[f] = c->kb.sing[f] = nullptr;
+        }
+    }
+    drop_kept(c);   // what is kept (or left of it) does not belong to these
+    for (int64_t f = 0; f < nf && !adopt; f++) {
+        if ((rc = dalloc_bytes(c, &H[f]->dense, (size_t)c->dense_total * esz))) return fail_pair(rc);
+        if (n64 && (rc = dalloc_bytes(c, &H[f]->dense64, (size_t)c->dense_total * sizeof(bem_c64))))
+            return fail_pair(rc);
+        if ((rc = dalloc(c, &sing[f], (size_t)nitems, &own))) return fail_pair(rc);
+        if (pairing) {
+            bem_c64* t = nullptr;
+            if ((rc = dalloc_bytes(c, &pd[f], (size_t)c->dense_total * esz))) return fail_pair(rc);
+            if (n64 && (rc = dalloc_bytes(c, &pd64[f], (size_t)c->dense_total * sizeof(bem_c64))))
+                return fail_pair(rc);
+            if ((rc = dalloc(c, &t, (size_t)nitems, nullptr))) return fail_pair(rc);
+            ps[f] = t;
+        }
     }
     hipEvent_t ev0, ev1, ev2;
     HIPC(c, hipEventCreate(&ev0)); HIPC(c, hipEventCreate(&ev1)); HIPC(c, hipEventCreate(&ev2));
     HIPC(c, hipEventRecord(ev0, c->stream));
-    for (int64_t f = 0; f < nf; f++) {   // self and near-singular entries per frequency
+    for (int64_t f = 0; f < nf && !adopt; f++) {   // self and near-singular entries per frequency
         use(f);
         Timer t(c, 1);
         hipLaunchKernelGGL(k_singular, dim3(nblk(nitems, kWG)), dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), kind,
-                           c->d_sing_order, nitems, c->nnz, sing[f], nullptr);
-        if ((rc = launch_check(c, "k_singular"))) return fail(rc);
+                           c->d_sing_order, nitems, c->nnz, sing[f], (bem_c64*)ps[f]);
+        if ((rc = launch_check(c, "k_singular"))) return fail_pair(rc);
     }
     // the dense tiles of every frequency, on the main stream while the large
     // ACA classes run on the side streams
     auto dense_work = [&]() -> int {
         int rc2;
-        for (int64_t f = 0; f < nf; f++) {
+        for (int64_t f = 0; f < nf && !adopt; f++) {
             use(f);
             bem_c64* d64 = (bem_c64*)(n64 ? H[f]->dense64 : H[f]->dense);   // the FP64 near field, if any
+            bem_c64* p64 = (bem_c64*)(n64 ? pd64[f] : pd[f]);                // and the paired DPhi one
             if (!c->tiles.empty()) {
                 Timer t(c, 0);
                 const dim3 g((unsigned)c->tiles.size());
                 if (f32 && !n64)
                     hipLaunchKernelGGL(k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq32(c), c->d_subst,
-                                       kind, c->d_tiles, (bem_c32*)H[f]->dense, (bem_c32*)nullptr);
+                                       kind, c->d_tiles, (bem_c32*)H[f]->dense, (bem_c32*)pd[f]);
                 else
                     hipLaunchKernelGGL(k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), c->d_tiles,
-                                       kind == BEM_KIND_PHI ? d64 : nullptr, kind == BEM_KIND_DPHI ? d64 : nullptr);
+                                       kind == BEM_KIND_PHI ? d64 : nullptr, kind == BEM_KIND_DPHI ? d64 : p64);
                 if ((rc2 = launch_check(c, "k_dense"))) return rc2;
             }
             if (!c->fixes.empty()) {
                 const dim3 g(nblk((int64_t)c->fixes.size(), kWG));
-                if (f32 && !n64)
-                    hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
-                                       (int64_t)c->fixes.size(), c->nnz, sing[f], (bem_c32*)H[f]->dense);
-                else
-                    hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
-                                       (int64_t)c->fixes.size(), c->nnz, sing[f], d64);
-                if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
+                for (int q = 0; q < (pairing ? 2 : 1); q++) {
+                    const bem_c64* tab = q ? (const bem_c64*)ps[f] : sing[f];
+                    if (f32 && !n64)
+                        hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                           (int64_t)c->fixes.size(), c->nnz, tab, (bem_c32*)(q ? pd[f] : H[f]->dense));
+                    else
+                        hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes,
+                                           (int64_t)c->fixes.size(), c->nnz, tab, q ? p64 : d64);
+                    if ((rc2 = launch_check(c, "k_scatter"))) return rc2;
+                }
             }
             if (n64 && c->dense_total) {
-                hipLaunchKernelGGL(k_round32, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream, c->dense_total,
-                                   (const bem_c64*)d64, (bem_c32*)H[f]->dense);
+                const dim3 g(nblk(c->dense_total, 256));
+                hipLaunchKernelGGL(k_round32, g, dim3(256), 0, c->stream, c->dense_total, (const bem_c64*)d64,
+                                   (bem_c32*)H[f]->dense);
+                if (pairing)
+                    hipLaunchKernelGGL(k_round32, g, dim3(256), 0, c->stream, c->dense_total, (const bem_c64*)p64,
+                                       (bem_c32*)pd[f]);
                 if ((rc2 = launch_check(c, "k_round32"))) return rc2;
             }
         }
@@ -3819,7 +4067,7 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
     int64_t* d_pivoff = nullptr;
     int32_t* d_piv = nullptr;
     if (kind == BEM_KIND_DPHI) {
-        if ((rc = ensure_pivot_lists(c, seed))) return fail(rc);
+        if ((rc = ensure_pivot_lists(c, seed))) return fail_pair(rc);
         d_pivoff = c->d_pivoff;
         d_piv = c->d_piv;
     }
@@ -3837,13 +4085,13 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
     std::vector<std::vector<int64_t>> evs;
     if ((rc = run_aca_batch(c, kind, precision, tol, maxrank, seed, jobs, c->adm_list, cap, d_pivoff, d_piv, dense_work,
                             S1, rank, flag, evs)))
-        return fail(rc);
+        return fail_pair(rc);
     HIPC(c, hipEventRecord(ev1, c->stream));
     for (int64_t f = 0; f < nf; f++) {
         std::vector<int32_t> over, keep;
         for (int32_t b : c->adm_list) {
             if (flag[f][b] == 2)
-                return fail(set_err(c, BEM_ERR_NAN, "bem_assemble_batch: non-finite value in ACA block " +
+                return fail_pair(set_err(c, BEM_ERR_NAN, "bem_assemble_batch: non-finite value in ACA block " +
                                                         std::to_string(b)));
             (flag[f][b] == 1 ? over : keep).push_back(b);
         }
@@ -3857,18 +4105,18 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
             rc = run_aca(c, kind, precision, tol, maxrank, seed, over, cap2, d_pivoff, d_piv, [] { return BEM_OK; }, S2,
                          rank2, flag2, evs2);
             c->sing_cur = nullptr;
-            if (rc) return fail(rc);
+            if (rc) return fail_pair(rc);
             for (int32_t b : over) {
                 rank[f][b] = rank2[b];
                 evs[f][b] = evs2[b];
                 if (flag2[b])
-                    return fail(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
+                    return fail_pair(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
                                         "bem_assemble_batch: ACA block " + std::to_string(b) +
                                             (flag2[b] == 2 ? " non-finite" : " needs rank > 1024")));
             }
             c->vp.give(rank2); c->vp.give(flag2); c->vp.give(evs2);
         }
-        if ((rc = finish_hmat(c, H[f], f32, S1[f], S2, rank[f], evs[f], over, keep, nullptr))) return fail(rc);
+        if ((rc = finish_hmat(c, H[f], f32, S1[f], S2, rank[f], evs[f], over, keep, nullptr))) return fail_pair(rc);
         c->vp.give(rank[f]); c->vp.give(flag[f]); c->vp.give(evs[f]);
     }
     HIPC(c, hipEventRecord(ev2, c->stream));
@@ -3885,9 +4133,25 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
         outs[f] = H[f];
         H[f] = nullptr;
     }
+    if (pairing) {   // the DPhi near fields wait for the next batched DPhi assembly of this state
+        c->kb.kcc = kccv;
+        c->kb.dense = pd;
+        c->kb.dense64 = pd64;
+        c->kb.sing = ps;
+        c->kb.gen = c->gen;
+        c->kb.prec = precision;
+        c->kb.near64 = n64;
+    }
     c->k = k0;
     c->cc = cc0;
     free_list(c, own);
+    return BEM_OK;
+}
+
+int bem_set_fuse_ops(bem_ctx_t c, int on)
+{
+    if (!c) return BEM_ERR_ARG;
+    c->fuse_ops = on != 0;
     return BEM_OK;
 }
 
@@ -4332,6 +4596,7 @@ struct OpSet {
     std::vector<bem_hmat*> ops;
     int R = 0;
     bool refine = false;
+    size_t active = 0;   // groups beyond the first `active` are padding (zero right-hand sides); 0: none
 };
 template <int Q>
 int gmres_lockstep(bem_ctx* c, const OpSet& S, const bem_c64* B, bem_c64* X, const double* tol, int64_t m,
@@ -4516,18 +4781,28 @@ namespace {
 
 int pow2_q(int64_t nrhs) { return nrhs <= 1 ? 1 : nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : 8; }
 
-// Q-interleaved partial vectors (context) and t vectors (operator) of at
-// least Q right-hand sides
-int ensure_multi(bem_ctx* c, bem_hmat* h, int Q)
+// Q-interleaved partial vectors (context) of at least Q right-hand sides
+int ensure_partq(bem_ctx* c, int Q)
 {
     int rc;
     if (c->partq_cap < Q) {
         pfree(c, c->d_partq);
         c->d_partq = nullptr;
         c->partq_cap = 0;
+        pfree(c, c->d_xq);
+        c->d_xq = nullptr;
         if ((rc = dalloc(c, &c->d_partq, (size_t)std::max<int64_t>(c->part_total, 1) * Q, nullptr))) return rc;
+        if ((rc = dalloc(c, &c->d_xq, (size_t)c->n * Q, nullptr))) return rc;
         c->partq_cap = Q;
     }
+    return BEM_OK;
+}
+
+// ... and the operator's t vectors of at least Q right-hand sides
+int ensure_multi(bem_ctx* c, bem_hmat* h, int Q)
+{
+    int rc;
+    if ((rc = ensure_partq(c, Q))) return rc;
     if (h->tq_cap < Q) {
         pfree(c, h->d_tq);
         h->d_tq = nullptr;
@@ -4612,15 +4887,113 @@ int hmvm_multi(bem_ctx* c, bem_hmat* h, bool refine, const bem_c64* X, bem_c64* 
 
 int hmvm_multi_any(bem_ctx* c, bem_hmat* h, int Q, bool refine, const bem_c64* X, bem_c64* Y, int acc);
 
-// buffers of every operator of S for products of its column count
+// buffers of every operator of S for products of its column count (and
+// the context's Q-wide partial vectors of the fused product)
 int ensure_ops(bem_ctx* c, const OpSet& S, int Q)
 {
+    int rc;
+    if (S.ops.size() > 1 && Q > 1 && (rc = ensure_partq(c, Q))) return rc;
     const int w = S.ops.size() == 1 ? Q : S.R;
     if (w == 1) return BEM_OK;
-    int rc;
     for (bem_hmat* h : S.ops)
         if ((rc = ensure_multi(c, h, w))) return rc;
     return BEM_OK;
+}
+
+// Y = A X with every operator of S in one pass (k_*_ops): DS the near-field
+// storage read, LS the low-rank storage; S.ops.size() R == Q
+template <class DS, class LS, int R, int Q>
+int hmvm_ops(bem_ctx* c, const OpSet& S, const bem_c64* X, bem_c64* Y)
+{
+    const int G = (int)S.ops.size();
+    OpsArg O{};
+    O.n = c->n;
+    O.P = std::max<int64_t>(c->part_total, 1);
+    int64_t mtt = 0, mut = 0;
+    int maxrank = 0;
+    for (int g = 0; g < G; g++) {
+        const bem_hmat* h = S.ops[g];
+        O.pad[g] = S.active && (size_t)g >= S.active;
+        O.F[g] = sizeof(DS) == sizeof(bem_c64) && h->prec == BEM_PREC_FP32 ? h->dense64 : h->dense;
+        O.tt[g] = h->d_ttask;
+        O.ut[g] = h->d_utask;
+        O.ntt[g] = O.pad[g] ? 0 : h->nttask;
+        O.nut[g] = h->nutask;
+        O.rank[g] = h->d_rank;
+        O.voff[g] = h->d_voff;
+        O.aoff[g] = h->d_aoff;
+        O.toff[g] = h->d_toff;
+        O.uoff[g] = h->d_uoff;
+        O.U[g] = h->U;
+        O.V[g] = h->V;
+        O.A[g] = h->A;
+        O.t[g] = R == 1 ? h->d_t : h->d_tq;
+        mtt = std::max(mtt, h->nttask);
+        mut = std::max(mut, h->nutask);
+        maxrank = std::max(maxrank, h->maxrank);
+    }
+    int rc;
+    hipLaunchKernelGGL((k_group_cols<R, Q>), dim3(nblk(c->n * Q, 256)), dim3(256), 0, c->stream, c->n, X, c->d_xq);
+    if (!c->tiles.empty()) {
+        Timer t(c, 3);
+        hipLaunchKernelGGL((k_mvm_dense_ops<DS, R>), dim3((unsigned)c->tiles.size(), G), dim3(kWG), 0, c->stream,
+                           c->d_tiles, O, c->d_xq, c->d_partq);
+        if ((rc = launch_check(c, "k_mvm_dense_ops"))) return rc;
+    }
+    if (mut > 0) {
+        Timer t(c, 4);
+        if (mtt > 0)
+            hipLaunchKernelGGL((k_lr_t_ops<LS, R>), dim3(nblk(mtt, kWG / 64), G), dim3(kWG), 0, c->stream, O,
+                               c->d_col0, c->d_cols, c->d_xq);
+        hipLaunchKernelGGL((k_lr_u_ops<LS, R>), dim3((unsigned)mut, G), dim3(kWG),
+                           R == 1 ? maxrank * sizeof(bem_c64) : 0, c->stream, O, c->d_rows, c->d_lr_part, c->d_partq);
+        if ((rc = launch_check(c, "k_lr_ops"))) return rc;
+    }
+    Timer t(c, 5);
+    hipLaunchKernelGGL((k_mvm_gather_ops<R, Q>), dim3(nblk(16 * c->n, 256)), dim3(256), 0, c->stream, c->n, O.P,
+                       c->d_gptr, c->d_gidx, c->d_partq, Y);
+    return launch_check(c, "k_mvm_gather_ops");
+}
+
+// whether S's operators take the fused product: at most kMaxOps of one
+// storage precision (and, for the refinement form of FP32 ones, each with
+// its FP64 near field), no operator twice among the active groups (each
+// group writes its operator's t vectors)
+bool fusable(const OpSet& S)
+{
+    if (S.ops.size() < 2 || S.ops.size() > (size_t)kMaxOps) return false;
+    const size_t na = S.active ? S.active : S.ops.size();
+    for (size_t g = 0; g < S.ops.size(); g++) {
+        const bem_hmat* h = S.ops[g];
+        if (h->prec != S.ops[0]->prec) return false;
+        if (S.refine && h->prec == BEM_PREC_FP32 && !h->dense64) return false;
+        for (size_t e = 0; e < g && g < na; e++)
+            if (S.ops[e] == h) return false;
+    }
+    return true;
+}
+
+template <int R, int Q>
+int hmvm_ops_prec(bem_ctx* c, const OpSet& S, const bem_c64* X, bem_c64* Y)
+{
+    if (S.ops[0]->prec == BEM_PREC_FP64) return hmvm_ops<bem_c64, bem_c64, R, Q>(c, S, X, Y);
+    if (S.refine) return hmvm_ops<bem_c64, bem_c32, R, Q>(c, S, X, Y);
+    return hmvm_ops<bem_c32, bem_c32, R, Q>(c, S, X, Y);
+}
+
+template <int Q>
+int hmvm_ops_any(bem_ctx* c, const OpSet& S, const bem_c64* X, bem_c64* Y)
+{
+    if constexpr (Q >= 2) {
+        if (S.R == 1) return hmvm_ops_prec<1, Q>(c, S, X, Y);
+    }
+    if constexpr (Q >= 4) {
+        if (S.R == 2) return hmvm_ops_prec<2, Q>(c, S, X, Y);
+    }
+    if constexpr (Q >= 8) {
+        if (S.R == 4) return hmvm_ops_prec<4, Q>(c, S, X, Y);
+    }
+    return set_err(c, BEM_ERR_ARG, "fused product: unsupported column split");
 }
 
 // Y = A X for Q interleaved columns: one product of the single operator, or
@@ -4630,6 +5003,7 @@ template <int Q>
 int apply_ops(bem_ctx* c, const OpSet& S, const bem_c64* X, bem_c64* Y, bem_c64* tx, bem_c64* ty)
 {
     if (S.ops.size() == 1) return hmvm_multi<Q>(c, S.ops[0], S.refine, X, Y, 0);
+    if (c->fuse_ops && fusable(S) && (int)S.ops.size() * S.R == Q) return hmvm_ops_any<Q>(c, S, X, Y);
     int rc;
     const int R = S.R;
     for (size_t g = 0; g < S.ops.size(); g++) {
@@ -5088,6 +5462,7 @@ int bem_gmres_batch(bem_ctx_t c, int64_t nops, const bem_hmat_t* hs, const bem_h
     // right-hand sides) reuse the first operator
     OpSet in, rs;
     in.R = rs.R = (int)nrhs;
+    in.active = rs.active = (size_t)nops;
     rs.refine = true;
     for (int g = 0; g < Q / nrhs; g++) {
         const int64_t o = g < nops ? g : 0;

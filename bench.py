@@ -468,11 +468,12 @@ def main():
         # the other scheduling of the same FP32 points: lanes, or batched chunks of 8
         ob = 0 if batched else 8
         ol = args.lanes if batched else 1
-        prb, _, _, elb, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, ol, ob)
+        prb, _, stb, elb, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, ol, ob)
         key = "lanes_leg" if batched else "sweep_batch_leg"
         print(f"# {key}: {1e3 * elb / points:.2f} ms/point", file=sys.stderr, flush=True)
         out[key] = {"value": round(points / elb, 4), "ms_per_point": round(1e3 * elb / points, 3),
                     "lanes": ol, "sweep_batch": ob,
+                    "phase_ms": {k: round(stb[ic][k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
                     "max_rel_diff_vs_headline": float(max(np.linalg.norm(prb[i, 0] - pr[i, 0]) /
                                                           np.linalg.norm(pr[i, 0]) for i in range(len(freqs))))}
     if refine and not args.no_norefine_leg:

@@ -138,8 +138,11 @@ int bem_hmat_destroy(bem_hmat_t h);
    assembles one operator at a time, bemsolvercontroller.cpp:59-79): the
    operator `kind` at nf frequencies (kcc3: nf x (k, cc re, cc im)) on the
    context's mesh, partition and boundary conditions, returned as nf
-   handles in outs, each equal to bem_assemble at that frequency without
-   near-field pairing (the FP64 near field of bem_set_near_fp64 is kept).  The ACA runs as one
+   handles in outs, each equal to bem_assemble at that frequency (the FP64
+   near field of bem_set_near_fp64 is kept).  With near-field pairing on, a
+   batched Phi assembly also forms the DPhi near field of every frequency
+   and the next batched DPhi assembly of the same state and frequency list
+   takes them.  The ACA runs as one
    launch per size class over the flattened (frequency, block) list, the
    same-size blocks of all frequencies adjacent and the largest first; the
    self / near-singular tables and dense tiles per frequency.  Not with an
@@ -234,6 +237,11 @@ int bem_gmres_multi(bem_ctx_t ctx, bem_hmat_t h, bem_hmat_t hres, int64_t nrhs, 
 int bem_gmres_batch(bem_ctx_t ctx, int64_t nops, const bem_hmat_t* hs, const bem_hmat_t* hres, int64_t nrhs,
                     const void* b_dev, void* x_dev, int x_is_zero, double tol, int64_t restart, int64_t maxit,
                     int64_t max_steps, int64_t* iters, double* rel_resid, int64_t* steps);
+/* bem_set_fuse_ops(ctx, 1) (the default): the product of bem_gmres_batch
+   applies all its operators (of one storage precision) in one launch per
+   kernel, operator o taking columns o * nrhs .. of the interleaved vectors;
+   0: one operator after another through column gathers. */
+int bem_set_fuse_ops(bem_ctx_t ctx, int on);
 
 /* Leaf-block preconditioner of an assembled operator: LU with partial
    pivoting (FP64) of every diagonal leaf block, i.e. the leaf level of the

@@ -140,3 +140,82 @@ def test_batched_sweep(bem, O, meshes):
         for k in range(2):
             assert rel(pb[fi, k], pl[fi, k]) < 1e-8, (fi, k, rel(pb[fi, k], pl[fi, k]))
     S.close()
+
+
+@pytest.mark.parametrize("prec,near64", [(0, False), (1, False), (1, True)])
+def test_batch_pairing(bem, O, meshes, prec, near64):
+    """Near-field pairing across batches: a batched Phi assembly also forms
+    the DPhi near fields, and the next batched DPhi assembly of the same
+    frequency list takes them (no near-field tile launch of its own); both
+    batches equal the unpaired single assemblies bit for bit."""
+    import ctypes
+    P = sphere_problem(O, meshes, level=3, leaf=32, eta=2.0, freq=FREQS[0])
+    ctx = upload_problem(bem, P)
+    ctx.set_near_fp64(near64)
+    nb = len(P.blocks)
+    kcc = []
+    for f in FREQS:
+        P.set_frequency(f, 343.0, 1)
+        prm = P.params()
+        kcc.append([prm["k"], prm["cc"].real, prm["cc"].imag])
+    kcc = np.array(kcc)
+    single = {}
+    for f, (k, cr, ci) in enumerate(kcc):
+        ctx.set_frequency(k, complex(cr, ci))
+        for kind in (0, 1):
+            single[f, kind] = ctx.assemble(kind, 1e-6, 0, precision=prec, seed=3)
+    ctx.set_near_pairing(True)
+    phi = ctx.assemble_batch(0, kcc, 1e-6, precision=prec, seed=3)
+    prof = np.zeros(14)
+    bem.lib().bem_profile(ctx.h, 1, prof.ctypes.data_as(ctypes.c_void_p))
+    dphi = ctx.assemble_batch(1, kcc, 1e-6, precision=prec, seed=3)
+    bem.lib().bem_profile(ctx.h, 0, prof.ctypes.data_as(ctypes.c_void_p))
+    assert prof[7] == 0, prof            # no near-field tile launch in the DPhi batch
+    for f in range(len(FREQS)):
+        for kind, G in ((0, phi[f]), (1, dphi[f])):
+            ok, b = same_blocks(G, single[f, kind], nb)
+            assert ok, (f, kind, b)
+        if near64:
+            x = np.exp(0.3j * np.arange(P.n))
+            assert np.array_equal(dphi[f].mvm_refine(x), single[f, 1].mvm_refine(x))
+    for G in phi + dphi + list(single.values()):
+        G.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fused_product_equals_operator_loop(bem, O, meshes, prec):
+    """bem_gmres_batch with every operator in one pass per kernel (fused)
+    against one operator after another: bit for bit with one or two
+    right-hand sides per operator (the same per-entry arithmetic), to 1e-12
+    with four (the loop takes the matrix-core kernels there); FP32 with the
+    refinement form too."""
+    P = sphere_problem(O, meshes, level=3, leaf=32, eta=2.0, freq=FREQS[0])
+    ctx = upload_problem(bem, P)
+    ctx.set_near_fp64(prec == bem.FP32)
+    kcc = []
+    for f in FREQS:
+        P.set_frequency(f, 343.0, 1)
+        prm = P.params()
+        kcc.append([prm["k"], prm["cc"].real, prm["cc"].imag])
+    Hs = ctx.assemble_batch(1, np.array(kcc), 1e-6, precision=prec, seed=3)
+    n = P.n
+    rng = np.random.default_rng(9)
+    steps = 10 if prec == bem.FP32 else 0
+    for nops, nrhs in ((3, 1), (3, 2), (2, 4)):
+        B = rng.standard_normal((n, nops * nrhs)) + 1j * rng.standard_normal((n, nops * nrhs))
+        out = {}
+        for fuse in (True, False):
+            ctx.set_fuse_ops(fuse)
+            print("fused product: prec", prec, "nops", nops, "nrhs", nrhs, "fuse", fuse, flush=True)
+            out[fuse] = ctx.gmres_batch(Hs[:nops], B, 1e-11, max_steps=steps)
+            assert out[fuse][4] == 0
+        if nrhs < 4:
+            assert np.array_equal(out[True][0].view(np.uint64), out[False][0].view(np.uint64)), (nops, nrhs)
+            assert np.array_equal(out[True][1], out[False][1])
+        else:
+            assert rel(out[True][0], out[False][0]) < 1e-9, rel(out[True][0], out[False][0])
+    ctx.set_fuse_ops(True)
+    for H in Hs:
+        H.close()
+    ctx.close()

@@ -61,6 +61,9 @@ _SIGS = {
     "bem_solver_set_refinement": (_int, [_p, _int, _i64]),
     "bem_solver_set_sweep_batch": (_int, [_p, _int]),
     "bem_set_fuse_ops": (_int, [_p, _int]),
+    "bem_autotune_aca": (_int, [_p, _int, _int, _i64, _p, _dbl, _i64, _u64, _p, _p]),
+    "bem_solver_set_autotune": (_int, [_p, _int]),
+    "bem_solver_aca_schedule": (_int, [_p, _int, _p]),
     "bem_gmres_batch": (_int, [_p, _i64, _p, _p, _i64, _p, _p, _int, _dbl, _i64, _i64, _i64, _p, _p,
                                ctypes.POINTER(_i64)]),
     "bem_set_mfma": (_int, [_p, _int]),
@@ -259,6 +262,16 @@ class Context:
         self._check(lib().bem_assemble_batch(self.h, int(kind), len(kcc), _ptr(kcc), float(tol), int(maxrank),
                                              int(precision), int(seed), ctypes.cast(outs, _p)))
         return [HMatrix(self, _p(h)) for h in outs]
+
+    def autotune_aca(self, kind, kcc, tol, maxrank=0, precision=FP64, seed=1):
+        """bem_autotune_aca: (small bound, huge bound, queue), best ms; the
+        schedule stays on the context"""
+        kcc = _c(np.asarray(kcc).reshape(-1, 3), np.float64)
+        ch = np.zeros(3, np.int64)
+        ms = np.zeros(1)
+        self._check(lib().bem_autotune_aca(self.h, int(kind), int(precision), len(kcc), _ptr(kcc), float(tol),
+                                           int(maxrank), int(seed), _ptr(ch), _ptr(ms)))
+        return tuple(int(v) for v in ch), float(ms[0])
 
     def set_fuse_ops(self, on):
         """bem_set_fuse_ops: products of several operators in one pass (default on)"""
@@ -644,6 +657,22 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return phi, dphi, pr, dict(zip(STAT_KEYS, st.tolist()))
+
+    def set_autotune(self, on):
+        """bem_solver_set_autotune: tune the ACA schedule on the first batched chunk"""
+        rc = lib().bem_solver_set_autotune(self.h, int(bool(on)))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_autotune")
+
+    def aca_schedule(self, precision):
+        """the tuned ACA schedule of a precision (bem_solver_aca_schedule) or None"""
+        out = np.zeros(4)
+        rc = lib().bem_solver_aca_schedule(self.h, int(precision), _ptr(out))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_aca_schedule")
+        if out[0] < 0:
+            return None
+        return {"small_bound": int(out[0]), "huge_bound": int(out[1]), "queue": int(out[2]), "ms": float(out[3])}
 
     def set_sweep_batch(self, batch):
         """sweeps in frequency chunks of batch with batched assembly and

@@ -4148,6 +4148,88 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
     return BEM_OK;
 }
 
+int bem_autotune_aca(bem_ctx_t c, int kind, int precision, int64_t nf, const double* kcc3, double tol,
+                     int64_t maxrank, uint64_t seed, int64_t* choice3, double* best_ms)
+{
+    if (!c || !kcc3 || nf < 1 || nf > 64 || (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32))
+        return set_err(c, BEM_ERR_ARG, "bem_autotune_aca: bad arguments");
+    const int pi = precision == BEM_PREC_FP32 ? 1 : 0;
+    const int64_t keep_small = c->cls_small[pi], keep_huge = c->cls_huge[pi];
+    const bool keep_queue = c->aca_queue;
+    // one timed assembly (the fastest of two runs) under the schedule s
+    auto timed = [&](int64_t small, int64_t huge, bool queue, double* ms) -> int {
+        c->cls_small[pi] = small;
+        c->cls_huge[pi] = huge;
+        c->aca_queue = queue;
+        double best = 1e300;
+        for (int rep = 0; rep < 2; rep++) {
+            std::vector<bem_hmat_t> hs((size_t)nf, nullptr);
+            const double t0 = now_ms();
+            int rc;
+            if (nf == 1) {
+                const double k0 = c->k;
+                const bem_c64 cc0 = c->cc;
+                c->k = kcc3[0];
+                c->cc = c64(kcc3[1], kcc3[2]);
+                rc = bem_assemble(c, kind, tol, maxrank, precision, seed, &hs[0]);
+                c->k = k0;
+                c->cc = cc0;
+            } else {
+                rc = bem_assemble_batch(c, kind, nf, kcc3, tol, maxrank, precision, seed, hs.data());
+            }
+            const double dt = now_ms() - t0;
+            for (bem_hmat_t h : hs)
+                if (h) bem_hmat_destroy(h);
+            if (rc) return rc;
+            best = std::min(best, dt);
+        }
+        *ms = best;
+        return BEM_OK;
+    };
+    auto restore = [&](int rc) {
+        c->cls_small[pi] = keep_small;
+        c->cls_huge[pi] = keep_huge;
+        c->aca_queue = keep_queue;
+        drop_kept(c);
+        return rc;
+    };
+    drop_kept(c);   // nothing kept before is adopted by a timed run, nothing a timed run keeps survives
+    const int64_t inf = (int64_t)1 << 40;
+    const std::vector<int64_t> smalls = {128, 256, 384, 512};
+    const std::vector<int64_t> huges = pi ? std::vector<int64_t>{1024, 2048, 4096, inf} : std::vector<int64_t>{inf};
+    int64_t bs = kSmallBlock[pi], bh = kHugeBlock[pi];
+    bool bq = false;
+    double bt = 0;
+    int rc;
+    if ((rc = timed(bs, bh, bq, &bt))) return restore(rc);   // the built-in schedule (also the warm-up)
+    if ((rc = timed(bs, bh, bq, &bt))) return restore(rc);
+    // coordinate descent: the one-wavefront bound, then the huge bound, then the queue
+    for (int64_t v : smalls) {
+        double t;
+        if (v == bs) continue;
+        if ((rc = timed(v, bh, bq, &t))) return restore(rc);
+        if (t < bt) { bt = t; bs = v; }
+    }
+    for (int64_t v : huges) {
+        double t;
+        if (v == bh || v <= bs) continue;
+        if ((rc = timed(bs, v, bq, &t))) return restore(rc);
+        if (t < bt) { bt = t; bh = v; }
+    }
+    if (nf > 1) {
+        double t;
+        if ((rc = timed(bs, bh, !bq, &t))) return restore(rc);
+        if (t < bt) { bt = t; bq = !bq; }
+    }
+    c->cls_small[pi] = bs;
+    c->cls_huge[pi] = bh;
+    c->aca_queue = bq;
+    drop_kept(c);
+    if (choice3) { choice3[0] = bs; choice3[1] = bh; choice3[2] = bq; }
+    if (best_ms) *best_ms = bt;
+    return BEM_OK;
+}
+
 int bem_set_fuse_ops(bem_ctx_t c, int on)
 {
     if (!c) return BEM_ERR_ARG;

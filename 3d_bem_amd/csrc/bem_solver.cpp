@@ -671,6 +671,23 @@ void BoundaryElementSolver::calculateBoundarySolutionsBatch(const std::vector<do
     destroy(dres);
 }
 
+void BoundaryElementSolver::tuneAca(const std::vector<double>& freqs, int64_t choice[3], double* ms)
+{
+    const auto& P = bemSolverParameters;
+    check(ctx_, bem_set_near_fp64(ctx_, refineMode(P) == 1), "bem_set_near_fp64");
+    std::vector<double> kcc(3 * freqs.size());
+    for (size_t f = 0; f < freqs.size(); f++) {
+        setFrequency(freqs[f]);
+        setWavenumberAndCoupling();
+        kcc[3 * f] = wavenumber;
+        kcc[3 * f + 1] = couplingParameter.real();
+        kcc[3 * f + 2] = couplingParameter.imag();
+    }
+    check(ctx_, bem_autotune_aca(ctx_, BEM_KIND_DPHI, P.precision, (int64_t)freqs.size(), kcc.data(),
+                                 P.acaRelativeError, (int64_t)P.acaMaxRank, P.pivotSeed, choice, ms),
+          "bem_autotune_aca");
+}
+
 void BoundaryElementSolver::regularSolve(double tol, long restart)
 {
     setWavenumberAndCoupling();
@@ -832,6 +849,12 @@ struct bem_solver {
     int refine = 0;
     long refineMaxSteps = 10;
     int sweepBatch = 0;   // bem_solver_set_sweep_batch
+    // bem_solver_set_autotune: the ACA schedule of each precision, timed on
+    // the first batched chunk of the first sweep and then used on every lane
+    bool autotune = false;
+    bool tuned[2] = {false, false};
+    int64_t schedule[2][3] = {};
+    double tunedMs[2] = {0, 0};
 };
 
 extern "C" {
@@ -998,6 +1021,27 @@ int bem_solver_set_refinement(bem_solver_t s, int mode, int64_t max_steps)
     return BEM_OK;
 }
 
+int bem_solver_set_autotune(bem_solver_t s, int on)
+{
+    if (!s || !s->s) return BEM_ERR_ARG;
+    s->autotune = on != 0;
+    if (!on) s->tuned[0] = s->tuned[1] = false;
+    return BEM_OK;
+}
+
+int bem_solver_aca_schedule(bem_solver_t s, int precision, double* out4)
+{
+    if (!s || !s->s || !out4 || (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)) return BEM_ERR_ARG;
+    const int pi = precision == BEM_PREC_FP32 ? 1 : 0;
+    if (!s->tuned[pi]) {
+        out4[0] = out4[1] = out4[2] = out4[3] = -1;
+        return BEM_OK;
+    }
+    for (int q = 0; q < 3; q++) out4[q] = (double)s->schedule[pi][q];
+    out4[3] = s->tunedMs[pi];
+    return BEM_OK;
+}
+
 int bem_solver_set_sweep_batch(bem_solver_t s, int batch)
 {
     if (!s || !s->s || batch < 0) return BEM_ERR_ARG;
@@ -1098,45 +1142,37 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 rhs[k].planeWaves = s->s->planeWaves;
             }
         }
-        if (s->sweepBatch > 1) {   // batched assembly and lockstep GMRES over frequency chunks, one context
+        // batch > 1: frequency chunks of up to `chunk` points, each solved by
+        // one batched call; the lanes take chunks instead of single points
+        const int64_t chunk = s->sweepBatch > 1 ? std::max<int64_t>(1, std::min<int64_t>(s->sweepBatch, 8 / nrhs)) : 1;
+        const int64_t nchunks = (nf + chunk - 1) / chunk;
+        const int L = (int)std::min<int64_t>(lanes, std::max<int64_t>(nchunks, 1));
+        const int pi = precision == BEM_PREC_FP32 ? 1 : 0;
+        if (s->autotune && chunk > 1 && nf > 0 && !s->tuned[pi]) {
             auto& S = *s->s;
             S.setWaveSpeed(c);
-            S.setAirDensity(rho);
             auto& P = S.bemSolverParameters;
             P.acaRelativeError = aca_tol;
             P.acaMaxRank = (unsigned long)max_rank;
             P.coupling = (bem::BemCoupling)coupling;
-            P.gmresTolerance = gmres_tol;
-            P.gmresRestart = (long)restart;
             P.pivotSeed = seed;
             P.precision = precision;
-            P.preconditioner = s->preconditioner;
             P.refine = s->refine;
-            P.refineMaxSteps = s->refineMaxSteps;
-            const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(s->sweepBatch, 8 / nrhs));
-            for (int64_t f0 = 0; f0 < nf; f0 += chunk) {
-                const int64_t f1 = std::min(nf, f0 + chunk);
-                std::vector<double> fr(freqs + f0, freqs + f1);
-                std::vector<std::vector<std::vector<bem::cplx>>> pr;
-                std::vector<std::vector<int64_t>> its;
-                std::vector<double> st;
-                S.calculateBoundarySolutionsBatch(fr, rhs, pr, its, st);
-                for (int64_t f = f0; f < f1; f++) {
-                    for (int64_t k = 0; k < nrhs; k++) {
-                        std::memcpy(pressure + 2 * ((f * nrhs + k) * n), pr[f - f0][k].data(), n * sizeof(bem::cplx));
-                        if (iters) iters[f * nrhs + k] = its[f - f0][k];
-                    }
-                    if (stats16) std::memcpy(stats16 + 16 * f, st.data() + 16 * (f - f0), 16 * sizeof(double));
-                }
-            }
-            return BEM_OK;
+            std::vector<double> fr(freqs, freqs + std::min(nf, chunk));
+            S.tuneAca(fr, s->schedule[pi], &s->tunedMs[pi]);
+            s->tuned[pi] = true;
         }
-        const int L = (int)std::min<int64_t>(lanes, std::max<int64_t>(nf, 1));
         while ((int)s->lanes.size() < L - 1)
             s->lanes.push_back(std::make_unique<bem::BoundaryElementSolver>(s->device, s->input, s->leaf, s->eta));
         std::vector<bem::BoundaryElementSolver*> solver(L);
         solver[0] = s->s.get();
         for (int q = 1; q < L; q++) solver[q] = s->lanes[q - 1].get();
+        if (s->tuned[pi])
+            for (int q = 0; q < L; q++)
+                bem::check(solver[q]->context(),
+                           bem_set_aca_schedule(solver[q]->context(), precision, s->schedule[pi][0],
+                                                s->schedule[pi][1], (int)s->schedule[pi][2]),
+                           "bem_set_aca_schedule");
         std::vector<std::string> errs(L);
         std::vector<int> codes(L, BEM_OK);
         auto work = [&](int q) {
@@ -1157,6 +1193,25 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 P.preconditionerRelError = s->hluRelError;
                 P.refine = s->refine;
                 P.refineMaxSteps = s->refineMaxSteps;
+                if (chunk > 1) {
+                    for (int64_t ch = q; ch < nchunks; ch += L) {
+                        const int64_t f0 = ch * chunk, f1 = std::min(nf, f0 + chunk);
+                        std::vector<double> fr(freqs + f0, freqs + f1);
+                        std::vector<std::vector<std::vector<bem::cplx>>> prb;
+                        std::vector<std::vector<int64_t>> itb;
+                        std::vector<double> st;
+                        S.calculateBoundarySolutionsBatch(fr, rhs, prb, itb, st);
+                        for (int64_t f = f0; f < f1; f++) {
+                            for (int64_t k = 0; k < nrhs; k++) {
+                                std::memcpy(pressure + 2 * ((f * nrhs + k) * n), prb[f - f0][k].data(),
+                                            n * sizeof(bem::cplx));
+                                if (iters) iters[f * nrhs + k] = itb[f - f0][k];
+                            }
+                            if (stats16) std::memcpy(stats16 + 16 * f, st.data() + 16 * (f - f0), 16 * sizeof(double));
+                        }
+                    }
+                    return;
+                }
                 std::vector<std::vector<bem::cplx>> pr;
                 std::vector<int64_t> its;
                 for (int64_t f = q; f < nf; f += L) {

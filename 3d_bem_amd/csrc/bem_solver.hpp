@@ -179,6 +179,9 @@ public:
     void calculateBoundarySolutionsBatch(const std::vector<double>& freqs, const std::vector<RightHandSide>& rhs,
                                          std::vector<std::vector<std::vector<cplx>>>& pressure,
                                          std::vector<std::vector<int64_t>>& iterations, std::vector<double>& stats16);
+    // bem_autotune_aca of the DPhi assembly at these frequencies with the
+    // current parameters; the chosen schedule stays on this solver's context
+    void tuneAca(const std::vector<double>& freqs, int64_t choice[3], double* ms);
     // calculateFieldSolution with the direct sum (calculateFieldSolutionRegular,
     // boundaryelementsolver.cpp:2192-2316): sound pressure at observation
     // points from the last boundary solution, incident sources included

@@ -358,6 +358,10 @@ def main():
     ap.add_argument("--sweep-batch", type=int, default=0, metavar="B",
                     help="FP32 form: frequency chunks of B points with batched assembly and lockstep GMRES on one "
                          "context instead of the lanes (0: lanes)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched sweeps: tune the ACA schedule on the first chunk (in the warm-up)")
+    ap.add_argument("--leg-batch", type=int, default=8, metavar="B",
+                    help="chunk size of the batched comparison leg (when the headline runs on lanes)")
     ap.add_argument("--no-batch-leg", action="store_true",
                     help="skip the comparison run of the other FP32 scheduling (lanes or batched chunks)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
@@ -416,15 +420,16 @@ def main():
     freqs = batch_freqs(args.freq, args.batch)
     ic = int(np.argmin(np.abs(freqs - args.freq)))
     batched = prec == bem.FP32 and args.sweep_batch > 1 and args.plane is None
-    lanes = args.lanes if prec == bem.FP32 and not batched else 1
+    lanes = min(args.lanes, -(-args.batch // args.sweep_batch)) if batched else args.lanes if prec == bem.FP32 else 1
     sb = args.sweep_batch if batched else 0
     refine = args.refine if prec == bem.FP32 else 0
     S.set_refinement(refine)
+    S.set_autotune(args.autotune)
 
     pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sb)
     rstats = S.refine_stats()
     points = args.steps * len(freqs)
-    sched = f"batches of {sb}" if batched else f"{lanes} lanes"
+    sched = f"batches of {sb} on {lanes} lanes" if batched else f"{lanes} lanes"
     print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, {sched}", file=sys.stderr, flush=True)
     st = stats[ic]
     points0 = args.steps * len(range(0, len(freqs), max(lanes, 1)))   # points whose launches lane 0 records
@@ -448,6 +453,7 @@ def main():
                                + ("" if args.plane is None else f", rigid plane {args.plane:g} m below the centre"),
                    "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "sweep_batch": sb, "setup_s": round(setup_s, 3),
                    "refine": refine, "refine_stats_last_point_lane0": rstats,
+                   "aca_schedule": S.aca_schedule(prec) if args.autotune else None,
                    "ms_per_point": round(1e3 * elapsed / points, 3),
                    "gmres_iters": int(its[ic, 0]), "residual_estimate": st["residual"],
                    "max_rank_phi": st["max_rank_phi"], "max_rank_dphi": st["max_rank_dphi"],
@@ -465,9 +471,10 @@ def main():
     p64 = None
     pr0 = None
     if prec == bem.FP32 and args.plane is None and not args.no_batch_leg:
-        # the other scheduling of the same FP32 points: lanes, or batched chunks of 8
-        ob = 0 if batched else 8
-        ol = args.lanes if batched else 1
+        # the other scheduling of the same FP32 points: lanes, or batched
+        # chunks of --leg-batch on as many lanes as there are chunks
+        ob = 0 if batched else args.leg_batch
+        ol = args.lanes if batched else min(args.lanes, -(-args.batch // args.leg_batch))
         prb, _, stb, elb, _, _ = timed_run(bem, S, ctx_h, args, prec, freqs, ol, ob)
         key = "lanes_leg" if batched else "sweep_batch_leg"
         print(f"# {key}: {1e3 * elb / points:.2f} ms/point", file=sys.stderr, flush=True)

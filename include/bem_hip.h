@@ -155,6 +155,17 @@ int bem_hmat_destroy(bem_hmat_t h);
 int bem_assemble_batch(bem_ctx_t ctx, int kind, int64_t nf, const double* kcc3, double aca_tol, int64_t max_rank,
                        int precision, uint64_t seed, bem_hmat_t* outs);
 int bem_set_aca_schedule(bem_ctx_t ctx, int precision, int64_t small_bound, int64_t huge_bound, int queue);
+/* ACA schedule autotuner (no counterpart in the reference, whose per-block
+   tasks go to a thread pool, hmatrix.cpp:793-846): times the assembly of
+   `kind` at the nf frequencies kcc3 (bem_assemble for nf = 1,
+   bem_assemble_batch otherwise; the fastest of two runs each) under
+   candidate schedules by coordinate descent -- the one-wavefront bound
+   (128, 256, 384, 512), then for FP32 the huge bound (1024, 2048, 4096,
+   none), then for nf > 1 the work queue -- and keeps the fastest on the
+   context.  choice3: small bound, huge bound, queue; best_ms: its time.
+   Operators kept for near-field pairing are dropped. */
+int bem_autotune_aca(bem_ctx_t ctx, int kind, int precision, int64_t nf, const double* kcc3, double tol,
+                     int64_t maxrank, uint64_t seed, int64_t* choice3, double* best_ms);
 /* HArithm::MVM: y (+)= H x on device pointers (complex double, cluster
    order); accumulate = 1 adds to y as the reference does. */
 int bem_hmvm(bem_ctx_t ctx, bem_hmat_t h, const void* x_dev, void* y_dev, int accumulate);
@@ -445,12 +456,21 @@ int bem_refine_triangles(int64_t n, const double* tri9, double edge_length, int6
    order), iters nf x nrhs, stats nf x 16 (bem_solver_solve's layout, for the
    last right-hand side). */
 /* batch > 1: bem_solver_sweep processes the frequencies in chunks of
-   min(batch, 8 / nrhs) on the solver's one context: Phi and DPhi of a
-   chunk from one batched assembly each (bem_assemble_batch), the GMRES of
-   all its (frequency, right-hand side) pairs in one lockstep solve
-   (bem_gmres_batch), so no host lanes and no extra hardware queues; nrhs
-   1, 2 or 4, no preconditioner, no impedance plane.  0 or 1: the lanes. */
+   min(batch, 8 / nrhs): Phi and DPhi of a chunk from one batched assembly
+   each (bem_assemble_batch), the GMRES of all its (frequency, right-hand
+   side) pairs in one lockstep solve (bem_gmres_batch); the sweep's lanes
+   take whole chunks (one lane: no extra hardware queues needed); nrhs 1, 2
+   or 4, no preconditioner, no impedance plane.  0 or 1: one point per lane
+   at a time. */
 int bem_solver_set_sweep_batch(bem_solver_t s, int batch);
+/* on: the first batched sweep of each precision first runs
+   bem_autotune_aca on its first chunk (DPhi, the sweep's parameters, timed
+   alone on the solver's context) and every lane then assembles with the
+   chosen schedule; off: the built-in schedule, tuning forgotten.
+   bem_solver_aca_schedule: small bound, huge bound, queue, tuned ms of a
+   precision's chosen schedule (all -1 before tuning). */
+int bem_solver_set_autotune(bem_solver_t s, int on);
+int bem_solver_aca_schedule(bem_solver_t s, int precision, double* out4);
 int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, double rho, int coupling,
                      double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      int64_t nrhs, const double* g, const double* planes5, int lanes, double* pressure,

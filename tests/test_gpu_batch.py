@@ -219,3 +219,41 @@ def test_fused_product_equals_operator_loop(bem, O, meshes, prec):
     for H in Hs:
         H.close()
     ctx.close()
+
+
+def test_autotune_keeps_results(bem, O, meshes):
+    """bem_autotune_aca picks one of its candidate schedules and keeps it on
+    the context; assemblies under it equal the built-in schedule's bit for
+    bit.  A tuned batched sweep reports its schedule and gives the untuned
+    sweep's pressures bit for bit."""
+    P = sphere_problem(O, meshes, level=3, leaf=32, eta=2.0, freq=FREQS[0])
+    ctx = upload_problem(bem, P)
+    nb = len(P.blocks)
+    kcc = []
+    for f in FREQS:
+        P.set_frequency(f, 343.0, 1)
+        prm = P.params()
+        kcc.append([prm["k"], prm["cc"].real, prm["cc"].imag])
+    kcc = np.array(kcc)
+    ref = ctx.assemble_batch(1, kcc, 1e-6, precision=bem.FP32, seed=3)
+    (small, huge, queue), ms = ctx.autotune_aca(1, kcc, 1e-6, precision=bem.FP32, seed=3)
+    print("tuned schedule", small, huge, queue, ms, flush=True)
+    assert small in (128, 256, 384, 512) and huge in (1024, 2048, 4096, 1 << 40) and queue in (0, 1) and ms > 0
+    tuned = ctx.assemble_batch(1, kcc, 1e-6, precision=bem.FP32, seed=3)
+    for f in range(len(FREQS)):
+        ok, b = same_blocks(tuned[f], ref[f], nb)
+        assert ok, (f, b)
+    for G in ref + tuned:
+        G.close()
+    ctx.close()
+    tri = meshes.icosphere(3, 0.5)
+    n = len(tri)
+    S = bem.Solver(tri, meshes.robin_bc(n, 0, 1, 1), leaf=32, eta=2.0)
+    S.set_sweep_batch(4)
+    p0, _, _ = S.sweep(np.array(FREQS), aca_tol=1e-6, gmres_tol=1e-10, precision=bem.FP32)
+    assert S.aca_schedule(bem.FP32) is None
+    S.set_autotune(True)
+    p1, _, _ = S.sweep(np.array(FREQS), aca_tol=1e-6, gmres_tol=1e-10, precision=bem.FP32)
+    assert S.aca_schedule(bem.FP32) is not None
+    assert np.array_equal(p0.view(np.uint64), p1.view(np.uint64))
+    S.close()

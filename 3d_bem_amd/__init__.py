@@ -77,6 +77,8 @@ _SIGS = {
     "bem_read_msh_error": (ctypes.c_char_p, []),
     "bem_refine_triangles": (_int, [_i64, _p, _dbl, _i64, _p, ctypes.POINTER(_i64)]),
     "bem_solver_field": (_int, [_p, _i64, _p, _p]),
+    "bem_solver_field_hmat": (_int, [_p, _i64, _p, _dbl, _i64, _i64, _dbl, _u64, _p, _p]),
+    "bem_field_hmat": (_int, [_p, _i64, _p, _i64, _p, _dbl, _i64, _u64, _p, _p, _p, _p]),
     "bem_solver_set_preconditioner": (_int, [_p, _int]),
     "bem_solver_set_hlu": (_int, [_p, _i64, _dbl]),
     "bem_solver_precond_stats": (_int, [_p, _p]),
@@ -272,6 +274,18 @@ class Context:
         self._check(lib().bem_autotune_aca(self.h, int(kind), int(precision), len(kcc), _ptr(kcc), float(tol),
                                            int(maxrank), int(seed), _ptr(ch), _ptr(ms)))
         return tuple(int(v) for v in ch), float(ms[0])
+
+    def field_hmat(self, obs, blocks5, phi, dphi, tol, maxrank=0, seed=1):
+        """bem_field_hmat: boundary term at obs (nobs, 3, in the points'
+        cluster order) through the partition blocks5 (nb, 5); phi / dphi
+        device arrays; returns (values, stats4)"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        b5 = _c(np.asarray(blocks5).reshape(-1, 5), np.int64)
+        out = np.empty(len(obs), np.complex128)
+        st = np.zeros(4)
+        self._check(lib().bem_field_hmat(self.h, len(obs), _ptr(obs), len(b5), _ptr(b5), float(tol), int(maxrank),
+                                         int(seed), phi.ptr, dphi.ptr, _ptr(out), _ptr(st)))
+        return out, st
 
     def set_fuse_ops(self, on):
         """bem_set_fuse_ops: products of several operators in one pass (default on)"""
@@ -744,6 +758,19 @@ class Solver:
         if rc != BEM_OK:
             raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
         return out
+
+    def field_hmat(self, obs, aca_tol=0.01, maxrank=0, leaf=50, eta=3.0, seed=1):
+        """sound pressure at observation points (nobs, 3) from the last solve
+        through the observation x boundary H-matrix (bem_solver_field_hmat,
+        the reference's calculateFieldSolutionFast); returns (pressure, stats)"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        out = np.empty(len(obs), np.complex128)
+        st = np.zeros(4)
+        rc = lib().bem_solver_field_hmat(self.h, len(obs), _ptr(obs), float(aca_tol), int(maxrank), int(leaf),
+                                         float(eta), int(seed), _ptr(out), _ptr(st))
+        if rc != BEM_OK:
+            raise BemError(rc, lib().bem_solver_last_error(self.h).decode())
+        return out, {"ms": st[0], "admissible_blocks": int(st[1]), "max_rank": int(st[2]), "evaluated": int(st[3])}
 
     def sweep(self, freqs, g=None, planes=None, c=343.0, rho=1.21, coupling=1, aca_tol=1e-6, maxrank=0,
               gmres_tol=None, restart=300, seed=1, precision=FP64, lanes=1):

@@ -270,12 +270,13 @@ __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int6
 __constant__ static const double kTri7[7][3] = BEM_TRI7;
 constexpr int64_t kFieldChunk = 1024;   // boundary elements per partial sum of the field
 
-// M phi - L dphi of one element (vertices and normal from LDS column c)
-__device__ __forceinline__ bem_c64 field_term(double k, double px, double py, double pz, const double (*sv)[kTile],
-                                              const double (*sn)[kTile], int c, double area, bem_c64 phi,
-                                              bem_c64 dphi)
+// L and M of one element for the observation point p (BemOperatorField,
+// boundaryelementsolver.cpp:2319-2353; vertices and normal from LDS column c)
+__device__ __forceinline__ void field_ops(double k, double px, double py, double pz, const double (*sv)[kTile],
+                                          const double (*sn)[kTile], int c, double area, bem_c64* L, bem_c64* Mk)
 {
-    bem_c64 L = c64(0, 0), Mk = c64(0, 0);
+    *L = c64(0, 0);
+    *Mk = c64(0, 0);
 #pragma unroll
     for (int q = 0; q < 7; q++) {
         double x1 = kTri7[q][0], x2 = kTri7[q][1];
@@ -283,10 +284,19 @@ __device__ __forceinline__ bem_c64 field_term(double k, double px, double py, do
         double qx = (x1 * sv[0][c] + x2 * sv[3][c]) + x3 * sv[6][c];
         double qy = (x1 * sv[1][c] + x2 * sv[4][c]) + x3 * sv[7][c];
         double qz = (x1 * sv[2][c] + x2 * sv[5][c]) + x3 * sv[8][c];
-        bem_field_point_accumulate(&L, &Mk, k, px, py, pz, sn[0][c], sn[1][c], sn[2][c], qx, qy, qz, kTri7[q][2]);
+        bem_field_point_accumulate(L, Mk, k, px, py, pz, sn[0][c], sn[1][c], sn[2][c], qx, qy, qz, kTri7[q][2]);
     }
-    L = c64_scale(L, area);
-    Mk = c64_scale(Mk, area);
+    *L = c64_scale(*L, area);
+    *Mk = c64_scale(*Mk, area);
+}
+
+// M phi - L dphi of one element
+__device__ __forceinline__ bem_c64 field_term(double k, double px, double py, double pz, const double (*sv)[kTile],
+                                              const double (*sn)[kTile], int c, double area, bem_c64 phi,
+                                              bem_c64 dphi)
+{
+    bem_c64 L, Mk;
+    field_ops(k, px, py, pz, sv, sn, c, area, &L, &Mk);
     return bem_field_term(L, Mk, phi, dphi);
 }
 
@@ -345,6 +355,58 @@ __global__ __launch_bounds__(kWG) void k_field(DevMesh M, double k, int64_t nobs
         part[i * nchunk + blockIdx.y] = c64_add(c64_add(red[0][r], red[1][r]), c64_add(red[2][r], red[3][r]));
 }
 
+// ---- field through the observation x boundary H-matrix
+// (calculateFieldSolutionFast, boundaryelementsolver.cpp:2397-2476).
+// A dense block's 64 x 64 tile: partial[t][r] = sum_c M phi_c - sum_c L dphi_c
+// over the tile's columns (column c taken by thread group c mod 4; the four
+// sums of each operator combined (s0 + s1) + (s2 + s3) before the difference).
+struct FieldTile {
+    int64_t row0, col0, part;
+    int32_t rows, cols;
+};
+
+__global__ __launch_bounds__(kWG) void k_field_dense(DevMesh M, double k, int64_t nobs, const double* __restrict__ obs,
+                                                     const FieldTile* __restrict__ tiles,
+                                                     const bem_c64* __restrict__ phi, const bem_c64* __restrict__ dphi,
+                                                     bem_c64* __restrict__ part)
+{
+    __shared__ double sv[9][kTile];
+    __shared__ double sn[3][kTile], sa[kTile];
+    __shared__ bem_c64 sphi[kTile], sdphi[kTile];
+    __shared__ bem_c64 redM[4][kTile], redL[4][kTile];
+    const FieldTile T = tiles[blockIdx.x];
+    const int tid = threadIdx.x, r = tid & (kTile - 1), cg = tid >> 6;
+    const int64_t n = M.n;
+    for (int idx = tid; idx < 9 * kTile; idx += kWG) {
+        int d = idx / kTile, c = idx % kTile;
+        if (c < T.cols) sv[d][c] = M.vtx[(int64_t)d * n + T.col0 + c];
+    }
+    for (int c = tid; c < T.cols; c += kWG) {
+        sn[0][c] = M.nx[T.col0 + c]; sn[1][c] = M.ny[T.col0 + c]; sn[2][c] = M.nz[T.col0 + c];
+        sa[c] = M.area[T.col0 + c];
+        sphi[c] = phi[T.col0 + c];
+        sdphi[c] = dphi[T.col0 + c];
+    }
+    __syncthreads();
+    bem_c64 sM = c64(0, 0), sL = c64(0, 0);
+    if (r < T.rows) {
+        const int64_t i = T.row0 + r;
+        const double px = obs[i], py = obs[nobs + i], pz = obs[2 * nobs + i];
+        for (int c = cg; c < T.cols; c += 4) {
+            bem_c64 L, Mk;
+            field_ops(k, px, py, pz, sv, sn, c, sa[c], &L, &Mk);
+            sM = c64_fma(Mk, sphi[c], sM);
+            sL = c64_fma(L, sdphi[c], sL);
+        }
+    }
+    redM[cg][r] = sM;
+    redL[cg][r] = sL;
+    __syncthreads();
+    if (cg == 0 && r < T.rows)
+        part[T.part + r] = c64_sub(c64_add(c64_add(redM[0][r], redM[1][r]), c64_add(redM[2][r], redM[3][r])),
+                                   c64_add(c64_add(redL[0][r], redL[1][r]), c64_add(redL[2][r], redL[3][r])));
+}
+
 __global__ void k_field_reduce(int64_t nobs, int nchunk, const bem_c64* __restrict__ part, bem_c64* __restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -376,6 +438,43 @@ __device__ __forceinline__ bem_c64 wave_sum(bem_c64 x)
     }
     return x;
 }
+
+// An admissible block's two low-rank operators (the M kernel's factors and
+// the L kernel's, from the ACA scratch): partial[i] = (U0 diag(A0) V0 phi_J)_i
+// - (U1 diag(A1) V1 dphi_J)_i, one workgroup per block; t_l = A_l (V_l . x_J)
+// by wavefront l mod 4 (lane-strided sums and the wavefront tree), then the
+// rows in order l = 0, 1, ...
+struct FieldLR {
+    const bem_c64 *U0, *V0, *A0, *U1, *V1, *A1;
+    int32_t r0, r1;
+    int64_t rows, col0, cols, part;
+};
+
+__global__ __launch_bounds__(kWG) void k_field_lr(const FieldLR* __restrict__ blocks, const bem_c64* __restrict__ phi,
+                                                  const bem_c64* __restrict__ dphi, bem_c64* __restrict__ part)
+{
+    extern __shared__ bem_c64 st[];   // t of the M kernel (r0), then of the L kernel (r1)
+    const FieldLR B = blocks[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int l = w; l < B.r0 + B.r1; l += kWG / 64) {
+        const bool second = l >= B.r0;
+        const int ll = second ? l - B.r0 : l;
+        const bem_c64* v = (second ? B.V1 : B.V0) + (int64_t)ll * B.cols;
+        const bem_c64* x = (second ? dphi : phi) + B.col0;
+        bem_c64 p = c64(0, 0);
+        for (int64_t j = lane; j < B.cols; j += 64) p = c64_fma(v[j], x[j], p);
+        p = wave_sum(p);
+        if (lane == 0) st[l] = c64_mul((second ? B.A1 : B.A0)[ll], p);
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < B.rows; i += kWG) {
+        bem_c64 a0 = c64(0, 0), a1 = c64(0, 0);
+        for (int l = 0; l < B.r0; l++) a0 = c64_fma(B.U0[(int64_t)l * B.rows + i], st[l], a0);
+        for (int l = 0; l < B.r1; l++) a1 = c64_fma(B.U1[(int64_t)l * B.rows + i], st[B.r0 + l], a1);
+        part[B.part + i] = c64_sub(a0, a1);
+    }
+}
+
 
 // ---- complex arithmetic over the ACA storage type: bem_c64 uses the
 // bit-exact FP64 contract of bem_detmath.h, bem_c32 the same formulas in FP32
@@ -479,6 +578,12 @@ struct AcaArgs {
     // (zero at launch) until nitems are taken; null: one workgroup per item
     int* queue;
     int nitems;
+    // field operators (FIELD instantiations): rows are observation points
+    // obs[d * nobs + i] (cluster order of the observation tree), columns
+    // boundary elements; kind 0 the M kernel (multiplies phi), 1 the L
+    // kernel (multiplies dphi)
+    const double* obs = nullptr;
+    int64_t nobs = 0;
 };
 
 // The dot products are processed kDotGroup at a time per wavefront: the
@@ -615,6 +720,28 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
     return true;
 }
 
+// field operator entry of observation point i and element j: the 7-point
+// rule of BemOperatorField (boundaryelementsolver.cpp:2319-2353) with the
+// points formed as k_field forms them; kind 0 implicitPhiMatrixField (M,
+// :2647-2683), kind 1 implicitDPhiMatrixField (L, :2685-2719)
+__device__ __forceinline__ bem_c64 field_entry(const AcaArgs& a, int64_t i, int64_t j)
+{
+    const DevMesh& M = a.M;
+    const int64_t n = M.n;
+    const double px = a.obs[i], py = a.obs[a.nobs + i], pz = a.obs[2 * a.nobs + i];
+    const double nqx = M.nx[j], nqy = M.ny[j], nqz = M.nz[j];
+    bem_c64 L = c64(0, 0), Mk = c64(0, 0);
+    for (int q = 0; q < 7; q++) {
+        double x1 = kTri7[q][0], x2 = kTri7[q][1];
+        double x3 = (1.0 - x2) - x1;
+        double qx = (x1 * M.vtx[j] + x2 * M.vtx[3 * n + j]) + x3 * M.vtx[6 * n + j];
+        double qy = (x1 * M.vtx[n + j] + x2 * M.vtx[4 * n + j]) + x3 * M.vtx[7 * n + j];
+        double qz = (x1 * M.vtx[2 * n + j] + x2 * M.vtx[5 * n + j]) + x3 * M.vtx[8 * n + j];
+        bem_field_point_accumulate(&L, &Mk, a.F.k, px, py, pz, nqx, nqy, nqz, qx, qy, qz, kTri7[q][2]);
+    }
+    return c64_scale(a.kind == 0 ? Mk : L, M.area[j]);
+}
+
 // self / near-singular entry of (i, j) from the table; t = pair index or
 // -1 for the self entry
 template <class C>
@@ -624,9 +751,10 @@ __device__ __forceinline__ C singular_entry(const AcaArgs& a, int64_t i, int64_t
 }
 
 // any entry (the pivot probe of the extra variant)
-template <bool PLANE, class C>
+template <bool PLANE, class C, bool FIELD = false>
 __device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
 {
+    if constexpr (FIELD) return from64<C>(field_entry(a, i, j));
     C e;
     if (regular_entry<PLANE>(a, i, j, &e)) return e;
     return singular_entry<C>(a, i, i == j ? -1 : near_find(a.M, i, (int32_t)j));
@@ -636,7 +764,7 @@ __device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
 // Regular entries first; the few self / near-singular entries of the row
 // (CSR row r0 + i) in a second pass, so that the main loop has no divergent
 // branch.
-template <int NT, class C, bool PLANE>
+template <int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
                         const C* U, const C* V, const C* A, C* dst, C* sw)
 {
@@ -645,8 +773,13 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     const int64_t gi = r0 + i;
     for (int64_t j = threadIdx.x; j < n; j += NT) {
         C e;
-        if (!regular_entry<PLANE>(a, gi, c0 + j, &e)) continue;
+        if constexpr (FIELD) e = from64<C>(field_entry(a, gi, c0 + j));
+        else if (!regular_entry<PLANE>(a, gi, c0 + j, &e)) continue;
         dst[j] = cx_sub(e, cross_sum<true>(sw, V, n, j, r));
+    }
+    if constexpr (FIELD) {   // no self or near-singular entries between observation points and elements
+        __syncthreads();
+        return;
     }
     const int64_t p0 = a.M.nr_ptr[gi], p1 = a.M.nr_ptr[gi + 1];
     for (int64_t t = p0 + threadIdx.x; t <= p1; t += NT) {
@@ -662,7 +795,7 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // column j of the block minus the current cross approximation, into dst;
 // the near-singular rows of column c0 + j are the near list of row c0 + j
 // (the distance criterion is symmetric)
-template <int NT, class C, bool PLANE>
+template <int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
                         const C* U, const C* V, const C* A, C* dst, C* sw)
 {
@@ -671,8 +804,13 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     const int64_t gj = c0 + j;
     for (int64_t i = threadIdx.x; i < m; i += NT) {
         C e;
-        if (!regular_entry<PLANE>(a, r0 + i, gj, &e)) continue;
+        if constexpr (FIELD) e = from64<C>(field_entry(a, r0 + i, gj));
+        else if (!regular_entry<PLANE>(a, r0 + i, gj, &e)) continue;
         dst[i] = cx_sub(e, cross_sum<false>(sw, U, m, i, r));
+    }
+    if constexpr (FIELD) {
+        __syncthreads();
+        return;
     }
     const int64_t p0 = a.M.nr_ptr[gj], p1 = a.M.nr_ptr[gj + 1];
     for (int64_t t = p0 + threadIdx.x; t <= p1; t += NT) {
@@ -690,7 +828,7 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // variant = BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra
 // variant = BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).
 // Dynamic LDS: sw[cap] (type C) then sdots[2 cap + 2] (FP64).
-template <bool EXTRA, int NT, class C, bool PLANE>
+template <bool EXTRA, int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred)
 {
     const int64_t m = a.rows[b], n = a.cols[b], r0 = a.row0[b], c0 = a.col0[b];
@@ -708,7 +846,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
     bem_c64 S = c64(0, 0);
     if (!EXTRA) {
         int64_t j = bem_rand_bounded(a.seed, 0, (uint64_t)b, 0, n);
-        aca_col<NT, C, PLANE>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
+        aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
         ev += m;
         int64_t i = -1;
         while (r < maxRank) {
@@ -718,12 +856,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             i = am.i;
             if (am.v <= tiny2 && r >= 1) break;
             C* row = V + (int64_t)r * n;
-            aca_row<NT, C, PLANE>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+            aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
             ev += n;
             ArgMax ar = block_argmax<NT>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
             C* col = U + (int64_t)r * m;
-            aca_col<NT, C, PLANE>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+            aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
             ev += m;
             C alpha = cx_inv(row[ar.i]);
             if (threadIdx.x == 0) A[r] = alpha;
@@ -740,7 +878,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
         bool done = false;
         for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
             int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
-            C pe = entry_any<PLANE, C>(a, r0 + i, c0 + j);
+            C pe = entry_any<PLANE, C, FIELD>(a, r0 + i, c0 + j);
             ev += 1;
             if (cx_abs2(pe) < tiny2) continue;
             C approx = cx_zero<C>();
@@ -748,7 +886,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 approx = cx_add(approx, cx_mul(cx_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
             if (cx_abs2(cx_sub(pe, approx)) < tol2 * cx_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
-            aca_col<NT, C, PLANE>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
+            aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
             ev += m;
             bool first = true;
             while (r < maxRank) {
@@ -759,12 +897,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 first = false;
                 if (cx_abs2(wcol[i]) <= tiny2 && r >= 1) break;
                 C* row = V + (int64_t)r * n;
-                aca_row<NT, C, PLANE>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+                aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
                 ev += n;
                 ArgMax ar = block_argmax<NT>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
                 C* col = U + (int64_t)r * m;
-                aca_col<NT, C, PLANE>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+                aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
                 ev += m;
                 C alpha = cx_inv(row[ar.i]);
                 if (threadIdx.x == 0) A[r] = alpha;
@@ -829,13 +967,13 @@ __device__ __forceinline__ void aca_item(const AcaArgs& a, int item, char* dyn, 
 // listed (frequency, block) item; 2: persistent workgroups taking items
 // from the queue counter (separate instantiations, so that the plain kernel
 // keeps its register budget)
-template <bool EXTRA, int NT, class C, bool PLANE, int MODE>
+template <bool EXTRA, int NT, class C, bool PLANE, int MODE, bool FIELD = false>
 __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) char dyn[];
     __shared__ ArgMax sred[NT / 64];
     if constexpr (MODE == 0) {
-        aca_block<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
+        aca_block<EXTRA, NT, C, PLANE, FIELD>(a, a.blist[blockIdx.x], dyn, sred);
         return;
     }
     if constexpr (MODE == 1) {
@@ -3313,24 +3451,48 @@ template <class C>
 void launch_aca(int q, bool extra, unsigned nblocks, size_t shm, hipStream_t st, const AcaArgs& aq)
 {
     const dim3 g(nblocks);
+    if constexpr (sizeof(C) == sizeof(bem_c64)) {
+        if (aq.obs) {   // field operators (FP64, one workgroup per block)
+            const int nt = q == 0 ? kHugeThreads<C> : q == 1 ? kMidThreads<C> : 64;
+            if (nt == 64) {
+                if (extra) hipLaunchKernelGGL((k_aca<true, 64, C, false, 0, true>), g, dim3(64), shm, st, aq);
+                else hipLaunchKernelGGL((k_aca<false, 64, C, false, 0, true>), g, dim3(64), shm, st, aq);
+            } else {
+                if (extra) hipLaunchKernelGGL((k_aca<true, 256, C, false, 0, true>), g, dim3(256), shm, st, aq);
+                else hipLaunchKernelGGL((k_aca<false, 256, C, false, 0, true>), g, dim3(256), shm, st, aq);
+            }
+            return;
+        }
+    }
     const bool plane = aq.M.rvtx != nullptr;
     if (q == 0) launch_aca_nt<kHugeThreads<C>, C>(extra, plane, g, shm, st, aq);
     else if (q == 1) launch_aca_nt<kMidThreads<C>, C>(extra, plane, g, shm, st, aq);
     else launch_aca_nt<64, C>(extra, plane, g, shm, st, aq);
 }
 
+// A block partition other than the context's (the field's observation x
+// boundary partition): host extents, device extent arrays, observation points
+struct PartView {
+    const std::vector<bemsetup::Block>* blocks;
+    const int64_t *row0, *rows, *col0, *cols;
+    const double* obs;
+    int64_t nobs;
+};
+
 int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
             const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv,
             const std::function<int()>& main_work, AcaScratch& S,
-            std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev)
+            std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev,
+            const PartView* pv = nullptr)
 {
-    int64_t nb = (int64_t)c->blocks.size();
+    const std::vector<bemsetup::Block>& blocks = pv ? *pv->blocks : c->blocks;
+    int64_t nb = (int64_t)blocks.size();
     S.ctx = c;
     S.uoff = c->vp.take<int64_t>(nb, 0); S.voff = c->vp.take<int64_t>(nb, 0); S.aoff = c->vp.take<int64_t>(nb, 0);
     int64_t tu = 0, tv = 0, ta = 0;
     int maxcap = 1;
     for (int32_t b : list) {
-        const auto& B = c->blocks[b];
+        const auto& B = blocks[b];
         S.uoff[b] = tu; S.voff[b] = tv; S.aoff[b] = ta;
         tu += B.rows * cap[b]; tv += B.cols * cap[b]; ta += cap[b];
         maxcap = std::max(maxcap, cap[b]);
@@ -3353,7 +3515,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     // tiles); within each class the largest blocks start first.
     std::vector<int32_t> cls[3];
     for (int32_t b : list) {
-        int64_t s2 = c->blocks[b].rows + c->blocks[b].cols;
+        int64_t s2 = blocks[b].rows + blocks[b].cols;
         const int pi = prec == BEM_PREC_FP32 ? 1 : 0;
         cls[s2 > c->cls_huge[pi] ? 0 : (s2 > c->cls_small[pi] ? 1 : 2)].push_back(b);
     }
@@ -3380,7 +3542,12 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     a.F32 = dev_freq32(c);
     a.kind = kind;
     a.blist = S.d_list;
-    a.row0 = c->d_row0; a.rows = c->d_rows; a.col0 = c->d_col0; a.cols = c->d_cols;
+    if (pv) {
+        a.row0 = pv->row0; a.rows = pv->rows; a.col0 = pv->col0; a.cols = pv->cols;
+        a.obs = pv->obs; a.nobs = pv->nobs;
+    } else {
+        a.row0 = c->d_row0; a.rows = c->d_rows; a.col0 = c->d_col0; a.cols = c->d_cols;
+    }
     a.uoff = S.d_uoff; a.voff = S.d_voff; a.aoff = S.d_aoff; a.cap = S.d_cap;
     a.U = S.U; a.V = S.V; a.A = S.A;
     a.rank_out = S.d_rank; a.flag_out = S.d_flag; a.evals_out = S.d_ev;
@@ -5769,6 +5936,210 @@ int bem_field_noalloc(bem_ctx_t c, int64_t nobs, const double* obs_dev, const vo
     hipLaunchKernelGGL(k_field_reduce, dim3(nblk(nobs, 256)), dim3(256), 0, c->stream, nobs, (int)nchunk, part,
                        (bem_c64*)out_dev);
     return launch_check(c, "k_field (noalloc)");
+}
+
+int bem_field_hmat(bem_ctx_t c, int64_t nobs, const double* obs3, int64_t nblocks, const int64_t* blocks5,
+                   double tol, int64_t maxrank, uint64_t seed, const void* phi, const void* dphi, double* out2,
+                   double* stats4)
+{
+    if (!c || nobs < 0 || (nobs && (!obs3 || !out2 || !blocks5)) || nblocks < 0 || !phi || !dphi || !(tol >= 0) ||
+        maxrank < 0)
+        return set_err(c, BEM_ERR_ARG, "bem_field_hmat: bad arguments");
+    if (!c->have_freq || c->n <= 0) return set_err(c, BEM_ERR_STATE, "bem_field_hmat: mesh or frequency missing");
+    if (c->mesh.rvtx) return set_err(c, BEM_ERR_ARG, "bem_field_hmat: not with an impedance plane (use bem_field)");
+    if (nobs == 0) return BEM_OK;
+    HIPC(c, hipSetDevice(c->device));
+    const double t0 = now_ms();
+    const int64_t n = c->n;
+    // the partition, checked against both index ranges and for covering
+    // every (observation, element) pair exactly once by area
+    std::vector<bemsetup::Block> blocks((size_t)nblocks);
+    double covered = 0;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const int64_t* q = blocks5 + 5 * b;
+        if (q[0] < 0 || q[1] < 1 || q[0] + q[1] > nobs || q[2] < 0 || q[3] < 1 || q[2] + q[3] > n)
+            return set_err(c, BEM_ERR_ARG, "bem_field_hmat: block outside the observation or element range");
+        blocks[b] = bemsetup::Block{q[0], q[1], q[2], q[3], q[4] != 0};
+        covered += (double)q[1] * (double)q[3];
+    }
+    if (covered != (double)nobs * (double)n)
+        return set_err(c, BEM_ERR_ARG, "bem_field_hmat: the blocks do not cover the observation x element range");
+    std::vector<void*> own;
+    auto fail = [&](int rc) {
+        free_list(c, own);
+        return rc;
+    };
+    int rc;
+    // observation points SoA, block extents
+    std::vector<double> soa(3 * nobs);
+    for (int64_t i = 0; i < nobs; i++)
+        for (int d = 0; d < 3; d++) soa[d * nobs + i] = obs3[3 * i + d];
+    std::vector<int64_t> ext(4 * nblocks);
+    for (int64_t b = 0; b < nblocks; b++) {
+        ext[b] = blocks[b].row0;
+        ext[nblocks + b] = blocks[b].rows;
+        ext[2 * nblocks + b] = blocks[b].col0;
+        ext[3 * nblocks + b] = blocks[b].cols;
+    }
+    double* d_obs;
+    int64_t* d_ext;
+    if ((rc = dupload(c, &d_obs, soa.data(), soa.size(), &own)) ||
+        (rc = dupload(c, &d_ext, ext.data(), ext.size(), &own)))
+        return fail(rc);
+    const PartView pv{&blocks, d_ext, d_ext + nblocks, d_ext + 2 * nblocks, d_ext + 3 * nblocks, d_obs, nobs};
+    // admissible blocks, largest first; the L kernel's pivot lists
+    std::vector<int32_t> adm;
+    for (int64_t b = 0; b < nblocks; b++)
+        if (blocks[b].adm) adm.push_back((int32_t)b);
+    std::stable_sort(adm.begin(), adm.end(), [&](int32_t x, int32_t y) {
+        return blocks[x].rows + blocks[x].cols > blocks[y].rows + blocks[y].cols;
+    });
+    int64_t* d_pivoff = nullptr;
+    int32_t* d_piv = nullptr;
+    if (!adm.empty()) {
+        bemsetup::Elements E;
+        E.n = n;
+        E.normal.resize(n);
+        for (int64_t e = 0; e < n; e++)
+            E.normal[e] = bemsetup::Vec3{c->h_normal[3 * e], c->h_normal[3 * e + 1], c->h_normal[3 * e + 2]};
+        std::vector<int64_t> pivoff;
+        std::vector<int32_t> piv;
+        bemsetup::build_field_pivot_lists(E, blocks, seed, pivoff, piv);
+        if (piv.empty()) piv.push_back(0);
+        if ((rc = dupload(c, &d_pivoff, pivoff.data(), pivoff.size(), &own)) ||
+            (rc = dupload(c, &d_piv, piv.data(), piv.size(), &own)))
+            return fail(rc);
+    }
+    // ACA of both field operators (partialPivotACA with implicitPhiMatrixField,
+    // partialPivotACAextra with implicitDPhiMatrixField, :2446-2453), first
+    // with a modest rank capacity, the overflowing blocks again with their full one
+    auto full_cap = [&](int32_t b) {
+        int64_t m = std::min(blocks[b].rows, blocks[b].cols);
+        if (maxrank > 0) m = std::min(m, maxrank);
+        return (int32_t)m;
+    };
+    AcaScratch S[2][2];
+    std::vector<int32_t> rank[2], over_of[2];
+    std::vector<int64_t> evs[2];
+    int64_t evaluated = 0;
+    int maxr = 0;
+    for (int kind = 0; kind < 2; kind++) {
+        std::vector<int32_t> cap((size_t)nblocks, 0), flag, rank2, flag2;
+        std::vector<int64_t> evs2;
+        for (int32_t b : adm) cap[b] = std::min<int32_t>(full_cap(b), kCapFirst);
+        if (adm.empty()) break;
+        if ((rc = run_aca(c, kind, BEM_PREC_FP64, tol, maxrank, seed, adm, cap, kind ? d_pivoff : nullptr,
+                          kind ? d_piv : nullptr, [] { return BEM_OK; }, S[kind][0], rank[kind], flag, evs[kind],
+                          &pv)))
+            return fail(rc);
+        std::vector<int32_t> over;
+        for (int32_t b : adm) {
+            if (flag[b] == 2) return fail(set_err(c, BEM_ERR_NAN, "bem_field_hmat: non-finite ACA value"));
+            if (flag[b] == 1) over.push_back(b);
+        }
+        if (!over.empty()) {
+            std::vector<int32_t> cap2((size_t)nblocks, 0);
+            for (int32_t b : over) cap2[b] = std::min<int32_t>(full_cap(b), kCapMax);
+            if ((rc = run_aca(c, kind, BEM_PREC_FP64, tol, maxrank, seed, over, cap2, kind ? d_pivoff : nullptr,
+                              kind ? d_piv : nullptr, [] { return BEM_OK; }, S[kind][1], rank2, flag2, evs2, &pv)))
+                return fail(rc);
+            for (int32_t b : over) {
+                if (flag2[b])
+                    return fail(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
+                                        "bem_field_hmat: ACA block " + std::to_string(b) +
+                                            (flag2[b] == 2 ? " non-finite" : " needs rank > 1024")));
+                rank[kind][b] = rank2[b];
+                evs[kind][b] = evs2[b];
+            }
+        }
+        over_of[kind] = std::vector<int32_t>((size_t)nblocks, 0);
+        for (int32_t b : over) over_of[kind][b] = 1;
+        for (int32_t b : adm) {
+            evaluated += evs[kind][b];
+            maxr = std::max(maxr, rank[kind][b]);
+        }
+    }
+    // partial vectors: per admissible block its rows, per dense block one
+    // per 64 x 64 tile; each row sums its partials in partition order
+    std::vector<FieldLR> lr;
+    std::vector<FieldTile> tiles;
+    std::vector<std::vector<int64_t>> cover((size_t)nobs);
+    int64_t ptot = 0, maxr2 = 1;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const auto& B = blocks[b];
+        if (B.adm) {
+            FieldLR L{};
+            const bem_c64* f[2][3];
+            int32_t r[2];
+            for (int kind = 0; kind < 2; kind++) {
+                const AcaScratch& Sk = S[kind][over_of[kind][b]];
+                f[kind][0] = (const bem_c64*)Sk.U + Sk.uoff[b];
+                f[kind][1] = (const bem_c64*)Sk.V + Sk.voff[b];
+                f[kind][2] = (const bem_c64*)Sk.A + Sk.aoff[b];
+                r[kind] = rank[kind][b];
+            }
+            L.U0 = f[0][0]; L.V0 = f[0][1]; L.A0 = f[0][2];
+            L.U1 = f[1][0]; L.V1 = f[1][1]; L.A1 = f[1][2];
+            L.r0 = r[0];
+            L.r1 = r[1];
+            L.rows = B.rows;
+            L.col0 = B.col0;
+            L.cols = B.cols;
+            L.part = ptot;
+            maxr2 = std::max<int64_t>(maxr2, (int64_t)r[0] + r[1]);
+            lr.push_back(L);
+            for (int64_t i = 0; i < B.rows; i++) cover[B.row0 + i].push_back(ptot + i);
+            ptot += B.rows;
+            continue;
+        }
+        for (int64_t r0 = 0; r0 < B.rows; r0 += kTile) {
+            const int32_t rr = (int32_t)std::min<int64_t>(kTile, B.rows - r0);
+            for (int64_t c0 = 0; c0 < B.cols; c0 += kTile) {
+                const int32_t cc = (int32_t)std::min<int64_t>(kTile, B.cols - c0);
+                tiles.push_back(FieldTile{B.row0 + r0, B.col0 + c0, ptot, rr, cc});
+                for (int64_t i = 0; i < rr; i++) cover[B.row0 + r0 + i].push_back(ptot + i);
+                ptot += rr;
+            }
+        }
+    }
+    // (a row's partials: tiles of one dense block in column order, blocks in partition order)
+    std::vector<int64_t> gptr((size_t)nobs + 1, 0), gidx;
+    for (int64_t i = 0; i < nobs; i++) gptr[i + 1] = gptr[i] + (int64_t)cover[i].size();
+    gidx.reserve(gptr[nobs]);
+    for (auto& v : cover) gidx.insert(gidx.end(), v.begin(), v.end());
+    FieldLR* d_lr = nullptr;
+    FieldTile* d_tiles = nullptr;
+    int64_t *d_gptr, *d_gidx;
+    bem_c64 *d_part, *d_out;
+    if ((!lr.empty() && (rc = dupload(c, &d_lr, lr.data(), lr.size(), &own))) ||
+        (!tiles.empty() && (rc = dupload(c, &d_tiles, tiles.data(), tiles.size(), &own))) ||
+        (rc = dupload(c, &d_gptr, gptr.data(), gptr.size(), &own)) ||
+        (rc = dupload(c, &d_gidx, gidx.data(), std::max<size_t>(gidx.size(), 1), &own)) ||
+        (rc = dalloc(c, &d_part, (size_t)std::max<int64_t>(ptot, 1), &own)) ||
+        (rc = dalloc(c, &d_out, (size_t)nobs, &own)))
+        return fail(rc);
+    {
+        Timer t(c, 6);
+        if (!tiles.empty())
+            hipLaunchKernelGGL(k_field_dense, dim3((unsigned)tiles.size()), dim3(kWG), 0, c->stream, c->mesh, c->k, nobs,
+                               d_obs, d_tiles, (const bem_c64*)phi, (const bem_c64*)dphi, d_part);
+        if (!lr.empty())
+            hipLaunchKernelGGL(k_field_lr, dim3((unsigned)lr.size()), dim3(kWG), (size_t)maxr2 * sizeof(bem_c64),
+                               c->stream, d_lr, (const bem_c64*)phi, (const bem_c64*)dphi, d_part);
+    }
+    hipLaunchKernelGGL(k_mvm_gather, dim3(nblk(16 * nobs, 256)), dim3(256), 0, c->stream, nobs, d_gptr, d_gidx, d_part,
+                       d_out, 0);
+    if ((rc = launch_check(c, "bem_field_hmat"))) return fail(rc);
+    hipError_t e = hipMemcpyAsync(out2, d_out, (size_t)nobs * sizeof(bem_c64), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_err(c, BEM_ERR_HIP, std::string("bem_field_hmat: ") + hipGetErrorString(e)));
+    if (stats4) {
+        stats4[0] = now_ms() - t0;
+        stats4[1] = (double)adm.size();
+        stats4[2] = (double)maxr;
+        stats4[3] = (double)evaluated;
+    }
+    return fail(BEM_OK);
 }
 
 int bem_reconstruct(bem_ctx_t c, const void* x, double rho, double freq, void* phi, void* dphi, void* pr)

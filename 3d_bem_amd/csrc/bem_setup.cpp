@@ -178,9 +178,21 @@ void reflect_elements(int64_t n, const double* nodes9, const double point[3], co
 
 ClusterTree build_cluster_tree(int64_t n, const double* nodes9, int64_t leaf)
 {
+    std::vector<double> mid(3 * n);
+    for (int64_t e = 0; e < n; e++) {
+        const Vec3 c = centroid(tri_of(nodes9 + 9 * e));
+        mid[3 * e] = c.x;
+        mid[3 * e + 1] = c.y;
+        mid[3 * e + 2] = c.z;
+    }
+    return build_cluster_tree_points(n, mid.data(), leaf);
+}
+
+ClusterTree build_cluster_tree_points(int64_t n, const double* pts3, int64_t leaf)
+{
     ClusterTree T;
     std::vector<Vec3> mid(n);
-    for (int64_t e = 0; e < n; e++) mid[e] = centroid(tri_of(nodes9 + 9 * e));
+    for (int64_t e = 0; e < n; e++) mid[e] = v3(pts3[3 * e], pts3[3 * e + 1], pts3[3 * e + 2]);
     std::vector<std::vector<int64_t>> members;
     const double inf = (double)std::numeric_limits<float>::infinity();  // Cuboid() (cuboid.h:17-20)
     // explicit stack instead of recursion; children are created son1 first so
@@ -266,14 +278,19 @@ double box_distance(const Cluster& a, const Cluster& b)
 
 std::vector<Block> build_partition(const ClusterTree& T, double eta)
 {
+    return build_partition(T, T, eta);
+}
+
+std::vector<Block> build_partition(const ClusterTree& TR, const ClusterTree& TC, double eta)
+{
     std::vector<Block> out;
-    if (T.clusters.empty()) return out;
+    if (TR.clusters.empty() || TC.clusters.empty()) return out;
     std::vector<std::pair<int, int>> st{{0, 0}};
     while (!st.empty()) {
         auto [rc, cc] = st.back();
         st.pop_back();
-        const Cluster& R = T.clusters[rc];
-        const Cluster& C = T.clusters[cc];
+        const Cluster& R = TR.clusters[rc];
+        const Cluster& C = TC.clusters[cc];
         double dist = box_distance(R, C);
         bool adm = dist != 0 && std::min(norm(R.bmax - R.bmin), norm(C.bmax - C.bmin)) / dist < eta;
         if (adm || R.leaf || C.leaf) {
@@ -444,6 +461,34 @@ void build_pivot_lists(const Elements& E, const std::vector<Block>& blocks, uint
                 int64_t ri = bem_rand_bounded(seed, 1, (uint64_t)b, draw++, (int64_t)rg.size());
                 int64_t cj = bem_rand_bounded(seed, 1, (uint64_t)b, draw++, (int64_t)cg.size());
                 per[b].push_back((int32_t)(rg[ri] - B.row0));
+                per[b].push_back((int32_t)(cg[cj] - B.col0));
+            }
+    }
+    offsets.assign(nb + 1, 0);
+    for (size_t b = 0; b < nb; b++) offsets[b + 1] = offsets[b] + (int64_t)per[b].size() / 2;
+    pairs.resize(2 * offsets[nb]);
+    for (size_t b = 0; b < nb; b++) std::copy(per[b].begin(), per[b].end(), pairs.begin() + 2 * offsets[b]);
+}
+
+void build_field_pivot_lists(const Elements& E, const std::vector<Block>& blocks, uint64_t seed,
+                             std::vector<int64_t>& offsets, std::vector<int32_t>& pairs)
+{
+    size_t nb = blocks.size();
+    std::vector<std::vector<int32_t>> per(nb);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t b = 0; b < (int64_t)nb; b++) {
+        const Block& B = blocks[b];
+        if (!B.adm) continue;
+        auto C = orientation_groups(E, B.col0, B.cols);
+        // one random row per column group, drawn first (:2773-2777); then
+        // for every (row, group) a random member of the group (:2781-2789)
+        uint64_t draw = 0;
+        std::vector<int64_t> rows(C.size());
+        for (auto& r : rows) r = bem_rand_bounded(seed, kFieldPivotStream, (uint64_t)b, draw++, B.rows);
+        for (int64_t r : rows)
+            for (auto& cg : C) {
+                int64_t cj = bem_rand_bounded(seed, kFieldPivotStream, (uint64_t)b, draw++, (int64_t)cg.size());
+                per[b].push_back((int32_t)r);
                 per[b].push_back((int32_t)(cg[cj] - B.col0));
             }
     }

@@ -59,10 +59,17 @@ struct ClusterTree {
     std::vector<int64_t> perm;
 };
 ClusterTree build_cluster_tree(int64_t n, const double* nodes9, int64_t leaf);
+// the same over points (ClusterTree(QVector<Eigen::Vector3d>*) and the
+// observation-triangle tree, whose members are their midpoints,
+// clustertree.h:96-113; pts3: n x 3)
+ClusterTree build_cluster_tree_points(int64_t n, const double* pts3, int64_t leaf);
 
 // HMatrix::populateBlockClusterTree (hmatrix.cpp:922-1007) with the
 // admissibility min(diam)/dist < eta (hmatrix.cpp:1052-1068).
 std::vector<Block> build_partition(const ClusterTree& T, double eta);
+// rows from TR, columns from TC (HMatrix(rowTree, columnTree, true), the
+// field's observation x boundary partition, boundaryelementsolver.cpp:2425)
+std::vector<Block> build_partition(const ClusterTree& TR, const ClusterTree& TC, double eta);
 
 // Near-singular pairs |p_i - p_j| < 4 hbar, i != j
 // (boundaryelementsolver.cpp:851), as CSR over rows with columns sorted, plus
@@ -92,5 +99,16 @@ std::vector<double> build_self(const Elements& E);
 // block origin; offsets per block (non-admissible blocks get empty lists).
 void build_pivot_lists(const Elements& E, const std::vector<Block>& blocks, uint64_t seed,
                        std::vector<int64_t>& offsets, std::vector<int32_t>& pairs);
+
+// Pivot lists of the field's partialPivotACAextra
+// (getNormalFilteredPivotIndicesForField, boundaryelementsolver.cpp:
+// 2721-2807): orientation groups of the block's boundary columns only (cone
+// 0.44 rad, at most 20 groups); as many random observation rows as groups,
+// and for every (row, group) pair a random member of the group.  Blocks
+// index rows by observation point and columns by element of E.  Draws from
+// stream kFieldPivotStream of the counter-based generator.
+constexpr uint64_t kFieldPivotStream = 2;
+void build_field_pivot_lists(const Elements& E, const std::vector<Block>& blocks, uint64_t seed,
+                             std::vector<int64_t>& offsets, std::vector<int32_t>& pairs);
 
 }  // namespace bemsetup

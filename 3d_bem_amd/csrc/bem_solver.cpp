@@ -288,6 +288,52 @@ std::vector<cplx> BoundaryElementSolver::calculateFieldSolution(const std::vecto
     dphi.upload(dPhiSolution);
     std::vector<cplx> field(nobs);
     check(ctx_, bem_field(ctx_, nobs, obs3.data(), phi.data(), dphi.data(), (double*)field.data()), "bem_field");
+    return fieldToPressure(obs3, field);
+}
+
+std::vector<cplx> BoundaryElementSolver::calculateFieldSolutionFast(const std::vector<double>& obs3,
+                                                                    double relativeError, long maxRank, int64_t leaf,
+                                                                    double eta, uint64_t seed, double* stats4)
+{
+    auto& E = boundaryElements;
+    if ((int64_t)phiSolution.size() != E.n || (int64_t)dPhiSolution.size() != E.n)
+        throw Error(BEM_ERR_STATE, "calculateFieldSolutionFast: no boundary solution yet");
+    if (!E.impedancePlanes.empty())
+        throw Error(BEM_ERR_ARG, "calculateFieldSolutionFast: not with an impedance plane (use the direct sum)");
+    if (leaf < 1 || !(eta > 0)) throw Error(BEM_ERR_ARG, "calculateFieldSolutionFast: leaf >= 1 and eta > 0");
+    const int64_t nobs = (int64_t)obs3.size() / 3;
+    if (nobs == 0) return {};
+    // the observation points' cluster tree (ClusterTree(&triangles) on the
+    // observation triangles' midpoints, :2420) against the boundary tree of
+    // the solve (HMatrix(fieldTree, boundaryTree, true), :2425)
+    bemsetup::ClusterTree T = bemsetup::build_cluster_tree_points(nobs, obs3.data(), leaf);
+    std::vector<double> obsc(3 * nobs);
+    for (int64_t i = 0; i < nobs; i++)
+        for (int d = 0; d < 3; d++) obsc[3 * i + d] = obs3[3 * T.perm[i] + d];
+    std::vector<bemsetup::Block> blocks = bemsetup::build_partition(T, clusterTree->tree_, eta);
+    std::vector<int64_t> b5(5 * blocks.size());
+    for (size_t b = 0; b < blocks.size(); b++) {
+        b5[5 * b] = blocks[b].row0;
+        b5[5 * b + 1] = blocks[b].rows;
+        b5[5 * b + 2] = blocks[b].col0;
+        b5[5 * b + 3] = blocks[b].cols;
+        b5[5 * b + 4] = blocks[b].adm;
+    }
+    DeviceVector phi(ctx_, E.n), dphi(ctx_, E.n);
+    phi.upload(phiSolution);
+    dphi.upload(dPhiSolution);
+    std::vector<cplx> fc(nobs), field(nobs);
+    check(ctx_, bem_field_hmat(ctx_, nobs, obsc.data(), (int64_t)blocks.size(), b5.data(), relativeError,
+                               (int64_t)maxRank, seed, phi.data(), dphi.data(), (double*)fc.data(), stats4),
+          "bem_field_hmat");
+    for (int64_t i = 0; i < nobs; i++) field[T.perm[i]] = fc[i];
+    return fieldToPressure(obs3, field);
+}
+
+std::vector<cplx> BoundaryElementSolver::fieldToPressure(const std::vector<double>& obs3,
+                                                         const std::vector<cplx>& field) const
+{
+    const int64_t nobs = (int64_t)field.size();
     // incident field of the sources (sourcePhiTerm, :3443-3450; plane waves A e^{ik d.x})
     const double k = wavenumber;
     const bem_c64 s = c64(0.0, airDensity * (2.0 * BEM_PI) * frequency);   // p = i rho 2 pi f phi (:2234-2240)
@@ -1103,6 +1149,25 @@ int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* p
     try {
         std::vector<double> obs(obs3, obs3 + 3 * nobs);
         std::vector<bem::cplx> p = s->s->calculateFieldSolution(obs);
+        if (nobs) std::memcpy(pressure2, p.data(), nobs * sizeof(bem::cplx));
+    } catch (const bem::Error& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return BEM_ERR_HIP;
+    }
+    return BEM_OK;
+}
+
+int bem_solver_field_hmat(bem_solver_t s, int64_t nobs, const double* obs3, double aca_tol, int64_t max_rank,
+                          int64_t leaf, double eta, uint64_t seed, double* pressure2, double* stats4)
+{
+    if (!s || !s->s || nobs < 0 || (nobs && (!obs3 || !pressure2)) || !(aca_tol >= 0) || max_rank < 0)
+        return BEM_ERR_ARG;
+    try {
+        std::vector<double> obs(obs3, obs3 + 3 * nobs);
+        std::vector<bem::cplx> p = s->s->calculateFieldSolutionFast(obs, aca_tol, (long)max_rank, leaf, eta, seed, stats4);
         if (nobs) std::memcpy(pressure2, p.data(), nobs * sizeof(bem::cplx));
     } catch (const bem::Error& e) {
         s->err = e.msg;

@@ -186,6 +186,22 @@ public:
     // boundaryelementsolver.cpp:2192-2316): sound pressure at observation
     // points from the last boundary solution, incident sources included
     std::vector<cplx> calculateFieldSolution(const std::vector<double>& observationPoints3);
+    // calculateFieldSolutionFast (boundaryelementsolver.cpp:2397-2476): the
+    // boundary term through the observation x boundary H-matrix -- a cluster
+    // tree over the observation points (leaf; the reference's is 50), the
+    // admissible blocks as ACA of the M kernel times phi minus ACA (normal-
+    // filtered pivots) of the L kernel times dphi, relativeError / maxRank
+    // (fieldACARelativeError, default 0.01, bemparameters.h:29-30), the
+    // dense blocks directly -- on the GPU (bem_field_hmat); incident sources
+    // and pressure as calculateFieldSolution.  No impedance plane.
+    // stats4: ms, admissible blocks, largest rank, evaluated entries.
+    std::vector<cplx> calculateFieldSolutionFast(const std::vector<double>& observationPoints3, double relativeError,
+                                                 long maxRank, int64_t leaf, double eta, uint64_t seed,
+                                                 double* stats4);
+    // the incident field of the sources added to the boundary term, times
+    // i rho 2 pi f: the sound pressure at the points
+    std::vector<cplx> fieldToPressure(const std::vector<double>& observationPoints3,
+                                      const std::vector<cplx>& field) const;
     BoundaryElements boundaryElements;     // cluster order after construction
     BemSolverParameters bemSolverParameters;
     std::vector<PointSource> pointSources;

@@ -15,9 +15,18 @@ FP32; self and near-singular entries and (with --refine 1, the default) the
 whole near field in FP64; GMRES in FP64 arithmetic iterating on the FP32
 operator inside an iterative refinement whose residuals b - A x are taken in
 FP64 with the FP64 near field and the stored low-rank factors (--refine 2:
-with FP64 H-matrices; 0: no refinement); the batch's points solved
-concurrently on --lanes solver lanes (default 8, with 16 hardware queues:
-the benchmark runs as a child process with GPU_MAX_HW_QUEUES=16).  The plain
+with FP64 H-matrices; 0: no refinement).  The batch's points go in chunks
+of --sweep-batch (default 4): Phi and DPhi of a chunk's frequencies from
+one batched assembly each (one ACA launch per size class over the
+flattened frequency x block items, the near field of both operators from
+one quadrature), and one lockstep GMRES over the chunk whose Arnoldi
+columns replay a captured HIP graph with every operator's product fused
+into one launch per kernel; the chunks run concurrently on up to --lanes
+solver lanes (two for the default 8 points), with the ACA schedule tuned on
+the first chunk during the warm-up.  The earlier scheduling, one point per
+lane on eight lanes, is timed beside it (lanes_leg); the benchmark runs as
+a child process with GPU_MAX_HW_QUEUES=16 so that lanes do not share
+hardware queues.  The plain
 FP64 path (one point after the other) is timed beside it on the same points
 (fp64_value, speedup_vs_fp64), the unrefined FP32 form too (fp32_norefine);
 the 1000 Hz surface pressures are compared with the FP64 H-matrix path, with
@@ -355,11 +364,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=8, help="frequency points per step")
     ap.add_argument("--lanes", type=int, default=8, help="concurrent solver lanes of the FP32 form")
-    ap.add_argument("--sweep-batch", type=int, default=0, metavar="B",
-                    help="FP32 form: frequency chunks of B points with batched assembly and lockstep GMRES on one "
-                         "context instead of the lanes (0: lanes)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="batched sweeps: tune the ACA schedule on the first chunk (in the warm-up)")
+    ap.add_argument("--sweep-batch", type=int, default=4, metavar="B",
+                    help="FP32 form: frequency chunks of B points, each with batched assembly and one lockstep GMRES, "
+                         "the chunks on up to --lanes lanes (0: one point per lane)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched sweeps: keep the built-in ACA schedule (default: tuned on the first chunk, in the "
+                         "warm-up)")
     ap.add_argument("--leg-batch", type=int, default=8, metavar="B",
                     help="chunk size of the batched comparison leg (when the headline runs on lanes)")
     ap.add_argument("--no-batch-leg", action="store_true",
@@ -424,7 +434,7 @@ def main():
     sb = args.sweep_batch if batched else 0
     refine = args.refine if prec == bem.FP32 else 0
     S.set_refinement(refine)
-    S.set_autotune(args.autotune)
+    S.set_autotune(batched and not args.no_autotune)
 
     pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sb)
     rstats = S.refine_stats()
@@ -453,7 +463,7 @@ def main():
                                + ("" if args.plane is None else f", rigid plane {args.plane:g} m below the centre"),
                    "triangles": n, "points_per_step": len(freqs), "lanes": lanes, "sweep_batch": sb, "setup_s": round(setup_s, 3),
                    "refine": refine, "refine_stats_last_point_lane0": rstats,
-                   "aca_schedule": S.aca_schedule(prec) if args.autotune else None,
+                   "aca_schedule": S.aca_schedule(prec),
                    "ms_per_point": round(1e3 * elapsed / points, 3),
                    "gmres_iters": int(its[ic, 0]), "residual_estimate": st["residual"],
                    "max_rank_phi": st["max_rank_phi"], "max_rank_dphi": st["max_rank_dphi"],

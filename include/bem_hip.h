@@ -306,6 +306,21 @@ int bem_reconstruct_g(bem_ctx_t ctx, const void* x_dev, const void* g_dev, doubl
    calculateFieldSolutionFast, :2397-2570, approximates this sum).  obs3:
    nobs x 3 host; phi_dev, dphi_dev: n complex on the device (cluster order);
    out2: nobs complex host.  Sources are not included. */
+/* The boundary term of calculateFieldSolutionFast (boundaryelementsolver.cpp:
+   2397-2476) on the GPU: obs3 (nobs x 3) the observation points in the
+   order of their cluster tree, blocks5 (nblocks x (row0, rows, col0, cols,
+   admissible)) the observation x element partition (minPartition order of
+   HMatrix(fieldTree, boundaryTree, true)).  Admissible blocks: ACA of
+   implicitPhiMatrixField (the M kernel, :2647-2683) and, with the pivots of
+   getNormalFilteredPivotIndicesForField (:2721-2807, counter-based draws
+   from `seed`), of implicitDPhiMatrixField (the L kernel, :2685-2719), to
+   tol / max_rank; the block's term U A V phi - U' A' V' dphi right after
+   (the blocks are not kept, as in the reference).  Dense blocks: M phi -
+   L dphi directly.  Row sums in partition order; out2 in the points' order.
+   No impedance plane (bem_field covers it).  stats4 as bem_solver_field_hmat. */
+int bem_field_hmat(bem_ctx_t ctx, int64_t nobs, const double* obs3, int64_t nblocks, const int64_t* blocks5,
+                   double tol, int64_t max_rank, uint64_t seed, const void* phi_dev, const void* dphi_dev,
+                   double* out2, double* stats4);
 int bem_field(bem_ctx_t ctx, int64_t nobs, const double* obs3, const void* phi_dev, const void* dphi_dev,
               double* out2);
 
@@ -413,6 +428,16 @@ int bem_solver_set_impedance_plane(bem_solver_t s, const double* point3, const d
    the direct sum, boundaryelementsolver.cpp:2192-2316).  pressure2: nobs
    complex. */
 int bem_solver_field(bem_solver_t s, int64_t nobs, const double* obs3, double* pressure2);
+/* calculateFieldSolutionFast (boundaryelementsolver.cpp:2397-2476, the
+   reference's default field path, bemparameters.h:29-30): the sound
+   pressure at nobs points (input order) from the last boundary solution,
+   the boundary term through an observation x boundary H-matrix (cluster
+   tree over the points with leaf size `leaf`, admissibility eta, ACA to
+   aca_tol / max_rank; bem_field_hmat) plus the incident sources.  No
+   impedance plane.  stats4 (optional): ms, admissible blocks, largest rank,
+   evaluated entries. */
+int bem_solver_field_hmat(bem_solver_t s, int64_t nobs, const double* obs3, double aca_tol, int64_t max_rank,
+                          int64_t leaf, double eta, uint64_t seed, double* pressure2, double* stats4);
 
 /* Boundary mesh from a GMSH .msh file (mshReader::readMsh,
    mshreader.cpp:10-428): ASCII format 2 (as the reference parses) or 4.1.

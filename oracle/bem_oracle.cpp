@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <vector>
@@ -462,6 +463,34 @@ struct TreeBuild {
     }
 };
 
+/* a cluster tree on the points mid, then depth-first reordering
+   (clustertree.cpp:265-283): the depth-first leaf order gives perm
+   (perm[new] = old); every cluster becomes a contiguous range */
+void build_tree(const std::vector<V3>& mid, int64_t leaf, std::vector<Cluster>& clusters, std::vector<int64_t>& perm)
+{
+    const int64_t n = (int64_t)mid.size();
+    std::vector<int64_t> all(n);
+    for (int64_t e = 0; e < n; e++) all[e] = e;
+    TreeBuild tb{mid, leaf, clusters, {}};
+    tb.make(std::move(all), 0);
+    int64_t pos = 0;
+    struct Rec {
+        std::vector<Cluster>& cl; std::vector<int64_t>& perm; TreeBuild* tb; int64_t* pos;
+        void go(int c)
+        {
+            Cluster& C = cl[c];
+            if (C.leaf) {
+                C.start = *pos;
+                for (int64_t e : tb->idx[c]) perm.push_back(e), (*pos)++;
+            } else {
+                go(C.son1); go(C.son2);
+                C.start = cl[C.son1].start;
+            }
+        }
+    } rec{clusters, perm, &tb, &pos};
+    rec.go(0);
+}
+
 /* HMatrix::clusterDistance (hmatrix.cpp:1070-1129) */
 double cluster_distance(const Cluster& c1, const Cluster& c2)
 {
@@ -491,7 +520,28 @@ double cluster_distance(const Cluster& c1, const Cluster& c2)
 /* HMatrix::clusterSize (hmatrix.cpp:1131-1135) */
 double cluster_size(const Cluster& c) { return norm(sub(c.bmax, c.bmin)); }
 
-/* HMatrix::createBlockClusterChildren + isBlockAdmissible (hmatrix.cpp:958-1068) */
+/* HMatrix::createBlockClusterChildren + isBlockAdmissible (hmatrix.cpp:958-1068),
+   rows from the tree R, columns from the tree C */
+void make_blocks2(const std::vector<Cluster>& R, const std::vector<Cluster>& C, int rc, int cc, double eta,
+                  std::vector<Block>& out)
+{
+    const Cluster& r = R[rc];
+    const Cluster& c = C[cc];
+    double dist = cluster_distance(r, c);
+    bool adm = false;
+    if (dist != 0) adm = std::min(cluster_size(r), cluster_size(c)) / dist < eta;
+    if (adm || r.leaf || c.leaf) {
+        Block b;
+        b.row0 = r.start; b.rows = r.size; b.col0 = c.start; b.cols = c.size; b.adm = adm; b.rc = rc; b.cc = cc;
+        out.push_back(b);
+        return;
+    }
+    make_blocks2(R, C, r.son1, c.son1, eta, out);
+    make_blocks2(R, C, r.son1, c.son2, eta, out);
+    make_blocks2(R, C, r.son2, c.son1, eta, out);
+    make_blocks2(R, C, r.son2, c.son2, eta, out);
+}
+
 void make_blocks(orc_problem& P, int rc, int cc, double eta)
 {
     const Cluster& r = P.clusters[rc];
@@ -557,13 +607,16 @@ struct ACAState {
     cplx S = {0, 0};
 };
 
-void eval_row(const orc_problem& P, int kind, const Block& B, int64_t i, cplx* row)
+/* the implicit matrix an ACA reads: entry (row, column), global indices */
+using EntryFn = std::function<cplx(int64_t, int64_t)>;
+
+void eval_row(const EntryFn& E, const Block& B, int64_t i, cplx* row)
 {
-    for (int64_t j = 0; j < B.cols; j++) row[j] = entry(P, kind, B.row0 + i, B.col0 + j);
+    for (int64_t j = 0; j < B.cols; j++) row[j] = E(B.row0 + i, B.col0 + j);
 }
-void eval_col(const orc_problem& P, int kind, const Block& B, int64_t j, cplx* col)
+void eval_col(const EntryFn& E, const Block& B, int64_t j, cplx* col)
 {
-    for (int64_t i = 0; i < B.rows; i++) col[i] = entry(P, kind, B.row0 + i, B.col0 + j);
+    for (int64_t i = 0; i < B.rows; i++) col[i] = E(B.row0 + i, B.col0 + j);
 }
 /* row_j -= sum_l (U[i,l] alpha_l) V[l,j] */
 void sub_row(ACAState& st, int64_t i, cplx* row)
@@ -615,13 +668,13 @@ double append_cross(ACAState& st, const cplx* col, const cplx* row, cplx alpha)
 }
 
 /* BlockCluster::partialPivotACA (hmatrix.cpp:471-593) */
-int64_t aca_plain(const orc_problem& P, int kind, const Block& B, int64_t bidx, double tol, int64_t maxrank,
-                  uint64_t seed, ACAState& st)
+int64_t aca_plain(const EntryFn& E, const Block& B, int64_t bidx, double tol, int64_t maxrank, uint64_t seed,
+                  ACAState& st)
 {
     int64_t m = B.rows, n = B.cols, ev = 0;
     std::vector<cplx> row(n), col(m);
     int64_t j = bem_rand_bounded(seed, 0, (uint64_t)bidx, 0, n);
-    eval_col(P, kind, B, j, col.data()); ev += m;
+    eval_col(E, B, j, col.data()); ev += m;
     int64_t i = 0;
     double tiny2 = TINY * TINY;
     while (st.rank < maxrank) {
@@ -629,12 +682,12 @@ int64_t aca_plain(const orc_problem& P, int kind, const Block& B, int64_t bidx, 
         double mv;
         i = argmax_abs2(col.data(), m, &mv);
         if (mv <= tiny2 && st.rank >= 1) break;
-        eval_row(P, kind, B, i, row.data()); ev += n;
+        eval_row(E, B, i, row.data()); ev += n;
         sub_row(st, i, row.data());
         double mr;
         j = argmax_abs2(row.data(), n, &mr);
         if (mr <= tiny2 && st.rank >= 1) break;
-        eval_col(P, kind, B, j, col.data()); ev += m;
+        eval_col(E, B, j, col.data()); ev += m;
         cplx alpha = c64_div(c64(1.0, 0.0), row[j]);
         sub_col(st, j, col.data());
         double gain = append_cross(st, col.data(), row.data(), alpha);
@@ -689,23 +742,63 @@ std::vector<std::pair<int64_t, int64_t>> pivot_list(const orc_problem& P, const 
     return out;
 }
 
-/* BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745) */
-int64_t aca_extra(const orc_problem& P, int kind, const Block& B, int64_t bidx, double tol, int64_t maxrank,
-                  uint64_t seed, ACAState& st)
+/* getNormalFilteredPivotIndicesForField (boundaryelementsolver.cpp:2721-2807):
+   orientation groups of the block's boundary columns only; as many random
+   rows as groups, drawn first; then for every (row, group) a random member
+   of the group.  Draws from stream 2 of the counter-based generator. */
+std::vector<std::pair<int64_t, int64_t>> field_pivot_list(const orc_problem& P, const Block& B, int64_t bidx,
+                                                          uint64_t seed)
+{
+    auto C = normal_groups(P, B.col0, B.cols);
+    uint64_t draw = 0;
+    std::vector<int64_t> rows(C.size());
+    for (auto& r : rows) r = bem_rand_bounded(seed, 2, (uint64_t)bidx, draw++, B.rows);
+    std::vector<std::pair<int64_t, int64_t>> out;
+    for (int64_t r : rows)
+        for (auto& g : C) {
+            int64_t cj = bem_rand_bounded(seed, 2, (uint64_t)bidx, draw++, (int64_t)g.size());
+            out.push_back({r, g[cj] - B.col0});
+        }
+    return out;
+}
+
+/* implicitPhiMatrixField (kind 0: M, :2647-2683) / implicitDPhiMatrixField
+   (kind 1: L, :2685-2719) of observation point p and element j: the 7-point
+   rule of BemOperatorField (:2319-2353) */
+cplx field_entry(const orc_problem& P, int kind, V3 p, int64_t j)
+{
+    static const double rule[7][3] = BEM_TRI7;
+    const Tri& t = P.tri[j];
+    const V3 nq = P.normal[j];
+    cplx L = c64(0, 0), M = c64(0, 0);
+    for (int q = 0; q < 7; q++) {
+        double x1 = rule[q][0], x2 = rule[q][1];
+        double x3 = (1.0 - x2) - x1;
+        double qx = (x1 * t.a.x + x2 * t.b.x) + x3 * t.c.x;
+        double qy = (x1 * t.a.y + x2 * t.b.y) + x3 * t.c.y;
+        double qz = (x1 * t.a.z + x2 * t.b.z) + x3 * t.c.z;
+        bem_field_point_accumulate(&L, &M, P.k, p.x, p.y, p.z, nq.x, nq.y, nq.z, qx, qy, qz, rule[q][2]);
+    }
+    return c64_scale(kind == 0 ? M : L, P.area[j]);
+}
+
+/* BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745) over the pivot
+   list piv (pairs relative to the block origin) */
+int64_t aca_extra(const EntryFn& E, const std::vector<std::pair<int64_t, int64_t>>& piv, const Block& B, double tol,
+                  int64_t maxrank, ACAState& st)
 {
     int64_t m = B.rows, n = B.cols, ev = 0;
     std::vector<cplx> row(n), col(m), colabs(m);
-    auto piv = pivot_list(P, B, bidx, seed);
     double tiny2 = TINY * TINY;
     for (size_t pi = 0; pi < piv.size() && st.rank < maxrank; pi++) {
         int64_t i = piv[pi].first, j = piv[pi].second;
-        cplx a = entry(P, kind, B.row0 + i, B.col0 + j); ev++;
+        cplx a = E(B.row0 + i, B.col0 + j); ev++;
         if (c64_abs2(a) < tiny2) continue;
         cplx approx = c64(0, 0);
         for (int64_t l = 0; l < st.rank; l++)
             approx = c64_add(approx, c64_mul(c64_mul(st.U[l * m + i], st.alpha[l]), st.V[l * n + j]));
         if (c64_abs2(c64_sub(a, approx)) < (tol * tol) * c64_abs2(a)) continue;
-        eval_col(P, kind, B, j, col.data()); ev += m;
+        eval_col(E, B, j, col.data()); ev += m;
         sub_col(st, j, col.data());
         bool first = true;
         while (st.rank < maxrank) {
@@ -715,12 +808,12 @@ int64_t aca_extra(const orc_problem& P, int kind, const Block& B, int64_t bidx, 
             i = argmax_abs2(colabs.data(), m, &mv);
             first = false;
             if (c64_abs2(col[i]) <= tiny2 && st.rank >= 1) break;
-            eval_row(P, kind, B, i, row.data()); ev += n;
+            eval_row(E, B, i, row.data()); ev += n;
             sub_row(st, i, row.data());
             double mr;
             j = argmax_abs2(row.data(), n, &mr);
             if (mr <= tiny2 && st.rank >= 1) break;
-            eval_col(P, kind, B, j, col.data()); ev += m;
+            eval_col(E, B, j, col.data()); ev += m;
             cplx alpha = c64_div(c64(1.0, 0.0), row[j]);
             sub_col(st, j, col.data());
             cplx oldS = st.S;
@@ -892,33 +985,8 @@ orc_problem* orc_new(int64_t n, const double* nodes, const double* bc, int64_t l
         tin[e].a = v3(v[0], v[1], v[2]); tin[e].b = v3(v[3], v[4], v[5]); tin[e].c = v3(v[6], v[7], v[8]);
         mid[e] = tri_mid(tin[e]);
     }
-    /* ClusterTree(BoundaryElements*) (clustertree.h:77-86): tree on
-       midpoints, then depth-first reordering (clustertree.cpp:265-283) */
-    std::vector<int64_t> all(n);
-    for (int64_t e = 0; e < n; e++) all[e] = e;
-    TreeBuild tb{mid, leaf, P->clusters, {}};
-    tb.make(std::move(all), 0);
-    /* depth-first leaf order gives the permutation; every cluster becomes a
-       contiguous range */
-    int64_t pos = 0;
-    std::vector<int64_t> st{0};
-    std::vector<int64_t> startpos(P->clusters.size(), -1);
-    /* recursive assignment */
-    struct Rec {
-        orc_problem* P; TreeBuild* tb; int64_t* pos;
-        void go(int c)
-        {
-            Cluster& C = P->clusters[c];
-            if (C.leaf) {
-                C.start = *pos;
-                for (int64_t e : tb->idx[c]) P->perm.push_back(e), (*pos)++;
-            } else {
-                go(C.son1); go(C.son2);
-                C.start = P->clusters[C.son1].start;
-            }
-        }
-    } rec{P, &tb, &pos};
-    rec.go(0);
+    /* ClusterTree(BoundaryElements*) (clustertree.h:77-86) */
+    build_tree(mid, leaf, P->clusters, P->perm);
     P->tri.resize(n); P->normal.resize(n); P->colloc.resize(n); P->area.resize(n);
     P->a.resize(n); P->b.resize(n); P->g.resize(n);
     for (int64_t e = 0; e < n; e++) {
@@ -1082,6 +1150,107 @@ void orc_field(const orc_problem* p, int64_t nobs, const double* obs3, const dou
     }
 }
 
+/* calculateFieldSolutionFast (boundaryelementsolver.cpp:2397-2476): the
+   boundary term M phi - L dphi at the points obs3 (input order) through the
+   observation x boundary H-matrix.  The points' own cluster tree (leaf),
+   partition against the boundary tree (eta); per admissible block ACA of
+   the M kernel and, with the field pivots, of the L kernel, each block's
+   U alpha V x applied right away; dense blocks row by row; every block's
+   term added to its rows in partition order.  out2 in input order. */
+void orc_field_hmat(const orc_problem* p, int64_t nobs, const double* obs3, const double* phi2, const double* dphi2,
+                    double tol, int64_t maxrank, uint64_t seed, int64_t leaf, double eta, double* out2, double* stats3)
+{
+    const cplx* phi = (const cplx*)phi2;
+    const cplx* dphi = (const cplx*)dphi2;
+    std::vector<V3> pts(nobs);
+    for (int64_t i = 0; i < nobs; i++) pts[i] = v3(obs3[3 * i], obs3[3 * i + 1], obs3[3 * i + 2]);
+    std::vector<Cluster> fcl;
+    std::vector<int64_t> fperm;
+    build_tree(pts, leaf, fcl, fperm);
+    std::vector<V3> obs(nobs);
+    for (int64_t i = 0; i < nobs; i++) obs[i] = pts[fperm[i]];
+    std::vector<Block> blocks;
+    make_blocks2(fcl, p->clusters, 0, 0, eta, blocks);
+    const int64_t nb = (int64_t)blocks.size();
+    std::vector<std::vector<cplx>> term(nb);
+    int64_t nadm = 0, maxr = 0, evaluated = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nadm, evaluated) reduction(max : maxr)
+    for (int64_t b = 0; b < nb; b++) {
+        const Block& B = blocks[b];
+        auto& T = term[b];
+        T.assign(B.rows, c64(0, 0));
+        if (!B.adm) {
+            for (int64_t i = 0; i < B.rows; i++) {
+                cplx sm = c64(0, 0), sl = c64(0, 0);
+                for (int64_t j = 0; j < B.cols; j++) {
+                    sm = c64_fma(field_entry(*p, 0, obs[B.row0 + i], B.col0 + j), phi[B.col0 + j], sm);
+                    sl = c64_fma(field_entry(*p, 1, obs[B.row0 + i], B.col0 + j), dphi[B.col0 + j], sl);
+                }
+                T[i] = c64_sub(sm, sl);
+            }
+            continue;
+        }
+        nadm++;
+        int64_t mr = std::min(B.rows, B.cols);
+        if (maxrank > 0) mr = std::min(mr, maxrank);
+        for (int kind = 0; kind < 2; kind++) {
+            const EntryFn E = [&](int64_t i, int64_t j) { return field_entry(*p, kind, obs[i], j); };
+            ACAState st;
+            st.m = B.rows; st.n = B.cols; st.cap = mr;
+            st.U.assign(mr * B.rows, c64(0, 0)); st.alpha.assign(mr, c64(0, 0)); st.V.assign(mr * B.cols, c64(0, 0));
+            evaluated += kind == 0 ? aca_plain(E, B, b, tol, mr, seed, st)
+                                   : aca_extra(E, field_pivot_list(*p, B, b, seed), B, tol, mr, st);
+            maxr = std::max(maxr, st.rank);
+            const cplx* x = (kind == 0 ? phi : dphi) + B.col0;
+            std::vector<cplx> t(st.rank);
+            for (int64_t l = 0; l < st.rank; l++) {
+                cplx d = c64(0, 0);
+                for (int64_t j = 0; j < B.cols; j++) d = c64_fma(st.V[l * B.cols + j], x[j], d);
+                t[l] = c64_mul(st.alpha[l], d);
+            }
+            for (int64_t i = 0; i < B.rows; i++) {
+                cplx a = c64(0, 0);
+                for (int64_t l = 0; l < st.rank; l++) a = c64_fma(st.U[l * B.rows + i], t[l], a);
+                T[i] = kind == 0 ? a : c64_sub(T[i], a);
+            }
+        }
+    }
+    std::vector<cplx> f(nobs, c64(0, 0));
+    for (int64_t b = 0; b < nb; b++)
+        for (int64_t i = 0; i < blocks[b].rows; i++) f[blocks[b].row0 + i] = c64_add(f[blocks[b].row0 + i], term[b][i]);
+    for (int64_t i = 0; i < nobs; i++) {
+        out2[2 * fperm[i]] = f[i].re;
+        out2[2 * fperm[i] + 1] = f[i].im;
+    }
+    if (stats3) {
+        stats3[0] = (double)nadm;
+        stats3[1] = (double)maxr;
+        stats3[2] = (double)evaluated;
+    }
+}
+
+/* the points' cluster tree and the observation x boundary partition of
+   orc_field_hmat: perm (perm[new] = old), blocks5 (row0, rows, col0, cols,
+   admissible) up to cap blocks; returns the number of blocks */
+int64_t orc_field_partition(const orc_problem* p, int64_t nobs, const double* obs3, int64_t leaf, double eta,
+                            int64_t* perm, int64_t* blocks5, int64_t cap)
+{
+    std::vector<V3> pts(nobs);
+    for (int64_t i = 0; i < nobs; i++) pts[i] = v3(obs3[3 * i], obs3[3 * i + 1], obs3[3 * i + 2]);
+    std::vector<Cluster> fcl;
+    std::vector<int64_t> fperm;
+    build_tree(pts, leaf, fcl, fperm);
+    std::copy(fperm.begin(), fperm.end(), perm);
+    std::vector<Block> blocks;
+    make_blocks2(fcl, p->clusters, 0, 0, eta, blocks);
+    for (int64_t b = 0; b < (int64_t)blocks.size() && b < cap; b++) {
+        const Block& B = blocks[b];
+        int64_t* q = blocks5 + 5 * b;
+        q[0] = B.row0; q[1] = B.rows; q[2] = B.col0; q[3] = B.cols; q[4] = B.adm;
+    }
+    return (int64_t)blocks.size();
+}
+
 /* MeshFunctions::reflectTriangles (meshfunctions.cpp:349-378) through
    BoundaryElements::reflectGeometry (boundaryelements.cpp:151-155): the
    plane normal normalised, x' = R [x; 1] with R = [I - 2 n n^T | -2 n d],
@@ -1182,7 +1351,8 @@ orc_hmat* orc_assemble(const orc_problem* p, int kind, double tol, int64_t maxra
             ACAState st;
             st.m = B.rows; st.n = B.cols; st.cap = mr;
             st.U.assign(mr * B.rows, c64(0, 0)); st.alpha.assign(mr, c64(0, 0)); st.V.assign(mr * B.cols, c64(0, 0));
-            ev += kind == 0 ? aca_plain(*p, kind, B, b, tol, mr, seed, st) : aca_extra(*p, kind, B, b, tol, mr, seed, st);
+            const EntryFn E = [&](int64_t i, int64_t j) { return entry(*p, kind, i, j); };
+            ev += kind == 0 ? aca_plain(E, B, b, tol, mr, seed, st) : aca_extra(E, pivot_list(*p, B, b, seed), B, tol, mr, st);
             if (st.rank == 0) { /* zero rank-1 block (hmatrix.cpp:587-592) */
                 K.rank = 1;
                 K.U.assign(B.rows, c64(0, 0)); K.alpha.assign(1, c64(0, 0)); K.V.assign(B.cols, c64(0, 0));

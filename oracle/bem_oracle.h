@@ -123,6 +123,17 @@ void orc_set_plane(orc_problem* p, const double* point3, const double* normal3);
    element index mod 4 combined (s0 + s1) + (s2 + s3).  Sources excluded. */
 void orc_field(const orc_problem* p, int64_t nobs, const double* obs3, const double* phi2, const double* dphi2,
                double* out2);
+/* The same boundary term through the observation x boundary H-matrix of
+   calculateFieldSolutionFast (boundaryelementsolver.cpp:2397-2476): the
+   points' cluster tree (leaf), the partition against the boundary tree
+   (eta), ACA of the M kernel and (field pivots) of the L kernel per
+   admissible block to tol / maxrank, dense blocks directly; obs3 and out2
+   in input order.  stats3: admissible blocks, largest rank, evaluated
+   entries. */
+int64_t orc_field_partition(const orc_problem* p, int64_t nobs, const double* obs3, int64_t leaf, double eta,
+                            int64_t* perm, int64_t* blocks5, int64_t cap);
+void orc_field_hmat(const orc_problem* p, int64_t nobs, const double* obs3, const double* phi2, const double* dphi2,
+                    double tol, int64_t maxrank, uint64_t seed, int64_t leaf, double eta, double* out2, double* stats3);
 
 /* quadrature tables used (for tests): order-8 triangle rule 16x3 and the
    5-point Gauss-Legendre rule 5x2, the 16-point half Gauss-Legendre 16x2 */

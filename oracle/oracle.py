@@ -57,6 +57,10 @@ def lib():
                                          _p, _p, _p, _p]),
             "orc_dense_solve": (ctypes.c_int, [_p, _dbl, _dbl, _p, _p, _p]),
             "orc_field": (None, [_p, ctypes.c_int64, _p, _p, _p, _p]),
+            "orc_field_partition": (ctypes.c_int64, [_p, ctypes.c_int64, _p, ctypes.c_int64, ctypes.c_double, _p,
+                                                     _p, ctypes.c_int64]),
+            "orc_field_hmat": (None, [_p, ctypes.c_int64, _p, _p, _p, ctypes.c_double, ctypes.c_int64,
+                                      ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, _p, _p]),
             "orc_set_plane": (None, [_p, _p, _p]),
             "orc_tables": (None, [_p, _p, _p]),
             "orc_sincos": (None, [_i64, _p, _p, _p]),
@@ -207,6 +211,29 @@ class Problem:
         out = np.empty(len(obs), np.complex128)
         lib().orc_field(self.h, len(obs), _ptr(obs), _ptr(phi), _ptr(dphi), _ptr(out))
         return out
+
+    def field_hmat(self, obs, phi, dphi, tol, maxrank=0, seed=1, leaf=50, eta=3.0):
+        """boundary term through the observation x boundary H-matrix (input
+        order of obs); returns (values, (admissible blocks, max rank, evaluated))"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        phi, dphi = _c(phi, np.complex128), _c(dphi, np.complex128)
+        out = np.empty(len(obs), np.complex128)
+        st = np.zeros(3)
+        lib().orc_field_hmat(self.h, len(obs), _ptr(obs), _ptr(phi), _ptr(dphi), float(tol), int(maxrank), int(seed),
+                             int(leaf), float(eta), _ptr(out), _ptr(st))
+        return out, tuple(int(v) for v in st)
+
+    def field_partition(self, obs, leaf=50, eta=3.0):
+        """(perm, blocks (nb, 5)) of the points' tree and the observation x
+        boundary partition"""
+        obs = _c(obs, np.float64).reshape(-1, 3)
+        perm = np.empty(len(obs), np.int64)
+        cap = 1 << 16
+        b5 = np.empty((cap, 5), np.int64)
+        nb = lib().orc_field_partition(self.h, len(obs), _ptr(obs), int(leaf), float(eta), _ptr(perm), _ptr(b5), cap)
+        if nb > cap:
+            raise RuntimeError("too many field blocks")
+        return perm, b5[:nb].copy()
 
     def dense_solve(self, rho, freq):
         n = self.n

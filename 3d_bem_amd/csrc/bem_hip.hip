@@ -36,7 +36,7 @@ namespace {
 constexpr int kWG = 256;        // workgroup size of the block kernels (4 wavefronts)
 constexpr int kTile = 64;       // dense tile edge
 constexpr int kCapFirst = 48;   // first-pass ACA rank capacity per block
-constexpr int kCapMax = 1024;   // second-pass capacity limit (LDS: (3 cap + 2) x 16 B <= 48 KiB)
+constexpr int kCapMax = 1016;   // second-pass capacity limit (FP64 LDS: (4 cap + 2) x 16 B <= 64 KiB)
 // ACA workgroup size classes by m + n, per storage precision [FP64, FP32]:
 // up to kSmallBlock one wavefront, up to kHugeBlock kMidThreads, above kHugeThreads
 constexpr int64_t kSmallBlock[2] = {384, 384};
@@ -827,7 +827,9 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // ACA of one admissible block b by a workgroup of NT threads.  Plain
 // variant = BlockCluster::partialPivotACA (hmatrix.cpp:471-593); extra
 // variant = BlockCluster::partialPivotACAextra (hmatrix.cpp:595-745).
-// Dynamic LDS: sw[cap] (type C) then sdots[2 cap + 2] (FP64).
+// Dynamic LDS (aca_shm_bytes): sw[cap] and sA[cap] (type C: cross-sum
+// weights / probe terms, and a copy of alpha for the loops every thread
+// runs in rank order), then sdots[2 cap + 2] (FP64).
 template <bool EXTRA, int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred)
 {
@@ -838,8 +840,10 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
     C* U = (C*)a.U + a.uoff[b];
     C* V = (C*)a.V + a.voff[b];
     C* A = (C*)a.A + a.aoff[b];
+    const size_t cbytes = ((size_t)cap * sizeof(C) + 15) & ~(size_t)15;
     C* sw = (C*)dyn;
-    bem_c64* sdots = (bem_c64*)(dyn + (((size_t)cap * sizeof(C) + 15) & ~(size_t)15));
+    C* sA = (C*)(dyn + cbytes);
+    bem_c64* sdots = (bem_c64*)(dyn + 2 * cbytes);
     const double tiny2 = kTiny * kTiny, tol2 = a.tol * a.tol;
     int r = 0, flag = 0;
     int64_t ev = 0;
@@ -864,10 +868,10 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
             ev += m;
             C alpha = cx_inv(row[ar.i]);
-            if (threadIdx.x == 0) A[r] = alpha;
+            if (threadIdx.x == 0) A[r] = sA[r] = alpha;
             aca_dots<NT>(col, row, U, V, m, n, r, sdots);
             __syncthreads();
-            double gain = aca_update_norm(alpha, A, r, sdots, S);
+            double gain = aca_update_norm(alpha, sA, r, sdots, S);
             r++;
             __syncthreads();
             if (!(isfinite(gain))) { flag = 2; break; }
@@ -881,9 +885,14 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             C pe = entry_any<PLANE, C, FIELD>(a, r0 + i, c0 + j);
             ev += 1;
             if (cx_abs2(pe) < tiny2) continue;
+            // the approximation at (i, j): its rank terms formed in parallel,
+            // then summed in rank order by every thread
+            for (int l = threadIdx.x; l < r; l += NT)
+                sw[l] = cx_mul(cx_mul(U[(int64_t)l * m + i], sA[l]), V[(int64_t)l * n + j]);
+            __syncthreads();
             C approx = cx_zero<C>();
-            for (int l = 0; l < r; l++)
-                approx = cx_add(approx, cx_mul(cx_mul(U[(int64_t)l * m + i], A[l]), V[(int64_t)l * n + j]));
+            for (int l = 0; l < r; l++) approx = cx_add(approx, sw[l]);
+            __syncthreads();
             if (cx_abs2(cx_sub(pe, approx)) < tol2 * cx_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
             aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
@@ -905,11 +914,11 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
                 ev += m;
                 C alpha = cx_inv(row[ar.i]);
-                if (threadIdx.x == 0) A[r] = alpha;
+                if (threadIdx.x == 0) A[r] = sA[r] = alpha;
                 aca_dots<NT>(col, row, U, V, m, n, r, sdots);
                 __syncthreads();
                 bem_c64 oldS = S;
-                double gain = aca_update_norm(alpha, A, r, sdots, S);
+                double gain = aca_update_norm(alpha, sA, r, sdots, S);
                 r++;
                 __syncthreads();
                 if (!(isfinite(gain))) { flag = 2; done = true; break; }
@@ -3479,6 +3488,13 @@ struct PartView {
     int64_t nobs;
 };
 
+// dynamic LDS of aca_block for rank capacity cap and storage size esz
+size_t aca_shm_bytes(int cap, size_t esz)
+{
+    const size_t cbytes = ((size_t)cap * esz + 15) & ~(size_t)15;
+    return 2 * cbytes + (size_t)(2 * cap + 2) * sizeof(bem_c64);
+}
+
 int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
             const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv,
             const std::function<int()>& main_work, AcaScratch& S,
@@ -3559,7 +3575,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     a.queue = nullptr; a.nitems = 0;
     // dynamic LDS of aca_block: sw[cap] of the storage type, then the FP64
     // dot products sdots[2 cap + 2]
-    size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
+    size_t shm = aca_shm_bytes(maxcap, esz);
     HIPC(c, hipEventRecord(c->ev_fork, c->stream));
     size_t off = 0;
     for (int q = 0; q < 3; q++) {
@@ -3831,7 +3847,7 @@ int run_aca_batch(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, u
     a.singf = pk.ptr<const bem_c64*>(k_sp);
     a.Uf = pk.ptr<void*>(k_up); a.Vf = pk.ptr<void*>(k_vp); a.Af = pk.ptr<void*>(k_ap);
     a.queue = nullptr; a.nitems = 0;
-    size_t shm = (((size_t)maxcap * esz + 15) & ~(size_t)15) + (size_t)(2 * maxcap + 2) * sizeof(bem_c64);
+    size_t shm = aca_shm_bytes(maxcap, esz);
     HIPC(c, hipEventRecord(c->ev_fork, c->stream));
     size_t off = 0;
     // resident workgroups of each class on the chip (256 CUs x 4 SIMDs x
@@ -4064,7 +4080,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
                 return fail(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
                                     "bem_assemble: ACA block " + std::to_string(b) +
                                         (flag2[b] == 2 ? " non-finite"
-                                                       : " needs rank > 1024; lower the tolerance or set max_rank")));
+                                                       : " needs rank > " + std::to_string(kCapMax) + "; lower the tolerance or set max_rank")));
             }
         }
         c->vp.give(cap2);
@@ -4279,7 +4295,7 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
                 if (flag2[b])
                     return fail_pair(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
                                         "bem_assemble_batch: ACA block " + std::to_string(b) +
-                                            (flag2[b] == 2 ? " non-finite" : " needs rank > 1024")));
+                                            (flag2[b] == 2 ? " non-finite" : " needs rank > " + std::to_string(kCapMax))));
             }
             c->vp.give(rank2); c->vp.give(flag2); c->vp.give(evs2);
         }
@@ -4370,23 +4386,26 @@ int bem_autotune_aca(bem_ctx_t c, int kind, int precision, int64_t nf, const dou
     int rc;
     if ((rc = timed(bs, bh, bq, &bt))) return restore(rc);   // the built-in schedule (also the warm-up)
     if ((rc = timed(bs, bh, bq, &bt))) return restore(rc);
-    // coordinate descent: the one-wavefront bound, then the huge bound, then the queue
+    // coordinate descent: the one-wavefront bound, then the huge bound, then
+    // the queue; a candidate replaces the current choice only when it is
+    // kTuneGain faster (run-to-run noise does not move the schedule)
+    constexpr double kTuneGain = 0.97;
     for (int64_t v : smalls) {
         double t;
         if (v == bs) continue;
         if ((rc = timed(v, bh, bq, &t))) return restore(rc);
-        if (t < bt) { bt = t; bs = v; }
+        if (t < kTuneGain * bt) { bt = t; bs = v; }
     }
     for (int64_t v : huges) {
         double t;
         if (v == bh || v <= bs) continue;
         if ((rc = timed(bs, v, bq, &t))) return restore(rc);
-        if (t < bt) { bt = t; bh = v; }
+        if (t < kTuneGain * bt) { bt = t; bh = v; }
     }
     if (nf > 1) {
         double t;
         if ((rc = timed(bs, bh, !bq, &t))) return restore(rc);
-        if (t < bt) { bt = t; bq = !bq; }
+        if (t < kTuneGain * bt) { bt = t; bq = !bq; }
     }
     c->cls_small[pi] = bs;
     c->cls_huge[pi] = bh;
@@ -6047,7 +6066,7 @@ int bem_field_hmat(bem_ctx_t c, int64_t nobs, const double* obs3, int64_t nblock
                 if (flag2[b])
                     return fail(set_err(c, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
                                         "bem_field_hmat: ACA block " + std::to_string(b) +
-                                            (flag2[b] == 2 ? " non-finite" : " needs rank > 1024")));
+                                            (flag2[b] == 2 ? " non-finite" : " needs rank > " + std::to_string(kCapMax))));
                 rank[kind][b] = rank2[b];
                 evs[kind][b] = evs2[b];
             }

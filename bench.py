@@ -272,7 +272,7 @@ def timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sweep_batch=0):
     return pr, its, stats, elapsed, prof[:7].copy(), prof[7:].copy()
 
 
-def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload, quads=1, points_per_s=None):
+def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, traffic_sched, quads=1, points_per_s=None):
     """Dominant kernel family (largest total device time) against its bound.
     Assembly families: algorithmic flops (F_REG per regular quadrature,
     quads of them per regular entry: 2 with an impedance plane) over the
@@ -314,18 +314,19 @@ def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, measured_workload, 
         rl = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
               "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
               "algorithmic_per_launch": f"{byts:.4g} B"}
-    if dom == 2 and prec_name == "fp32" and measured_workload:
-        rl["traffic"] = measured_traffic("aca", "fp32")
+    if dom == 2 and prec_name == "fp32" and traffic_sched:
+        rl["traffic"] = measured_traffic("aca", "fp32", traffic_sched)
     rl["kernel"] = FAMILIES[dom]
     rl["avg_ms"] = round(float(avg_ms[dom]), 4)
     rl["launches_per_point"] = round(float(launches), 2)
     return rl
 
 
-def measured_traffic(family, prec_name):
-    """HBM bytes per launch of a kernel family from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration), or None"""
-    key = f"traffic_bytes_per_launch {family} {prec_name} ="
+def measured_traffic(family, prec_name, sched):
+    """HBM bytes per launch of a kernel family under one scheduling (sched:
+    e.g. "batch4 s384 h2048 q0") from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by the copy calibration), or None"""
+    key = f"traffic_bytes_per_launch {family} {prec_name} {sched} ="
     path = os.path.join(ROOT, "profiles", f"{family}_{prec_name}_sphere20480_traffic.md")
     try:
         for line in open(path):
@@ -440,6 +441,14 @@ def main():
     rstats = S.refine_stats()
     points = args.steps * len(freqs)
     sched = f"batches of {sb} on {lanes} lanes" if batched else f"{lanes} lanes"
+    # the scheduling the committed traffic measurement must match: workload,
+    # chunking and the ACA schedule in use (tuned, else the built-in one)
+    acs = S.aca_schedule(prec) or {"small_bound": 384, "huge_bound": 2048, "queue": 0}
+    huge = acs["huge_bound"] if acs["huge_bound"] < (1 << 40) else "none"
+    traffic_sched = None
+    if (args.level, args.leaf, args.eta, args.aca_tol, args.plane, args.batch) == (5, 64, 2.0, 1e-5, None, 8):
+        traffic_sched = (f"batch{sb}" if batched else f"lanes{lanes}") + \
+            f" s{acs['small_bound']} h{huge} q{acs['queue']}"
     print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, {sched}", file=sys.stderr, flush=True)
     st = stats[ic]
     points0 = args.steps * len(range(0, len(freqs), max(lanes, 1)))   # points whose launches lane 0 records
@@ -470,8 +479,7 @@ def main():
                    "evaluated_entries": [st["evaluated_phi"], st["evaluated_dphi"]],
                    "phase_ms": {k: round(st[k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
                    "kernel_ms_lane0": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
-        "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0,
-                             (args.level, args.leaf, args.eta, args.aca_tol, args.plane) == (5, 64, 2.0, 1e-5, None),
+        "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0, traffic_sched,
                              quads=1 if args.plane is None else 2, points_per_s=points / elapsed),
     }
     out["roofline"]["duration_source"] = ("HIP events around each launch on lane 0's launching streams during the "

@@ -161,7 +161,8 @@ int bem_set_aca_schedule(bem_ctx_t ctx, int precision, int64_t small_bound, int6
    bem_assemble_batch otherwise; the fastest of two runs each) under
    candidate schedules by coordinate descent -- the one-wavefront bound
    (128, 256, 384, 512), then for FP32 the huge bound (1024, 2048, 4096,
-   none), then for nf > 1 the work queue -- and keeps the fastest on the
+   none), then for nf > 1 the work queue, a candidate taking over only when
+   3 % faster than the current choice -- and keeps the choice on the
    context.  choice3: small bound, huge bound, queue; best_ms: its time.
    Operators kept for near-field pairing are dropped. */
 int bem_autotune_aca(bem_ctx_t ctx, int kind, int precision, int64_t nf, const double* kcc3, double tol,

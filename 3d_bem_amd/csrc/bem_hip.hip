@@ -711,7 +711,10 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
 {
     const DevMesh& M = a.M;
     if (near_or_self(M, i, j)) return false;
-    bem_c32f4 o = regular_ops_f32_edge<1>(a.F32, M, i, j);
+    // only the operators the entry uses (the others stay zero, and the
+    // entry formula then gives the same value)
+    const int need = bc_needs_f32(a.F32, a.F.subst, a.kind, j);
+    bem_c32f4 o = regular_ops_f32_edge<1>(a.F32, M, i, j, need & 1, need & 2);
     *e = bc_entry_f32(a.F32, a.F.subst, a.kind, o, j);
     if (PLANE) {
         bem_c32 r = bc_entry_f32(a.F32, a.F.subst, a.kind, refl_ops_f32(a.F32, M, i, j), j);

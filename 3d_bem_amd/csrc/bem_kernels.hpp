@@ -314,7 +314,8 @@ __device__ __forceinline__ bem_c32 c32_fma(bem_c32 a, bem_c32 b, bem_c32 acc)
 // bem_point_accumulate
 __device__ __forceinline__ void point_accumulate_f32_r(bem_c32f4* acc, const DevFreq32& F, double Rx, double Ry,
                                                        double Rz, float npx, float npy, float npz, float nqx,
-                                                       float nqy, float nqz, float upnq, float w);
+                                                       float nqy, float nqz, float upnq, float w, bool needX = true,
+                                                       bool needY = true);
 
 // one quadrature point, same formulas as bem_point_accumulate
 __device__ __forceinline__ void point_accumulate_f32(bem_c32f4* acc, const DevFreq32& F, double px, double py,
@@ -324,9 +325,13 @@ __device__ __forceinline__ void point_accumulate_f32(bem_c32f4* acc, const DevFr
     point_accumulate_f32_r(acc, F, px - qx, py - qy, pz - qz, npx, npy, npz, nqx, nqy, nqz, upnq, w);
 }
 
+// needX / needY: only the operators of X = L + cc Mt / Y = M + cc N are
+// accumulated (the others keep their value); each one that is accumulated
+// gets the same operations as with both
 __device__ __forceinline__ void point_accumulate_f32_r(bem_c32f4* acc, const DevFreq32& F, double Rx, double Ry,
                                                        double Rz, float npx, float npy, float npz, float nqx,
-                                                       float nqy, float nqz, float upnq, float w)
+                                                       float nqy, float nqz, float upnq, float w, bool needX,
+                                                       bool needY)
 {
     double rr = fma(Rz, Rz, fma(Ry, Ry, Rx * Rx));
     float inv_r = __builtin_amdgcn_rsqf((float)rr);
@@ -337,30 +342,34 @@ __device__ __forceinline__ void point_accumulate_f32_r(bem_c32f4* acc, const Dev
     float s = __builtin_amdgcn_sinf(rev), c = __builtin_amdgcn_cosf(rev);
     float kr = F.k * (float)r;
     float fx = (float)Rx, fy = (float)Ry, fz = (float)Rz;
-    float rnq = -fmaf(fz, nqz, fmaf(fy, nqy, fx * nqx)) * inv_r;
     float rnp = fmaf(fz, npz, fmaf(fy, npy, fx * npx)) * inv_r;
     float g = inv_r * (float)BEM_INV_4PI;
     float Gre = c * g, Gim = s * g;
     float Grre = Gre * inv_r, Grim = Gim * inv_r;
     float dGre = fmaf(-Grim, kr, -Grre);
     float dGim = fmaf(Grre, kr, -Grim);
-    float Grrre = Grre * inv_r, Grrim = Grim * inv_r;
-    float a = fmaf(-kr, kr, 2.0f);
-    float b = -2.0f * kr;
-    float d2Gre = fmaf(Grrre, a, -(Grrim * b));
-    float d2Gim = fmaf(Grrre, b, Grrim * a);
-    float pp = rnp * rnq;
-    float tt = -(upnq + pp) * inv_r;
-    float Nre = fmaf(dGre, tt, d2Gre * pp);
-    float Nim = fmaf(dGim, tt, d2Gim * pp);
-    acc->L.re = fmaf(w, Gre, acc->L.re);
-    acc->L.im = fmaf(w, Gim, acc->L.im);
-    acc->M.re = fmaf(w, dGre * rnq, acc->M.re);
-    acc->M.im = fmaf(w, dGim * rnq, acc->M.im);
-    acc->Mt.re = fmaf(w, dGre * rnp, acc->Mt.re);
-    acc->Mt.im = fmaf(w, dGim * rnp, acc->Mt.im);
-    acc->N.re = fmaf(w, Nre, acc->N.re);
-    acc->N.im = fmaf(w, Nim, acc->N.im);
+    if (needX) {
+        acc->L.re = fmaf(w, Gre, acc->L.re);
+        acc->L.im = fmaf(w, Gim, acc->L.im);
+        acc->Mt.re = fmaf(w, dGre * rnp, acc->Mt.re);
+        acc->Mt.im = fmaf(w, dGim * rnp, acc->Mt.im);
+    }
+    if (needY) {
+        float rnq = -fmaf(fz, nqz, fmaf(fy, nqy, fx * nqx)) * inv_r;
+        float Grrre = Grre * inv_r, Grrim = Grim * inv_r;
+        float a = fmaf(-kr, kr, 2.0f);
+        float b = -2.0f * kr;
+        float d2Gre = fmaf(Grrre, a, -(Grrim * b));
+        float d2Gim = fmaf(Grrre, b, Grrim * a);
+        float pp = rnp * rnq;
+        float tt = -(upnq + pp) * inv_r;
+        float Nre = fmaf(dGre, tt, d2Gre * pp);
+        float Nim = fmaf(dGim, tt, d2Gim * pp);
+        acc->M.re = fmaf(w, dGre * rnq, acc->M.re);
+        acc->M.im = fmaf(w, dGim * rnq, acc->M.im);
+        acc->N.re = fmaf(w, Nre, acc->N.re);
+        acc->N.im = fmaf(w, Nim, acc->N.im);
+    }
 }
 
 // order-8 16-point rule weights in FP32
@@ -415,7 +424,8 @@ __device__ __forceinline__ bem_c32f4 regular_ops_f32_vtx(const DevFreq32& F, dou
 template <int UNROLL = 1>
 __device__ __forceinline__ bem_c32f4 regular_ops_f32_edge_of(const DevFreq32& F, const DevMesh& M, int64_t i,
                                                              int64_t j, const double* edg, const double* qnx,
-                                                             const double* qny, const double* qnz)
+                                                             const double* qny, const double* qnz,
+                                                             bool needX = true, bool needY = true)
 {
     const int64_t n = M.n;
     const double* g = edg + j;
@@ -430,7 +440,8 @@ __device__ __forceinline__ bem_c32f4 regular_ops_f32_edge_of(const DevFreq32& F,
     for (int q = 0; q < 16; q++) {
         const double x1 = kTriX[q][0], x2 = kTriX[q][1];
         point_accumulate_f32_r(&o, F, fma(-x1, ax, fma(-x2, bx, dx)), fma(-x1, ay, fma(-x2, by, dy)),
-                               fma(-x1, az, fma(-x2, bz, dz)), npx, npy, npz, nqx, nqy, nqz, upnq, kTriWf[q]);
+                               fma(-x1, az, fma(-x2, bz, dz)), npx, npy, npz, nqx, nqy, nqz, upnq, kTriWf[q], needX,
+                               needY);
     }
     const float area = (float)M.area[j];
     o.L = c32(o.L.re * area, o.L.im * area);
@@ -441,9 +452,21 @@ __device__ __forceinline__ bem_c32f4 regular_ops_f32_edge_of(const DevFreq32& F,
 }
 
 template <int UNROLL = 1>
-__device__ __forceinline__ bem_c32f4 regular_ops_f32_edge(const DevFreq32& F, const DevMesh& M, int64_t i, int64_t j)
+__device__ __forceinline__ bem_c32f4 regular_ops_f32_edge(const DevFreq32& F, const DevMesh& M, int64_t i, int64_t j,
+                                                          bool needX = true, bool needY = true)
 {
-    return regular_ops_f32_edge_of<UNROLL>(F, M, i, j, M.edg, M.nx, M.ny, M.nz);
+    return regular_ops_f32_edge_of<UNROLL>(F, M, i, j, M.edg, M.nx, M.ny, M.nz, needX, needY);
+}
+
+// which of X and Y an off-diagonal entry of column j uses (bc_entry_f32):
+// Phi takes X (subst) or Y; DPhi takes both, except cd = 0, where the
+// other one is multiplied by zero.  Returns the X bit 1 and the Y bit 2.
+__device__ __forceinline__ int bc_needs_f32(const DevFreq32& F, const int32_t* subst, int kind, int64_t j)
+{
+    if (kind == 0) return subst[j] ? 1 : 2;
+    const bem_c32 cd = F.cd[j];
+    if (cd.re == 0.0f && cd.im == 0.0f) return subst[j] ? 2 : 1;
+    return 3;
 }
 
 // the FP32 form of refl_ops (mirror image of element j, edge form)

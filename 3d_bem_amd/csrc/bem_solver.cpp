@@ -634,12 +634,17 @@ void BoundaryElementSolver::calculateBoundarySolutionsBatch(const std::vector<do
         for (int64_t i = 0; i < n; i++) Gi[i * nr + k] = rhs[k].g[i];
     G.upload(Gi);
     double t1 = now_ms(), t2, t3;
+    std::vector<double> phiRank(nf, 0.0), phiEvaluated(nf, 0.0);
     try {
         if (anyG) {   // b = s + Phi g, Phi of each frequency read once for its right-hand sides
             check(ctx_, bem_assemble_batch(ctx_, BEM_KIND_PHI, nf, kcc.data(), P.acaRelativeError, (int64_t)P.acaMaxRank,
                                            R == 2 ? BEM_PREC_FP64 : P.precision, P.pivotSeed, phis.data()),
                   "Phi batch");
             t1 = now_ms();
+            for (int64_t f = 0; f < nf; f++) {
+                phiRank[f] = bem_hmat_stats_value(phis[f], 1);
+                phiEvaluated[f] = bem_hmat_stats_value(phis[f], 0);
+            }
         }
         std::vector<cplx> sf((size_t)(n * nr));
         for (int64_t f = 0; f < nf; f++) {
@@ -697,6 +702,8 @@ void BoundaryElementSolver::calculateBoundarySolutionsBatch(const std::vector<do
             st[4] = (t4 - t0) / nf;
             st[5] = (double)its[f * nr + nr - 1];
             st[6] = rel[f * nr + nr - 1];
+            st[7] = phiRank[f];
+            st[9] = phiEvaluated[f];
             st[8] = bem_hmat_stats_value(dphis[f], 1);
             st[10] = bem_hmat_stats_value(dphis[f], 0);
             st[15] = bem_hmat_stats_value(dphis[f], 3);

@@ -123,6 +123,7 @@ def test_batched_sweep(bem, O, meshes):
     S.set_sweep_batch(4)
     pr, its, st = S.sweep(freqs, g=g, planes=planes, aca_tol=1e-6, gmres_tol=1e-10, seed=1)
     assert pr.shape == (3, 2, n) and np.all(its > 0)
+    assert all(x["evaluated_phi"] > 0 and x["evaluated_dphi"] > 0 and x["max_rank_phi"] > 0 for x in st)
     Pp = O.Problem(tri, meshes.robin_bc(n, 0, 1, 1), leaf=32, eta=2.0)
     Ps = O.Problem(tri, meshes.robin_bc(n, 0, 1, 0), leaf=32, eta=2.0)
     for fi, f in enumerate(freqs):

@@ -61,6 +61,7 @@ _SIGS = {
     "bem_solver_set_refinement": (_int, [_p, _int, _i64]),
     "bem_solver_set_sweep_batch": (_int, [_p, _int]),
     "bem_set_fuse_ops": (_int, [_p, _int]),
+    "bem_assemble_multi": (_int, [_p, _i64, _int, _dbl, _i64, _int, _u64, _p]),
     "bem_autotune_aca": (_int, [_p, _int, _int, _i64, _p, _dbl, _i64, _u64, _p, _p]),
     "bem_solver_set_autotune": (_int, [_p, _int]),
     "bem_solver_aca_schedule": (_int, [_p, _int, _p]),
@@ -354,6 +355,18 @@ class Context:
         out = np.zeros(14)
         self._check(lib().bem_profile(self.h, int(enable), _ptr(out)))
         return out
+
+
+def assemble_multi(ctxs, kind, tol, maxrank=0, precision=FP64, seed=1):
+    """bem_assemble_multi: one operator per context (each its own mesh and
+    frequency) from one ACA launch per size class; HMatrix per context"""
+    hv = (_p * len(ctxs))(*[c.h for c in ctxs])
+    outs = (_p * len(ctxs))()
+    rc = lib().bem_assemble_multi(ctypes.cast(hv, _p), len(ctxs), int(kind), float(tol), int(maxrank), int(precision),
+                                  int(seed), ctypes.cast(outs, _p))
+    if rc != BEM_OK:
+        ctxs[0]._check(rc)
+    return [HMatrix(c, _p(h)) for c, h in zip(ctxs, outs)]
 
 
 class HMatrix:

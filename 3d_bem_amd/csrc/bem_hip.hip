@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -541,6 +542,8 @@ __device__ ArgMax block_argmax(const C* v, int64_t len, int64_t excl, ArgMax* sr
     return r;
 }
 
+struct MeshJob;
+
 struct AcaArgs {
     DevMesh M;
     DevFreq F;
@@ -584,7 +587,32 @@ struct AcaArgs {
     // kernel (multiplies dphi)
     const double* obs = nullptr;
     int64_t nobs = 0;
+    // several contexts (meshes) in one launch: list entries are
+    // (job << kJobShift) | block, job j's mesh, frequency, partition, ACA
+    // scratch and results in jobs[j]
+    const MeshJob* jobs = nullptr;
 };
+
+// One context's part of a multi-mesh ACA launch: the AcaArgs fields that
+// differ between contexts
+struct MeshJob {
+    DevMesh M;
+    DevFreq F;
+    DevFreq32 F32;
+    const int64_t *row0, *rows, *col0, *cols;
+    const int64_t *uoff, *voff, *aoff;
+    const int32_t* cap;
+    void *U, *V, *A;
+    int32_t* rank_out;
+    int32_t* flag_out;
+    int64_t* evals_out;
+    const int64_t* piv_off;
+    const int32_t* piv;
+    const bem_c64* sing;
+    int64_t nnz;
+};
+constexpr int kJobShift = 20;                       // blocks per context < 2^20
+constexpr int32_t kJobMask = (1 << kJobShift) - 1;
 
 // The dot products are processed kDotGroup at a time per wavefront: the
 // new vector is loaded once per group and the shuffle trees of the group are
@@ -975,10 +1003,30 @@ __device__ __forceinline__ void aca_item(const AcaArgs& a, int item, char* dyn, 
     aca_block<EXTRA, NT, C, PLANE>(ai, b, dyn, sred);
 }
 
+// one listed (context, block) item of a multi-mesh launch
+template <bool EXTRA, int NT, class C, bool PLANE>
+__device__ __forceinline__ void aca_job(const AcaArgs& a, int item, char* dyn, ArgMax* sred)
+{
+    const MeshJob& J = a.jobs[item >> kJobShift];
+    AcaArgs ai = a;
+    ai.M = J.M;
+    ai.F = J.F;
+    ai.F32 = J.F32;
+    ai.row0 = J.row0; ai.rows = J.rows; ai.col0 = J.col0; ai.cols = J.cols;
+    ai.uoff = J.uoff; ai.voff = J.voff; ai.aoff = J.aoff; ai.cap = J.cap;
+    ai.U = J.U; ai.V = J.V; ai.A = J.A;
+    ai.rank_out = J.rank_out; ai.flag_out = J.flag_out; ai.evals_out = J.evals_out;
+    ai.piv_off = J.piv_off; ai.piv = J.piv;
+    ai.sing = J.sing; ai.nnz = J.nnz;
+    ai.nf = 0;
+    aca_block<EXTRA, NT, C, PLANE>(ai, item & kJobMask, dyn, sred);
+}
+
 // MODE 0: one workgroup per listed block of the single operator; 1: one per
-// listed (frequency, block) item; 2: persistent workgroups taking items
-// from the queue counter (separate instantiations, so that the plain kernel
-// keeps its register budget)
+// listed (frequency, block) item; 3: one per listed (context, block) item;
+// 2 / 4: persistent workgroups taking the items of 1 / 3 from the queue
+// counter (separate instantiations, so that the plain kernel keeps its
+// register budget)
 template <bool EXTRA, int NT, class C, bool PLANE, int MODE, bool FIELD = false>
 __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
 {
@@ -992,6 +1040,10 @@ __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
         aca_item<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
         return;
     }
+    if constexpr (MODE == 3) {
+        aca_job<EXTRA, NT, C, PLANE>(a, a.blist[blockIdx.x], dyn, sred);
+        return;
+    }
     __shared__ int s_pos;
     while (true) {   // every workgroup leaves once the counter passes nitems
         if (threadIdx.x == 0) s_pos = atomicAdd(a.queue, 1);
@@ -999,7 +1051,8 @@ __global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
         const int pos = s_pos;
         __syncthreads();
         if (pos >= a.nitems) break;
-        aca_item<EXTRA, NT, C, PLANE>(a, a.blist[pos], dyn, sred);
+        if constexpr (MODE == 4) aca_job<EXTRA, NT, C, PLANE>(a, a.blist[pos], dyn, sred);
+        else aca_item<EXTRA, NT, C, PLANE>(a, a.blist[pos], dyn, sred);
         __syncthreads();
     }
 }
@@ -3438,7 +3491,18 @@ template <> constexpr int kMidThreads<bem_c64> = 256;
 template <int NT, class C>
 void launch_aca_nt(bool extra, bool plane, dim3 g, size_t shm, hipStream_t st, const AcaArgs& aq)
 {
-    const int mode = aq.nf == 0 ? 0 : (aq.queue ? 2 : 1);   // batches run without a plane (checked by the caller)
+    // batches run without a plane (checked by the caller)
+    const int mode = aq.jobs ? (aq.queue ? 4 : 3) : aq.nf == 0 ? 0 : (aq.queue ? 2 : 1);
+    if (mode == 3) {
+        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 3>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C, false, 3>), g, dim3(NT), shm, st, aq);
+        return;
+    }
+    if (mode == 4) {
+        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 4>), g, dim3(NT), shm, st, aq);
+        else hipLaunchKernelGGL((k_aca<false, NT, C, false, 4>), g, dim3(NT), shm, st, aq);
+        return;
+    }
     if (mode == 1) {
         if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
         else hipLaunchKernelGGL((k_aca<false, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
@@ -3498,11 +3562,13 @@ size_t aca_shm_bytes(int cap, size_t esz)
     return 2 * cbytes + (size_t)(2 * cap + 2) * sizeof(bem_c64);
 }
 
-int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
-            const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv,
-            const std::function<int()>& main_work, AcaScratch& S,
-            std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev,
-            const PartView* pv = nullptr)
+// The set-up half of run_aca: scratch at the given capacities, offsets,
+// the listed blocks ordered by size class (cls, largest first within each),
+// the results array, and the kernel arguments (blist = the ordered list)
+int aca_prepare(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed,
+                const std::vector<int32_t>& list, const std::vector<int32_t>& cap, const int64_t* d_pivoff,
+                const int32_t* d_piv, AcaScratch& S, AcaArgs& a, std::vector<int32_t> (&cls)[3], size_t& shm,
+                const PartView* pv = nullptr)
 {
     const std::vector<bemsetup::Block>& blocks = pv ? *pv->blocks : c->blocks;
     int64_t nb = (int64_t)blocks.size();
@@ -3532,7 +3598,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     // own streams so
     // that their long tails overlap the small blocks (and the caller's dense
     // tiles); within each class the largest blocks start first.
-    std::vector<int32_t> cls[3];
+    for (auto& v : cls) v.clear();
     for (int32_t b : list) {
         int64_t s2 = blocks[b].rows + blocks[b].cols;
         const int pi = prec == BEM_PREC_FP32 ? 1 : 0;
@@ -3555,7 +3621,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     S.d_ev = (int64_t*)res;
     S.d_rank = (int32_t*)((char*)res + (size_t)nb * 8);
     S.d_flag = S.d_rank + nb;
-    AcaArgs a;
+    a = AcaArgs();
     a.M = c->mesh;
     a.F = dev_freq(c);
     a.F32 = dev_freq32(c);
@@ -3576,9 +3642,40 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
     a.nf = 0; a.nbk = nb; a.kf = nullptr; a.ccf = nullptr; a.singf = nullptr;
     a.Uf = a.Vf = a.Af = nullptr;
     a.queue = nullptr; a.nitems = 0;
-    // dynamic LDS of aca_block: sw[cap] of the storage type, then the FP64
-    // dot products sdots[2 cap + 2]
-    size_t shm = aca_shm_bytes(maxcap, esz);
+    shm = aca_shm_bytes(maxcap, esz);
+    return BEM_OK;
+}
+
+// The result half of run_aca: ranks, flags and evaluation counts of the
+// nb blocks of S's partition (after the launches, on c's stream)
+int aca_collect(bem_ctx* c, const AcaScratch& S, int64_t nb, std::vector<int32_t>& rank, std::vector<int32_t>& flag,
+                std::vector<int64_t>& ev)
+{
+    int rc;
+    rank = c->vp.take<int32_t>(nb, 0); flag = c->vp.take<int32_t>(nb, 0); ev = c->vp.take<int64_t>(nb, 0);
+    char* h = nullptr;
+    if ((rc = stage_get(c, (size_t)nb * 16, &h))) return rc;
+    HIPC(c, hipMemcpyAsync(h, S.d_ev, (size_t)nb * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    std::memcpy(ev.data(), h, (size_t)nb * 8);
+    std::memcpy(rank.data(), h + (size_t)nb * 8, (size_t)nb * 4);
+    std::memcpy(flag.data(), h + (size_t)nb * 12, (size_t)nb * 4);
+    return BEM_OK;
+}
+
+int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_t seed, const std::vector<int32_t>& list,
+            const std::vector<int32_t>& cap, const int64_t* d_pivoff, const int32_t* d_piv,
+            const std::function<int()>& main_work, AcaScratch& S,
+            std::vector<int32_t>& rank, std::vector<int32_t>& flag, std::vector<int64_t>& ev,
+            const PartView* pv = nullptr)
+{
+    const int64_t nb = (int64_t)(pv ? pv->blocks->size() : c->blocks.size());
+    AcaArgs a;
+    std::vector<int32_t> cls[3];
+    size_t shm = 0;
+    int rc;
+    if ((rc = aca_prepare(c, kind, prec, tol, maxrank, seed, list, cap, d_pivoff, d_piv, S, a, cls, shm, pv)))
+        return rc;
     HIPC(c, hipEventRecord(c->ev_fork, c->stream));
     size_t off = 0;
     for (int q = 0; q < 3; q++) {
@@ -3599,15 +3696,7 @@ int run_aca(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uint64_
         HIPC(c, hipEventRecord(c->ev_join[q], c->side[q]));
         HIPC(c, hipStreamWaitEvent(c->stream, c->ev_join[q], 0));
     }
-    rank = c->vp.take<int32_t>(nb, 0); flag = c->vp.take<int32_t>(nb, 0); ev = c->vp.take<int64_t>(nb, 0);
-    char* h = nullptr;
-    if ((rc = stage_get(c, (size_t)nb * 16, &h))) return rc;
-    HIPC(c, hipMemcpyAsync(h, res, (size_t)nb * 16, hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
-    std::memcpy(ev.data(), h, (size_t)nb * 8);
-    std::memcpy(rank.data(), h + (size_t)nb * 8, (size_t)nb * 4);
-    std::memcpy(flag.data(), h + (size_t)nb * 12, (size_t)nb * 4);
-    return BEM_OK;
+    return aca_collect(c, S, nb, rank, flag, ev);
 }
 
 // Packs the ACA results of h's assembly (first pass S1, re-run of the
@@ -4331,6 +4420,241 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
     c->k = k0;
     c->cc = cc0;
     free_list(c, own);
+    return BEM_OK;
+}
+
+int bem_assemble_multi(const bem_ctx_t* ctxs, int64_t nctx, int kind, double tol, int64_t maxrank, int precision,
+                       uint64_t seed, bem_hmat_t* outs)
+{
+    if (!ctxs || !outs || nctx < 1 || nctx > 2048 || !ctxs[0]) return BEM_ERR_ARG;
+    bem_ctx* lead = ctxs[0];
+    if (kind != BEM_KIND_PHI && kind != BEM_KIND_DPHI) return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: bad kind");
+    if (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)
+        return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: bad precision");
+    if (!(tol >= 0)) return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: negative tolerance");
+    for (int64_t q = 0; q < nctx; q++) {
+        bem_ctx* c = ctxs[q];
+        if (!c) return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: null context");
+        if (c->device != lead->device) return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: contexts on other devices");
+        if (!c->have_part || !c->have_freq)
+            return set_err(lead, BEM_ERR_STATE, "bem_assemble_multi: partition or frequency missing");
+        if (c->mesh.rvtx) return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: not with an impedance plane");
+        if ((int64_t)c->blocks.size() > kJobMask)
+            return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: more than 2^20 blocks in a context");
+        for (int64_t e = 0; e < q; e++)
+            if (ctxs[e] == c) return set_err(lead, BEM_ERR_ARG, "bem_assemble_multi: a context listed twice");
+    }
+    HIPC(lead, hipSetDevice(lead->device));
+    const double t0 = now_ms();
+    const bool f32 = precision == BEM_PREC_FP32;
+    const int pi = f32 ? 1 : 0;
+    const size_t esz = f32 ? sizeof(bem_c32) : sizeof(bem_c64);
+    std::vector<bem_hmat*> H((size_t)nctx, nullptr);
+    std::vector<AcaScratch> S1((size_t)nctx);
+    std::vector<AcaArgs> A((size_t)nctx);
+    std::vector<std::vector<int32_t>> caps((size_t)nctx);
+    std::vector<std::array<std::vector<int32_t>, 3>> cls((size_t)nctx);
+    std::vector<void*> lead_own;
+    auto fail = [&](int code) {
+        for (auto* h : H)
+            if (h) bem_hmat_destroy(h);
+        free_list(lead, lead_own);
+        return code;
+    };
+    int rc;
+    size_t shm = 0;
+    for (int64_t q = 0; q < nctx; q++) {   // per context: near field, singular table, ACA scratch
+        bem_ctx* c = ctxs[q];
+        drop_kept(c);
+        const bool n64 = f32 && c->near64;
+        H[q] = new bem_hmat();
+        H[q]->ctx = c;
+        H[q]->kind = kind;
+        H[q]->prec = precision;
+        H[q]->esz = esz;
+        if ((rc = dalloc_bytes(c, &H[q]->dense, (size_t)c->dense_total * esz))) return fail(rc);
+        if (n64 && (rc = dalloc_bytes(c, &H[q]->dense64, (size_t)c->dense_total * sizeof(bem_c64)))) return fail(rc);
+        const int64_t nitems = c->nnz + c->n;
+        {
+            Timer t(c, 1);
+            hipLaunchKernelGGL(k_singular, dim3(nblk(nitems, kWG)), dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), kind,
+                               c->d_sing_order, nitems, c->nnz, c->d_sing, nullptr);
+            if ((rc = launch_check(c, "k_singular"))) return fail(rc);
+        }
+        bem_c64* d64 = (bem_c64*)(n64 ? H[q]->dense64 : H[q]->dense);
+        if (!c->tiles.empty()) {
+            Timer t(c, 0);
+            const dim3 g((unsigned)c->tiles.size());
+            if (f32 && !n64)
+                hipLaunchKernelGGL(k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq32(c), c->d_subst, kind,
+                                   c->d_tiles, (bem_c32*)H[q]->dense, (bem_c32*)nullptr);
+            else
+                hipLaunchKernelGGL(k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), c->d_tiles,
+                                   kind == BEM_KIND_PHI ? d64 : nullptr, kind == BEM_KIND_DPHI ? d64 : nullptr);
+            if ((rc = launch_check(c, "k_dense"))) return fail(rc);
+        }
+        if (!c->fixes.empty()) {
+            const dim3 g(nblk((int64_t)c->fixes.size(), kWG));
+            if (f32 && !n64)
+                hipLaunchKernelGGL(k_scatter<bem_c32>, g, dim3(kWG), 0, c->stream, c->d_fixes, (int64_t)c->fixes.size(),
+                                   c->nnz, c->d_sing, (bem_c32*)H[q]->dense);
+            else
+                hipLaunchKernelGGL(k_scatter<bem_c64>, g, dim3(kWG), 0, c->stream, c->d_fixes, (int64_t)c->fixes.size(),
+                                   c->nnz, c->d_sing, d64);
+            if ((rc = launch_check(c, "k_scatter"))) return fail(rc);
+        }
+        if (n64 && c->dense_total) {
+            hipLaunchKernelGGL(k_round32, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream, c->dense_total,
+                               (const bem_c64*)d64, (bem_c32*)H[q]->dense);
+            if ((rc = launch_check(c, "k_round32"))) return fail(rc);
+        }
+        int64_t* d_pivoff = nullptr;
+        int32_t* d_piv = nullptr;
+        if (kind == BEM_KIND_DPHI) {
+            if ((rc = ensure_pivot_lists(c, seed))) return fail(rc);
+            d_pivoff = c->d_pivoff;
+            d_piv = c->d_piv;
+        }
+        const int64_t nb = (int64_t)c->blocks.size();
+        caps[q].assign(nb, 0);
+        for (int32_t b : c->adm_list) {
+            int64_t m = std::min(c->blocks[b].rows, c->blocks[b].cols);
+            if (maxrank > 0) m = std::min(m, maxrank);
+            caps[q][b] = (int32_t)std::min<int64_t>(m, kCapFirst);
+        }
+        // every context classifies its blocks by the lead's schedule
+        const int64_t keep_s = c->cls_small[pi], keep_h = c->cls_huge[pi];
+        c->cls_small[pi] = lead->cls_small[pi];
+        c->cls_huge[pi] = lead->cls_huge[pi];
+        size_t shm_q = 0;
+        std::vector<int32_t> cq[3];
+        c->sing_cur = c->d_sing;
+        rc = aca_prepare(c, kind, precision, tol, maxrank, seed, c->adm_list, caps[q], d_pivoff, d_piv, S1[q], A[q], cq,
+                         shm_q);
+        c->sing_cur = nullptr;
+        c->cls_small[pi] = keep_s;
+        c->cls_huge[pi] = keep_h;
+        if (rc) return fail(rc);
+        for (int k = 0; k < 3; k++) cls[q][k] = std::move(cq[k]);
+        shm = std::max(shm, shm_q);
+        // the lead's launches wait for this context's near field, table and uploads
+        HIPC(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPC(lead, hipStreamWaitEvent(lead->stream, c->ev_fork, 0));
+    }
+    // the jobs, and the items of every size class: all contexts' blocks,
+    // the largest first
+    std::vector<MeshJob> jobs((size_t)nctx);
+    for (int64_t q = 0; q < nctx; q++) {
+        const AcaArgs& a = A[q];
+        MeshJob& J = jobs[q];
+        J.M = a.M; J.F = a.F; J.F32 = a.F32;
+        J.row0 = a.row0; J.rows = a.rows; J.col0 = a.col0; J.cols = a.cols;
+        J.uoff = a.uoff; J.voff = a.voff; J.aoff = a.aoff; J.cap = a.cap;
+        J.U = a.U; J.V = a.V; J.A = a.A;
+        J.rank_out = a.rank_out; J.flag_out = a.flag_out; J.evals_out = a.evals_out;
+        J.piv_off = a.piv_off; J.piv = a.piv;
+        J.sing = a.sing; J.nnz = a.nnz;
+    }
+    std::vector<int32_t> items;
+    size_t ncls[3];
+    for (int k = 0; k < 3; k++) {
+        std::vector<std::pair<int64_t, int32_t>> v;
+        for (int64_t q = 0; q < nctx; q++)
+            for (int32_t b : cls[q][k])
+                v.push_back({ctxs[q]->blocks[b].rows + ctxs[q]->blocks[b].cols, (int32_t)((q << kJobShift) | b)});
+        std::stable_sort(v.begin(), v.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        for (auto& e : v) items.push_back(e.second);
+        ncls[k] = v.size();
+    }
+    MeshJob* d_jobs = nullptr;
+    int32_t* d_items = nullptr;
+    int* d_queue = nullptr;
+    if ((rc = dupload(lead, &d_jobs, jobs.data(), jobs.size(), &lead_own)) ||
+        (!items.empty() && (rc = dupload(lead, &d_items, items.data(), items.size(), &lead_own))) ||
+        (rc = dalloc(lead, &d_queue, (size_t)3, &lead_own)))
+        return fail(rc);
+    if (lead->aca_queue) HIPC(lead, hipMemsetAsync(d_queue, 0, 3 * sizeof(int), lead->stream));
+    AcaArgs a = A[0];
+    a.jobs = d_jobs;
+    HIPC(lead, hipEventRecord(lead->ev_fork, lead->stream));
+    const int nt[3] = {pi ? kHugeThreads<bem_c32> : kHugeThreads<bem_c64>,
+                       pi ? kMidThreads<bem_c32> : kMidThreads<bem_c64>, 64};
+    size_t off = 0;
+    for (int k = 0; k < 3; k++) {
+        AcaArgs aq = a;
+        aq.blist = d_items + off;
+        off += ncls[k];
+        if (!ncls[k]) continue;
+        hipStream_t st = k < 2 ? lead->side[k] : lead->stream;
+        if (k < 2) HIPC(lead, hipStreamWaitEvent(st, lead->ev_fork, 0));
+        unsigned grid = (unsigned)ncls[k];
+        if (lead->aca_queue) {
+            aq.queue = d_queue + k;
+            aq.nitems = (int)ncls[k];
+            grid = (unsigned)std::min<size_t>(ncls[k], (size_t)(256 * 4 * 4 * 64 / nt[k]));
+        }
+        Timer t(lead, 2, st);
+        const bool x = kind == BEM_KIND_DPHI;
+        if (f32) launch_aca<bem_c32>(k, x, grid, shm, st, aq);
+        else launch_aca<bem_c64>(k, x, grid, shm, st, aq);
+        if ((rc = launch_check(lead, "k_aca (multi)"))) return fail(rc);
+    }
+    for (int k = 0; k < 2; k++) {
+        HIPC(lead, hipEventRecord(lead->ev_join[k], lead->side[k]));
+        HIPC(lead, hipStreamWaitEvent(lead->stream, lead->ev_join[k], 0));
+    }
+    HIPC(lead, hipStreamSynchronize(lead->stream));
+    const double t1 = now_ms();
+    // per context: results, re-run of the overflowing blocks, packing
+    for (int64_t q = 0; q < nctx; q++) {
+        bem_ctx* c = ctxs[q];
+        const int64_t nb = (int64_t)c->blocks.size();
+        std::vector<int32_t> rank, flag;
+        std::vector<int64_t> evs;
+        if ((rc = aca_collect(c, S1[q], nb, rank, flag, evs))) return fail(rc);
+        std::vector<int32_t> over, keep;
+        for (int32_t b : c->adm_list) {
+            if (flag[b] == 2)
+                return fail(set_err(lead, BEM_ERR_NAN, "bem_assemble_multi: non-finite value in ACA block " +
+                                                          std::to_string(b) + " of context " + std::to_string(q)));
+            (flag[b] == 1 ? over : keep).push_back(b);
+        }
+        AcaScratch S2;
+        if (!over.empty()) {
+            std::vector<int32_t> cap2(nb, 0), rank2, flag2;
+            std::vector<int64_t> evs2;
+            for (int32_t b : over) {
+                int64_t m = std::min(c->blocks[b].rows, c->blocks[b].cols);
+                if (maxrank > 0) m = std::min(m, maxrank);
+                cap2[b] = (int32_t)std::min<int64_t>(m, kCapMax);
+            }
+            c->sing_cur = c->d_sing;
+            rc = run_aca(c, kind, precision, tol, maxrank, seed, over, cap2, kind == BEM_KIND_DPHI ? c->d_pivoff : nullptr,
+                         kind == BEM_KIND_DPHI ? c->d_piv : nullptr, [] { return BEM_OK; }, S2, rank2, flag2, evs2);
+            c->sing_cur = nullptr;
+            if (rc) return fail(rc);
+            for (int32_t b : over) {
+                rank[b] = rank2[b];
+                evs[b] = evs2[b];
+                if (flag2[b])
+                    return fail(set_err(lead, flag2[b] == 2 ? BEM_ERR_NAN : BEM_ERR_ARG,
+                                        "bem_assemble_multi: ACA block " + std::to_string(b) + " of context " +
+                                            std::to_string(q) +
+                                            (flag2[b] == 2 ? " non-finite"
+                                                           : " needs rank > " + std::to_string(kCapMax))));
+            }
+        }
+        if ((rc = finish_hmat(c, H[q], f32, S1[q], S2, rank, evs, over, keep, nullptr))) return fail(rc);
+        HIPC(c, hipStreamSynchronize(c->stream));
+    }
+    const double wall = now_ms() - t0;
+    for (int64_t q = 0; q < nctx; q++) {   // the launch's times, shared out evenly
+        H[q]->stats[4] = wall / nctx;
+        H[q]->stats[7] = (t1 - t0) / nctx;
+        outs[q] = H[q];
+        H[q] = nullptr;
+    }
+    free_list(lead, lead_own);
     return BEM_OK;
 }
 

@@ -154,6 +154,17 @@ int bem_hmat_destroy(bem_hmat_t h);
    counter (1) or one workgroup per item (0).  Results do not depend on it. */
 int bem_assemble_batch(bem_ctx_t ctx, int kind, int64_t nf, const double* kcc3, double aca_tol, int64_t max_rank,
                        int precision, uint64_t seed, bem_hmat_t* outs);
+/* Several meshes at once (no counterpart in the reference, which solves
+   one mesh after another): the operator `kind` of every context in ctxs
+   (nctx, all on one device; each with its own mesh, partition, boundary
+   conditions and frequency), returned in outs[q] for ctxs[q] and equal to
+   bem_assemble on that context.  Near fields and singular tables per
+   context; the ACA of every (context, block) item in one launch per size
+   class on the first context's streams, sized by its schedule (and, with
+   its work queue on, as persistent workgroups).  No impedance plane, no
+   near-field pairing (a kept near field is dropped). */
+int bem_assemble_multi(const bem_ctx_t* ctxs, int64_t nctx, int kind, double tol, int64_t max_rank, int precision,
+                       uint64_t seed, bem_hmat_t* outs);
 int bem_set_aca_schedule(bem_ctx_t ctx, int precision, int64_t small_bound, int64_t huge_bound, int queue);
 /* ACA schedule autotuner (no counterpart in the reference, whose per-block
    tasks go to a thread pool, hmatrix.cpp:793-846): times the assembly of

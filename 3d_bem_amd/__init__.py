@@ -62,6 +62,8 @@ _SIGS = {
     "bem_solver_set_sweep_batch": (_int, [_p, _int]),
     "bem_set_fuse_ops": (_int, [_p, _int]),
     "bem_assemble_multi": (_int, [_p, _i64, _int, _dbl, _i64, _int, _u64, _p]),
+    "bem_solvers_solve_multi": (_int, [_p, _i64, _p, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _int, _p, _p,
+                                       _p]),
     "bem_autotune_aca": (_int, [_p, _int, _int, _i64, _p, _dbl, _i64, _u64, _p, _p]),
     "bem_solver_set_autotune": (_int, [_p, _int]),
     "bem_solver_aca_schedule": (_int, [_p, _int, _p]),
@@ -367,6 +369,27 @@ def assemble_multi(ctxs, kind, tol, maxrank=0, precision=FP64, seed=1):
     if rc != BEM_OK:
         ctxs[0]._check(rc)
     return [HMatrix(c, _p(h)) for c, h in zip(ctxs, outs)]
+
+
+def solve_multi(solvers, freqs, c=343.0, rho=1.21, coupling=1, aca_tol=1e-5, maxrank=0, gmres_tol=None, restart=300,
+                seed=1, precision=FP64, threads=4):
+    """bem_solvers_solve_multi: solver q at freqs[q], Phi and DPhi of all
+    meshes from one multi-mesh assembly each; pressures (cluster order),
+    GMRES iterations, stats per solver"""
+    freqs = _c(freqs, np.float64)
+    ns = len(solvers)
+    hv = (_p * ns)(*[S.h for S in solvers])
+    outs = [np.empty(S.n, np.complex128) for S in solvers]
+    pv = (_p * ns)(*[o.ctypes.data for o in outs])
+    its = np.zeros(ns, np.int64)
+    st = np.zeros((ns, 16))
+    gt = -1.0 if gmres_tol is None else float(gmres_tol)
+    rc = lib().bem_solvers_solve_multi(ctypes.cast(hv, _p), ns, _ptr(freqs), float(c), float(rho), int(coupling),
+                                       float(aca_tol), int(maxrank), gt, int(restart), int(seed), int(precision),
+                                       int(threads), ctypes.cast(pv, _p), _ptr(its), _ptr(st))
+    if rc != BEM_OK:
+        raise BemError(rc, lib().bem_solver_last_error(solvers[0].h).decode())
+    return outs, its, [dict(zip(STAT_KEYS, r.tolist())) for r in st]
 
 
 class HMatrix:

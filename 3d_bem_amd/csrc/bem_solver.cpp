@@ -121,6 +121,11 @@ void HMatrix::assembleBlocksExtra(long rank, double relErr, uint64_t seed, int p
 {
     assemble(BEM_KIND_DPHI, rank, relErr, seed, precision);
 }
+void HMatrix::adopt(bem_hmat_t h)
+{
+    clear();
+    h_ = h;
+}
 void HMatrix::clear()
 {
     if (h_) bem_hmat_destroy(h_);
@@ -371,6 +376,11 @@ void BoundaryElementSolver::assembleRhsOperator()
 {
     const auto& P = bemSolverParameters;
     const int R = refineMode(P);
+    if (presetPhi) {
+        phiHMat->adopt(presetPhi);
+        presetPhi = nullptr;
+        return;
+    }
     double t0 = now_ms();
     phiHMat->assembleBlocks((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, R == 2 ? BEM_PREC_FP64 : P.precision);
     if (R == 2) refineStats[3] += now_ms() - t0;
@@ -741,6 +751,76 @@ void BoundaryElementSolver::tuneAca(const std::vector<double>& freqs, int64_t ch
           "bem_autotune_aca");
 }
 
+void BoundaryElementSolver::solveMany(const std::vector<BoundaryElementSolver*>& solvers,
+                                      const std::vector<double>& freqs, int threads)
+{
+    const size_t ns = solvers.size();
+    if (ns == 0) return;
+    if (freqs.size() != ns) throw Error(BEM_ERR_ARG, "solveMany: one frequency per solver");
+    const auto& P0 = solvers[0]->bemSolverParameters;
+    const int R = refineMode(P0);
+    std::vector<bem_ctx_t> ctxs(ns);
+    for (size_t q = 0; q < ns; q++) {
+        auto& S = *solvers[q];
+        if (!S.boundaryElements.impedancePlanes.empty())
+            throw Error(BEM_ERR_ARG, "solveMany: not with an impedance plane");
+        S.bemSolverParameters = P0;
+        S.setFrequency(freqs[q]);
+        S.setWavenumberAndCoupling();
+        check(S.ctx_, bem_set_near_fp64(S.ctx_, R == 1), "bem_set_near_fp64");
+        ctxs[q] = S.ctx_;
+    }
+    std::vector<bem_hmat_t> phi(ns, nullptr), dphi(ns, nullptr);
+    auto destroy = [](std::vector<bem_hmat_t>& v) {
+        for (bem_hmat_t h : v)
+            if (h) bem_hmat_destroy(h);
+    };
+    int rc = bem_assemble_multi(ctxs.data(), (int64_t)ns, BEM_KIND_PHI, P0.acaRelativeError, (int64_t)P0.acaMaxRank,
+                                R == 2 ? BEM_PREC_FP64 : P0.precision, P0.pivotSeed, phi.data());
+    if (rc == BEM_OK)
+        rc = bem_assemble_multi(ctxs.data(), (int64_t)ns, BEM_KIND_DPHI, P0.acaRelativeError, (int64_t)P0.acaMaxRank,
+                                P0.precision, P0.pivotSeed, dphi.data());
+    if (rc != BEM_OK) {
+        destroy(phi);
+        destroy(dphi);
+        check(ctxs[0], rc, "bem_assemble_multi");
+    }
+    for (size_t q = 0; q < ns; q++) {
+        solvers[q]->presetPhi = phi[q];
+        solvers[q]->presetDPhi = dphi[q];
+    }
+    // the solves: every one takes its preset operators (the frequency is set again, unchanged)
+    const int L = std::max(1, std::min<int>(threads, (int)ns));
+    std::vector<std::string> errs(L);
+    std::vector<int> codes(L, BEM_OK);
+    auto work = [&](int t) {
+        for (size_t q = t; q < ns; q += L) {
+            try {
+                solvers[q]->calculateBoundarySolution();
+            } catch (const Error& e) {
+                codes[t] = e.code;
+                errs[t] = e.msg;
+                return;
+            } catch (const std::exception& e) {
+                codes[t] = BEM_ERR_HIP;
+                errs[t] = e.what();
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < L; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    for (size_t q = 0; q < ns; q++) {   // operators a failed solve did not take
+        if (solvers[q]->presetPhi) bem_hmat_destroy(solvers[q]->presetPhi);
+        if (solvers[q]->presetDPhi) bem_hmat_destroy(solvers[q]->presetDPhi);
+        solvers[q]->presetPhi = solvers[q]->presetDPhi = nullptr;
+    }
+    for (int t = 0; t < L; t++)
+        if (codes[t] != BEM_OK) throw Error(codes[t], errs[t]);
+}
+
 void BoundaryElementSolver::regularSolve(double tol, long restart)
 {
     setWavenumberAndCoupling();
@@ -830,7 +910,12 @@ void BoundaryElementSolver::hMatrixSolve()
     phiHMat->clear();
     double t2 = now_ms();
     assembleResidualOperator();
-    dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    if (presetDPhi) {
+        dPhiHMat->adopt(presetDPhi);
+        presetDPhi = nullptr;
+    } else {
+        dPhiHMat->assembleBlocksExtra((long)P.acaMaxRank, P.acaRelativeError, P.pivotSeed, P.precision);
+    }
     check(ctx_, bem_synchronize(ctx_), "sync");
     double t3 = now_ms();
     stats[8] = dPhiHMat->stat(1);
@@ -1021,6 +1106,54 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         return e.code;
     } catch (const std::exception& e) {
         s->err = e.what();
+        return BEM_ERR_HIP;
+    }
+    return BEM_OK;
+}
+
+int bem_solvers_solve_multi(const bem_solver_t* ss, int64_t ns, const double* freqs, double c, double rho, int coupling,
+                            double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed,
+                            int precision, int threads, double* const* pressure, int64_t* iters, double* stats16)
+{
+    if (!ss || ns < 1 || !freqs || coupling < 0 || coupling > 2 || max_rank < 0 || restart < 1 || threads < 1)
+        return BEM_ERR_ARG;
+    for (int64_t q = 0; q < ns; q++)
+        if (!ss[q] || !ss[q]->s) return BEM_ERR_ARG;
+    bem_solver_t s0 = ss[0];
+    try {
+        std::vector<bem::BoundaryElementSolver*> v(ns);
+        for (int64_t q = 0; q < ns; q++) {
+            auto& S = *ss[q]->s;
+            S.setWaveSpeed(c);
+            S.setAirDensity(rho);
+            v[q] = &S;
+        }
+        auto& P = v[0]->bemSolverParameters;
+        P.acaRelativeError = aca_tol;
+        P.acaMaxRank = (unsigned long)max_rank;
+        P.coupling = (bem::BemCoupling)coupling;
+        P.gmresTolerance = gmres_tol;
+        P.gmresRestart = (long)restart;
+        P.pivotSeed = seed;
+        P.precision = precision;
+        P.preconditioner = s0->preconditioner;
+        P.preconditionerRank = s0->hluRank;
+        P.preconditionerRelError = s0->hluRelError;
+        P.refine = s0->refine;
+        P.refineMaxSteps = s0->refineMaxSteps;
+        bem::BoundaryElementSolver::solveMany(v, std::vector<double>(freqs, freqs + ns), threads);
+        for (int64_t q = 0; q < ns; q++) {
+            auto& S = *v[q];
+            if (pressure && pressure[q])
+                std::memcpy(pressure[q], S.soundPressure.data(), S.boundaryElements.n * sizeof(bem::cplx));
+            if (iters) iters[q] = (int64_t)S.stats[5];
+            if (stats16) std::memcpy(stats16 + 16 * q, S.stats, sizeof S.stats);
+        }
+    } catch (const bem::Error& e) {
+        s0->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        s0->err = e.what();
         return BEM_ERR_HIP;
     }
     return BEM_OK;

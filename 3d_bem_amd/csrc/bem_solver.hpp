@@ -72,6 +72,8 @@ public:
     HMatrix(bem_ctx_t ctx) : ctx_(ctx) {}
     ~HMatrix() { clear(); }
     void setClusterTrees(std::shared_ptr<ClusterTree> rows, std::shared_ptr<ClusterTree> cols);
+    // takes an operator assembled elsewhere on this context (bem_assemble_multi)
+    void adopt(bem_hmat_t h);
     // HMatrix::populateBlockClusterTree (hmatrix.cpp:922-1007); admissibility
     // constant eta (reference constexpr 3, hmatrix.h:240)
     void populateBlockClusterTree(double eta = 3.0);
@@ -207,6 +209,16 @@ public:
     std::vector<PointSource> pointSources;
     std::vector<PlaneWave> planeWaves;
     std::vector<cplx> phiSolution, dPhiSolution, soundPressure;
+    // The next solve's Phi / DPhi already assembled on this solver's context
+    // (solveMany): taken instead of assembling, once.
+    bem_hmat_t presetPhi = nullptr, presetDPhi = nullptr;
+    // BemSolverController::run over several meshes: each solver at its own
+    // frequency, Phi and DPhi of all of them from one multi-mesh assembly
+    // each (bem_assemble_multi; the solvers' parameters, seed and precision
+    // of the first), then the solves on up to `threads` host threads.
+    // Refinement mode 2 assembles its FP64 operators per solver.
+    static void solveMany(const std::vector<BoundaryElementSolver*>& solvers, const std::vector<double>& freqs,
+                          int threads);
     double frequency = 0, waveSpeed = 343.0, airDensity = 1.2041;
     double stats[16] = {0};
     double precondStats[5] = {0};   // bem_precond_stats of the last solve's preconditioner

@@ -57,38 +57,53 @@ def test_sixteen_ellipsoids(bem, O, meshes):
     """16 ellipsoids, semi-axes uniform in [0.2, 1.0] m (seeded), 5120 or
     20480 triangles (250880 in all), each at a frequency uniform in
     [200, 2000] Hz with k max(axes) <= 20, normal velocity N(0, 1) per
-    element, Burton-Miller, FP32: every solve converges, FP32 agrees with
-    FP64 on four of them, and the FP64 path equals the CPU restatement on the
-    smallest one."""
+    element, Burton-Miller, FP32, solved together (bem_solvers_solve_multi:
+    Phi and DPhi of all 16 meshes from one multi-mesh ACA launch per size
+    class each): every solve converges, each pressure equals the same
+    solver's own FP32 solve bit for bit on four of them, FP32 agrees with FP64
+    there, and the FP64 path equals the CPU restatement on a 5120-triangle one."""
+    import time
     rng = np.random.default_rng(5)
     c, rho = 343.0, 1.21
     levels = rng.permutation([5] * 11 + [4] * 5)
-    total = 0
-    checked = 0
+    solvers, freqs, bcs, tris = [], [], [], []
     for q, level in enumerate(levels):
         axes = rng.uniform(0.2, 1.0, 3)
         f = rng.uniform(200.0, 2000.0)
         f = min(f, 20.0 * c / (2 * np.pi * axes.max()))
         tri = meshes.ellipsoid(int(level), axes)
-        n = len(tri)
-        total += n
-        bc = meshes.robin_bc(n, 0.0, 1.0, rng.standard_normal(n))
-        S = bem.Solver(tri, bc, leaf=64, eta=2.0)
-        _, _, p32, st = S.solve(f, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=q, precision=bem.FP32)
-        assert st["residual"] < 1e-7, (q, st["residual"])
-        if q % 4 == 0:
-            _, _, p64, _ = S.solve(f, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=q, precision=bem.FP64)
-            e = rel(p32, p64)
-            print("ellipsoid", q, "n", n, "f", round(f, 1), "axes", np.round(axes, 3), "FP32 vs FP64", e)
-            assert e < 1e-4
-            if level == 4 and checked == 0:
-                P = O.Problem(tri, bc, leaf=64, eta=2.0)
-                P.set_frequency(f, c, 1)
-                _, _, pc, _ = P.solve(1e-5, gmres_tol=1e-8, seed=q, rho=rho, freq=f)
-                assert rel(p64, pc) < 1e-10
-                checked += 1
+        bc = meshes.robin_bc(len(tri), 0.0, 1.0, rng.standard_normal(len(tri)))
+        solvers.append(bem.Solver(tri, bc, leaf=64, eta=2.0))
+        freqs.append(f)
+        bcs.append(bc)
+        tris.append(tri)
+    assert sum(len(t) for t in tris) == 11 * 20480 + 5 * 5120
+    t0 = time.perf_counter()
+    p32, its, st = bem.solve_multi(solvers, freqs, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=7,
+                                   precision=bem.FP32, threads=4)
+    t_multi = time.perf_counter() - t0
+    assert all(x["residual"] < 1e-7 for x in st), [x["residual"] for x in st]
+    checked = 0
+    t_single = 0.0
+    for q in range(0, 16, 4):
+        S = solvers[q]
+        t0 = time.perf_counter()
+        _, _, ps, _ = S.solve(freqs[q], c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=7, precision=bem.FP32)
+        t_single += time.perf_counter() - t0
+        assert np.array_equal(ps.view(np.uint64), p32[q].view(np.uint64)), q
+        _, _, p64, _ = S.solve(freqs[q], c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=7, precision=bem.FP64)
+        e = rel(p32[q], p64)
+        print("ellipsoid", q, "n", len(tris[q]), "f", round(freqs[q], 1), "FP32 vs FP64", e, flush=True)
+        assert e < 1e-4
+        if levels[q] == 4 and checked == 0:
+            P = O.Problem(tris[q], bcs[q], leaf=64, eta=2.0)
+            P.set_frequency(freqs[q], c, 1)
+            _, _, pc, _ = P.solve(1e-5, gmres_tol=1e-8, seed=7, rho=rho, freq=freqs[q])
+            assert rel(p64, pc) < 1e-10
+            checked += 1
+    print("16 meshes solved together: %.3f s; four of them one by one: %.3f s" % (t_multi, t_single), flush=True)
+    for S in solvers:
         S.close()
-    assert total == 11 * 20480 + 5 * 5120
 
 
 def test_capsule76k_hlu_preconditioned(bem, meshes):

@@ -50,3 +50,23 @@ def test_multi_mesh_equals_single(bem, O, meshes):
                 H.close()
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("prec,refine", [(0, 0), (1, 1)])
+def test_solve_multi_equals_single_solves(bem, meshes, prec, refine):
+    """bem_solvers_solve_multi on three meshes at their own frequencies:
+    every pressure equals that solver's own bem_solver_solve bit for bit
+    (FP64; FP32 with refinement mode 1)."""
+    cases = [(meshes.icosphere(3, 0.5), 500.0), (meshes.ellipsoid(3, (0.5, 0.3, 0.2)), 900.0),
+             (meshes.icosphere(2, 0.3), 1300.0)]
+    solvers = [bem.Solver(t, meshes.robin_bc(len(t), 0, 1, 1), leaf=32, eta=2.0) for t, _ in cases]
+    for S in solvers:
+        S.set_refinement(refine)
+    freqs = [f for _, f in cases]
+    pm, its, st = bem.solve_multi(solvers, freqs, aca_tol=1e-6, gmres_tol=1e-10, seed=3, precision=prec, threads=2)
+    for q, S in enumerate(solvers):
+        _, _, ps, st1 = S.solve(freqs[q], aca_tol=1e-6, gmres_tol=1e-10, seed=3, precision=prec)
+        assert np.array_equal(ps.view(np.uint64), pm[q].view(np.uint64)), q
+        assert its[q] == st1["gmres_iters"]
+    for S in solvers:
+        S.close()

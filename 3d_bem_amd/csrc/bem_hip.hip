@@ -504,6 +504,28 @@ __device__ __forceinline__ bem_c64 ld64(const bem_c32* p)
     bem_c32 v = *p;
     return c64((double)v.re, (double)v.im);
 }
+// loads from device global memory through a pointer the compiler cannot
+// place (factor storage reached through argument arrays): the explicit global
+// address space gives global_load with a scalar base instead of flat loads,
+// which also wait on the LDS counter
+typedef float bem_v2f __attribute__((ext_vector_type(2)));
+typedef double bem_v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bem_c32 ldg(const bem_c32* p)
+{
+    const bem_v2f x = *(const __attribute__((address_space(1))) bem_v2f*)p;
+    return c32(x.x, x.y);
+}
+__device__ __forceinline__ bem_c64 ldg(const bem_c64* p)
+{
+    const bem_v2d x = *(const __attribute__((address_space(1))) bem_v2d*)p;
+    return c64(x.x, x.y);
+}
+__device__ __forceinline__ bem_c64 ldg64(const bem_c64* p) { return ldg(p); }
+__device__ __forceinline__ bem_c64 ldg64(const bem_c32* p)
+{
+    bem_c32 v = ldg(p);
+    return c64((double)v.re, (double)v.im);
+}
 template <class C> __device__ __forceinline__ C from64(bem_c64 z);
 template <> __device__ __forceinline__ bem_c64 from64<bem_c64>(bem_c64 z) { return z; }
 template <> __device__ __forceinline__ bem_c32 from64<bem_c32>(bem_c64 z) { return c32((float)z.re, (float)z.im); }
@@ -519,12 +541,25 @@ __device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b)
 // first index of the largest |v_i|^2 over [0, len); index `excl` counts as 0.
 // NT threads (NT / 64 wavefronts); every thread gets the result.
 template <int NT, class C>
-__device__ ArgMax block_argmax(const C* v, int64_t len, int64_t excl, ArgMax* sred)
+__device__ ArgMax block_argmax(const C* __restrict__ v, int64_t len, int64_t excl, ArgMax* sred)
 {
     const int tid = threadIdx.x;
     ArgMax a{-1.0, (int64_t)1 << 62};
-    for (int64_t t = tid; t < len; t += NT) {
-        double x = t == excl ? 0.0 : cx_abs2(v[t]);
+    int64_t t = tid;
+    // four elements per thread loaded together, then compared in index order
+    for (; t + 3 * NT < len; t += 4 * NT) {
+        C e[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) e[u] = ldg(v + t + u * NT);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t tu = t + u * NT;
+            double x = tu == excl ? 0.0 : cx_abs2(e[u]);
+            if (x > a.v) { a.v = x; a.i = tu; }
+        }
+    }
+    for (; t < len; t += NT) {
+        double x = t == excl ? 0.0 : cx_abs2(ldg(v + t));
         if (x > a.v) { a.v = x; a.i = t; }
     }
     for (int s = 32; s > 0; s >>= 1) {
@@ -620,39 +655,87 @@ constexpr int32_t kJobMask = (1 << kJobShift) - 1;
 // lane-strided / pairwise order.  Accumulation is FP64 for both types.
 constexpr int kDotGroup = 4;
 
+// Totals over the 64 lanes of eight values per lane, each by the pairwise
+// tree of the device reduction order (partial p plus partial p + s for
+// s = 32, 16, ..., 1), with the values spread over the lanes as the tree
+// narrows: at s = 32, 16 and 8 a lane keeps half of its values and trades
+// the other half with its partner (lane ^ s), so that each addition pairs
+// the same two partials as the tree (addition commutes, so the sums are the
+// same bits) while 20 lane exchanges replace 96.  Lane 8v (v = 0..7) returns
+// the total of value v.
+__device__ __forceinline__ double wave_sum8_spread(const double (&x)[8], int lane)
+{
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+    double y[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const double send = h5 ? x[j] : x[4 + j], keep = h5 ? x[4 + j] : x[j];
+        y[j] = keep + __shfl_xor(send, 32, 64);
+    }
+    double z[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const double send = h4 ? y[j] : y[2 + j], keep = h4 ? y[2 + j] : y[j];
+        z[j] = keep + __shfl_xor(send, 16, 64);
+    }
+    double w;
+    {
+        const double send = h3 ? z[0] : z[1], keep = h3 ? z[1] : z[0];
+        w = keep + __shfl_xor(send, 8, 64);
+    }
+    w = w + __shfl_xor(w, 4, 64);
+    w = w + __shfl_xor(w, 2, 64);
+    w = w + __shfl_xor(w, 1, 64);
+    return w;
+}
+
 // tasks over vector v (length len) against basis W (stride len): k = 0 is
-// |v|^2 into out[base], k >= 1 is v^H W_{k-1} into out[base + 2k]
+// |v|^2 into out[base], k >= 1 is v^H W_{k-1} into out[base + 2k].  The
+// group's loads of one element are issued together, unconditionally: a task
+// without a basis vector (k = 0, or k >= ntask) reads v itself and discards
+// the value, so that no load sits behind a branch and the group pays one
+// memory latency per element instead of one per task.
 template <int NT, class C>
-__device__ __forceinline__ void dot_group_pass(const C* v, const C* W, int64_t len, int ntask, bem_c64* out,
-                                               int base)
+__device__ __forceinline__ void dot_group_pass(const C* __restrict__ v, const C* __restrict__ W, int64_t len,
+                                               int ntask, bem_c64* out, int base)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int kb = w * kDotGroup; kb < ntask; kb += (NT / 64) * kDotGroup) {
+        const C* src[kDotGroup];
         bem_c64 acc[kDotGroup];
 #pragma unroll
-        for (int g = 0; g < kDotGroup; g++) acc[g] = c64(0, 0);
+        for (int g = 0; g < kDotGroup; g++) {
+            const int k = kb + g;
+            src[g] = (k >= 1 && k < ntask) ? W + (int64_t)(k - 1) * len : v;
+            acc[g] = c64(0, 0);
+        }
+        // branch-free: every task accumulates (the products of tasks without
+        // a basis vector are never written out) and |v|^2 is accumulated
+        // beside them, taking task 0's place in the first group
+        double nrm = 0.0;
         for (int64_t t = lane; t < len; t += 64) {
-            bem_c64 cv = ld64(v + t);
+            const bem_c64 cv = ldg64(v + t);
+            bem_c64 wv[kDotGroup];
 #pragma unroll
-            for (int g = 0; g < kDotGroup; g++) {
-                int k = kb + g;
-                if (k == 0) acc[g].re = acc[g].re + c64_abs2(cv);
-                else if (k < ntask) acc[g] = c64_fma(c64_conj(cv), ld64(W + (int64_t)(k - 1) * len + t), acc[g]);
-            }
+            for (int g = 0; g < kDotGroup; g++) wv[g] = ldg64(src[g] + t);
+            nrm = nrm + c64_abs2(cv);
+#pragma unroll
+            for (int g = 0; g < kDotGroup; g++) acc[g] = c64_fma(c64_conj(cv), wv[g], acc[g]);
         }
-        for (int s = 32; s > 0; s >>= 1) {
+        if (kb == 0) acc[0] = c64(nrm, 0.0);
+        static_assert(kDotGroup == 4, "wave_sum8_spread reduces four complex values");
+        double x[2 * kDotGroup];
 #pragma unroll
-            for (int g = 0; g < kDotGroup; g++) {
-                acc[g].re = acc[g].re + __shfl_down(acc[g].re, s, 64);
-                acc[g].im = acc[g].im + __shfl_down(acc[g].im, s, 64);
-            }
+        for (int g = 0; g < kDotGroup; g++) {
+            x[2 * g] = acc[g].re;
+            x[2 * g + 1] = acc[g].im;
         }
-        if (lane == 0) {
-#pragma unroll
-            for (int g = 0; g < kDotGroup; g++) {
-                int k = kb + g;
-                if (k == 0) out[base] = c64(acc[g].re, 0.0);
-                else if (k < ntask) out[base + 2 * k] = acc[g];
+        const double tot = wave_sum8_spread(x, lane);
+        if ((lane & 7) == 0) {
+            const int v = lane >> 3, k = kb + (v >> 1);
+            if (k < ntask) {
+                double* o = (double*)(out + (k == 0 ? base : base + 2 * k));
+                o[v & 1] = tot;
             }
         }
     }
@@ -660,8 +743,8 @@ __device__ __forceinline__ void dot_group_pass(const C* v, const C* W, int64_t l
 
 // sdots[0] |row|^2, [1] |col|^2, [2 + 2l] row^H V_l, [3 + 2l] col^H U_l (l < r)
 template <int NT, class C>
-__device__ void aca_dots(const C* col, const C* row, const C* U, const C* V, int64_t m, int64_t n, int r,
-                         bem_c64* sdots)
+__device__ __forceinline__ void aca_dots(const C* col, const C* row, const C* U, const C* V, int64_t m, int64_t n,
+                                         int r, bem_c64* sdots)
 {
     dot_group_pass<NT>(col, U, m, r + 1, sdots, 1);
     dot_group_pass<NT>(row, V, n, r + 1, sdots, 0);
@@ -702,12 +785,12 @@ __device__ __forceinline__ C cross_sum(const C* w, const C* B, int64_t ld64, int
     for (; l + kCrossUnroll <= r; l += kCrossUnroll) {
         C v[kCrossUnroll];
 #pragma unroll
-        for (int u = 0; u < kCrossUnroll; u++) v[u] = B[(uint32_t)(l + u) * ld + idx];
+        for (int u = 0; u < kCrossUnroll; u++) v[u] = ldg(B + ((uint32_t)(l + u) * ld + idx));
 #pragma unroll
         for (int u = 0; u < kCrossUnroll; u++) s = W_FIRST ? cx_fma(w[l + u], v[u], s) : cx_fma(v[u], w[l + u], s);
     }
     for (; l < r; l++) {
-        C v = B[(uint32_t)l * ld + idx];
+        C v = ldg(B + ((uint32_t)l * ld + idx));
         s = W_FIRST ? cx_fma(w[l], v, s) : cx_fma(v, w[l], s);
     }
     return s;
@@ -799,7 +882,7 @@ template <int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
                         const C* U, const C* V, const C* A, C* dst, C* sw)
 {
-    for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(U[(int64_t)l * m + i], A[l]);
+    for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(ldg(U + (int64_t)l * m + i), ldg(A + l));
     __syncthreads();
     const int64_t gi = r0 + i;
     for (int64_t j = threadIdx.x; j < n; j += NT) {
@@ -830,7 +913,7 @@ template <int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
                         const C* U, const C* V, const C* A, C* dst, C* sw)
 {
-    for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(A[l], V[(int64_t)l * n + j]);
+    for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(ldg(A + l), ldg(V + (int64_t)l * n + j));
     __syncthreads();
     const int64_t gj = c0 + j;
     for (int64_t i = threadIdx.x; i < m; i += NT) {
@@ -853,6 +936,18 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
         dst[i] = cx_sub(e, cross_sum<false>(sw, U, m, i, r));
     }
     __syncthreads();
+}
+
+// The entry counter of an ACA is the same in every lane; it is kept in scalar
+// registers (readfirstlane after each update).  In one compiled form of the
+// batched one-workgroup-per-item kernel its vector-register copy was spilled,
+// its register reused and the count stored wrong while every factor was
+// right; the scalar form does not depend on that register allocation.
+__device__ __forceinline__ int64_t uni64(int64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // ACA of one admissible block b by a workgroup of NT threads.  Plain
@@ -882,7 +977,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
     if (!EXTRA) {
         int64_t j = bem_rand_bounded(a.seed, 0, (uint64_t)b, 0, n);
         aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
-        ev += m;
+        ev = uni64(ev + m);
         int64_t i = -1;
         while (r < maxRank) {
             if (r >= cap) { flag = 1; break; }
@@ -892,12 +987,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             if (am.v <= tiny2 && r >= 1) break;
             C* row = V + (int64_t)r * n;
             aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
-            ev += n;
+            ev = uni64(ev + n);
             ArgMax ar = block_argmax<NT>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
             C* col = U + (int64_t)r * m;
             aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
-            ev += m;
+            ev = uni64(ev + m);
             C alpha = cx_inv(row[ar.i]);
             if (threadIdx.x == 0) A[r] = sA[r] = alpha;
             aca_dots<NT>(col, row, U, V, m, n, r, sdots);
@@ -914,12 +1009,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
         for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
             int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
             C pe = entry_any<PLANE, C, FIELD>(a, r0 + i, c0 + j);
-            ev += 1;
+            ev = uni64(ev + 1);
             if (cx_abs2(pe) < tiny2) continue;
             // the approximation at (i, j): its rank terms formed in parallel,
             // then summed in rank order by every thread
             for (int l = threadIdx.x; l < r; l += NT)
-                sw[l] = cx_mul(cx_mul(U[(int64_t)l * m + i], sA[l]), V[(int64_t)l * n + j]);
+                sw[l] = cx_mul(cx_mul(ldg(U + (int64_t)l * m + i), sA[l]), ldg(V + (int64_t)l * n + j));
             __syncthreads();
             C approx = cx_zero<C>();
             for (int l = 0; l < r; l++) approx = cx_add(approx, sw[l]);
@@ -927,7 +1022,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             if (cx_abs2(cx_sub(pe, approx)) < tol2 * cx_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
             aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
-            ev += m;
+            ev = uni64(ev + m);
             bool first = true;
             while (r < maxRank) {
                 if (r >= cap) { flag = 1; done = true; break; }
@@ -938,12 +1033,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 if (cx_abs2(wcol[i]) <= tiny2 && r >= 1) break;
                 C* row = V + (int64_t)r * n;
                 aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
-                ev += n;
+                ev = uni64(ev + n);
                 ArgMax ar = block_argmax<NT>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
                 C* col = U + (int64_t)r * m;
                 aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
-                ev += m;
+                ev = uni64(ev + m);
                 C alpha = cx_inv(row[ar.i]);
                 if (threadIdx.x == 0) A[r] = sA[r] = alpha;
                 aca_dots<NT>(col, row, U, V, m, n, r, sdots);

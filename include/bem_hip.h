@@ -394,14 +394,6 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
                      int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      double* phi, double* dphi, double* pressure, double* stats16);
 
-/* regularSolveWithLU (boundaryelementsolver.cpp:140-331), the reference's
-   dense path: every entry of Phi and DPhi in FP64 (the same entry
-   generator as the H-matrix assembly) on a context of its own, b = s + Phi f,
-   the dense system solved by GMRES to gmres_tol (the reference factorises
-   it by partial-pivoting LU), reconstruction and pressure as
-   bem_solver_solve.  Needs 2 n^2 x 16 bytes of device memory (13.4 GB at
-   n = 20480).  stats16: [0] assembly ms, [3] GMRES ms, [4] total ms,
-   [5] iterations, [6] GMRES relative residual, the rest 0. */
 /* Several meshes solved together (no counterpart in the reference, which
    runs them one after another): solver ss[q] at freqs[q], every other
    parameter shared (and the first solver's preconditioner and refinement
@@ -413,6 +405,15 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
 int bem_solvers_solve_multi(const bem_solver_t* ss, int64_t ns, const double* freqs, double c, double rho, int coupling,
                             double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed,
                             int precision, int threads, double* const* pressure, int64_t* iters, double* stats16);
+
+/* regularSolveWithLU (boundaryelementsolver.cpp:140-331), the reference's
+   dense path: every entry of Phi and DPhi in FP64 (the same entry
+   generator as the H-matrix assembly) on a context of its own, b = s + Phi f,
+   the dense system solved by GMRES to gmres_tol (the reference factorises
+   it by partial-pivoting LU), reconstruction and pressure as
+   bem_solver_solve.  Needs 2 n^2 x 16 bytes of device memory (13.4 GB at
+   n = 20480).  stats16: [0] assembly ms, [3] GMRES ms, [4] total ms,
+   [5] iterations, [6] GMRES relative residual, the rest 0. */
 int bem_solver_solve_dense(bem_solver_t s, double freq, double c, double rho, int coupling, double gmres_tol,
                            int64_t restart, double* phi, double* dphi, double* pressure, double* stats16);
 

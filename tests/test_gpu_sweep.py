@@ -47,9 +47,13 @@ def test_sweep_equals_single_solves(bem, O, meshes):
 def test_sphere5120_sweep64_fp32(bem, meshes):
     """Icosphere L4 (5120 triangles), 64 log-spaced frequencies 100-2000 Hz,
     pulsating and plane-wave scattering right-hand sides (seeded direction),
-    FP32 assembly, leaf 64, eta 2, ACA 1e-5, GMRES 1e-8: pulsating pressure
-    against p(a) = i w rho a / (i k a - 1), scattering against the Mie series
-    of the rigid sphere, and four frequencies against the FP64 sweep."""
+    FP32 assembly, leaf 64, eta 2, ACA 1e-5, GMRES 1e-8, run as the batched
+    form: chunks of 4 frequencies (one batched assembly per operator and one
+    lockstep GMRES over the 8 (frequency, right-hand side) columns), FP64
+    near-field refinement, the ACA schedule tuned on the first chunk, two
+    lanes.  Pulsating pressure against p(a) = i w rho a / (i k a - 1),
+    scattering against the Mie series of the rigid sphere, and four
+    frequencies against the FP64 sweep."""
     a, c, rho = 0.5, 343.0, 1.21
     tri = meshes.icosphere(4, a)
     n = len(tri)
@@ -59,9 +63,16 @@ def test_sphere5120_sweep64_fp32(bem, meshes):
     freqs = 100.0 * 20.0 ** (np.arange(64) / 63.0)
     g = np.stack([np.ones(n, complex), np.zeros(n, complex)])
     planes = np.array([[0, 0, 0, 0, 0], [d[0], d[1], d[2], 1.0, 0.0]])
+    S.set_sweep_batch(4)
+    S.set_autotune(True)
+    S.set_refinement(1)
     pr, its, st = S.sweep(freqs, g=g, planes=planes, c=c, rho=rho, aca_tol=1e-5, gmres_tol=1e-8, seed=1,
-                          precision=bem.FP32, lanes=4)
+                          precision=bem.FP32, lanes=2)
     assert np.all(its > 0) and all(s["residual"] < 1e-7 for s in st)
+    assert S.aca_schedule(bem.FP32) is not None
+    print("tuned ACA schedule", S.aca_schedule(bem.FP32))
+    S.set_sweep_batch(0)
+    S.set_refinement(0)
     col = tri.reshape(n, 3, 3).mean(axis=1)[perm]          # centroids in cluster order
     cos_t = (col @ d) / np.linalg.norm(col, axis=1)
     worst_p, worst_s = 0.0, 0.0

@@ -626,6 +626,11 @@ struct AcaArgs {
     // (job << kJobShift) | block, job j's mesh, frequency, partition, ACA
     // scratch and results in jobs[j]
     const MeshJob* jobs = nullptr;
+    // per block: 0 when no (row, column) pair of the block can be a self or
+    // near-singular pair (its collocation boxes lie apart by more than the
+    // near limit), so that its rows and columns skip the near-pair checks;
+    // null: every block is checked
+    const int8_t* bnear = nullptr;
 };
 
 // One context's part of a multi-mesh ACA launch: the AcaArgs fields that
@@ -645,48 +650,51 @@ struct MeshJob {
     const int32_t* piv;
     const bem_c64* sing;
     int64_t nnz;
+    const int8_t* bnear;
 };
 constexpr int kJobShift = 20;                       // blocks per context < 2^20
 constexpr int32_t kJobMask = (1 << kJobShift) - 1;
 
-// The dot products are processed kDotGroup at a time per wavefront: the
-// new vector is loaded once per group and the shuffle trees of the group are
-// interleaved (independent latency chains); each task keeps the canonical
-// lane-strided / pairwise order.  Accumulation is FP64 for both types.
+// The dot products are processed kDotGroup at a time per wavefront: per
+// element the new vector and the group's basis vectors are loaded together
+// and the group's totals are reduced together (wave_sum_spread); each task
+// keeps the canonical lane-strided / pairwise order.  Accumulation is FP64
+// for both types.
 constexpr int kDotGroup = 4;
 
-// Totals over the 64 lanes of eight values per lane, each by the pairwise
-// tree of the device reduction order (partial p plus partial p + s for
-// s = 32, 16, ..., 1), with the values spread over the lanes as the tree
-// narrows: at s = 32, 16 and 8 a lane keeps half of its values and trades
-// the other half with its partner (lane ^ s), so that each addition pairs
-// the same two partials as the tree (addition commutes, so the sums are the
-// same bits) while 20 lane exchanges replace 96.  Lane 8v (v = 0..7) returns
-// the total of value v.
-__device__ __forceinline__ double wave_sum8_spread(const double (&x)[8], int lane)
+// Totals over the 64 lanes of NV values per lane (NV a power of two up to
+// 64), each by the pairwise tree of the device reduction order (partial p
+// plus partial p + s for s = 32, 16, ..., 1), with the values spread over
+// the lanes as the tree narrows: while a lane holds more than one value it
+// keeps half of them at each step and trades the other half with its partner
+// (lane ^ s), so that each addition pairs the same two partials as the tree
+// (addition commutes, so the sums are the same bits) with far fewer lane
+// exchanges (34 instead of 192 for 16 values).  Lane (64 / NV) v returns the
+// total of value v.
+template <int NV>
+__device__ __forceinline__ double wave_sum_spread(const double (&x)[NV], int lane)
 {
-    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
-    double y[4];
+    static_assert(NV >= 1 && NV <= 64 && (NV & (NV - 1)) == 0, "NV: a power of two up to 64");
+    double cur[NV];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const double send = h5 ? x[j] : x[4 + j], keep = h5 ? x[4 + j] : x[j];
-        y[j] = keep + __shfl_xor(send, 32, 64);
-    }
-    double z[2];
+    for (int v = 0; v < NV; v++) cur[v] = x[v];
+    int nv = NV;
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const double send = h4 ? y[j] : y[2 + j], keep = h4 ? y[2 + j] : y[j];
-        z[j] = keep + __shfl_xor(send, 16, 64);
+    for (int s = 32; s >= 1; s >>= 1) {
+        if (nv > 1) {
+            const bool hi = lane & s;
+            const int h = nv / 2;
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                const double send = hi ? cur[j] : cur[h + j], keep = hi ? cur[h + j] : cur[j];
+                cur[j] = keep + __shfl_xor(send, s, 64);
+            }
+            nv = h;
+        } else {
+            cur[0] = cur[0] + __shfl_xor(cur[0], s, 64);
+        }
     }
-    double w;
-    {
-        const double send = h3 ? z[0] : z[1], keep = h3 ? z[1] : z[0];
-        w = keep + __shfl_xor(send, 8, 64);
-    }
-    w = w + __shfl_xor(w, 4, 64);
-    w = w + __shfl_xor(w, 2, 64);
-    w = w + __shfl_xor(w, 1, 64);
-    return w;
+    return cur[0];
 }
 
 // tasks over vector v (length len) against basis W (stride len): k = 0 is
@@ -715,24 +723,24 @@ __device__ __forceinline__ void dot_group_pass(const C* __restrict__ v, const C*
         double nrm = 0.0;
         for (int64_t t = lane; t < len; t += 64) {
             const bem_c64 cv = ldg64(v + t);
-            bem_c64 wv[kDotGroup];
+            C wv[kDotGroup];
 #pragma unroll
-            for (int g = 0; g < kDotGroup; g++) wv[g] = ldg64(src[g] + t);
+            for (int g = 0; g < kDotGroup; g++) wv[g] = ldg(src[g] + t);
             nrm = nrm + c64_abs2(cv);
 #pragma unroll
-            for (int g = 0; g < kDotGroup; g++) acc[g] = c64_fma(c64_conj(cv), wv[g], acc[g]);
+            for (int g = 0; g < kDotGroup; g++) acc[g] = c64_fma(c64_conj(cv), ld64(&wv[g]), acc[g]);
         }
         if (kb == 0) acc[0] = c64(nrm, 0.0);
-        static_assert(kDotGroup == 4, "wave_sum8_spread reduces four complex values");
         double x[2 * kDotGroup];
 #pragma unroll
         for (int g = 0; g < kDotGroup; g++) {
             x[2 * g] = acc[g].re;
             x[2 * g + 1] = acc[g].im;
         }
-        const double tot = wave_sum8_spread(x, lane);
-        if ((lane & 7) == 0) {
-            const int v = lane >> 3, k = kb + (v >> 1);
+        const double tot = wave_sum_spread<2 * kDotGroup>(x, lane);
+        constexpr int kStride = 64 / (2 * kDotGroup);
+        if ((lane & (kStride - 1)) == 0) {
+            const int v = lane / kStride, k = kb + (v >> 1);
             if (k < ntask) {
                 double* o = (double*)(out + (k == 0 ? base : base + 2 * k));
                 o[v & 1] = tot;
@@ -807,10 +815,10 @@ __device__ __forceinline__ bool near_or_self(const DevMesh& M, int64_t i, int64_
 // plus the mirror image's entry (a template argument so that the plain
 // kernels keep their register budget)
 template <bool PLANE>
-__device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c64* e)
+__device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c64* e, bool nearb = true)
 {
     const DevMesh& M = a.M;
-    if (near_or_self(M, i, j)) return false;
+    if (nearb && near_or_self(M, i, j)) return false;
     bem_ops4 o = regular_ops_vtx<1>(a.F.k, M.px[i], M.py[i], M.pz[i], M.nx[i], M.ny[i], M.nz[i], M.nx[j], M.ny[j],
                                     M.nz[j], M.area[j], load_vtx(M, j));
     *e = bc_entry(a.F, a.kind, o, i, j);
@@ -818,10 +826,10 @@ __device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64
     return true;
 }
 template <bool PLANE>
-__device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c32* e)
+__device__ __forceinline__ bool regular_entry(const AcaArgs& a, int64_t i, int64_t j, bem_c32* e, bool nearb = true)
 {
     const DevMesh& M = a.M;
-    if (near_or_self(M, i, j)) return false;
+    if (nearb && near_or_self(M, i, j)) return false;
     // only the operators the entry uses (the others stay zero, and the
     // entry formula then gives the same value)
     const int need = bc_needs_f32(a.F32, a.F.subst, a.kind, j);
@@ -866,11 +874,11 @@ __device__ __forceinline__ C singular_entry(const AcaArgs& a, int64_t i, int64_t
 
 // any entry (the pivot probe of the extra variant)
 template <bool PLANE, class C, bool FIELD = false>
-__device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
+__device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j, bool nearb = true)
 {
     if constexpr (FIELD) return from64<C>(field_entry(a, i, j));
     C e;
-    if (regular_entry<PLANE>(a, i, j, &e)) return e;
+    if (regular_entry<PLANE>(a, i, j, &e, nearb)) return e;
     return singular_entry<C>(a, i, i == j ? -1 : near_find(a.M, i, (int32_t)j));
 }
 
@@ -880,7 +888,7 @@ __device__ __forceinline__ C entry_any(const AcaArgs& a, int64_t i, int64_t j)
 // branch.
 template <int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t i,
-                        const C* U, const C* V, const C* A, C* dst, C* sw)
+                        const C* U, const C* V, const C* A, C* dst, C* sw, bool nearb)
 {
     for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(ldg(U + (int64_t)l * m + i), ldg(A + l));
     __syncthreads();
@@ -888,10 +896,12 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     for (int64_t j = threadIdx.x; j < n; j += NT) {
         C e;
         if constexpr (FIELD) e = from64<C>(field_entry(a, gi, c0 + j));
-        else if (!regular_entry<PLANE>(a, gi, c0 + j, &e)) continue;
+        else if (!regular_entry<PLANE>(a, gi, c0 + j, &e, nearb)) continue;
         dst[j] = cx_sub(e, cross_sum<true>(sw, V, n, j, r));
     }
-    if constexpr (FIELD) {   // no self or near-singular entries between observation points and elements
+    // no self or near-singular entries between observation points and
+    // elements, nor in a block whose collocation boxes lie apart
+    if (FIELD || !nearb) {
         __syncthreads();
         return;
     }
@@ -911,7 +921,7 @@ __device__ void aca_row(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
 // (the distance criterion is symmetric)
 template <int NT, class C, bool PLANE, bool FIELD = false>
 __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int64_t n, int r, int64_t j,
-                        const C* U, const C* V, const C* A, C* dst, C* sw)
+                        const C* U, const C* V, const C* A, C* dst, C* sw, bool nearb)
 {
     for (int l = threadIdx.x; l < r; l += NT) sw[l] = cx_mul(ldg(A + l), ldg(V + (int64_t)l * n + j));
     __syncthreads();
@@ -919,10 +929,10 @@ __device__ void aca_col(const AcaArgs& a, int64_t r0, int64_t c0, int64_t m, int
     for (int64_t i = threadIdx.x; i < m; i += NT) {
         C e;
         if constexpr (FIELD) e = from64<C>(field_entry(a, r0 + i, gj));
-        else if (!regular_entry<PLANE>(a, r0 + i, gj, &e)) continue;
+        else if (!regular_entry<PLANE>(a, r0 + i, gj, &e, nearb)) continue;
         dst[i] = cx_sub(e, cross_sum<false>(sw, U, m, i, r));
     }
-    if constexpr (FIELD) {
+    if (FIELD || !nearb) {
         __syncthreads();
         return;
     }
@@ -961,6 +971,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
 {
     const int64_t m = a.rows[b], n = a.cols[b], r0 = a.row0[b], c0 = a.col0[b];
     const int cap = a.cap[b];
+    const bool nearb = !FIELD && (!a.bnear || a.bnear[b] != 0);
     int64_t maxRank = m < n ? m : n;
     if (a.maxrank > 0 && a.maxrank < maxRank) maxRank = a.maxrank;
     C* U = (C*)a.U + a.uoff[b];
@@ -976,7 +987,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
     bem_c64 S = c64(0, 0);
     if (!EXTRA) {
         int64_t j = bem_rand_bounded(a.seed, 0, (uint64_t)b, 0, n);
-        aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, 0, j, U, V, A, U, sw);
+        aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, 0, j, U, V, A, U, sw, nearb);
         ev = uni64(ev + m);
         int64_t i = -1;
         while (r < maxRank) {
@@ -986,12 +997,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             i = am.i;
             if (am.v <= tiny2 && r >= 1) break;
             C* row = V + (int64_t)r * n;
-            aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+            aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw, nearb);
             ev = uni64(ev + n);
             ArgMax ar = block_argmax<NT>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
             C* col = U + (int64_t)r * m;
-            aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+            aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw, nearb);
             ev = uni64(ev + m);
             C alpha = cx_inv(row[ar.i]);
             if (threadIdx.x == 0) A[r] = sA[r] = alpha;
@@ -1008,7 +1019,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
         bool done = false;
         for (int64_t p = p0; p < p1 && r < maxRank && !done; p++) {
             int64_t i = a.piv[2 * p], j = a.piv[2 * p + 1];
-            C pe = entry_any<PLANE, C, FIELD>(a, r0 + i, c0 + j);
+            C pe = entry_any<PLANE, C, FIELD>(a, r0 + i, c0 + j, nearb);
             ev = uni64(ev + 1);
             if (cx_abs2(pe) < tiny2) continue;
             // the approximation at (i, j): its rank terms formed in parallel,
@@ -1021,7 +1032,7 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             __syncthreads();
             if (cx_abs2(cx_sub(pe, approx)) < tol2 * cx_abs2(pe)) continue;
             if (r >= cap) { flag = 1; break; }
-            aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw);
+            aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, j, U, V, A, U + (int64_t)r * m, sw, nearb);
             ev = uni64(ev + m);
             bool first = true;
             while (r < maxRank) {
@@ -1032,12 +1043,12 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
                 first = false;
                 if (cx_abs2(wcol[i]) <= tiny2 && r >= 1) break;
                 C* row = V + (int64_t)r * n;
-                aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw);
+                aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw, nearb);
                 ev = uni64(ev + n);
                 ArgMax ar = block_argmax<NT>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
                 C* col = U + (int64_t)r * m;
-                aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw);
+                aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw, nearb);
                 ev = uni64(ev + m);
                 C alpha = cx_inv(row[ar.i]);
                 if (threadIdx.x == 0) A[r] = sA[r] = alpha;
@@ -1113,6 +1124,7 @@ __device__ __forceinline__ void aca_job(const AcaArgs& a, int item, char* dyn, A
     ai.rank_out = J.rank_out; ai.flag_out = J.flag_out; ai.evals_out = J.evals_out;
     ai.piv_off = J.piv_off; ai.piv = J.piv;
     ai.sing = J.sing; ai.nnz = J.nnz;
+    ai.bnear = J.bnear;
     ai.nf = 0;
     aca_block<EXTRA, NT, C, PLANE>(ai, item & kJobMask, dyn, sred);
 }
@@ -2696,6 +2708,7 @@ struct bem_ctx {
     // partition
     std::vector<bemsetup::Block> blocks;
     int64_t *d_row0 = nullptr, *d_rows = nullptr, *d_col0 = nullptr, *d_cols = nullptr;
+    int8_t* d_bnear = nullptr;   // per block: may hold self / near-singular pairs (aca_block)
     std::vector<int64_t> dense_off;          // per block (-1 if admissible)
     int64_t dense_total = 0;
     std::vector<Tile> tiles;
@@ -3507,6 +3520,33 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
             }
         }
     }
+    // admissible blocks whose row and column collocation boxes lie further
+    // apart than the near limit (with a margin far above any rounding of the
+    // device's distance) hold no self or near-singular pair
+    std::vector<int8_t> bnear(nb, 1);
+    {
+        auto box = [&](int64_t s0, int64_t cnt, double lo[3], double hi[3]) {
+            for (int d = 0; d < 3; d++) lo[d] = hi[d] = c->h_colloc[3 * s0 + d];
+            for (int64_t e = s0; e < s0 + cnt; e++)
+                for (int d = 0; d < 3; d++) {
+                    lo[d] = std::min(lo[d], c->h_colloc[3 * e + d]);
+                    hi[d] = std::max(hi[d], c->h_colloc[3 * e + d]);
+                }
+        };
+        const double lim = c->mesh.near_lim * (1.0 + 1e-9);
+        for (int32_t b : c->adm_list) {
+            const auto& B = c->blocks[b];
+            double rl[3], rh[3], cl[3], ch[3];
+            box(B.row0, B.rows, rl, rh);
+            box(B.col0, B.cols, cl, ch);
+            double d2 = 0;
+            for (int d = 0; d < 3; d++) {
+                const double g = std::max({0.0, cl[d] - rh[d], rl[d] - ch[d]});
+                d2 += g * g;
+            }
+            if (std::sqrt(d2) > lim) bnear[b] = 0;
+        }
+    }
     std::vector<int64_t> gptr(n + 1, 0), gidx;
     for (int64_t i = 0; i < n; i++) gptr[i + 1] = gptr[i] + (int64_t)contrib[i].size();
     gidx.reserve(gptr[n]);
@@ -3519,6 +3559,7 @@ int bem_upload_partition(bem_ctx_t c, int64_t nb, const int64_t* row0, const int
     if ((rc = dupload(c, &c->d_rows, rs.data(), nb, ap))) return rc;
     if ((rc = dupload(c, &c->d_col0, c0.data(), nb, ap))) return rc;
     if ((rc = dupload(c, &c->d_cols, cs.data(), nb, ap))) return rc;
+    if ((rc = dupload(c, &c->d_bnear, bnear.data(), nb, ap))) return rc;
     if ((rc = dupload(c, &c->d_tiles, c->tiles.data(), c->tiles.size(), ap))) return rc;
     if ((rc = dupload(c, &c->d_fixes, c->fixes.data(), c->fixes.size(), ap))) return rc;
     if ((rc = dupload(c, &c->d_adm_list, c->adm_list.data(), c->adm_list.size(), ap))) return rc;
@@ -3734,6 +3775,7 @@ int aca_prepare(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, uin
     a.tol = tol; a.maxrank = maxrank; a.seed = seed;
     a.piv_off = d_pivoff; a.piv = d_piv;
     a.sing = c->sing_cur ? c->sing_cur : c->d_sing; a.nnz = c->nnz;
+    a.bnear = pv ? nullptr : c->d_bnear;
     a.nf = 0; a.nbk = nb; a.kf = nullptr; a.ccf = nullptr; a.singf = nullptr;
     a.Uf = a.Vf = a.Af = nullptr;
     a.queue = nullptr; a.nitems = 0;
@@ -4029,6 +4071,7 @@ int run_aca_batch(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, u
     a.tol = tol; a.maxrank = maxrank; a.seed = seed;
     a.piv_off = d_pivoff; a.piv = d_piv;
     a.sing = nullptr; a.nnz = c->nnz;
+    a.bnear = c->d_bnear;
     a.nf = nf; a.nbk = nb;
     a.kf = pk.ptr<double>(k_kf); a.ccf = pk.ptr<bem_c64>(k_ccf);
     a.singf = pk.ptr<const bem_c64*>(k_sp);
@@ -4649,6 +4692,7 @@ int bem_assemble_multi(const bem_ctx_t* ctxs, int64_t nctx, int kind, double tol
         J.rank_out = a.rank_out; J.flag_out = a.flag_out; J.evals_out = a.evals_out;
         J.piv_off = a.piv_off; J.piv = a.piv;
         J.sing = a.sing; J.nnz = a.nnz;
+        J.bnear = a.bnear;
     }
     std::vector<int32_t> items;
     size_t ncls[3];

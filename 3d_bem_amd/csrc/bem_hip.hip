@@ -76,9 +76,12 @@ __global__ void k_quadpoints(int64_t n, const double* __restrict__ nodes, double
 
 // Dense near-field tiles (K1): regular quadrature of every (row, col) pair of
 // a 64 x 64 tile with the column data staged in LDS.  Self and
-// near-singular pairs are skipped here and written by k_fixup.
+// near-singular pairs are skipped here and written by k_fixup.  The
+// plane-free kernel is budgeted for four wavefronts per SIMD (116 VGPRs, no
+// spills; unbudgeted the compiler takes 138 and three); the plane's would
+// spill at that budget and keeps the compiler's choice.
 template <bool PLANE>
-__global__ __launch_bounds__(kWG) void k_dense(DevMesh M, DevFreq F, const Tile* __restrict__ tiles,
+__global__ __launch_bounds__(kWG, PLANE ? 1 : 4) void k_dense(DevMesh M, DevFreq F, const Tile* __restrict__ tiles,
                                                bem_c64* outPhi, bem_c64* outDPhi)
 {
     __shared__ double sq[3][16][kTile];

@@ -61,6 +61,8 @@ _SIGS = {
     "bem_solver_set_refinement": (_int, [_p, _int, _i64]),
     "bem_solver_set_sweep_batch": (_int, [_p, _int]),
     "bem_set_fuse_ops": (_int, [_p, _int]),
+    "bem_set_precond_side": (_int, [_p, _int]),
+    "bem_solver_set_precond_side": (_int, [_p, _int]),
     "bem_assemble_multi": (_int, [_p, _i64, _int, _dbl, _i64, _int, _u64, _p]),
     "bem_solvers_solve_multi": (_int, [_p, _i64, _p, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _int, _p, _p,
                                        _p]),
@@ -293,6 +295,11 @@ class Context:
     def set_fuse_ops(self, on):
         """bem_set_fuse_ops: products of several operators in one pass (default on)"""
         self._check(lib().bem_set_fuse_ops(self.h, int(bool(on))))
+
+    def set_precond_side(self, left):
+        """bem_set_precond_side: preconditioned GMRES on M^-1 A (left, the
+        reference's form) or on A M^-1 (right, the default)"""
+        self._check(lib().bem_set_precond_side(self.h, int(bool(left))))
 
     def gmres_batch(self, hs, B, tol, hres=None, restart=300, maxit=100000, max_steps=0):
         """lockstep GMRES over several operators (bem_gmres_batch): B (n,
@@ -785,6 +792,13 @@ class Solver:
             rc = lib().bem_solver_set_preconditioner(self.h, int(kind))
         if rc != BEM_OK:
             raise BemError(rc, "bem_solver_set_preconditioner")
+
+    def set_precond_side(self, left):
+        """bem_solver_set_precond_side: 1 the reference's left preconditioning
+        from x = M^-1 b (tolerance on the preconditioned residual), 0 right"""
+        rc = lib().bem_solver_set_precond_side(self.h, int(bool(left)))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_precond_side")
 
     def field(self, obs):
         """sound pressure at observation points (nobs, 3) from the last solve"""

@@ -542,8 +542,15 @@ __device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b)
 }
 
 // first index of the largest |v_i|^2 over [0, len); index `excl` counts as 0.
-// NT threads (NT / 64 wavefronts); every thread gets the result.
-template <int NT, class C>
+// NT threads (NT / 64 wavefronts); every thread gets the result.  GLOBAL: v
+// lies in device global memory (the ACA's factors) and is read through the
+// global address space; otherwise (e.g. LDS) through generic loads.
+template <bool GLOBAL, class C> __device__ __forceinline__ C ld_any(const C* p)
+{
+    if constexpr (GLOBAL) return ldg(p);
+    else return *p;
+}
+template <int NT, class C, bool GLOBAL = false>
 __device__ ArgMax block_argmax(const C* __restrict__ v, int64_t len, int64_t excl, ArgMax* sred)
 {
     const int tid = threadIdx.x;
@@ -553,7 +560,7 @@ __device__ ArgMax block_argmax(const C* __restrict__ v, int64_t len, int64_t exc
     for (; t + 3 * NT < len; t += 4 * NT) {
         C e[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) e[u] = ldg(v + t + u * NT);
+        for (int u = 0; u < 4; u++) e[u] = ld_any<GLOBAL>(v + t + u * NT);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int64_t tu = t + u * NT;
@@ -562,7 +569,7 @@ __device__ ArgMax block_argmax(const C* __restrict__ v, int64_t len, int64_t exc
         }
     }
     for (; t < len; t += NT) {
-        double x = t == excl ? 0.0 : cx_abs2(ldg(v + t));
+        double x = t == excl ? 0.0 : cx_abs2(ld_any<GLOBAL>(v + t));
         if (x > a.v) { a.v = x; a.i = t; }
     }
     for (int s = 32; s > 0; s >>= 1) {
@@ -996,13 +1003,13 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
         while (r < maxRank) {
             if (r >= cap) { flag = 1; break; }
             const C* wcol = r >= 1 ? U + (int64_t)(r - 1) * m : U;
-            ArgMax am = block_argmax<NT>(wcol, m, r >= 1 ? i : -1, sred);
+            ArgMax am = block_argmax<NT, C, true>(wcol, m, r >= 1 ? i : -1, sred);
             i = am.i;
             if (am.v <= tiny2 && r >= 1) break;
             C* row = V + (int64_t)r * n;
             aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw, nearb);
             ev = uni64(ev + n);
-            ArgMax ar = block_argmax<NT>(row, n, -1, sred);
+            ArgMax ar = block_argmax<NT, C, true>(row, n, -1, sred);
             if (ar.v <= tiny2 && r >= 1) break;
             C* col = U + (int64_t)r * m;
             aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw, nearb);
@@ -1041,14 +1048,14 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
             while (r < maxRank) {
                 if (r >= cap) { flag = 1; done = true; break; }
                 const C* wcol = first ? U + (int64_t)r * m : U + (int64_t)(r - 1) * m;
-                ArgMax am = block_argmax<NT>(wcol, m, first ? -1 : i, sred);
+                ArgMax am = block_argmax<NT, C, true>(wcol, m, first ? -1 : i, sred);
                 i = am.i;
                 first = false;
                 if (cx_abs2(wcol[i]) <= tiny2 && r >= 1) break;
                 C* row = V + (int64_t)r * n;
                 aca_row<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, i, U, V, A, row, sw, nearb);
                 ev = uni64(ev + n);
-                ArgMax ar = block_argmax<NT>(row, n, -1, sred);
+                ArgMax ar = block_argmax<NT, C, true>(row, n, -1, sred);
                 if (ar.v <= tiny2 && r >= 1) break;
                 C* col = U + (int64_t)r * m;
                 aca_col<NT, C, PLANE, FIELD>(a, r0, c0, m, n, r, ar.i, U, V, A, col, sw, nearb);
@@ -2729,6 +2736,7 @@ struct bem_ctx {
     // the DPhi near field (dense blocks and singular table) from the same
     // operators and keeps it for the next DPhi assembly of the same state
     bool pair_near = false;
+    bool prec_left = false;   // bem_set_precond_side: left preconditioning (the reference's form)
     uint64_t gen = 0;                    // bumped by every change of mesh, partition, BC, frequency, plane
     bem_c64* d_sing2 = nullptr;          // the kept DPhi singular table
     void* st_dense = nullptr;            // the kept DPhi dense blocks
@@ -5348,6 +5356,13 @@ int bem_gmres(bem_ctx_t c, bem_hmat_t h, const void* b, void* x, double tol, int
     return bem_gmres_prec(c, h, nullptr, b, x, tol, restart, maxit, iters, rel_resid);
 }
 
+int bem_set_precond_side(bem_ctx_t c, int left)
+{
+    if (!c) return BEM_ERR_ARG;
+    c->prec_left = left != 0;
+    return BEM_OK;
+}
+
 int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void* x, double tol, int64_t restart,
                    int64_t maxit, int64_t* iters, double* rel_resid)
 {
@@ -5371,7 +5386,7 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
     const int nchunk = (int)((n + kDotChunk - 1) / kDotChunk);
     // workspace (complex slots): V (m+1) n, w n, r n, h, hacc, y, g, cs_s
     // (m+2 each), H (m+1) m, dot partials (m+1) nchunk, then doubles
-    int64_t need = (m + 1) * n + 3 * n + 5 * (m + 2) + (m + 1) * m + (m + 1) * nchunk + (m + 2) + 640;
+    int64_t need = (m + 1) * n + 4 * n + 5 * (m + 2) + (m + 1) * m + (m + 1) * nchunk + (m + 2) + 640;
     if (c->gm_cap < need) {
         pfree(c, c->d_gm);
         c->d_gm = nullptr;
@@ -5390,8 +5405,9 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
     bem_c64* dcs_s = dg + (m + 2);
     bem_c64* dH = dcs_s + (m + 2);
     bem_c64* dpart = dH + (m + 1) * m;
-    bem_c64* zp = dpart + (m + 1) * nchunk;   // preconditioned vector (right preconditioning)
-    double* dcs_c = (double*)(zp + n);
+    bem_c64* zp = dpart + (m + 1) * nchunk;   // preconditioned vector
+    bem_c64* pb = zp + n;                      // M^-1 b (left preconditioning)
+    double* dcs_c = (double*)(pb + n);
     double* dn = dcs_c + 2 * (m + 2);      // norm partials (256) + result + residual estimate
     double* dnorm = dn + 512;
     double* dres = dnorm + 1;
@@ -5408,8 +5424,12 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
         return BEM_OK;
     };
     int rc;
+    // left preconditioning (gmresLUPreconditionedSolve, gmres.h:200-216): GMRES
+    // on M^-1 A x = M^-1 b, its residual and tolerance those of that system
+    const bool left = pc && c->prec_left;
     double bn = 0;
-    norm_dev(bd);
+    if (left && (rc = bem_precond_apply(c, pc, bd, pb))) return rc;
+    norm_dev(left ? pb : bd);
     if ((rc = launch_check(c, "norm")) || (rc = fetch(&bn, dnorm))) return rc;
     if (bn == 0) {
         HIPC(c, hipMemsetAsync(xd, 0, n * sizeof(bem_c64), c->stream));
@@ -5422,15 +5442,20 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
     int64_t it = 0;
     double rel = 1;
     while (true) {
-        // r = b - A x, beta = |r|
+        // r = b - A x (left: M^-1 (b - A x)), beta = |r|
         if ((rc = bem_hmvm(c, h, xd, r, 0))) return rc;
         hipLaunchKernelGGL(k_residual, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, bd, r);
-        norm_dev(r);
+        const bem_c64* rr = r;
+        if (left) {
+            if ((rc = bem_precond_apply(c, pc, r, zp))) return rc;
+            rr = zp;
+        }
+        norm_dev(rr);
         double beta;
         if ((rc = launch_check(c, "residual")) || (rc = fetch(&beta, dnorm))) return rc;
         rel = beta / bn;
         if (rel < tol || it >= maxit) break;
-        hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, r, dnorm, V);
+        hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, rr, dnorm, V);
         std::fill(g.begin(), g.end(), c64(0, 0));
         g[0] = c64(beta, 0);
         HIPC(c, hipMemcpyAsync(dg, g.data(), (m + 2) * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
@@ -5438,7 +5463,10 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
         // residual estimate is copied to the pinned ring slot jj & 1
         auto launch_col = [&](int64_t jj) -> int {
             int rc2;
-            if (pc) {   // w = A M^-1 v_jj
+            if (left) {   // w = M^-1 A v_jj
+                if ((rc2 = bem_hmvm(c, h, V + jj * n, zp, 0))) return rc2;
+                if ((rc2 = bem_precond_apply(c, pc, zp, w))) return rc2;
+            } else if (pc) {   // w = A M^-1 v_jj
                 if ((rc2 = bem_precond_apply(c, pc, V + jj * n, zp))) return rc2;
                 if ((rc2 = bem_hmvm(c, h, zp, w, 0))) return rc2;
             } else if ((rc2 = bem_hmvm(c, h, V + jj * n, w, 0))) return rc2;
@@ -5495,7 +5523,7 @@ int bem_gmres_prec(bem_ctx_t c, bem_hmat_t h, bem_prec_t pc, const void* b, void
         }
         if (j > 0) {
             HIPC(c, hipMemcpyAsync(dy, y.data(), j * sizeof(bem_c64), hipMemcpyHostToDevice, c->stream));
-            if (pc) {   // x += M^-1 (V y); w is free here
+            if (pc && !left) {   // x += M^-1 (V y); w is free here
                 hipLaunchKernelGGL(k_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, (int)j, V, dy, w, 0);
                 if ((rc = bem_precond_apply(c, pc, w, zp))) return rc;
                 hipLaunchKernelGGL(k_axpy1, dim3(nblk(n, 256)), dim3(256), 0, c->stream, n, zp, xd);

@@ -414,9 +414,26 @@ int64_t BoundaryElementSolver::solveSystem(DeviceVector& x, const DeviceVector& 
 {
     const auto& P = bemSolverParameters;
     const int R = refineMode(P);
+    check(ctx_, bem_set_precond_side(ctx_, P.preconditionLeft ? 1 : 0), "bem_set_precond_side");
     if (R == 0) {
-        x.upload(std::vector<cplx>(x.size(), cplx(0, 0)));
-        return GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, rel, pc);
+        const bool left = pc && P.preconditionLeft;
+        if (left)   // the reference's guess, LUSubstitutionSolve (boundaryelementsolver.cpp:689)
+            check(ctx_, bem_precond_apply(ctx_, pc, b.data(), x.data()), "bem_precond_apply");
+        else
+            x.upload(std::vector<cplx>(x.size(), cplx(0, 0)));
+        const int64_t its = GMRES::gmresSolve(*dPhiHMat, x, b, tol, P.gmresRestart, rel, pc);
+        if (left && rel) {   // GMRES reports the preconditioned residual; the true one, as the reference prints (:696-698)
+            DeviceVector y(ctx_, x.size());
+            check(ctx_, bem_hmvm(ctx_, dPhiHMat->handle(), x.data(), y.data(), 0), "bem_hmvm");
+            const std::vector<cplx> yv = y.download(), bv = b.download();
+            double num = 0, den = 0;
+            for (size_t i = 0; i < bv.size(); i++) {
+                num += std::norm(bv[i] - yv[i]);
+                den += std::norm(bv[i]);
+            }
+            *rel = den > 0 ? std::sqrt(num / den) : 0;
+        }
+        return its;
     }
     int64_t its = 0, steps = 0;
     double r = 0;
@@ -984,6 +1001,7 @@ struct bem_solver {
     int preconditioner = 0;
     long hluRank = 2;
     double hluRelError = 0;
+    bool precondLeft = false;   // bem_solver_set_precond_side
     int refine = 0;
     long refineMaxSteps = 10;
     int sweepBatch = 0;   // bem_solver_set_sweep_batch
@@ -1093,6 +1111,7 @@ int bem_solver_solve(bem_solver_t s, double freq, double c, double rho, int coup
         S.bemSolverParameters.preconditioner = s->preconditioner;
         S.bemSolverParameters.preconditionerRank = s->hluRank;
         S.bemSolverParameters.preconditionerRelError = s->hluRelError;
+        S.bemSolverParameters.preconditionLeft = s->precondLeft;
         S.bemSolverParameters.refine = s->refine;
         S.bemSolverParameters.refineMaxSteps = s->refineMaxSteps;
         S.calculateBoundarySolution();
@@ -1139,6 +1158,7 @@ int bem_solvers_solve_multi(const bem_solver_t* ss, int64_t ns, const double* fr
         P.preconditioner = s0->preconditioner;
         P.preconditionerRank = s0->hluRank;
         P.preconditionerRelError = s0->hluRelError;
+        P.preconditionLeft = s0->precondLeft;
         P.refine = s0->refine;
         P.refineMaxSteps = s0->refineMaxSteps;
         bem::BoundaryElementSolver::solveMany(v, std::vector<double>(freqs, freqs + ns), threads);
@@ -1239,6 +1259,13 @@ int bem_solver_refine_stats(bem_solver_t s, double* out4)
 {
     if (!s || !s->s || !out4) return BEM_ERR_ARG;
     std::memcpy(out4, s->s->refineStats, sizeof s->s->refineStats);
+    return BEM_OK;
+}
+
+int bem_solver_set_precond_side(bem_solver_t s, int left)
+{
+    if (!s || !s->s) return BEM_ERR_ARG;
+    s->precondLeft = left != 0;
     return BEM_OK;
 }
 
@@ -1396,6 +1423,7 @@ int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, 
                 P.preconditioner = s->preconditioner;
                 P.preconditionerRank = s->hluRank;
                 P.preconditionerRelError = s->hluRelError;
+                P.preconditionLeft = s->precondLeft;
                 P.refine = s->refine;
                 P.refineMaxSteps = s->refineMaxSteps;
                 if (chunk > 1) {

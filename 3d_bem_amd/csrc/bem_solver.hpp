@@ -122,6 +122,10 @@ struct BemSolverParameters {
     int preconditioner = 0;       // 0 none, 1 leaf-block LU (bem_precond_create), 2 H-LU (bem_precond_create_hlu)
     long preconditionerRank = 2;  // bemparameters.h:27
     double preconditionerRelError = 0;   // bemparameters.h:28
+    // left preconditioning from x = M^-1 b, tolerance on the preconditioned
+    // residual (gmresLUPreconditionedSolve, gmres.h:200-216); false: right
+    // preconditioning from 0, tolerance on the true residual
+    bool preconditionLeft = false;
     // FP32 operators only: iterative refinement of the GMRES solution against
     // an FP64-accurate operator (bem_gmres_ir): 0 none, 1 the FP32
     // operators' own FP64 near field + their low-rank factors

@@ -297,6 +297,15 @@ int bem_precond_create_hlu(bem_ctx_t ctx, bem_hmat_t h, int64_t nclusters, const
    part of the GMRES iterations), [3] stored complex entries, [4]
    factorisation ms (host H-LU; 0 for the leaf-block LU) */
 int bem_precond_stats(bem_prec_t p, double* out5);
+/* Side of the preconditioning in bem_gmres_prec (and the inner solves of
+   bem_gmres_ir) on this context: 0 (default) right, GMRES on A M^-1 with
+   the true residual |b - A x| / |b| against tol; 1 left, the reference's
+   gmresLUPreconditionedSolve (gmres.h:200-216): GMRES on M^-1 A x = M^-1 b
+   from the given x, the preconditioned residual |M^-1 (b - A x)| /
+   |M^-1 b| against tol (Eigen's criterion).  The reference starts it from
+   x = M^-1 b (hMatrixSolve, boundaryelementsolver.cpp:689-692), as the
+   solver does with bem_solver_set_precond_side. */
+int bem_set_precond_side(bem_ctx_t ctx, int left);
 int bem_gmres_prec(bem_ctx_t ctx, bem_hmat_t h, bem_prec_t p, const void* b_dev, void* x_dev, double tol,
                    int64_t restart, int64_t maxit, int64_t* iters, double* rel_resid);
 
@@ -421,6 +430,12 @@ int bem_solver_solve_dense(bem_solver_t s, double freq, double c, double rho, in
    default), 1 the leaf-block LU of bem_precond_create, 2 the H-LU of
    bem_precond_create_hlu (usePreconditioner, bemparameters.h:26) */
 int bem_solver_set_preconditioner(bem_solver_t s, int kind);
+/* 1: GMRES with the preconditioner as the reference's
+   gmresLUPreconditionedSolve (gmres.h:200-216): left preconditioning from
+   x = M^-1 b, the tolerance on the preconditioned residual (the true
+   residual is still reported in stats[6]); 0 (default): right
+   preconditioning from 0, the tolerance on the true residual */
+int bem_solver_set_precond_side(bem_solver_t s, int left);
 /* the H-LU preconditioner with preconditionerRank / preconditionerRelError
    (bemparameters.h:27-28; defaults 2 / 0) */
 int bem_solver_set_hlu(bem_solver_t s, int64_t rank, double rel_err);

@@ -111,3 +111,48 @@ def test_fp32_operator_with_hlu(bem, meshes):
     assert np.linalg.norm(p32h - p64) / np.linalg.norm(p64) < 1e-5
     assert st1["gmres_iters"] <= st0["gmres_iters"]
     S.close()
+
+
+def test_left_preconditioning_as_reference(bem, O, meshes):
+    """The reference's preconditioned GMRES (gmresLUPreconditionedSolve,
+    gmres.h:200-216): GMRES on M^-1 A x = M^-1 b from x = M^-1 b, stopped
+    on the preconditioned residual (bem_set_precond_side).  The reported
+    residual is |M^-1 (b - A x)| / |M^-1 b| recomputed on the host, the
+    solution agrees with the right-preconditioned one, and the solver form
+    (bem_solver_set_precond_side) reproduces the restatement's pressure."""
+    P = sphere_problem(O, meshes, level=3, leaf=32, eta=2.0, freq=700.0)
+    ctx = upload_problem(bem, P)
+    G = ctx.assemble(1, 1e-6, 0, seed=5)
+    M = G.hlu(P.clusters, 2, 0.0)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(P.n) + 1j * rng.standard_normal(P.n)
+    x_r, its_r, rel_r, _ = G.gmres(b, 1e-10, precond=M)
+    _, its_0, _, _ = G.gmres(b, 1e-10)
+    ctx.set_precond_side(True)
+    pb = M.apply(b)
+    x_l, its_l, rel_l, rc = G.gmres(b, 1e-10, x0=pb, precond=M)
+    ctx.set_precond_side(False)
+    assert rc == 0 and rel_l < 1e-10
+    r = M.apply(b - G.mvm(x_l))
+    rel_host = np.linalg.norm(r) / np.linalg.norm(pb)
+    print("left-preconditioned GMRES: iterations", its_l, "(right", its_r, ", none", its_0, ") residual", rel_l,
+          "host", rel_host, "vs right", np.linalg.norm(x_l - x_r) / np.linalg.norm(x_r))
+    assert rel_host < 1e-10 and abs(rel_host - rel_l) < 1e-3 * rel_l + 1e-14
+    assert np.linalg.norm(x_l - x_r) / np.linalg.norm(x_r) < 1e-8
+    assert its_l < its_0
+    M.close()
+    G.close()
+    ctx.close()
+    tri = meshes.icosphere(3, 0.5)
+    bc = meshes.robin_bc(len(tri), 0, 1, 1)
+    S = bem.Solver(tri, bc, leaf=32, eta=2.0)
+    S.set_preconditioner(2)
+    S.set_precond_side(True)
+    _, _, p1, st1 = S.solve(500.0, c=343.0, rho=1.21, aca_tol=1e-6, gmres_tol=1e-10, seed=1)
+    Q = O.Problem(tri, bc, leaf=32, eta=2.0)
+    Q.set_frequency(500.0, 343.0, 1)
+    _, _, pro, _ = Q.solve(1e-6, gmres_tol=1e-10, seed=1, rho=1.21, freq=500.0)
+    rel = np.linalg.norm(p1 - pro) / np.linalg.norm(pro)
+    print("solver, left H-LU: iterations", st1["gmres_iters"], "vs restatement", rel, "true residual", st1["residual"])
+    assert rel < 1e-9 and st1["residual"] < 1e-9
+    S.close()

@@ -63,6 +63,7 @@ _SIGS = {
     "bem_set_fuse_ops": (_int, [_p, _int]),
     "bem_set_precond_side": (_int, [_p, _int]),
     "bem_solver_set_precond_side": (_int, [_p, _int]),
+    "bem_solver_set_aca_schedule": (_int, [_p, _int, _i64, _i64, _int]),
     "bem_assemble_multi": (_int, [_p, _i64, _int, _dbl, _i64, _int, _u64, _p]),
     "bem_solvers_solve_multi": (_int, [_p, _i64, _p, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _int, _p, _p,
                                        _p]),
@@ -720,6 +721,12 @@ class Solver:
         rc = lib().bem_solver_set_autotune(self.h, int(bool(on)))
         if rc != BEM_OK:
             raise BemError(rc, "bem_solver_set_autotune")
+
+    def set_aca_schedule(self, precision, small_bound, huge_bound, queue=0):
+        """bem_solver_set_aca_schedule: a known ACA schedule, used without tuning"""
+        rc = lib().bem_solver_set_aca_schedule(self.h, int(precision), int(small_bound), int(huge_bound), int(queue))
+        if rc != BEM_OK:
+            raise BemError(rc, "bem_solver_set_aca_schedule")
 
     def aca_schedule(self, precision):
         """the tuned ACA schedule of a precision (bem_solver_aca_schedule) or None"""

@@ -1235,6 +1235,20 @@ int bem_solver_set_autotune(bem_solver_t s, int on)
     return BEM_OK;
 }
 
+int bem_solver_set_aca_schedule(bem_solver_t s, int precision, int64_t small_bound, int64_t huge_bound, int queue)
+{
+    if (!s || !s->s || (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32) || small_bound < 1 ||
+        huge_bound < small_bound || queue < 0 || queue > 1)
+        return BEM_ERR_ARG;
+    const int pi = precision == BEM_PREC_FP32 ? 1 : 0;
+    s->schedule[pi][0] = small_bound;
+    s->schedule[pi][1] = huge_bound;
+    s->schedule[pi][2] = queue;
+    s->tunedMs[pi] = 0;
+    s->tuned[pi] = true;
+    return BEM_OK;
+}
+
 int bem_solver_aca_schedule(bem_solver_t s, int precision, double* out4)
 {
     if (!s || !s->s || !out4 || (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)) return BEM_ERR_ARG;

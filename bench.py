@@ -368,6 +368,9 @@ def main():
     ap.add_argument("--sweep-batch", type=int, default=4, metavar="B",
                     help="FP32 form: frequency chunks of B points, each with batched assembly and one lockstep GMRES, "
                          "the chunks on up to --lanes lanes (0: one point per lane)")
+    ap.add_argument("--aca-schedule", default=None, metavar="S,H,Q",
+                    help="a fixed ACA schedule instead of tuning: one-wavefront bound S, largest-class bound H "
+                         "(0: none), work queue Q (0 / 1)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched sweeps: keep the built-in ACA schedule (default: tuned on the first chunk, in the "
                          "warm-up)")
@@ -435,7 +438,10 @@ def main():
     sb = args.sweep_batch if batched else 0
     refine = args.refine if prec == bem.FP32 else 0
     S.set_refinement(refine)
-    S.set_autotune(batched and not args.no_autotune)
+    S.set_autotune(batched and not args.no_autotune and not args.aca_schedule)
+    if batched and args.aca_schedule:
+        sm, hu, qu = (int(v) for v in args.aca_schedule.split(","))
+        S.set_aca_schedule(prec, sm, hu if hu > 0 else 1 << 40, qu)
 
     pr, its, stats, elapsed, avg_ms, cnt = timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sb)
     rstats = S.refine_stats()

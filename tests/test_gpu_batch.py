@@ -226,7 +226,8 @@ def test_autotune_keeps_results(bem, O, meshes):
     """bem_autotune_aca picks one of its candidate schedules and keeps it on
     the context; assemblies under it equal the built-in schedule's bit for
     bit.  A tuned batched sweep reports its schedule and gives the untuned
-    sweep's pressures bit for bit."""
+    sweep's pressures bit for bit, and so does a sweep with a schedule set
+    by the caller."""
     P = sphere_problem(O, meshes, level=3, leaf=32, eta=2.0, freq=FREQS[0])
     ctx = upload_problem(bem, P)
     nb = len(P.blocks)
@@ -257,4 +258,11 @@ def test_autotune_keeps_results(bem, O, meshes):
     p1, _, _ = S.sweep(np.array(FREQS), aca_tol=1e-6, gmres_tol=1e-10, precision=bem.FP32)
     assert S.aca_schedule(bem.FP32) is not None
     assert np.array_equal(p0.view(np.uint64), p1.view(np.uint64))
+    # a known schedule (bem_solver_set_aca_schedule): no tuning run, the same pressures
+    S.set_autotune(False)
+    S.set_aca_schedule(bem.FP32, 256, 1024, 1)
+    p2, _, _ = S.sweep(np.array(FREQS), aca_tol=1e-6, gmres_tol=1e-10, precision=bem.FP32)
+    sch = S.aca_schedule(bem.FP32)
+    assert (sch["small_bound"], sch["huge_bound"], sch["queue"]) == (256, 1024, 1)
+    assert np.array_equal(p0.view(np.uint64), p2.view(np.uint64))
     S.close()

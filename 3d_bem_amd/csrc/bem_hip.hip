@@ -1252,7 +1252,12 @@ __device__ __forceinline__ void lr_t_task(int64_t task, const int32_t* __restric
         const int u = lane >> 4, q = lane & 15;   // row l0 + u, columns q, q + 16, ...
         bem_c64 p = c64(0, 0);
         if (u < nl) {
+            // four independent accumulators (columns q + 16 e + 64 i in pe[e]),
+            // so that consecutive multiply-adds do not wait on each other
             const S* vr = v + (int64_t)u * n;
+            bem_c64 pe[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) pe[e] = c64(0, 0);
             int64_t j = q;
             for (; j + 48 < n; j += 64) {
                 S vv[4];
@@ -1263,9 +1268,13 @@ __device__ __forceinline__ void lr_t_task(int64_t task, const int32_t* __restric
                     xv[e] = xs[(j + 16 * e) * XS];
                 }
 #pragma unroll
-                for (int e = 0; e < 4; e++) p = c64_fma(ld64(&vv[e]), xv[e], p);
+                for (int e = 0; e < 4; e++) pe[e] = c64_fma(ld64(&vv[e]), xv[e], pe[e]);
             }
-            for (; j < n; j += 16) p = c64_fma(ld64(vr + j), xs[j * XS], p);
+            // at most three columns are left (j + 48 >= n)
+            if (j < n) pe[0] = c64_fma(ld64(vr + j), xs[j * XS], pe[0]);
+            if (j + 16 < n) pe[1] = c64_fma(ld64(vr + j + 16), xs[(j + 16) * XS], pe[1]);
+            if (j + 32 < n) pe[2] = c64_fma(ld64(vr + j + 32), xs[(j + 32) * XS], pe[2]);
+            p = c64_add(c64_add(pe[0], pe[1]), c64_add(pe[2], pe[3]));
         }
         for (int s = 8; s > 0; s >>= 1) {
             p.re = p.re + __shfl_xor(p.re, s, 16);

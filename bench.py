@@ -16,13 +16,13 @@ whole near field in FP64; GMRES in FP64 arithmetic iterating on the FP32
 operator inside an iterative refinement whose residuals b - A x are taken in
 FP64 with the FP64 near field and the stored low-rank factors (--refine 2:
 with FP64 H-matrices; 0: no refinement).  The batch's points go in chunks
-of --sweep-batch (default 4): Phi and DPhi of a chunk's frequencies from
+of --sweep-batch (default 2): Phi and DPhi of a chunk's frequencies from
 one batched assembly each (one ACA launch per size class over the
 flattened frequency x block items, the near field of both operators from
 one quadrature), and one lockstep GMRES over the chunk whose Arnoldi
 columns replay a captured HIP graph with every operator's product fused
 into one launch per kernel; the chunks run concurrently on up to --lanes
-solver lanes (two for the default 8 points), with the ACA schedule tuned on
+solver lanes (four for the default 8 points), with the ACA schedule tuned on
 the first chunk during the warm-up.  The earlier scheduling, one point per
 lane on eight lanes, is timed beside it (lanes_leg); the benchmark runs as
 a child process with GPU_MAX_HW_QUEUES=16 so that lanes do not share
@@ -365,7 +365,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=8, help="frequency points per step")
     ap.add_argument("--lanes", type=int, default=8, help="concurrent solver lanes of the FP32 form")
-    ap.add_argument("--sweep-batch", type=int, default=4, metavar="B",
+    ap.add_argument("--sweep-batch", type=int, default=2, metavar="B",
                     help="FP32 form: frequency chunks of B points, each with batched assembly and one lockstep GMRES, "
                          "the chunks on up to --lanes lanes (0: one point per lane)")
     ap.add_argument("--aca-schedule", default=None, metavar="S,H,Q",

@@ -1090,15 +1090,12 @@ template <int NT, class C> constexpr int kAcaWaves = 1;
 template <> constexpr int kAcaWaves<64, bem_c32> = 4;
 template <> constexpr int kAcaWaves<128, bem_c32> = 4;
 
-// one listed item: a block of the single operator, or (nf > 0) block b of
-// frequency f with that frequency's wavenumber, coupling, table and storage
+// one listed item of a batched launch (nf > 0; the single operator's
+// launches take aca_block directly): block b of frequency f with that
+// frequency's wavenumber, coupling, table and storage
 template <bool EXTRA, int NT, class C, bool PLANE>
 __device__ __forceinline__ void aca_item(const AcaArgs& a, int item, char* dyn, ArgMax* sred)
 {
-    if (a.nf == 0) {
-        aca_block<EXTRA, NT, C, PLANE>(a, item, dyn, sred);
-        return;
-    }
     const int f = (int)(item / a.nbk);
     const int b = (int)(item - (int64_t)f * a.nbk);
     AcaArgs ai = a;

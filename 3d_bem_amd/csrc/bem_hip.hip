@@ -1087,7 +1087,9 @@ __device__ void aca_block(const AcaArgs& a, const int b, char* dyn, ArgMax* sred
 }
 
 // ACA (K3): one workgroup of NT threads per listed admissible block
-// FP32 ACA limited to 128 VGPRs: four wavefronts per SIMD at every size
+// FP32 ACA limited to 128 VGPRs: four wavefronts per SIMD at every size; with
+// the impedance plane (its mirror-image quadrature in every entry) three, so
+// that the one-wavefront kernels spill less
 template <int NT, class C> constexpr int kAcaWaves = 1;
 template <> constexpr int kAcaWaves<64, bem_c32> = 4;
 template <> constexpr int kAcaWaves<128, bem_c32> = 4;
@@ -1144,7 +1146,7 @@ __device__ __forceinline__ void aca_job(const AcaArgs& a, int item, char* dyn, A
 // counter (separate instantiations, so that the plain kernel keeps its
 // register budget)
 template <bool EXTRA, int NT, class C, bool PLANE, int MODE, bool FIELD = false>
-__global__ __launch_bounds__(NT, (kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
+__global__ __launch_bounds__(NT, (PLANE && kAcaWaves<NT, C> > 3 ? 3 : kAcaWaves<NT, C>)) void k_aca(AcaArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) char dyn[];
     __shared__ ArgMax sred[NT / 64];

@@ -248,14 +248,12 @@ __global__ void k_scatter(const Fix* __restrict__ fx, int64_t nfix, int64_t nnz,
 
 // FP32 copy of an FP64 near field (the operator GMRES iterates with when the
 // FP64 one is kept for the refinement residual)
-// grid-stride (launched with nblk_stream workgroups of 256)
 __global__ void k_round32(int64_t len, const bem_c64* __restrict__ src, bem_c32* __restrict__ dst)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < len; t += stride) {
-        const bem_c64 v = src[t];
-        dst[t] = c32((float)v.re, (float)v.im);
-    }
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    const bem_c64 v = src[t];
+    dst[t] = c32((float)v.re, (float)v.im);
 }
 
 __global__ void k_entries(DevMesh M, DevFreq F, int mode, int64_t np, const int64_t* ij, bem_c64* out)
@@ -2027,12 +2025,8 @@ __global__ void k_reconstruct(int64_t n, const bem_c64* x, const int32_t* subst,
 template <class C>
 __global__ void k_has_nonfinite(int64_t len, const C* v, int32_t* flag)
 {
-    // grid-stride (nblk_stream workgroups of 256); one flag write per thread at most
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    bool bad = false;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride)
-        bad = bad || !(isfinite(v[i].re) && isfinite(v[i].im));
-    if (bad) *flag = 1;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len && !(isfinite(v[i].re) && isfinite(v[i].im))) *flag = 1;
 }
 
 // ------------------------------------------- several right-hand sides
@@ -3170,10 +3164,6 @@ int launch_check(bem_ctx* c, const char* what)
 }
 
 unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
-// workgroups of 256 for the grid-stride streaming kernels (k_round32,
-// k_has_nonfinite): enough to keep every CU streaming, far fewer than one
-// workgroup per 256 elements of a near field
-unsigned nblk_stream(int64_t n) { return std::min<unsigned>(nblk(n, 256), 2048u); }
 
 }  // namespace
 
@@ -3980,10 +3970,10 @@ int finish_hmat(bem_ctx* c, bem_hmat* h, bool f32, AcaScratch& S1, AcaScratch& S
     HIPC(c, hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
     if (c->dense_total) {
         if (f32)
-            hipLaunchKernelGGL(k_has_nonfinite<bem_c32>, dim3(nblk_stream(c->dense_total)), dim3(256), 0, c->stream,
+            hipLaunchKernelGGL(k_has_nonfinite<bem_c32>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
                                c->dense_total, (const bem_c32*)h->dense, d_flag);
         else
-            hipLaunchKernelGGL(k_has_nonfinite<bem_c64>, dim3(nblk_stream(c->dense_total)), dim3(256), 0, c->stream,
+            hipLaunchKernelGGL(k_has_nonfinite<bem_c64>, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream,
                                c->dense_total, (const bem_c64*)h->dense, d_flag);
     }
     int32_t hflag = 0;
@@ -4282,7 +4272,7 @@ int bem_assemble(bem_ctx_t c, int kind, double tol, int64_t maxrank, int precisi
             }
         }
         if (n64 && c->dense_total) {   // the FP32 copies GMRES iterates with
-            const dim3 g(nblk_stream(c->dense_total));
+            const dim3 g(nblk(c->dense_total, 256));
             hipLaunchKernelGGL(k_round32, g, dim3(256), 0, c->stream, c->dense_total, (const bem_c64*)h->dense64,
                                (bem_c32*)h->dense);
             if (dense2)
@@ -4495,7 +4485,7 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
                 }
             }
             if (n64 && c->dense_total) {
-                const dim3 g(nblk_stream(c->dense_total));
+                const dim3 g(nblk(c->dense_total, 256));
                 hipLaunchKernelGGL(k_round32, g, dim3(256), 0, c->stream, c->dense_total, (const bem_c64*)d64,
                                    (bem_c32*)H[f]->dense);
                 if (pairing)
@@ -4671,7 +4661,7 @@ int bem_assemble_multi(const bem_ctx_t* ctxs, int64_t nctx, int kind, double tol
             if ((rc = launch_check(c, "k_scatter"))) return fail(rc);
         }
         if (n64 && c->dense_total) {
-            hipLaunchKernelGGL(k_round32, dim3(nblk_stream(c->dense_total)), dim3(256), 0, c->stream, c->dense_total,
+            hipLaunchKernelGGL(k_round32, dim3(nblk(c->dense_total, 256)), dim3(256), 0, c->stream, c->dense_total,
                                (const bem_c64*)d64, (bem_c32*)H[q]->dense);
             if ((rc = launch_check(c, "k_round32"))) return fail(rc);
         }

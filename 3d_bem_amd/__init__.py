@@ -64,6 +64,7 @@ _SIGS = {
     "bem_set_precond_side": (_int, [_p, _int]),
     "bem_solver_set_precond_side": (_int, [_p, _int]),
     "bem_solver_set_aca_schedule": (_int, [_p, _int, _i64, _i64, _int]),
+    "bem_solver_profile": (_int, [_p, _int, _p]),
     "bem_assemble_multi": (_int, [_p, _i64, _int, _dbl, _i64, _int, _u64, _p]),
     "bem_solvers_solve_multi": (_int, [_p, _i64, _p, _dbl, _dbl, _int, _dbl, _i64, _dbl, _i64, _u64, _int, _int, _p, _p,
                                        _p]),

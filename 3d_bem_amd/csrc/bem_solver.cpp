@@ -1249,6 +1249,29 @@ int bem_solver_set_aca_schedule(bem_solver_t s, int precision, int64_t small_bou
     return BEM_OK;
 }
 
+int bem_solver_profile(bem_solver_t s, int enable, double* out14)
+{
+    if (!s || !s->s) return BEM_ERR_ARG;
+    std::vector<bem_ctx_t> cs{s->s->context()};
+    for (auto& l : s->lanes) cs.push_back(l->context());
+    double ms[7] = {0}, cnt[7] = {0};
+    for (bem_ctx_t c : cs) {
+        double o[14];
+        const int rc = bem_profile(c, enable, o);
+        if (rc != BEM_OK) return rc;
+        for (int q = 0; q < 7; q++) {
+            ms[q] += o[q] * o[7 + q];
+            cnt[q] += o[7 + q];
+        }
+    }
+    if (out14)
+        for (int q = 0; q < 7; q++) {
+            out14[q] = cnt[q] > 0 ? ms[q] / cnt[q] : 0.0;
+            out14[7 + q] = cnt[q];
+        }
+    return BEM_OK;
+}
+
 int bem_solver_aca_schedule(bem_solver_t s, int precision, double* out4)
 {
     if (!s || !s->s || !out4 || (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)) return BEM_ERR_ARG;

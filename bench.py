@@ -252,8 +252,8 @@ def batch_freqs(center, batch):
 
 def timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sweep_batch=0):
     """warmup + timed steps (one sweep of the batch each); kernel family
-    times from HIP events recorded on lane 0's launching streams (with
-    sweep_batch > 1: on the one context that runs every point)"""
+    times from HIP events recorded around every launch of the timed steps on
+    the launching streams of all the solver's lanes (bem_solver_profile)"""
     S.set_sweep_batch(sweep_batch)
 
     def step():
@@ -263,12 +263,12 @@ def timed_run(bem, S, ctx_h, args, prec, freqs, lanes, sweep_batch=0):
     for _ in range(args.warmup):
         step()
     prof = np.zeros(14)
-    bem.lib().bem_profile(ctx_h, 1, prof.ctypes.data_as(ctypes.c_void_p))
+    bem.lib().bem_solver_profile(S.h, 1, prof.ctypes.data_as(ctypes.c_void_p))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pr, its, stats = step()
     elapsed = time.perf_counter() - t0
-    bem.lib().bem_profile(ctx_h, 0, prof.ctypes.data_as(ctypes.c_void_p))
+    bem.lib().bem_solver_profile(S.h, 0, prof.ctypes.data_as(ctypes.c_void_p))
     return pr, its, stats, elapsed, prof[:7].copy(), prof[7:].copy()
 
 
@@ -283,7 +283,7 @@ def roofline(avg_ms, cnt, st, blocks, n, prec_name, points0, traffic_sched, quad
     s = 16 if prec_name == "fp64" else 8
     dense = blocks[blocks[:, 4] == 0]
     n_dense = int((dense[:, 1] * dense[:, 3]).sum())
-    launches = cnt[dom] / points0        # launches of this family per frequency point (lane 0)
+    launches = cnt[dom] / points0        # launches of this family per frequency point
     if dom in (0, 2):
         if dom == 0:
             entries = 2 * (n_dense - n)  # regular entries of the Phi and DPhi near field
@@ -457,7 +457,7 @@ def main():
             f" s{acs['small_bound']} h{huge} q{acs['queue']}"
     print(f"# {args.precision}: {1e3 * elapsed / points:.2f} ms/point, {sched}", file=sys.stderr, flush=True)
     st = stats[ic]
-    points0 = args.steps * len(range(0, len(freqs), max(lanes, 1)))   # points whose launches lane 0 records
+    points0 = args.steps * len(freqs)   # points whose launches the profile window records (every lane)
     out = {
         "metric": "freq-points solved/s at 20k tris (FP32+refine vs FP64); rel-L2 err vs FP64 oracle",
         "value": round(points / elapsed, 4),
@@ -484,13 +484,13 @@ def main():
                    "max_rank_phi": st["max_rank_phi"], "max_rank_dphi": st["max_rank_dphi"],
                    "evaluated_entries": [st["evaluated_phi"], st["evaluated_dphi"]],
                    "phase_ms": {k: round(st[k], 2) for k in ("t_phi_asm", "t_rhs_mvm", "t_dphi_asm", "t_gmres")},
-                   "kernel_ms_lane0": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
+                   "kernel_ms_all_lanes": {f: [round(float(a), 4), int(c)] for f, a, c in zip(FAMILIES, avg_ms, cnt)}},
         "roofline": roofline(avg_ms, cnt, st, blocks, n, args.precision, points0, traffic_sched,
                              quads=1 if args.plane is None else 2, points_per_s=points / elapsed),
     }
-    out["roofline"]["duration_source"] = ("HIP events around each launch on lane 0's launching streams during the "
-                                          "timed steps (a launch's duration includes time it shares the chip "
-                                          "with the other lanes' kernels)")
+    out["roofline"]["duration_source"] = ("HIP events around every launch of the family on all lanes' launching "
+                                          "streams during the timed steps (a launch's duration includes time it "
+                                          "shares the chip with the other lanes' kernels)")
     p_head = pr[ic, 0]
     p64 = None
     pr0 = None

@@ -539,6 +539,10 @@ int bem_solver_aca_schedule(bem_solver_t s, int precision, double* out4);
    bem_set_aca_schedule), taken as tuned: no tuning run, every lane uses it
    (the tuned ms reads 0); bem_solver_set_autotune(s, 0) forgets it */
 int bem_solver_set_aca_schedule(bem_solver_t s, int precision, int64_t small_bound, int64_t huge_bound, int queue);
+/* bem_profile over the solver's context and every lane's: the windows of
+   all of them merged (average ms and launch count per kernel family, the
+   layout of bem_profile); enable as bem_profile, applied to each */
+int bem_solver_profile(bem_solver_t s, int enable, double* out14);
 int bem_solver_sweep(bem_solver_t s, int64_t nf, const double* freqs, double c, double rho, int coupling,
                      double aca_tol, int64_t max_rank, double gmres_tol, int64_t restart, uint64_t seed, int precision,
                      int64_t nrhs, const double* g, const double* planes5, int lanes, double* pressure,

@@ -118,14 +118,21 @@ BEM_HD void det_sincos(double x, double* s_out, double* c_out)
     double hz = 0.5 * z;
     double w = 1.0 - hz;
     double c = w + (((1.0 - w) - hz) + (z * z) * pc);
-    int64_t q = (int64_t)n & 3;
-    double so, co;
-    if (q == 0)      { so = s;  co = c;  }
-    else if (q == 1) { so = c;  co = -s; }
-    else if (q == 2) { so = -s; co = -c; }
-    else             { so = -c; co = s;  }
-    *s_out = so;
-    *c_out = co;
+    /* quadrant q = n mod 4 (|n| < 2^31 for |x| < 1e6):
+       (sin, cos) = (s, c), (c, -s), (-s, -c), (-c, s) for q = 0..3.
+       Selected without branches (odd q swaps, q >= 2 flips both sign
+       bits), so that the device evaluates neighbouring points' chains
+       interleaved; negation is a sign-bit flip, so the values are those of
+       the four-way case split. */
+    int q = (int)n;
+    union { double d; uint64_t u; } so, co;
+    so.d = (q & 1) ? c : s;
+    co.d = (q & 1) ? -s : c;
+    uint64_t flip = (uint64_t)(q & 2) << 62;
+    so.u ^= flip;
+    co.u ^= flip;
+    *s_out = so.d;
+    *c_out = co.d;
 }
 
 /* e^{i x} */

@@ -74,12 +74,37 @@ __global__ void k_quadpoints(int64_t n, const double* __restrict__ nodes, double
     }
 }
 
+// The (row, column) pairs of a dense tile in column-major order, pair p on
+// thread p mod kWG, so that every wavefront's lanes are busy but in the
+// tile's last round.  (With one row per lane, blocks of fewer than 64 rows
+// left lanes idle in every round and the few-row tiles of blocks just over
+// 64 rows left most of them idle: three quarters of the lanes worked on the
+// 20480-triangle sphere's tiles, against 99 % for the pair order.)
+struct TilePairs {
+    int r, c, dr, dc;
+    __device__ TilePairs(int tid, int rows)
+    {
+        c = tid / rows;
+        r = tid - c * rows;
+        dc = kWG / rows;
+        dr = kWG - dc * rows;
+    }
+    __device__ void next(int rows)   // pair p + kWG
+    {
+        r += dr;
+        c += dc;
+        if (r >= rows) {
+            r -= rows;
+            c++;
+        }
+    }
+};
+
 // Dense near-field tiles (K1): regular quadrature of every (row, col) pair of
-// a 64 x 64 tile with the column data staged in LDS.  Self and
+// a 64 x 64 tile with the row and column data staged in LDS.  Self and
 // near-singular pairs are skipped here and written by k_fixup.  The
-// plane-free kernel is budgeted for four wavefronts per SIMD (116 VGPRs, no
-// spills; unbudgeted the compiler takes 138 and three); the plane's would
-// spill at that budget and keeps the compiler's choice.
+// plane-free kernel is budgeted for four wavefronts per SIMD (no spills);
+// the plane's would spill at that budget and keeps the compiler's choice.
 template <bool PLANE>
 __global__ __launch_bounds__(kWG, PLANE ? 1 : 4) void k_dense(DevMesh M, DevFreq F, const Tile* __restrict__ tiles,
                                                bem_c64* outPhi, bem_c64* outDPhi)
@@ -87,6 +112,7 @@ __global__ __launch_bounds__(kWG, PLANE ? 1 : 4) void k_dense(DevMesh M, DevFreq
     __shared__ double sq[3][16][kTile];
     __shared__ double sn[3][kTile], sp[3][kTile], sa[kTile];
     __shared__ double rv[PLANE ? 9 : 1][kTile], rn[PLANE ? 3 : 1][kTile];   // mirror images
+    __shared__ double srow[6][kTile];   // the rows' collocation points and normals
     const Tile T = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int64_t n = M.n;
@@ -109,17 +135,20 @@ __global__ __launch_bounds__(kWG, PLANE ? 1 : 4) void k_dense(DevMesh M, DevFreq
             rn[0][c] = M.rnx[j]; rn[1][c] = M.rny[j]; rn[2][c] = M.rnz[j];
         }
     }
+    for (int r = tid; r < T.rows; r += kWG) {
+        const int64_t i = T.row0 + r;
+        srow[0][r] = M.px[i]; srow[1][r] = M.py[i]; srow[2][r] = M.pz[i];
+        srow[3][r] = M.nx[i]; srow[4][r] = M.ny[i]; srow[5][r] = M.nz[i];
+    }
     __syncthreads();
-    const int r = tid & (kTile - 1), cg = tid >> 6;
-    if (r >= T.rows) return;
-    const int64_t i = T.row0 + r;
-    const double px = M.px[i], py = M.py[i], pz = M.pz[i];
-    const double npx = M.nx[i], npy = M.ny[i], npz = M.nz[i];
-    for (int c = cg; c < T.cols; c += 4) {
-        const int64_t j = T.col0 + c;
-        if (i == j) continue;
+    TilePairs w(tid, T.rows);
+    for (int p = tid; p < T.rows * T.cols; p += kWG, w.next(T.rows)) {
+        const int r = w.r, c = w.c;
+        const int64_t i = T.row0 + r, j = T.col0 + c;
+        const double px = srow[0][r], py = srow[1][r], pz = srow[2][r];
+        const double npx = srow[3][r], npy = srow[4][r], npz = srow[5][r];
         double dx = px - sp[0][c], dy = py - sp[1][c], dz = pz - sp[2][c];
-        if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
+        if (i == j || sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
         bem_ops4 o = regular_ops(F.k, px, py, pz, npx, npy, npz, sn[0][c], sn[1][c], sn[2][c], sa[c], &sq[0][0][c],
                                  &sq[1][0][c], &sq[2][0][c], kTile);
         bem_c64 X, Y;
@@ -184,6 +213,8 @@ __global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const i
     __shared__ double sq[3][16][kTile];
     __shared__ double sp[3][kTile];
     __shared__ float sn[3][kTile], sa[kTile];
+    __shared__ double spr[3][kTile];   // the rows' collocation points and normals
+    __shared__ float snr[3][kTile];
     const Tile T = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int64_t n = M.n;
@@ -202,17 +233,20 @@ __global__ __launch_bounds__(kWG) void k_dense32(DevMesh M, DevFreq32 F, const i
         sp[0][c] = M.px[j]; sp[1][c] = M.py[j]; sp[2][c] = M.pz[j];
         sa[c] = (float)M.area[j];
     }
+    for (int r = tid; r < T.rows; r += kWG) {
+        const int64_t i = T.row0 + r;
+        spr[0][r] = M.px[i]; spr[1][r] = M.py[i]; spr[2][r] = M.pz[i];
+        snr[0][r] = (float)M.nx[i]; snr[1][r] = (float)M.ny[i]; snr[2][r] = (float)M.nz[i];
+    }
     __syncthreads();
-    const int r = tid & (kTile - 1), cg = tid >> 6;
-    if (r >= T.rows) return;
-    const int64_t i = T.row0 + r;
-    const double px = M.px[i], py = M.py[i], pz = M.pz[i];
-    const float npx = (float)M.nx[i], npy = (float)M.ny[i], npz = (float)M.nz[i];
-    for (int c = cg; c < T.cols; c += 4) {
-        const int64_t j = T.col0 + c;
-        if (i == j) continue;
+    TilePairs w(tid, T.rows);
+    for (int p = tid; p < T.rows * T.cols; p += kWG, w.next(T.rows)) {
+        const int r = w.r, c = w.c;
+        const int64_t i = T.row0 + r, j = T.col0 + c;
+        const double px = spr[0][r], py = spr[1][r], pz = spr[2][r];
+        const float npx = snr[0][r], npy = snr[1][r], npz = snr[2][r];
         double dx = px - sp[0][c], dy = py - sp[1][c], dz = pz - sp[2][c];
-        if (sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
+        if (i == j || sqrt(fma(dz, dz, fma(dy, dy, dx * dx))) < M.near_lim) continue;
         bem_c32f4 o = regular_ops_f32(F, px, py, pz, npx, npy, npz, sn[0][c], sn[1][c], sn[2][c], sa[c],
                                       &sq[0][0][c], &sq[1][0][c], &sq[2][0][c], kTile);
         bem_c32 e = bc_entry_f32(F, subst, kind, o, j);

@@ -3445,8 +3445,6 @@ int bem_set_frequency(bem_ctx_t c, double k, double cc_re, double cc_im)
     if (!(k > 0) || !std::isfinite(k)) return set_err(c, BEM_ERR_ARG, "bem_set_frequency: k must be > 0");
     HIPC(c, hipSetDevice(c->device));
     int64_t n = c->n;
-    c->k = k;
-    c->cc = c64(cc_re, cc_im);
     // substitution rule and column coefficients (bem_detmath.h bem_xy notes)
     std::vector<int32_t> subst(n);
     std::vector<bem_c64> cphi(n), cd(n);
@@ -3460,7 +3458,17 @@ int bem_set_frequency(bem_ctx_t c, double k, double cc_re, double cc_im)
             cphi[j] = c64_div(c64(1.0, 0.0), al);
             cd[j] = c64_div(be, al);
         }
+        // a = b = 0 (or a non-finite a, b) leaves the column without finite
+        // coefficients; the kernels are never given such a column
+        if (!(std::isfinite(cphi[j].re) && std::isfinite(cphi[j].im) && std::isfinite(cd[j].re) &&
+              std::isfinite(cd[j].im))) {
+            c->have_freq = false;
+            return set_err(c, BEM_ERR_ARG, "bem_set_frequency: boundary condition of element " + std::to_string(j) +
+                                               " (cluster order) has a = b = 0 or a non-finite coefficient");
+        }
     }
+    c->k = k;
+    c->cc = c64(cc_re, cc_im);
     c->h_subst = subst;
     if (c->freq_n != n) {   // (re)size for the current mesh
         for (void** p : {(void**)&c->d_subst, (void**)&c->d_cphi, (void**)&c->d_cd, (void**)&c->d_cphi32,

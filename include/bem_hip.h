@@ -125,7 +125,10 @@ int bem_set_bc(bem_ctx_t ctx, const double* alpha_ri, const double* beta_ri, con
    hmatrix.cpp:922-1007). */
 int bem_upload_partition(bem_ctx_t ctx, int64_t nblocks, const int64_t* row0, const int64_t* rows,
                          const int64_t* col0, const int64_t* cols, const uint8_t* admissible);
-/* wavenumber k = 2 pi f / c and coupling parameter (boundaryelementsolver.cpp:30-59) */
+/* wavenumber k = 2 pi f / c and coupling parameter (boundaryelementsolver.cpp:30-59);
+   also forms the column coefficients of the boundary conditions and refuses
+   (BEM_ERR_ARG, naming the element) an element whose a and b are both zero
+   or not finite, before any kernel sees it. */
 int bem_set_frequency(bem_ctx_t ctx, double k, double cc_re, double cc_im);
 
 /* assembleBlocks (kind PHI: dense + partialPivotACA) or assembleBlocksExtra

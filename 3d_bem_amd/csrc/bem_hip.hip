@@ -3688,7 +3688,8 @@ template <> constexpr int kMidThreads<bem_c64> = 256;
 template <int NT, class C>
 void launch_aca_nt(bool extra, bool plane, dim3 g, size_t shm, hipStream_t st, const AcaArgs& aq)
 {
-    // batches run without a plane (checked by the caller)
+    // batched items (mode 1) take the impedance plane; the work-queue and
+    // multi-mesh modes run without one (the callers check)
     const int mode = aq.jobs ? (aq.queue ? 4 : 3) : aq.nf == 0 ? 0 : (aq.queue ? 2 : 1);
     if (mode == 3) {
         if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 3>), g, dim3(NT), shm, st, aq);
@@ -3701,8 +3702,13 @@ void launch_aca_nt(bool extra, bool plane, dim3 g, size_t shm, hipStream_t st, c
         return;
     }
     if (mode == 1) {
-        if (extra) hipLaunchKernelGGL((k_aca<true, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
-        else hipLaunchKernelGGL((k_aca<false, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
+        if (extra) {
+            if (plane) hipLaunchKernelGGL((k_aca<true, NT, C, true, 1>), g, dim3(NT), shm, st, aq);
+            else hipLaunchKernelGGL((k_aca<true, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
+        } else {
+            if (plane) hipLaunchKernelGGL((k_aca<false, NT, C, true, 1>), g, dim3(NT), shm, st, aq);
+            else hipLaunchKernelGGL((k_aca<false, NT, C, false, 1>), g, dim3(NT), shm, st, aq);
+        }
         return;
     }
     if (mode == 2) {
@@ -4119,7 +4125,9 @@ int run_aca_batch(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, u
     int32_t* d_rank = (int32_t*)((char*)res + (size_t)nf * nb * 8);
     int32_t* d_flag = d_rank + (size_t)nf * nb;
     int* d_queue = (int*)((char*)res + (size_t)nf * nb * 16);
-    if (c->aca_queue) HIPC(c, hipMemsetAsync(d_queue, 0, 3 * sizeof(int), c->stream));
+    // the work queue's persistent form is not built with the impedance plane
+    const bool useq = c->aca_queue && !c->mesh.rvtx;
+    if (useq) HIPC(c, hipMemsetAsync(d_queue, 0, 3 * sizeof(int), c->stream));
     AcaArgs a;
     a.M = c->mesh;
     a.F = dev_freq(c);
@@ -4154,7 +4162,7 @@ int run_aca_batch(bem_ctx* c, int kind, int prec, double tol, int64_t maxrank, u
         hipStream_t st = q < 2 ? c->side[q] : c->stream;
         if (q < 2) HIPC(c, hipStreamWaitEvent(st, c->ev_fork, 0));
         unsigned grid = (unsigned)ncls[q];
-        if (c->aca_queue) {
+        if (useq) {
             aq.queue = d_queue + q;
             aq.nitems = (int)ncls[q];
             grid = (unsigned)std::min<size_t>(ncls[q], (size_t)(256 * 4 * 4 * 64 / nt[q]));
@@ -4411,7 +4419,6 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
     if (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32)
         return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: bad precision");
     if (!(tol >= 0)) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: negative tolerance");
-    if (c->mesh.rvtx) return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: not with an impedance plane");
     for (int64_t f = 0; f < nf; f++)
         if (!(kcc3[3 * f] > 0) || !std::isfinite(kcc3[3 * f]))
             return set_err(c, BEM_ERR_ARG, "bem_assemble_batch: k must be > 0");
@@ -4505,12 +4512,15 @@ int bem_assemble_batch(bem_ctx_t c, int kind, int64_t nf, const double* kcc3, do
             if (!c->tiles.empty()) {
                 Timer t(c, 0);
                 const dim3 g((unsigned)c->tiles.size());
+                const bool plane = c->mesh.rvtx != nullptr;
                 if (f32 && !n64)
-                    hipLaunchKernelGGL(k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq32(c), c->d_subst,
-                                       kind, c->d_tiles, (bem_c32*)H[f]->dense, (bem_c32*)pd[f]);
+                    hipLaunchKernelGGL(plane ? k_dense32<true> : k_dense32<false>, g, dim3(kWG), 0, c->stream, c->mesh,
+                                       dev_freq32(c), c->d_subst, kind, c->d_tiles, (bem_c32*)H[f]->dense,
+                                       (bem_c32*)pd[f]);
                 else
-                    hipLaunchKernelGGL(k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh, dev_freq(c), c->d_tiles,
-                                       kind == BEM_KIND_PHI ? d64 : nullptr, kind == BEM_KIND_DPHI ? d64 : p64);
+                    hipLaunchKernelGGL(plane ? k_dense<true> : k_dense<false>, g, dim3(kWG), 0, c->stream, c->mesh,
+                                       dev_freq(c), c->d_tiles, kind == BEM_KIND_PHI ? d64 : nullptr,
+                                       kind == BEM_KIND_DPHI ? d64 : p64);
                 if ((rc2 = launch_check(c, "k_dense"))) return rc2;
             }
             if (!c->fixes.empty()) {

@@ -619,8 +619,7 @@ void BoundaryElementSolver::calculateBoundarySolutionsBatch(const std::vector<do
     const int64_t nf = (int64_t)freqs.size(), nr = (int64_t)rhs.size(), n = E.n, nq = nf * nr;
     if (nf < 1 || (nr != 1 && nr != 2 && nr != 4) || nq > 8)
         throw Error(BEM_ERR_ARG, "batched solve: 1, 2 or 4 right-hand sides and at most 8 (frequency, right-hand side) pairs");
-    if (P.preconditioner != 0 || !E.impedancePlanes.empty())
-        throw Error(BEM_ERR_ARG, "batched solve: no preconditioner and no impedance plane");
+    if (P.preconditioner != 0) throw Error(BEM_ERR_ARG, "batched solve: no preconditioner");
     const int R = refineMode(P);
     const double t0 = now_ms();
     check(ctx_, bem_set_near_fp64(ctx_, R == 1), "bem_set_near_fp64");

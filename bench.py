@@ -433,7 +433,7 @@ def main():
     prec = bem.FP32 if args.precision == "fp32" else bem.FP64
     freqs = batch_freqs(args.freq, args.batch)
     ic = int(np.argmin(np.abs(freqs - args.freq)))
-    batched = prec == bem.FP32 and args.sweep_batch > 1 and args.plane is None
+    batched = prec == bem.FP32 and args.sweep_batch > 1
     lanes = min(args.lanes, -(-args.batch // args.sweep_batch)) if batched else args.lanes if prec == bem.FP32 else 1
     sb = args.sweep_batch if batched else 0
     refine = args.refine if prec == bem.FP32 else 0
@@ -494,7 +494,7 @@ def main():
     p_head = pr[ic, 0]
     p64 = None
     pr0 = None
-    if prec == bem.FP32 and args.plane is None and not args.no_batch_leg:
+    if prec == bem.FP32 and not args.no_batch_leg:
         # the other scheduling of the same FP32 points: lanes, or batched
         # chunks of --leg-batch on as many lanes as there are chunks
         ob = 0 if batched else args.leg_batch

@@ -118,6 +118,72 @@ def test_plane_solver_parity_and_fp32(bem, O, L3):
     S.close()
 
 
+def same_blocks(G, H, nb):
+    for b in range(nb):
+        A, B = G.block(b), H.block(b)
+        if isinstance(A, tuple):
+            if not isinstance(B, tuple) or len(A) != len(B) or not all(same_bits(x, y) for x, y in zip(A, B)):
+                return False, b
+        elif not same_bits(A, B):
+            return False, b
+    return True, -1
+
+
+def test_plane_batched_sweep(bem, O, L3):
+    """Batched frequency chunks with the plane: two frequencies assembled in
+    one launch per size class (every entry with its mirror image's
+    quadrature) equal the one-by-one assemblies bit for bit in FP64 and FP32
+    (a work-queue request is ignored with the plane); a batched FP64 sweep
+    equals the restatement's single solves (1e-10), and the batched FP32
+    sweep with refinement mode 1 the lanes' (1e-8)."""
+    tri, bc, P, ctx = L3
+    kcc = []
+    for f in (400.0, 650.0):
+        P.set_frequency(f, 343.0, 1)
+        prm = P.params()
+        kcc.append([prm["k"], prm["cc"].real, prm["cc"].imag])
+    kcc = np.array(kcc)
+    nb = len(P.blocks)
+    for prec in (bem.FP64, bem.FP32):
+        ctx.set_aca_schedule(prec, 384, 2048 if prec == bem.FP32 else 1 << 40, 1)
+        for kind in (0, 1):
+            single = []
+            for k, cr, ci in kcc:
+                ctx.set_frequency(k, complex(cr, ci))
+                single.append(ctx.assemble(kind, 1e-6, 0, precision=prec, seed=3))
+            batch = ctx.assemble_batch(kind, kcc, 1e-6, precision=prec, seed=3)
+            for f in range(len(kcc)):
+                ok, b = same_blocks(batch[f], single[f], nb)
+                assert ok, (prec, kind, f, b)
+            for G in batch + single:
+                G.close()
+        ctx.set_aca_schedule(prec, 384, 2048 if prec == bem.FP32 else 1 << 40, 0)
+    P.set_frequency(500.0, 343.0, 1)
+    prm = P.params()
+    ctx.set_frequency(prm["k"], prm["cc"])
+    S = bem.Solver(tri, bc, leaf=32, eta=2.0)
+    S.set_impedance_plane(P0, NRM)
+    freqs = np.array([400.0, 500.0, 650.0])
+    S.set_sweep_batch(2)
+    pb, its, _ = S.sweep(freqs, c=343.0, rho=1.21, aca_tol=1e-6, gmres_tol=1e-10, seed=1)
+    assert pb.shape == (3, 1, len(tri)) and np.all(its > 0)
+    for fi, f in enumerate(freqs):
+        P.set_frequency(f, 343.0, 1)
+        _, _, ref, _ = P.solve(1e-6, gmres_tol=1e-10, seed=1, rho=1.21, freq=f)
+        e = np.linalg.norm(pb[fi, 0] - ref) / np.linalg.norm(ref)
+        assert e < 1e-10, (f, e)
+    P.set_frequency(500.0, 343.0, 1)
+    S.set_refinement(1)
+    p32b, _, _ = S.sweep(freqs, c=343.0, rho=1.21, aca_tol=1e-6, gmres_tol=1e-10, seed=1, precision=bem.FP32)
+    S.set_sweep_batch(0)
+    p32l, _, _ = S.sweep(freqs, c=343.0, rho=1.21, aca_tol=1e-6, gmres_tol=1e-10, seed=1, precision=bem.FP32,
+                         lanes=2)
+    for fi in range(3):
+        e = np.linalg.norm(p32b[fi, 0] - p32l[fi, 0]) / np.linalg.norm(p32l[fi, 0])
+        assert e < 1e-8, (fi, e)
+    S.close()
+
+
 def test_method_of_images_fp32(bem, meshes):
     """L4 sphere (5120 elements) over the plane vs the sphere and its mirror
     image (10240 elements) in free space, both FP32 H-matrix solves with ACA

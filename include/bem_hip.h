@@ -148,8 +148,11 @@ int bem_hmat_destroy(bem_hmat_t h);
    takes them.  The ACA runs as one
    launch per size class over the flattened (frequency, block) list, the
    same-size blocks of all frequencies adjacent and the largest first; the
-   self / near-singular tables and dense tiles per frequency.  Not with an
-   impedance plane; 1 <= nf <= 64.  The context's frequency is unchanged.
+   self / near-singular tables and dense tiles per frequency.  With an
+   impedance plane every entry carries the mirror image's quadrature as in
+   bem_assemble, and the ACA runs one workgroup per item (a work-queue
+   schedule is ignored); 1 <= nf <= 64.  The context's frequency is
+   unchanged.
    bem_set_aca_schedule: the ACA size-class bounds on m + n for one
    precision (one wavefront up to small_bound, the middle workgroup shape
    up to huge_bound, the largest above) and, for queue >= 0, whether batched
@@ -527,8 +530,8 @@ int bem_refine_triangles(int64_t n, const double* tri9, double edge_length, int6
    each (bem_assemble_batch), the GMRES of all its (frequency, right-hand
    side) pairs in one lockstep solve (bem_gmres_batch); the sweep's lanes
    take whole chunks (one lane: no extra hardware queues needed); nrhs 1, 2
-   or 4, no preconditioner, no impedance plane.  0 or 1: one point per lane
-   at a time. */
+   or 4, no preconditioner (an impedance plane is taken).  0 or 1: one
+   point per lane at a time. */
 int bem_solver_set_sweep_batch(bem_solver_t s, int batch);
 /* on: the first batched sweep of each precision first runs
    bem_autotune_aca on its first chunk (DPhi, the sweep's parameters, timed

@@ -9,8 +9,8 @@ import pytest
 
 def test_det_sincos_ulp(O):
     rng = np.random.default_rng(0)
-    x = np.concatenate([rng.uniform(0, 300, 200000), rng.uniform(-2, 2, 50000),
-                        (np.arange(400) * (np.pi / 2)) + rng.uniform(-1e-6, 1e-6, 400)])
+    x = np.concatenate([rng.uniform(0, 300, 200000), rng.uniform(-2, 2, 50000), rng.uniform(-300, 0, 50000),
+                        (np.arange(-400, 400) * (np.pi / 2)) + rng.uniform(-1e-6, 1e-6, 800)])
     s, c = O.sincos(x)
     xl = x.astype(np.longdouble)
     rs, rc = np.sin(xl), np.cos(xl)

@@ -3292,6 +3292,17 @@ int bem_upload_mesh(bem_ctx_t c, int64_t n, const double* nodes9, const double* 
 {
     if (!c || n <= 0 || !nodes9 || !normal3 || !area || !colloc3 || n > (int64_t)INT32_MAX)
         return set_err(c, BEM_ERR_ARG, "bem_upload_mesh: bad arguments");
+    if (!(std::isfinite(hbar) && hbar > 0)) return set_err(c, BEM_ERR_ARG, "bem_upload_mesh: hbar must be > 0");
+    // a degenerate or non-finite element would give its whole column
+    // non-finite entries; the kernels are never given one
+    for (int64_t e = 0; e < n; e++) {
+        bool ok = std::isfinite(area[e]) && area[e] > 0;
+        for (int d = 0; d < 9 && ok; d++) ok = std::isfinite(nodes9[9 * e + d]);
+        for (int d = 0; d < 3 && ok; d++) ok = std::isfinite(normal3[3 * e + d]) && std::isfinite(colloc3[3 * e + d]);
+        if (!ok)
+            return set_err(c, BEM_ERR_ARG, "bem_upload_mesh: element " + std::to_string(e) +
+                                               " is degenerate (area not > 0) or has a non-finite coordinate");
+    }
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
     free_list(c, c->allocs_mesh);

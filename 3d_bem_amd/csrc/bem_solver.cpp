@@ -1020,6 +1020,10 @@ int bem_solver_create(bem_solver_t* out, int device, int64_t n, const double* no
     if (!out) return BEM_ERR_ARG;
     *out = nullptr;
     if (n <= 0 || !nodes9 || !bc6 || leaf < 1 || !(eta > 0)) return BEM_ERR_ARG;
+    // the cluster tree bisects the elements' centroids: non-finite vertices
+    // are refused before it (degenerate elements are refused at the upload)
+    for (int64_t q = 0; q < 9 * n; q++)
+        if (!std::isfinite(nodes9[q])) return BEM_ERR_ARG;
     auto S = new bem_solver();
     try {
         bem::BoundaryElements E;

@@ -60,7 +60,9 @@ const char* bem_last_error(bem_ctx_t ctx);
    HMatrix/clustertree.cpp:265-283): nodes n x 9 (a, b, c), normal n x 3,
    area n, colloc n x 3 (centroids), hbar = averageTriangleDim
    (boundaryelements.cpp:35-46).  Replaces the geometry read by
-   BemOperatorsConst (boundaryelementsolver.cpp:844-894). */
+   BemOperatorsConst (boundaryelementsolver.cpp:844-894).  An element with
+   a non-finite coordinate or an area not > 0 is refused (BEM_ERR_ARG,
+   naming it) before the context changes. */
 int bem_upload_mesh(bem_ctx_t ctx, int64_t n, const double* nodes9, const double* normal3, const double* area,
                     const double* colloc3, double hbar);
 /* Host H-LU (the reference keeps the H-LU preconditioner on the host:
@@ -386,7 +388,9 @@ int bem_profile(bem_ctx_t ctx, int enable, double* out14);
 
 /* BoundaryElementSolver::hMatrixSolve (boundaryelementsolver.cpp:526-744) as
    one call: cluster tree + partition on the host, everything else on the
-   device.  Mesh in input order: nodes n x 9, bc n x 6 (a, b, g complex). */
+   device.  Mesh in input order: nodes n x 9, bc n x 6 (a, b, g complex).
+   Non-finite vertex coordinates are refused (BEM_ERR_ARG) before the
+   cluster tree is built or a device is touched. */
 int bem_solver_create(bem_solver_t* out, int device, int64_t n, const double* nodes9, const double* bc6,
                       int64_t leaf, double eta);
 int bem_solver_destroy(bem_solver_t s);

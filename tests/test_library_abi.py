@@ -45,3 +45,18 @@ def test_null_arguments_rejected(bem):
     assert L.bem_hmat_destroy(None) == bem.BEM_ERR_ARG
     assert L.bem_solver_destroy(None) == bem.BEM_ERR_ARG
     assert L.bem_create(None, 0) == bem.BEM_ERR_ARG
+
+
+def test_non_finite_vertices_refused_before_any_device_call(bem):
+    """bem_solver_create refuses a mesh with a non-finite vertex coordinate
+    (BEM_ERR_ARG) before it builds the cluster tree or touches a device, so
+    the refusal is the same with or without a GPU."""
+    import numpy as np
+    tri = bem.meshes.icosphere(2, 0.5)
+    bc = bem.meshes.robin_bc(len(tri), 0.0, 1.0, 1.0)
+    for bad in (np.nan, np.inf, -np.inf):
+        t = tri.copy()
+        t[37, 4] = bad
+        with pytest.raises(bem.BemError) as e:
+            bem.Solver(t, bc, leaf=16, eta=2.0)
+        assert e.value.code == bem.BEM_ERR_ARG
